@@ -1,0 +1,247 @@
+/*
+ * lmm.h - C ABI of the MI355X (gfx950) GMRES + staggered-multigrid Stokes
+ * solver and the fluctuating-flux stencils that feed it.
+ *
+ * The entry points keep the argument order and meaning of the reference's
+ * Fortran routines (BoxLib-Codes/LowMachMixtures); each declaration cites
+ * the routine it replaces.  A multifab argument of the reference becomes a
+ * device-resident field handle (lmm_field); the ml_layout, bc_tower, dx and
+ * the probin_* module globals become the context (lmm_ctx).
+ *
+ * Data model: one periodic box of n[0] x n[1] x n[2] cells (n[2] = 1 in 2D)
+ * per context.  Fields live in HBM, x fastest, without ghost cells; a face
+ * component stores the lower face of each cell, the periodic hi+1 face being
+ * the same unknown.  lmm_field_upload_fab / lmm_field_download_fab convert
+ * from / to the reference's own per-box Fortran arrays (lower bound lo-ng,
+ * column-major, hi+1 faces and ghost cells included).
+ *
+ * Every call returns an lmm_status.  Calls on one context are stream
+ * ordered on the context's HIP stream and are not re-entrant; a call that
+ * returns data to the host synchronises that stream.
+ */
+#ifndef LMM_H
+#define LMM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lmm_ctx lmm_ctx;
+typedef struct lmm_field lmm_field;
+
+enum lmm_status {
+  LMM_OK = 0,
+  LMM_ERR_ARG = 1,          /* bad argument (null, wrong kind, size mismatch) */
+  LMM_ERR_UNSUPPORTED = 2,  /* option outside the implemented set */
+  LMM_ERR_BREAKDOWN = 3,    /* GMRES orthogonalisation breakdown (gmres.f90:377) */
+  LMM_ERR_DEVICE = 4,       /* HIP runtime error */
+  LMM_ERR_NOMEM = 5,
+  LMM_NOT_CONVERGED = 6     /* reported in lmm_gmres_info only; the call returns LMM_OK */
+};
+
+enum lmm_kind {
+  LMM_CELL = 0,
+  LMM_FACE_X = 1,
+  LMM_FACE_Y = 2,
+  LMM_FACE_Z = 3,
+  LMM_NODE = 4,     /* 2D nodes: node (i,j) is the lower-left corner of cell (i,j) */
+  LMM_EDGE_XY = 5,  /* 3D edges, nodal in the two named directions */
+  LMM_EDGE_XZ = 6,
+  LMM_EDGE_YZ = 7
+};
+
+enum lmm_precision {
+  LMM_FP64 = 0,      /* the reference algorithm: FP64 left-preconditioned GMRES */
+  LMM_FP32_MG = 1,   /* FP32 V-cycles inside an FP64 right-preconditioned flexible GMRES */
+  LMM_LEARNED = 2    /* LMM_FP32_MG plus the learned coarse-grid correction stage */
+};
+
+/* Solver knobs.  Names, meaning and defaults follow
+ * src_gmres/probin_gmres.f90:119-157 and probin_common.f90 (visc_type). */
+typedef struct {
+  int visc_type;               /* +-1, +-2, +-3 (stag_applyop.f90) */
+  int precon_type;             /* 1 = projection preconditioner (apply_precon.f90:103) */
+  double p_norm_weight;
+  double scale_factor;
+  int mg_max_vcycles, mg_minwidth;
+  int mg_nsmooths_down, mg_nsmooths_up, mg_nsmooths_bottom;
+  double mg_rel_tol;
+  int stag_mg_max_vcycles, stag_mg_minwidth;
+  int stag_mg_nsmooths_down, stag_mg_nsmooths_up, stag_mg_nsmooths_bottom;
+  int stag_mg_smoother;        /* 0 = Jacobi, 1 = 2*dim-colour Gauss-Seidel */
+  double stag_mg_omega, stag_mg_rel_tol;
+  double gmres_rel_tol, gmres_abs_tol;
+  int gmres_max_outer, gmres_max_inner, gmres_max_iter, gmres_min_iter;
+  int gmres_verbose;
+  int precision;               /* enum lmm_precision */
+} lmm_params;
+
+typedef struct {
+  int total_iter;              /* inner iterations (gmres.f90 total_iter) */
+  int outer_iter;
+  double norm_pre_rhs;         /* |M^-1 b| (gmres.f90:125-134) */
+  double norm_resid;           /* final (preconditioned for LMM_FP64) residual */
+  double norm_b;
+  double norm_resid_true;      /* last un-preconditioned residual |b - Ax| */
+  int status;                  /* LMM_OK or LMM_NOT_CONVERGED */
+  int vcycles;                 /* V-cycles run (stag + cell-centred) */
+  double seconds;              /* wall time of the call */
+} lmm_gmres_info;
+
+/* ---------------- context ---------------- */
+
+/* Fills p with the reference defaults (probin_gmres.f90:119-157, visc_type 1). */
+void lmm_params_default(lmm_params *p);
+
+/* Replaces ml_layout + bc_tower + dx for one periodic box of n_cells cells.
+ * Every multigrid level must have an even number of cells per direction
+ * (periodic hi+1 faces then coincide with lo faces under coloured sweeps). */
+int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p, lmm_ctx **out);
+int lmm_ctx_destroy(lmm_ctx *ctx);
+int lmm_ctx_set_params(lmm_ctx *ctx, const lmm_params *p);
+int lmm_ctx_get_params(const lmm_ctx *ctx, lmm_params *p);
+int lmm_ctx_synchronize(lmm_ctx *ctx);
+/* Number of staggered multigrid levels (compute_nlevs_mg, stag_mg_layout.f90:179-222). */
+int lmm_ctx_nlevels(const lmm_ctx *ctx, int *nlevs_stag, int *nlevs_cc);
+/* Last HIP or argument error message of this context. */
+const char *lmm_ctx_last_error(const lmm_ctx *ctx);
+/* Kernel timing: when enabled, HIP events bracket every launch of the named
+ * kernel families on the context stream.  family: "stag_smooth", "stag_restrict",
+ * "stag_prolong", "cc_smooth", "apply_matrix", "krylov". */
+int lmm_ctx_set_profiling(lmm_ctx *ctx, int on);
+int lmm_ctx_kernel_stats(lmm_ctx *ctx, const char *family, int64_t *launches, double *total_ms,
+                         double *algorithmic_bytes);
+int lmm_ctx_reset_stats(lmm_ctx *ctx);
+
+/* ---------------- fields (multifab replacement) ---------------- */
+
+int lmm_field_create(lmm_ctx *ctx, int kind, int ncomp, lmm_field **out);
+int lmm_field_destroy(lmm_field *f);
+int lmm_field_info(const lmm_field *f, int *kind, int *ncomp, int64_t *count_per_comp);
+/* Deduplicated host layout: ncomp consecutive blocks of nx*ny*nz doubles. */
+int lmm_field_upload(lmm_field *f, const double *host);
+int lmm_field_download(const lmm_field *f, double *host);
+/* Reference fab layout for component comp (0-based): host array with lower
+ * bound lo-ng and upper bound hi+ng (hi includes the nodal +1), column-major.
+ * Upload reads the valid region (the hi+1 face must equal the lo face);
+ * download writes the valid region, both copies of periodic faces, and fills
+ * the ng ghost layers from the periodic images. */
+int lmm_field_upload_fab(lmm_field *f, int comp, const double *host, const int lo[3],
+                         const int hi[3], int ng);
+int lmm_field_download_fab(const lmm_field *f, int comp, double *host, const int lo[3],
+                           const int hi[3], int ng);
+int lmm_field_setval(lmm_field *f, double v);
+int lmm_field_copy(lmm_field *dst, const lmm_field *src);
+/* Device address of the field (ncomp blocks of nx*ny*nz doubles). */
+double *lmm_field_device_ptr(lmm_field *f);
+
+/* ---------------- operators and solvers ---------------- */
+
+/* gmres.f90:30-45  gmres(mla,the_bc_tower,dx,b_u,b_p,x_u,x_p,alpha_fc,beta,beta_ed,
+ *                        gamma,theta_alpha,norm_pre_rhs)
+ * x_u, x_p: initial guess in, solution out.  b_u, coefficients are read only
+ * (the reference scales and restores them when scale_factor != 1).
+ * norm_pre_rhs may be NULL.  info may be NULL. */
+int lmm_gmres(lmm_ctx *ctx, lmm_field *const b_u[3], lmm_field *b_p, lmm_field *x_u[3],
+              lmm_field *x_p, lmm_field *const alpha_fc[3], lmm_field *beta,
+              lmm_field *const beta_ed[3], lmm_field *gamma, double theta_alpha,
+              double *norm_pre_rhs, lmm_gmres_info *info);
+
+/* apply_matrix.f90:23-39  apply_matrix(mla,b_u,b_p,x_u,x_p,alpha_fc,beta,beta_ed,gamma,
+ *                                       theta_alpha,dx,the_bc_tower)
+ * [b_u; b_p] = [A G; -D 0][x_u; x_p], A = theta_alpha*alpha - L_beta. */
+int lmm_apply_matrix(lmm_ctx *ctx, lmm_field *b_u[3], lmm_field *b_p, lmm_field *const x_u[3],
+                     lmm_field *x_p, lmm_field *const alpha_fc[3], lmm_field *beta,
+                     lmm_field *const beta_ed[3], lmm_field *gamma, double theta_alpha);
+
+/* apply_precon.f90:29-43  apply_precon(mla,b_u,b_p,x_u,x_p,alpha_fc,beta,beta_ed,gamma,
+ *                                       theta_alpha,dx,the_bc_tower)
+ * x = M^{-1} b with the projection preconditioner (precon_type 1). */
+int lmm_apply_precon(lmm_ctx *ctx, lmm_field *const b_u[3], lmm_field *b_p, lmm_field *x_u[3],
+                     lmm_field *x_p, lmm_field *const alpha_fc[3], lmm_field *beta,
+                     lmm_field *const beta_ed[3], lmm_field *gamma, double theta_alpha);
+
+/* stag_mg_solver.f90:37-50  stag_mg_solver(mla,la_mg,alpha_fc,beta_cc,beta_ed,gamma_cc,
+ *                                          theta_alpha,phi_fc,rhs_fc,dx,the_bc_tower)
+ * Solves (theta_alpha*alpha - L)phi = rhs; phi_fc is the initial guess. */
+int lmm_stag_mg_solver(lmm_ctx *ctx, lmm_field *const alpha_fc[3], lmm_field *beta_cc,
+                       lmm_field *const beta_ed[3], lmm_field *gamma_cc, double theta_alpha,
+                       lmm_field *phi_fc[3], lmm_field *const rhs_fc[3]);
+
+/* macproject.f90:34-45  macproject(mla,phi,umac,alphainv_fc,mac_rhs,dx,the_bc_tower)
+ * Solves div(alphainv grad phi) = mac_rhs with one cell-centred V-cycle
+ * (mg_* knobs); phi is zeroed on entry.  umac feeds only an unused norm in
+ * the reference and may be NULL. */
+int lmm_macproject(lmm_ctx *ctx, lmm_field *phi, lmm_field *const umac[3],
+                   lmm_field *const alphainv_fc[3], lmm_field *mac_rhs);
+
+/* stag_applyop.f90:70-81  stag_applyop_level(la,the_bc_level,phi_fc,Lphi_fc,alpha_fc,beta_cc,
+ *                                            beta_ed,gamma_cc,dx,color)
+ * alpha_fc is used as given (stag_applyop multiplies by theta_alpha first; pass
+ * theta_alpha != 1 to get stag_applyop.f90:20-67). color 0 = every face. */
+int lmm_stag_applyop(lmm_ctx *ctx, lmm_field *const phi_fc[3], lmm_field *Lphi_fc[3],
+                     lmm_field *const alpha_fc[3], lmm_field *beta_cc,
+                     lmm_field *const beta_ed[3], lmm_field *gamma_cc, double theta_alpha,
+                     int color);
+
+/* norm_inner_product.f90:16-21 stag_inner_prod / :266-271 cc_inner_prod
+ * prod_val[d] per face component; box-boundary faces weighted 1.5 as in the reference. */
+int lmm_stag_inner_prod(lmm_ctx *ctx, lmm_field *const m1[3], lmm_field *const m2[3],
+                        double prod_val[3]);
+int lmm_cc_inner_prod(lmm_ctx *ctx, lmm_field *m1, lmm_field *m2, double *prod_val);
+
+/* div_and_grad.f90:304-310 compute_div (comp 0 only, increment flag) and
+ * div_and_grad.f90:16-19 compute_grad (periodic box). */
+int lmm_compute_div(lmm_ctx *ctx, lmm_field *const phi_fc[3], lmm_field *div, int increment);
+int lmm_compute_grad(lmm_ctx *ctx, lmm_field *phi, lmm_field *gradp[3]);
+
+/* convert_stag.f90:132 average_cc_to_face, :395 average_cc_to_node (2D), :612
+ * average_cc_to_edge (3D) for component 0 of a cell field. */
+int lmm_average_cc_to_face(lmm_ctx *ctx, lmm_field *cc, lmm_field *face[3]);
+int lmm_average_cc_to_node(lmm_ctx *ctx, lmm_field *cc, lmm_field *node);
+int lmm_average_cc_to_edge(lmm_ctx *ctx, lmm_field *cc, lmm_field *edge[3]);
+
+/* stochastic_m_fluxdiv.f90:41-51  stochastic_m_fluxdiv(mla,the_bc_level,stoch_m_force,eta,
+ *        eta_ed,temperature,temperature_ed,dx,dt,weights)
+ * Increments stoch_m_force by div(sqrt(variance_coef_mom*2*k_B/(dV*dt)) sqrt(eta*T) W)
+ * where W = sum_r weights[r] * W_r is the noise held by the context
+ * (lmm_init_m_stochastic / lmm_fill_m_stochastic).  Non-symmetric stress form,
+ * filtering_width 0. */
+int lmm_stochastic_m_fluxdiv(lmm_ctx *ctx, lmm_field *stoch_m_force[3], lmm_field *eta,
+                             lmm_field *const eta_ed[3], lmm_field *temperature,
+                             lmm_field *const temperature_ed[3], double dt,
+                             const double *weights, double variance_coef_mom, double k_B);
+/* stochastic_m_fluxdiv.f90:613 init_m_stochastic(mla,n_rngs) */
+int lmm_init_m_stochastic(lmm_ctx *ctx, int n_rngs);
+/* stochastic_m_fluxdiv.f90:556 fill_m_stochastic(mla): fills every stage with
+ * N(0,1) numbers from the host generator (MT19937 seeded with seed, Acklam
+ * inverse normal), in the order: stage, W_cc components, then node/edge
+ * arrays and their 2 components, x fastest. */
+int lmm_fill_m_stochastic(lmm_ctx *ctx, uint32_t seed);
+/* Same divergence with caller-provided noise: W_cc has dim components, W_ed
+ * has 2 components per node (2D, W_ed[0]) or edge array (3D). */
+int lmm_stoch_m_force(lmm_ctx *ctx, lmm_field *stoch_m_force[3], lmm_field *W_cc,
+                      lmm_field *const W_ed[3], lmm_field *eta, lmm_field *const eta_ed[3],
+                      lmm_field *temperature, lmm_field *const temperature_ed[3], double variance);
+
+/* diffusive_mass_fluxdiv.f90:26-42  diffusive_mass_fluxdiv(mla,rho,rhotot,molarconc,rhoWchi,
+ *        Gama,diff_fluxdiv,Temp,zeta_by_Temp,gradp_baro,flux_total,dx,the_bc_tower)
+ * Isothermal, no barodiffusion: Temp, zeta_by_Temp, gradp_baro must be NULL.
+ * rhoWchi and Gama hold nspecies^2 components (Fortran (ns,ns) order);
+ * flux_total[d] is incremented; diff_fluxdiv is overwritten. */
+int lmm_diffusive_mass_fluxdiv(lmm_ctx *ctx, int nspecies, lmm_field *rho, lmm_field *rhotot,
+                               lmm_field *molarconc, lmm_field *rhoWchi, lmm_field *Gama,
+                               lmm_field *diff_fluxdiv, lmm_field *Temp, lmm_field *zeta_by_Temp,
+                               lmm_field *const gradp_baro[3], lmm_field *flux_total[3],
+                               int correct_flux);
+
+/* Host Gaussian generator of the noise fields (RNGs.c:247 genrandn stream:
+ * MT19937 init_genrand(seed) + Acklam inverse normal). */
+int lmm_fill_normal_host(uint32_t seed, int64_t count, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

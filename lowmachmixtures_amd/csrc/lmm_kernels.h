@@ -1,0 +1,129 @@
+// Host-side launchers of the gfx950 kernels (lmm_kernels.hip).
+#pragma once
+
+#include "lmm_common.h"
+
+namespace lmm {
+
+// Device scalars used by Krylov kernels live in one small array per context.
+struct DotOut {
+  double *partial;   // [nblk_red * 4]
+  double *scalars;   // result slots
+};
+
+constexpr int RED_BLOCKS = 1024;
+
+// Launch bookkeeping: every launch is followed by post_launch(), which checks
+// hipGetLastError() and, when debug_sync is set (environment LMM_DEBUG_SYNC=1),
+// synchronises the stream so a failing kernel is named in first_error().
+void set_debug_sync(int on);
+void post_launch(hipStream_t s, const char *name);
+const char *first_error();
+void clear_first_error();
+constexpr int RED_TPB = 256;
+
+template <typename T>
+struct Faces {
+  T *c[3];
+};
+
+// ---- staggered operator ----
+template <typename T>
+void launch_stag_gs_colour(hipStream_t s, const Grid &g, int visc, int a, int par, T omega,
+                           T *const phi[3], const StagCoef<T> &K, const T *rhs);
+template <typename T>
+void launch_stag_applyop(hipStream_t s, const Grid &g, int visc, int color, T *const phi[3],
+                         T *const Lphi[3], const StagCoef<T> &K);
+template <typename T>
+void launch_stag_jacobi(hipStream_t s, const Grid &g, int visc, T omega, T *const phi[3],
+                        T *const rhs[3], T *const tmp[3], const StagCoef<T> &K);
+template <typename T>
+void launch_stag_residual(hipStream_t s, const Grid &g, int visc, T *const phi[3], T *const rhs[3],
+                          T *const res[3], const StagCoef<T> &K);
+template <typename T>
+void launch_stag_restrict(hipStream_t s, const Grid &gc, T *const fine[3], T *const coarse[3],
+                          int simple);
+template <typename T>
+void launch_stag_prolong(hipStream_t s, const Grid &gf, T *const fine[3], T *const coarse[3]);
+template <typename T>
+void launch_cc_restrict(hipStream_t s, const Grid &gc, const T *fine, T *coarse);
+template <typename T>
+void launch_edge_restrict(hipStream_t s, const Grid &gc, T *const fine[3], T *const coarse[3]);
+template <typename T>
+void launch_nodal_restrict(hipStream_t s, const Grid &gc, const T *fine, T *coarse);
+
+// ---- cell-centred multigrid ----
+template <typename T>
+void launch_cc_rb_colour(hipStream_t s, const Grid &g, int colour, T *phi, const T *rhs,
+                         T *const a[3]);
+template <typename T>
+void launch_cc_residual(hipStream_t s, const Grid &g, const T *phi, const T *rhs, T *res,
+                        T *const a[3]);
+template <typename T>
+void launch_cc_prolong_add(hipStream_t s, const Grid &gf, T *fine, const T *coarse);
+
+// ---- elementwise / conversions ----
+template <typename T>
+void launch_fill(hipStream_t s, T *x, long n, T v);
+template <typename T>
+void launch_scale(hipStream_t s, T *x, long n, T v);
+template <typename TD, typename TS>
+void launch_convert(hipStream_t s, TD *dst, const TS *src, long n);
+void launch_reciprocal(hipStream_t s, double *dst, const double *src, long n);
+void launch_scale_copy(hipStream_t s, double *dst, const double *src, long n, double v);
+
+// ---- Stokes pieces (FP64 unless templated) ----
+template <typename T>
+void launch_apply_matrix(hipStream_t s, const Grid &g, int visc, T *const x_u[3], const T *x_p,
+                         T *const b_u[3], T *b_p, const StagCoef<T> &K);
+template <typename T>
+void launch_div_plus(hipStream_t s, const Grid &g, T *const u[3], const T *b_p, T *div,
+                     int mode);  // mode 0: div; 1: div + b_p; 2: div*(-1); 3: div_in + div
+template <typename T>
+void launch_precon_final(hipStream_t s, const Grid &g, int visc, T *const x_u[3], T *x_p,
+                         const T *phi, const T *mac_rhs, T *const ainv[3], const T *beta,
+                         const T *gamma, double theta_alpha, double *partial);
+void launch_grad(hipStream_t s, const Grid &g, const double *p, double *const gp[3]);
+void launch_avg_cc_to_face(hipStream_t s, const Grid &g, const double *cc, double *const face[3]);
+void launch_avg_cc_to_node(hipStream_t s, const Grid &g, const double *cc, double *node);
+void launch_avg_cc_to_edge(hipStream_t s, const Grid &g, const double *cc, double *const edge[3]);
+
+// ---- reductions (deterministic two-stage) ----
+// Segmented weighted dot over the Stokes vector layout [u0|u1|(u2)|p]:
+// face segments weight face index 0 along their direction by 3 (the lo and
+// hi+1 copies of the reference, each 1 + 0.5).
+void launch_stokes_dot(hipStream_t s, const Grid &g, const double *a, const double *b,
+                       double *partial);
+// Finalize: op 0 -> out[0] = (s0 + s1 (+ s2)) + pnw2*s3
+//           op 1 -> out[0] = sqrt(sqrt(su)^2 + (pnw*sqrt(s3))^2), out[1] = sqrt(su), out[2] = pnw*sqrt(s3)
+//           op 2 -> out[0..3] = raw segment sums
+//           op 3 -> out[0] = s3 / total_volume (pressure mean)
+//           op 4 -> out[d] = face-sum_d / total_volume (velocity means)
+void launch_finalize(hipStream_t s, const Grid &g, const double *partial, double *out, int op,
+                     double pnw);
+void launch_face_maxabs(hipStream_t s, const Grid &g, const double *const r[3], double *partial);
+void launch_finalize_max(hipStream_t s, const double *partial, double *out);
+template <typename T>
+void launch_sum_faces_weighted(hipStream_t s, const Grid &g, T *const u[3], double *partial);
+
+// ---- Krylov vector kernels on [u0|u1|(u2)|p] of length (dim+1)*N ----
+void launch_bmax_neg(hipStream_t s, double *tmp, const double *b, long n);       // tmp = -(tmp - b)
+void launch_div_by(hipStream_t s, double *v, const double *src, long n, const double *scal);  // v = src / *scal
+void launch_axmy(hipStream_t s, double *w, const double *v, long n, const double *scal);     // w = w - v * *scal
+void launch_update_sol(hipStream_t s, double *x, double *const V[], const double *y, int i, long n);
+template <typename T>
+void launch_sub_scalar(hipStream_t s, T *x, long n, const double *scal);
+template <typename T>
+void launch_sub_scalar3(hipStream_t s, const Grid &g, T *const u[3], const double *scal);
+
+// ---- fluctuating and diffusive fluxes ----
+void launch_stoch_m_force(hipStream_t s, const Grid &g, double *const force[3], const double *Wcc,
+                          double *const Wed[3], int nstages, const double *weights, long stage_stride_cc,
+                          long stage_stride_ed, const double *eta, double *const eta_ed[3],
+                          const double *temp, double *const temp_ed[3], double variance);
+void launch_diffusive_mass_flux(hipStream_t s, const Grid &g, int ns, const double *molarconc,
+                                const double *rhoWchi, const double *Gama, double *flux,
+                                double *const flux_total[3], int correct_flux);
+void launch_div_species(hipStream_t s, const Grid &g, int ns, const double *flux, double *div);
+
+}  // namespace lmm

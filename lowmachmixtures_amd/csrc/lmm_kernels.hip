@@ -1,0 +1,1077 @@
+// gfx950 kernels of the Stokes library: staggered and cell-centred multigrid
+// pieces, the fused Stokes operator, preconditioner assembly, deterministic
+// reductions, Krylov vector kernels and the flux stencils.
+//
+// Arithmetic follows the reference expression order (see lmm_common.h) and
+// the library is built with -ffp-contract=off, so the FP64 kernels reproduce
+// the reference formulas bit for bit.
+#include "lmm_kernels.h"
+
+namespace lmm {
+
+constexpr int TPB = 256;
+
+static int g_debug_sync = 0;
+static char g_first_error[256] = {0};
+void set_debug_sync(int on) { g_debug_sync = on; }
+const char *first_error() { return g_first_error; }
+void clear_first_error() { g_first_error[0] = 0; }
+void post_launch(hipStream_t s, const char *name) {
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess && g_debug_sync) e = hipStreamSynchronize(s);
+  if (e != hipSuccess && g_first_error[0] == 0)
+    snprintf(g_first_error, sizeof(g_first_error), "%s: %s", name, hipGetErrorString(e));
+}
+#define LAUNCH(K, G, B, SHM, S, ...)                      \
+  do {                                                   \
+    hipLaunchKernelGGL(K, G, B, SHM, S, __VA_ARGS__);    \
+    post_launch(S, #K);                                  \
+  } while (0)
+static inline int nblk(long n, int tpb = TPB) { return (int)((n + tpb - 1) / tpb); }
+
+__device__ __forceinline__ void decode(const Grid &g, long t, int &i, int &j, int &k) {
+  i = (int)(t % g.n[0]);
+  long r = t / g.n[0];
+  j = (int)(r % g.n[1]);
+  k = (int)(r / g.n[1]);
+}
+
+// ------------------------------------------------------------------------
+// staggered smoother / operator
+// ------------------------------------------------------------------------
+
+template <typename T>
+struct Ptr3 {
+  T *c[3];
+};
+
+template <typename T>
+static Ptr3<T> p3(T *const a[3]) {
+  Ptr3<T> r;
+  for (int d = 0; d < 3; d++) r.c[d] = a[d];
+  return r;
+}
+
+// One colour of the 2*dim-colour Gauss-Seidel: applyop at the colour's points
+// followed by phi += omega*(rhs - Lphi)/fac, fused.  Points of one colour
+// read only points of other colours, so the in-place update equals the
+// reference's two passes (stag_mg_solver.f90:373-382).
+template <typename T>
+__global__ void __launch_bounds__(256) k_stag_gs_colour(Grid g, int visc, int a, int par, T omega,
+                                                        T dxsqinv, Ptr3<T> ph, StagCoef<T> K,
+                                                        const T *__restrict__ rhs) {
+  const int half = g.n[0] >> 1;
+  const long total = (long)half * g.n[1] * g.n[2];
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= total) return;
+  int ii = (int)(t % half);
+  long r = t / half;
+  int j = (int)(r % g.n[1]);
+  int k = (int)(r / g.n[1]);
+  int i = 2 * ii + ((par + j + k) & 1);
+  Pt P = make_pt(g, i, j, k);
+  const T *phr[3] = {ph.c[0], ph.c[1], ph.c[2]};
+  T L, fac;
+  stag_point<T>(visc, g, a, P, phr, K, dxsqinv, L, fac);
+  long p = i + g.s[1] * j + g.s[2] * k;
+  ph.c[a][p] = ph.c[a][p] + omega * (rhs[p] - L) / fac;
+}
+
+template <typename T>
+void launch_stag_gs_colour(hipStream_t s, const Grid &g, int visc, int a, int par, T omega,
+                           T *const phi[3], const StagCoef<T> &K, const T *rhs) {
+  long total = (long)(g.n[0] / 2) * g.n[1] * g.n[2];
+  T dxsqinv = (T)(1.0 / (g.dx * g.dx));
+  LAUNCH(k_stag_gs_colour<T>, dim3(nblk(total)), dim3(TPB), 0, s, g, visc, a, par, omega,
+                     dxsqinv, p3(phi), K, rhs);
+}
+
+// Lphi at the points of one colour (or every point for colour 0).  The
+// component is blockIdx.y, uniform across a workgroup.
+template <typename T>
+__global__ void __launch_bounds__(256) k_stag_applyop(Grid g, int visc, int color, T dxsqinv, Ptr3<T> ph,
+                                                      Ptr3<T> lp, StagCoef<T> K) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  int i, j, k, par = (color + 1) % 2;
+  const int a = (color == 0) ? (int)blockIdx.y : (color - 1) / 2;
+  const long per = (color == 0) ? g.N : g.N / 2;
+  if (t >= per) return;
+  if (color == 0) {
+    decode(g, t, i, j, k);
+  } else {
+    int half = g.n[0] >> 1;
+    int ii = (int)(t % half);
+    long r = t / half;
+    j = (int)(r % g.n[1]);
+    k = (int)(r / g.n[1]);
+    i = 2 * ii + ((par + j + k) & 1);
+  }
+  Pt P = make_pt(g, i, j, k);
+  const T *phr[3] = {ph.c[0], ph.c[1], ph.c[2]};
+  T L, fac;
+  stag_point<T>(visc, g, a, P, phr, K, dxsqinv, L, fac);
+  lp.c[a][i + g.s[1] * j + g.s[2] * k] = L;
+}
+
+template <typename T>
+void launch_stag_applyop(hipStream_t s, const Grid &g, int visc, int color, T *const phi[3],
+                         T *const Lphi[3], const StagCoef<T> &K) {
+  long per = (color == 0) ? g.N : g.N / 2;
+  dim3 grid(nblk(per), color == 0 ? g.dim : 1);
+  T dxsqinv = (T)(1.0 / (g.dx * g.dx));
+  LAUNCH(k_stag_applyop<T>, grid, dim3(TPB), 0, s, g, visc, color, dxsqinv, p3(phi), p3(Lphi), K);
+}
+
+// Jacobi sweep (stag_mg_smoother = 0): tmp = phi + omega*(rhs - L phi)/fac with
+// L phi from the old phi at every point; the caller copies tmp back.
+template <typename T>
+__global__ void __launch_bounds__(256) k_stag_jacobi(Grid g, int visc, T omega, T dxsqinv, Ptr3<T> ph,
+                                                     Ptr3<T> rh, Ptr3<T> tmp, StagCoef<T> K) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  const int a = (int)blockIdx.y;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  Pt P = make_pt(g, i, j, k);
+  const T *phr[3] = {ph.c[0], ph.c[1], ph.c[2]};
+  T L, fac;
+  stag_point<T>(visc, g, a, P, phr, K, dxsqinv, L, fac);
+  long p = i + g.s[1] * j + g.s[2] * k;
+  tmp.c[a][p] = ph.c[a][p] + omega * (rh.c[a][p] - L) / fac;
+}
+
+template <typename T>
+void launch_stag_jacobi(hipStream_t s, const Grid &g, int visc, T omega, T *const phi[3],
+                        T *const rhs[3], T *const tmp[3], const StagCoef<T> &K) {
+  T dxsqinv = (T)(1.0 / (g.dx * g.dx));
+  LAUNCH(k_stag_jacobi<T>, dim3(nblk(g.N), g.dim), dim3(TPB), 0, s, g, visc, omega, dxsqinv,
+                     p3(phi), p3(rhs), p3(tmp), K);
+}
+
+// res = rhs - L phi  (stag_mg_solver.f90:404-416: -(Lphi - rhs))
+template <typename T>
+__global__ void __launch_bounds__(256) k_stag_residual(Grid g, int visc, T dxsqinv, Ptr3<T> ph, Ptr3<T> rh,
+                                                       Ptr3<T> res, StagCoef<T> K) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  const int a = (int)blockIdx.y;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  Pt P = make_pt(g, i, j, k);
+  const T *phr[3] = {ph.c[0], ph.c[1], ph.c[2]};
+  T L, fac;
+  stag_point<T>(visc, g, a, P, phr, K, dxsqinv, L, fac);
+  long p = i + g.s[1] * j + g.s[2] * k;
+  res.c[a][p] = rh.c[a][p] - L;
+}
+
+template <typename T>
+void launch_stag_residual(hipStream_t s, const Grid &g, int visc, T *const phi[3], T *const rhs[3],
+                          T *const res[3], const StagCoef<T> &K) {
+  T dxsqinv = (T)(1.0 / (g.dx * g.dx));
+  LAUNCH(k_stag_residual<T>, dim3(nblk(g.N), g.dim), dim3(TPB), 0, s, g, visc, dxsqinv, p3(phi),
+                     p3(rhs), p3(res), K);
+}
+
+// ------------------------------------------------------------------------
+// restriction / prolongation (stag_mg_solver.f90:873-1633)
+// ------------------------------------------------------------------------
+
+// fine index of coarse point with fine offsets (along a: oa, transverse t1: o1, t2: o2)
+__device__ __forceinline__ long fidx(const Grid &gf, int a, int t1, int t2, const int C[3], int oa, int o1,
+                                     int o2) {
+  int f[3] = {2 * C[0], 2 * C[1], 2 * C[2]};
+  if (gf.dim == 2) f[2] = 0;
+  f[a] += oa;
+  f[t1] += o1;
+  if (t2 >= 0) f[t2] += o2;
+  for (int d = 0; d < gf.dim; d++) {
+    if (f[d] < 0) f[d] += gf.n[d];
+    if (f[d] >= gf.n[d]) f[d] -= gf.n[d];
+  }
+  return f[0] + gf.s[1] * f[1] + gf.s[2] * f[2];
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_stag_restrict(Grid gc, Grid gf, int simple, Ptr3<T> fi, Ptr3<T> co) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= gc.N) return;
+  const int a = (int)blockIdx.y;
+  int C[3];
+  decode(gc, t, C[0], C[1], C[2]);
+  int t1, t2 = -1;
+  if (gc.dim == 2) {
+    t1 = 1 - a;
+  } else {
+    t1 = (a == 0) ? 1 : 0;
+    t2 = (a == 2) ? 1 : 2;
+  }
+  const T *f = fi.c[a];
+  T v;
+  if (gc.dim == 2) {
+    if (simple) {
+      v = (T)0.5 * (f[fidx(gf, a, t1, t2, C, 0, 0, 0)] + f[fidx(gf, a, t1, t2, C, 0, 1, 0)]);
+    } else {
+      v = (T)0.25 * (f[fidx(gf, a, t1, t2, C, 0, 0, 0)] + f[fidx(gf, a, t1, t2, C, 0, 1, 0)]) +
+          (T)0.125 * (f[fidx(gf, a, t1, t2, C, 1, 0, 0)] + f[fidx(gf, a, t1, t2, C, 1, 1, 0)] +
+                      f[fidx(gf, a, t1, t2, C, -1, 0, 0)] + f[fidx(gf, a, t1, t2, C, -1, 1, 0)]);
+    }
+  } else {
+    T g0 = f[fidx(gf, a, t1, t2, C, 0, 0, 0)] + f[fidx(gf, a, t1, t2, C, 0, 1, 0)] +
+           f[fidx(gf, a, t1, t2, C, 0, 0, 1)] + f[fidx(gf, a, t1, t2, C, 0, 1, 1)];
+    if (simple) {
+      v = (T)0.25 * g0;
+    } else {
+      T gp = f[fidx(gf, a, t1, t2, C, 1, 0, 0)] + f[fidx(gf, a, t1, t2, C, 1, 1, 0)] +
+             f[fidx(gf, a, t1, t2, C, 1, 0, 1)] + f[fidx(gf, a, t1, t2, C, 1, 1, 1)];
+      T gm = f[fidx(gf, a, t1, t2, C, -1, 0, 0)] + f[fidx(gf, a, t1, t2, C, -1, 1, 0)] +
+             f[fidx(gf, a, t1, t2, C, -1, 0, 1)] + f[fidx(gf, a, t1, t2, C, -1, 1, 1)];
+      v = (T)0.125 * g0 + (T)0.0625 * gp + (T)0.0625 * gm;
+    }
+  }
+  co.c[a][C[0] + gc.s[1] * C[1] + gc.s[2] * C[2]] = v;
+}
+
+template <typename T>
+void launch_stag_restrict(hipStream_t s, const Grid &gc, T *const fine[3], T *const coarse[3], int simple) {
+  Grid gf = gc;
+  for (int d = 0; d < gc.dim; d++) gf.n[d] = gc.n[d] * 2;
+  gf = make_grid(gc.dim, gf.n, gc.dx * 0.5);
+  LAUNCH(k_stag_restrict<T>, dim3(nblk(gc.N), gc.dim), dim3(TPB), 0, s, gc, gf, simple, p3(fine),
+                     p3(coarse));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_stag_prolong(Grid gf, Grid gc, Ptr3<T> fi, Ptr3<T> co) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= gf.N) return;
+  const int a = (int)blockIdx.y;
+  int c[3];
+  decode(gf, t, c[0], c[1], c[2]);
+  int t1, t2 = -1;
+  if (gf.dim == 2) {
+    t1 = 1 - a;
+  } else {
+    t1 = (a == 0) ? 1 : 0;
+    t2 = (a == 2) ? 1 : 2;
+  }
+  int A[3] = {c[0] / 2, c[1] / 2, c[2] / 2};
+  int o1 = (c[t1] % 2 == 0) ? -1 : 1;
+  int o2 = (t2 >= 0 && c[t2] % 2 == 0) ? -1 : 1;
+  const T *C = co.c[a];
+  auto cidx = [&](int sa, int s1, int s2) -> long {
+    int q[3] = {A[0], A[1], A[2]};
+    q[a] += sa;
+    q[t1] += s1;
+    if (t2 >= 0) q[t2] += s2;
+    for (int d = 0; d < gc.dim; d++) {
+      if (q[d] < 0) q[d] += gc.n[d];
+      if (q[d] >= gc.n[d]) q[d] -= gc.n[d];
+    }
+    return q[0] + gc.s[1] * q[1] + gc.s[2] * q[2];
+  };
+  long p = c[0] + gf.s[1] * c[1] + gf.s[2] * c[2];
+  T v = fi.c[a][p];
+  bool even = (c[a] % 2 == 0);
+  if (gf.dim == 2) {
+    if (even) {
+      v = v + (T)0.75 * C[cidx(0, 0, 0)] + (T)0.25 * C[cidx(0, o1, 0)];
+    } else {
+      v = v + (T)0.375 * C[cidx(0, 0, 0)] + (T)0.125 * C[cidx(0, o1, 0)] + (T)0.375 * C[cidx(1, 0, 0)] +
+          (T)0.125 * C[cidx(1, o1, 0)];
+    }
+  } else {
+    const T nine16 = (T)0.5625, three16 = (T)0.1875, one16 = (T)0.0625;
+    const T nine32 = (T)0.28125, three32 = (T)0.09375, one32 = (T)0.03125;
+    if (even) {
+      v = v + nine16 * C[cidx(0, 0, 0)] + three16 * C[cidx(0, o1, 0)] + three16 * C[cidx(0, 0, o2)] +
+          one16 * C[cidx(0, o1, o2)];
+    } else {
+      v = v + nine32 * C[cidx(0, 0, 0)] + three32 * C[cidx(0, o1, 0)] + three32 * C[cidx(0, 0, o2)] +
+          one32 * C[cidx(0, o1, o2)] + nine32 * C[cidx(1, 0, 0)] + three32 * C[cidx(1, o1, 0)] +
+          three32 * C[cidx(1, 0, o2)] + one32 * C[cidx(1, o1, o2)];
+    }
+  }
+  fi.c[a][p] = v;
+}
+
+template <typename T>
+void launch_stag_prolong(hipStream_t s, const Grid &gf, T *const fine[3], T *const coarse[3]) {
+  Grid gc = coarsen(gf);
+  LAUNCH(k_stag_prolong<T>, dim3(nblk(gf.N), gf.dim), dim3(TPB), 0, s, gf, gc, p3(fine),
+                     p3(coarse));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_cc_restrict(Grid gc, Grid gf, const T *__restrict__ f, T *c) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= gc.N) return;
+  int i, j, k;
+  decode(gc, t, i, j, k);
+  long b = 2 * i + gf.s[1] * (2 * j) + gf.s[2] * (2 * k);
+  long sy = gf.s[1], sz = gf.s[2];
+  T v;
+  if (gc.dim == 2) {
+    v = (T)0.25 * (f[b] + f[b + 1] + f[b + sy] + f[b + 1 + sy]);
+  } else {
+    v = (T)0.125 * (f[b] + f[b + 1] + f[b + sy] + f[b + 1 + sy] + f[b + sz] + f[b + 1 + sz] + f[b + sy + sz] +
+                    f[b + 1 + sy + sz]);
+  }
+  c[t] = v;
+}
+
+template <typename T>
+void launch_cc_restrict(hipStream_t s, const Grid &gc, const T *fine, T *coarse) {
+  int n[3] = {gc.n[0] * 2, gc.n[1] * 2, gc.n[2] * 2};
+  Grid gf = make_grid(gc.dim, n, gc.dx * 0.5);
+  LAUNCH(k_cc_restrict<T>, dim3(nblk(gc.N)), dim3(TPB), 0, s, gc, gf, fine, coarse);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_edge_restrict(Grid gc, Grid gf, Ptr3<T> fi, Ptr3<T> co) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= gc.N) return;
+  const int e = (int)blockIdx.y;
+  int i, j, k;
+  decode(gc, t, i, j, k);
+  long b = 2 * i + gf.s[1] * (2 * j) + gf.s[2] * (2 * k);
+  const T *f = fi.c[e];
+  T v;
+  if (e == 0)
+    v = (T)0.5 * (f[b] + f[b + gf.s[2]]);   // xy edges: average in z
+  else if (e == 1)
+    v = (T)0.5 * (f[b] + f[b + gf.s[1]]);   // xz edges: average in y
+  else
+    v = (T)0.5 * (f[b] + f[b + 1]);         // yz edges: average in x
+  co.c[e][t] = v;
+}
+
+template <typename T>
+void launch_edge_restrict(hipStream_t s, const Grid &gc, T *const fine[3], T *const coarse[3]) {
+  int n[3] = {gc.n[0] * 2, gc.n[1] * 2, gc.n[2] * 2};
+  Grid gf = make_grid(3, n, gc.dx * 0.5);
+  LAUNCH(k_edge_restrict<T>, dim3(nblk(gc.N), 3), dim3(TPB), 0, s, gc, gf, p3(fine), p3(coarse));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_nodal_restrict(Grid gc, Grid gf, const T *__restrict__ f, T *c) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= gc.N) return;
+  int i, j, k;
+  decode(gc, t, i, j, k);
+  c[t] = f[2 * i + gf.s[1] * (2 * j)];
+}
+
+template <typename T>
+void launch_nodal_restrict(hipStream_t s, const Grid &gc, const T *fine, T *coarse) {
+  int n[3] = {gc.n[0] * 2, gc.n[1] * 2, 1};
+  Grid gf = make_grid(2, n, gc.dx * 0.5);
+  LAUNCH(k_nodal_restrict<T>, dim3(nblk(gc.N)), dim3(TPB), 0, s, gc, gf, fine, coarse);
+}
+
+// ------------------------------------------------------------------------
+// cell-centred multigrid: div(a grad phi) = rhs
+//   (L phi)(p) = ( -(sum of the 2*dim face coefficients) phi(p)
+//                  + a_lo,x phi(p-x) + a_hi,x phi(p+x) + ... ) / dx^2
+// ------------------------------------------------------------------------
+
+template <typename T>
+__device__ __forceinline__ T cc_apply(const Grid &g, int i, int j, int k, const T *phi, const Ptr3<T> &A,
+                                      T inv, T &diag) {
+  long p = i + g.s[1] * j + g.s[2] * k;
+  long xm = wm(i, g.n[0]) + g.s[1] * j + g.s[2] * k, xp = wp(i, g.n[0]) + g.s[1] * j + g.s[2] * k;
+  long ym = i + g.s[1] * wm(j, g.n[1]) + g.s[2] * k, yp = i + g.s[1] * wp(j, g.n[1]) + g.s[2] * k;
+  T axl = A.c[0][p], axh = A.c[0][xp], ayl = A.c[1][p], ayh = A.c[1][yp];
+  T s = axl + axh + ayl + ayh;
+  T off = axl * phi[xm] + axh * phi[xp] + ayl * phi[ym] + ayh * phi[yp];
+  if (g.dim == 3) {
+    long zm = i + g.s[1] * j + g.s[2] * wm(k, g.n[2]), zp = i + g.s[1] * j + g.s[2] * wp(k, g.n[2]);
+    T azl = A.c[2][p], azh = A.c[2][zp];
+    s = s + azl + azh;
+    off = off + azl * phi[zm] + azh * phi[zp];
+  }
+  diag = -s * inv;
+  return (-s * phi[p] + off) * inv;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_cc_rb(Grid g, int colour, T inv, T *phi, const T *__restrict__ rhs,
+                                               Ptr3<T> A) {
+  const int half = g.n[0] >> 1;
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)half * g.n[1] * g.n[2]) return;
+  int ii = (int)(t % half);
+  long r = t / half;
+  int j = (int)(r % g.n[1]);
+  int k = (int)(r / g.n[1]);
+  int i = 2 * ii + ((colour + j + k) & 1);
+  T diag;
+  T lp = cc_apply<T>(g, i, j, k, phi, A, inv, diag);
+  long p = i + g.s[1] * j + g.s[2] * k;
+  phi[p] = phi[p] + (rhs[p] - lp) / diag;
+}
+
+template <typename T>
+void launch_cc_rb_colour(hipStream_t s, const Grid &g, int colour, T *phi, const T *rhs, T *const a[3]) {
+  long total = (long)(g.n[0] / 2) * g.n[1] * g.n[2];
+  T inv = (T)(1.0 / (g.dx * g.dx));
+  LAUNCH(k_cc_rb<T>, dim3(nblk(total)), dim3(TPB), 0, s, g, colour, inv, phi, rhs, p3(a));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_cc_residual(Grid g, T inv, const T *phi, const T *__restrict__ rhs,
+                                                     T *res, Ptr3<T> A) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  T diag;
+  res[t] = rhs[t] - cc_apply<T>(g, i, j, k, phi, A, inv, diag);
+}
+
+template <typename T>
+void launch_cc_residual(hipStream_t s, const Grid &g, const T *phi, const T *rhs, T *res, T *const a[3]) {
+  T inv = (T)(1.0 / (g.dx * g.dx));
+  LAUNCH(k_cc_residual<T>, dim3(nblk(g.N)), dim3(TPB), 0, s, g, inv, phi, rhs, res, p3(a));
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_cc_prolong_add(Grid gf, Grid gc, T *f, const T *__restrict__ c) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= gf.N) return;
+  int i, j, k;
+  decode(gf, t, i, j, k);
+  f[t] = f[t] + c[i / 2 + gc.s[1] * (j / 2) + gc.s[2] * (k / 2)];
+}
+
+template <typename T>
+void launch_cc_prolong_add(hipStream_t s, const Grid &gf, T *fine, const T *coarse) {
+  Grid gc = coarsen(gf);
+  LAUNCH(k_cc_prolong_add<T>, dim3(nblk(gf.N)), dim3(TPB), 0, s, gf, gc, fine, coarse);
+}
+
+// ------------------------------------------------------------------------
+// elementwise
+// ------------------------------------------------------------------------
+
+template <typename T>
+__global__ void k_fill(T *x, long n, T v) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = v;
+}
+template <typename T>
+void launch_fill(hipStream_t s, T *x, long n, T v) {
+  if (n > 0) LAUNCH(k_fill<T>, dim3(nblk(n)), dim3(TPB), 0, s, x, n, v);
+}
+
+template <typename T>
+__global__ void k_scale(T *x, long n, T v) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = x[t] * v;
+}
+template <typename T>
+void launch_scale(hipStream_t s, T *x, long n, T v) {
+  if (n > 0) LAUNCH(k_scale<T>, dim3(nblk(n)), dim3(TPB), 0, s, x, n, v);
+}
+
+template <typename TD, typename TS>
+__global__ void k_convert(TD *d, const TS *__restrict__ src, long n) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) d[t] = (TD)src[t];
+}
+template <typename TD, typename TS>
+void launch_convert(hipStream_t s, TD *dst, const TS *src, long n) {
+  if (n > 0) LAUNCH((k_convert<TD, TS>), dim3(nblk(n)), dim3(TPB), 0, s, dst, src, n);
+}
+
+__global__ void k_recip(double *d, const double *__restrict__ src, long n) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) d[t] = 1.0 / src[t];
+}
+void launch_reciprocal(hipStream_t s, double *dst, const double *src, long n) {
+  LAUNCH(k_recip, dim3(nblk(n)), dim3(TPB), 0, s, dst, src, n);
+}
+
+__global__ void k_scale_copy(double *d, const double *__restrict__ src, long n, double v) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) d[t] = src[t] * v;
+}
+void launch_scale_copy(hipStream_t s, double *dst, const double *src, long n, double v) {
+  LAUNCH(k_scale_copy, dim3(nblk(n)), dim3(TPB), 0, s, dst, src, n, v);
+}
+
+// ------------------------------------------------------------------------
+// Stokes operator (apply_matrix.f90:60-112), fused per cell:
+//   b_u[a] = (theta alpha - L) x_u + (p - p(-e_a))/dx ;  b_p = -(D x_u)
+// ------------------------------------------------------------------------
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_apply_matrix(Grid g, int visc, T dxsqinv, T dxinv, Ptr3<T> xu,
+                                                      const T *__restrict__ xp, Ptr3<T> bu, T *bp,
+                                                      StagCoef<T> K) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  Pt P = make_pt(g, i, j, k);
+  const T *phr[3] = {xu.c[0], xu.c[1], xu.c[2]};
+  for (int a = 0; a < g.dim; a++) {
+    T L, fac;
+    stag_point<T>(visc, g, a, P, phr, K, dxsqinv, L, fac);
+    long pm = at_sh(g, P, a, -1, -1, 0);
+    T gx = (xp[t] - xp[pm]) * dxinv;
+    bu.c[a][t] = L + gx;
+  }
+  T dv = (xu.c[0][at3(g, P, 1, 0, 0)] - xu.c[0][t]) * dxinv;
+  dv = dv + (xu.c[1][at3(g, P, 0, 1, 0)] - xu.c[1][t]) * dxinv;
+  if (g.dim == 3) dv = dv + (xu.c[2][at3(g, P, 0, 0, 1)] - xu.c[2][t]) * dxinv;
+  bp[t] = dv * (T)-1.0;
+}
+
+template <typename T>
+void launch_apply_matrix(hipStream_t s, const Grid &g, int visc, T *const x_u[3], const T *x_p,
+                         T *const b_u[3], T *b_p, const StagCoef<T> &K) {
+  T dxsqinv = (T)(1.0 / (g.dx * g.dx)), dxinv = (T)(1.0 / g.dx);
+  LAUNCH(k_apply_matrix<T>, dim3(nblk(g.N)), dim3(TPB), 0, s, g, visc, dxsqinv, dxinv, p3(x_u), x_p,
+                     p3(b_u), b_p, K);
+}
+
+// div kernel (div_and_grad.f90:400-451) with combination modes
+template <typename T>
+__global__ void __launch_bounds__(256) k_div(Grid g, T dxinv, Ptr3<T> u, const T *b_p, T *div, int mode) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  long xp = wp(i, g.n[0]) + g.s[1] * j + g.s[2] * k;
+  long yp = i + g.s[1] * wp(j, g.n[1]) + g.s[2] * k;
+  T v = (u.c[0][xp] - u.c[0][t]) * dxinv;
+  if (mode == 3) v = div[t] + v;
+  v = v + (u.c[1][yp] - u.c[1][t]) * dxinv;
+  if (g.dim == 3) {
+    long zp = i + g.s[1] * j + g.s[2] * wp(k, g.n[2]);
+    v = v + (u.c[2][zp] - u.c[2][t]) * dxinv;
+  }
+  if (mode == 1) v = v + b_p[t];
+  if (mode == 2) v = v * (T)-1.0;
+  div[t] = v;
+}
+
+template <typename T>
+void launch_div_plus(hipStream_t s, const Grid &g, T *const u[3], const T *b_p, T *div, int mode) {
+  T dxinv = (T)(1.0 / g.dx);
+  LAUNCH(k_div<T>, dim3(nblk(g.N)), dim3(TPB), 0, s, g, dxinv, p3(u), b_p, div, mode);
+}
+
+// Block sum into partial[blockIdx * 4 + seg] (seg fixed per call site).
+__device__ __forceinline__ double block_sum(double v, double *sh) {
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+  int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    int nw = (blockDim.x + 63) >> 6;
+    for (int w = 0; w < nw; w++) r += sh[w];
+  }
+  __syncthreads();
+  return r;
+}
+
+// P1 steps 3-4 (apply_precon.f90:131-188, macproject.f90:320-364), fused per cell:
+//   x_u[a] = x_u[a] + (-1)*(((phi - phi(-e_a))/dx) * ainv[a])
+//   x_p    = -(mac_rhs)*beta*c + theta*phi      (c per |visc_type|)
+// and the per-block partial sums of x_p for the pressure mean.
+template <typename T>
+__global__ void __launch_bounds__(256) k_precon_final(Grid g, int visc, T dxinv, Ptr3<T> xu, T *xp,
+                                                      const T *__restrict__ phi, const T *__restrict__ mrhs,
+                                                      Ptr3<T> ainv, const T *__restrict__ beta,
+                                                      const T *__restrict__ gamma, T theta, double *partial) {
+  __shared__ double sh[8];
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  double mine = 0.0;
+  if (t < g.N) {
+    int i, j, k;
+    decode(g, t, i, j, k);
+    Pt P = make_pt(g, i, j, k);
+    for (int a = 0; a < g.dim; a++) {
+      long pm = at_sh(g, P, a, -1, -1, 0);
+      T gp = (phi[t] - phi[pm]) * dxinv;
+      T gq = gp * ainv.c[a][t];
+      xu.c[a][t] = xu.c[a][t] + (T)-1.0 * gq;
+    }
+    int av = visc < 0 ? -visc : visc;
+    T x = mrhs[t] * (T)-1.0;
+    if (av == 1 || av == 2) {
+      x = x * beta[t];
+      if (av == 2) x = x * (T)2.0;
+    } else {
+      T tt = x * gamma[t];
+      x = x * beta[t];
+      x = x * (T)(4.0 / 3.0);
+      x = x + tt;
+    }
+    T ph = phi[t] * theta;
+    x = x + ph;
+    xp[t] = x;
+    mine = (double)x;
+  }
+  if (partial) {
+    double r = block_sum(mine, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+  }
+}
+
+template <typename T>
+void launch_precon_final(hipStream_t s, const Grid &g, int visc, T *const x_u[3], T *x_p, const T *phi,
+                         const T *mac_rhs, T *const ainv[3], const T *beta, const T *gamma,
+                         double theta_alpha, double *partial) {
+  T dxinv = (T)(1.0 / g.dx);
+  LAUNCH(k_precon_final<T>, dim3(nblk(g.N)), dim3(TPB), 0, s, g, visc, dxinv, p3(x_u), x_p, phi,
+                     mac_rhs, p3(ainv), beta, gamma, (T)theta_alpha, partial);
+}
+
+__global__ void k_grad(Grid g, double dxinv, const double *__restrict__ p, Ptr3<double> gp) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  Pt P = make_pt(g, i, j, k);
+  for (int a = 0; a < g.dim; a++) gp.c[a][t] = (p[t] - p[at_sh(g, P, a, -1, -1, 0)]) * dxinv;
+}
+void launch_grad(hipStream_t s, const Grid &g, const double *p, double *const gp[3]) {
+  LAUNCH(k_grad, dim3(nblk(g.N)), dim3(TPB), 0, s, g, 1.0 / g.dx, p, p3(gp));
+}
+
+// convert_stag.f90 interior formulas
+__global__ void k_avg_face(Grid g, const double *__restrict__ cc, Ptr3<double> f) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  Pt P = make_pt(g, i, j, k);
+  for (int a = 0; a < g.dim; a++) f.c[a][t] = 0.5 * (cc[t] + cc[at_sh(g, P, a, -1, -1, 0)]);
+}
+void launch_avg_cc_to_face(hipStream_t s, const Grid &g, const double *cc, double *const face[3]) {
+  LAUNCH(k_avg_face, dim3(nblk(g.N)), dim3(TPB), 0, s, g, cc, p3(face));
+}
+__global__ void k_avg_node(Grid g, const double *__restrict__ cc, double *nd) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  Pt P = make_pt(g, i, j, k);
+  nd[t] = 0.25 * (cc[t] + cc[at3(g, P, -1, 0, 0)] + cc[at3(g, P, 0, -1, 0)] + cc[at3(g, P, -1, -1, 0)]);
+}
+void launch_avg_cc_to_node(hipStream_t s, const Grid &g, const double *cc, double *node) {
+  LAUNCH(k_avg_node, dim3(nblk(g.N)), dim3(TPB), 0, s, g, cc, node);
+}
+__global__ void k_avg_edge(Grid g, const double *__restrict__ cc, Ptr3<double> e) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  Pt P = make_pt(g, i, j, k);
+  e.c[0][t] = 0.25 * (cc[t] + cc[at3(g, P, -1, 0, 0)] + cc[at3(g, P, 0, -1, 0)] + cc[at3(g, P, -1, -1, 0)]);
+  e.c[1][t] = 0.25 * (cc[t] + cc[at3(g, P, -1, 0, 0)] + cc[at3(g, P, 0, 0, -1)] + cc[at3(g, P, -1, 0, -1)]);
+  e.c[2][t] = 0.25 * (cc[t] + cc[at3(g, P, 0, -1, 0)] + cc[at3(g, P, 0, 0, -1)] + cc[at3(g, P, 0, -1, -1)]);
+}
+void launch_avg_cc_to_edge(hipStream_t s, const Grid &g, const double *cc, double *const edge[3]) {
+  LAUNCH(k_avg_edge, dim3(nblk(g.N)), dim3(TPB), 0, s, g, cc, p3(edge));
+}
+
+// ------------------------------------------------------------------------
+// deterministic reductions
+// ------------------------------------------------------------------------
+
+// partial[blk*4 + seg]: seg d < dim face components (face index 0 along d
+// weighted 3), seg 3 = cells.  a, b: Stokes vectors [u0|u1|(u2)|p].
+__global__ void __launch_bounds__(RED_TPB) k_stokes_dot(Grid g, const double *__restrict__ a,
+                                                        const double *__restrict__ b, double *partial) {
+  __shared__ double sh[8];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < g.N; q += stride) {
+    int i, j, k;
+    decode(g, q, i, j, k);
+    int c[3] = {i, j, k};
+    for (int d = 0; d < g.dim; d++) {
+      double w = (c[d] == 0) ? 3.0 : 1.0;
+      acc[d] += w * (a[d * g.N + q] * b[d * g.N + q]);
+    }
+    acc[3] += a[g.dim * g.N + q] * b[g.dim * g.N + q];
+  }
+  for (int seg = 0; seg < 4; seg++) {
+    double r = block_sum(acc[seg], sh);
+    if (threadIdx.x == 0) partial[blockIdx.x * 4 + seg] = r;
+  }
+}
+
+void launch_stokes_dot(hipStream_t s, const Grid &g, const double *a, const double *b, double *partial) {
+  LAUNCH(k_stokes_dot, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, a, b, partial);
+}
+
+__global__ void __launch_bounds__(256) k_finalize(Grid g, const double *__restrict__ partial, int nparts,
+                                                  double *out, int op, double pnw) {
+  __shared__ double sh[8];
+  double seg[4];
+  for (int q = 0; q < 4; q++) {
+    double v = 0.0;
+    for (int b = threadIdx.x; b < nparts; b += blockDim.x) v += partial[b * 4 + q];
+    seg[q] = block_sum(v, sh);
+  }
+  if (threadIdx.x != 0) return;
+  double su = seg[0] + seg[1];
+  if (g.dim == 3) su = su + seg[2];
+  double tv = (double)g.N;
+  if (op == 0) {
+    out[0] = su + (pnw * pnw) * seg[3];
+  } else if (op == 1) {
+    double nu = sqrt(su);
+    double np = pnw * sqrt(seg[3]);
+    out[0] = sqrt(nu * nu + np * np);
+    out[1] = nu;
+    out[2] = np;
+  } else if (op == 2) {
+    for (int q = 0; q < 4; q++) out[q] = seg[q];
+  } else if (op == 3) {
+    out[0] = seg[3] / tv;
+  } else if (op == 4) {
+    for (int q = 0; q < 3; q++) out[q] = seg[q] / tv;
+  } else if (op == 5) {
+    out[0] = seg[0];
+  }
+}
+
+void launch_finalize(hipStream_t s, const Grid &g, const double *partial, double *out, int op, double pnw) {
+  LAUNCH(k_finalize, dim3(1), dim3(256), 0, s, g, partial, RED_BLOCKS, out, op, pnw);
+}
+
+// sum of a cell field into partial[blk*4+3] (pressure mean); reads partial
+// from k_precon_final (one value per block of 256 cells).
+__global__ void k_fold_cells(const double *__restrict__ src, int nsrc, double *partial) {
+  __shared__ double sh[8];
+  // RED_BLOCKS blocks: block b folds src[b], src[b+RED_BLOCKS], ...
+  double v = 0.0;
+  for (int q = blockIdx.x + RED_BLOCKS * threadIdx.x; q < nsrc; q += RED_BLOCKS * blockDim.x) v += src[q];
+  double r = block_sum(v, sh);
+  if (threadIdx.x == 0) {
+    partial[blockIdx.x * 4 + 0] = 0.0;
+    partial[blockIdx.x * 4 + 1] = 0.0;
+    partial[blockIdx.x * 4 + 2] = 0.0;
+    partial[blockIdx.x * 4 + 3] = r;
+  }
+}
+
+__global__ void __launch_bounds__(RED_TPB) k_face_maxabs(Grid g, Ptr3<double> r, double *partial) {
+  __shared__ double sh[8];
+  double m[3] = {0.0, 0.0, 0.0};
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < g.N; q += stride)
+    for (int d = 0; d < g.dim; d++) m[d] = fmax(m[d], fabs(r.c[d][q]));
+  for (int d = 0; d < 3; d++) {
+    double v = m[d];
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double x = 0.0;
+      for (int w = 0; w < (int)(blockDim.x >> 6); w++) x = fmax(x, sh[w]);
+      partial[blockIdx.x * 4 + d] = x;
+    }
+    __syncthreads();
+  }
+}
+void launch_face_maxabs(hipStream_t s, const Grid &g, const double *const r[3], double *partial) {
+  Ptr3<double> p;
+  for (int d = 0; d < 3; d++) p.c[d] = const_cast<double *>(r[d]);
+  LAUNCH(k_face_maxabs, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, p, partial);
+}
+__global__ void k_finalize_max(const double *__restrict__ partial, double *out) {
+  if (threadIdx.x >= 3) return;
+  double m = 0.0;
+  for (int b = 0; b < RED_BLOCKS; b++) m = fmax(m, partial[b * 4 + threadIdx.x]);
+  out[threadIdx.x] = m;
+}
+void launch_finalize_max(hipStream_t s, const double *partial, double *out) {
+  LAUNCH(k_finalize_max, dim3(1), dim3(64), 0, s, partial, out);
+}
+
+// sum_umac_press velocity part for a tile spanning the box: every face once,
+// plus 0.5*u on the lo and hi+1 copies -> face index 0 weighted 3.
+template <typename T>
+__global__ void __launch_bounds__(RED_TPB) k_sum_faces_w(Grid g, Ptr3<T> u, double *partial) {
+  __shared__ double sh[8];
+  double acc[3] = {0.0, 0.0, 0.0};
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < g.N; q += stride) {
+    int i, j, k;
+    decode(g, q, i, j, k);
+    int c[3] = {i, j, k};
+    for (int d = 0; d < g.dim; d++) acc[d] += (c[d] == 0 ? 3.0 : 1.0) * (double)u.c[d][q];
+  }
+  for (int seg = 0; seg < 4; seg++) {
+    double r = block_sum(seg < 3 ? acc[seg] : 0.0, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x * 4 + seg] = r;
+  }
+}
+template <typename T>
+void launch_sum_faces_weighted(hipStream_t s, const Grid &g, T *const u[3], double *partial) {
+  LAUNCH(k_sum_faces_w<T>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, p3(u), partial);
+}
+
+// ------------------------------------------------------------------------
+// Krylov vector kernels
+// ------------------------------------------------------------------------
+
+__global__ void k_bmax_neg(double *tmp, const double *__restrict__ b, long n) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) tmp[t] = (tmp[t] - b[t]) * -1.0;
+}
+void launch_bmax_neg(hipStream_t s, double *tmp, const double *b, long n) {
+  LAUNCH(k_bmax_neg, dim3(nblk(n)), dim3(TPB), 0, s, tmp, b, n);
+}
+
+__global__ void k_div_by(double *v, const double *__restrict__ src, long n, const double *scal) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) v[t] = src[t] / scal[0];
+}
+void launch_div_by(hipStream_t s, double *v, const double *src, long n, const double *scal) {
+  LAUNCH(k_div_by, dim3(nblk(n)), dim3(TPB), 0, s, v, src, n, scal);
+}
+
+__global__ void k_axmy(double *w, const double *__restrict__ v, long n, const double *scal) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) w[t] = w[t] - v[t] * scal[0];
+}
+void launch_axmy(hipStream_t s, double *w, const double *v, long n, const double *scal) {
+  LAUNCH(k_axmy, dim3(nblk(n)), dim3(TPB), 0, s, w, v, n, scal);
+}
+
+struct VList {
+  const double *v[32];
+};
+// update_sol (gmres.f90:481-515): x = x + V(k)*y(k), k = 1..i in order.
+__global__ void k_update_sol(double *x, VList V, const double *__restrict__ y, int i, long n) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double xv = x[t];
+  for (int k = 0; k < i; k++) xv = xv + V.v[k][t] * y[k];
+  x[t] = xv;
+}
+void launch_update_sol(hipStream_t s, double *x, double *const V[], const double *y, int i, long n) {
+  VList L = {};
+  for (int k = 0; k < i && k < 32; k++) L.v[k] = V[k];
+  LAUNCH(k_update_sol, dim3(nblk(n)), dim3(TPB), 0, s, x, L, y, i, n);
+}
+
+template <typename T>
+__global__ void k_sub_scalar(T *x, long n, const double *scal) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) x[t] = (T)((double)x[t] - scal[0]);
+}
+template <typename T>
+void launch_sub_scalar(hipStream_t s, T *x, long n, const double *scal) {
+  LAUNCH(k_sub_scalar<T>, dim3(nblk(n)), dim3(TPB), 0, s, x, n, scal);
+}
+template <typename T>
+__global__ void k_sub_scalar3(Grid g, Ptr3<T> u, const double *scal) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  for (int d = 0; d < g.dim; d++) u.c[d][t] = (T)((double)u.c[d][t] - scal[d]);
+}
+template <typename T>
+void launch_sub_scalar3(hipStream_t s, const Grid &g, T *const u[3], const double *scal) {
+  LAUNCH(k_sub_scalar3<T>, dim3(nblk(g.N)), dim3(TPB), 0, s, g, p3(u), scal);
+}
+
+// ------------------------------------------------------------------------
+// fluctuating momentum flux (stochastic_m_fluxdiv.f90:41-553, periodic)
+// ------------------------------------------------------------------------
+
+struct Noise {
+  const double *cc;
+  const double *ed[3];
+  int nstages;
+  long scc, sed;  // stage strides
+  const double *w;
+};
+
+// sum_r w_r W_r (saxpy from zero), times variance, times sqrt(eta*T)
+__device__ __forceinline__ double wsum(const Noise &nz, const double *base, long stride, long idx) {
+  double v = 0.0;
+  for (int r = 0; r < nz.nstages; r++) v = v + nz.w[r] * base[r * stride + idx];
+  return v;
+}
+
+__global__ void __launch_bounds__(256) k_stoch_force(Grid g, Ptr3<double> force, Noise nz,
+                                                     const double *__restrict__ eta, Ptr3<double> eta_ed,
+                                                     const double *__restrict__ temp, Ptr3<double> temp_ed,
+                                                     double variance) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  Pt P = make_pt(g, i, j, k);
+  const long N = g.N;
+  double dxinv = 1.0 / g.dx;
+  auto WC = [&](int comp, long q) -> double {
+    double v = variance * wsum(nz, nz.cc + comp * N, nz.scc, q);
+    return v * sqrt(eta[q] * temp[q]);
+  };
+  auto WE = [&](int e, int comp, long q) -> double {
+    double v = variance * wsum(nz, nz.ed[e] + comp * N, nz.sed, q);
+    return v * sqrt(eta_ed.c[e][q] * temp_ed.c[e][q]);
+  };
+  long xm = at3(g, P, -1, 0, 0), yp = at3(g, P, 0, 1, 0), xp = at3(g, P, 1, 0, 0), ym = at3(g, P, 0, -1, 0);
+  if (g.dim == 2) {
+    force.c[0][t] = force.c[0][t] + (WC(0, t) - WC(0, xm)) * dxinv + (WE(0, 0, yp) - WE(0, 0, t)) * dxinv;
+    force.c[1][t] = force.c[1][t] + (WE(0, 1, xp) - WE(0, 1, t)) * dxinv + (WC(1, t) - WC(1, ym)) * dxinv;
+    return;
+  }
+  long zp = at3(g, P, 0, 0, 1), zm = at3(g, P, 0, 0, -1);
+  force.c[0][t] = force.c[0][t] + (WC(0, t) - WC(0, xm)) * dxinv + (WE(0, 0, yp) - WE(0, 0, t)) * dxinv +
+                  (WE(1, 0, zp) - WE(1, 0, t)) * dxinv;
+  force.c[1][t] = force.c[1][t] + (WE(0, 1, xp) - WE(0, 1, t)) * dxinv + (WC(1, t) - WC(1, ym)) * dxinv +
+                  (WE(2, 0, zp) - WE(2, 0, t)) * dxinv;
+  force.c[2][t] = force.c[2][t] + (WE(1, 1, xp) - WE(1, 1, t)) * dxinv + (WE(2, 1, yp) - WE(2, 1, t)) * dxinv +
+                  (WC(2, t) - WC(2, zm)) * dxinv;
+}
+
+void launch_stoch_m_force(hipStream_t s, const Grid &g, double *const force[3], const double *Wcc,
+                          double *const Wed[3], int nstages, const double *weights, long scc, long sed,
+                          const double *eta, double *const eta_ed[3], const double *temp,
+                          double *const temp_ed[3], double variance) {
+  Noise nz = {};
+  nz.cc = Wcc;
+  for (int e = 0; e < 3; e++) nz.ed[e] = Wed[e];
+  nz.nstages = nstages;
+  nz.scc = scc;
+  nz.sed = sed;
+  nz.w = weights;
+  LAUNCH(k_stoch_force, dim3(nblk(g.N)), dim3(TPB), 0, s, g, p3(force), nz, eta, p3(eta_ed), temp,
+                     p3(temp_ed), variance);
+}
+
+// ------------------------------------------------------------------------
+// species diffusion flux (diffusive_mass_fluxdiv.f90:92-262, isothermal)
+// flux layout: [dir][species][N]
+// ------------------------------------------------------------------------
+
+constexpr int MAXNS = 8;
+
+__global__ void __launch_bounds__(256) k_diff_mass_flux(Grid g, int ns, const double *__restrict__ x,
+                                                        const double *__restrict__ rw,
+                                                        const double *__restrict__ gm, double *flux,
+                                                        Ptr3<double> ft, int correct) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  Pt P = make_pt(g, i, j, k);
+  const long N = g.N;
+  double dxinv = 1.0 / g.dx;
+  for (int d = 0; d < g.dim; d++) {
+    long m = at_sh(g, P, d, -1, -1, 0);
+    double gx[MAXNS], y[MAXNS];
+    for (int q = 0; q < ns; q++) gx[q] = (x[q * N + t] - x[q * N + m]) * dxinv;
+    for (int r = 0; r < ns; r++) {
+      double acc = 0.0;
+      for (int c = 0; c < ns; c++) {
+        long q = r + ns * c;
+        acc = acc + 0.5 * (gm[q * N + t] + gm[q * N + m]) * gx[c];
+      }
+      y[r] = acc;
+    }
+    for (int r = 0; r < ns; r++) {
+      double acc = 0.0;
+      for (int c = 0; c < ns; c++) {
+        long q = r + ns * c;
+        acc = acc + 0.5 * (rw[q * N + t] + rw[q * N + m]) * y[c];
+      }
+      gx[r] = acc;
+    }
+    if (correct && ns > 1) {
+      double sum = 0.0;
+      for (int q = 0; q < ns - 1; q++) sum = sum + gx[q];
+      gx[ns - 1] = -sum;
+    }
+    for (int q = 0; q < ns; q++) {
+      flux[((long)d * ns + q) * N + t] = gx[q];
+      ft.c[d][q * N + t] = ft.c[d][q * N + t] + gx[q];
+    }
+  }
+}
+
+void launch_diffusive_mass_flux(hipStream_t s, const Grid &g, int ns, const double *molarconc,
+                                const double *rhoWchi, const double *Gama, double *flux,
+                                double *const flux_total[3], int correct_flux) {
+  LAUNCH(k_diff_mass_flux, dim3(nblk(g.N)), dim3(TPB), 0, s, g, ns, molarconc, rhoWchi, Gama, flux,
+                     p3(flux_total), correct_flux);
+}
+
+__global__ void k_div_species(Grid g, int ns, const double *__restrict__ flux, double *div) {
+  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= g.N) return;
+  const int q = (int)blockIdx.y;
+  const long t = q * g.N + c;
+  int i, j, k;
+  decode(g, c, i, j, k);
+  Pt P = make_pt(g, i, j, k);
+  const long N = g.N;
+  double dxinv = 1.0 / g.dx;
+  const double *fx = flux + ((long)0 * ns + q) * N, *fy = flux + ((long)1 * ns + q) * N;
+  double v = (fx[at3(g, P, 1, 0, 0)] - fx[c]) * dxinv;
+  v = v + (fy[at3(g, P, 0, 1, 0)] - fy[c]) * dxinv;
+  if (g.dim == 3) {
+    const double *fz = flux + ((long)2 * ns + q) * N;
+    v = v + (fz[at3(g, P, 0, 0, 1)] - fz[c]) * dxinv;
+  }
+  div[t] = v;
+}
+void launch_div_species(hipStream_t s, const Grid &g, int ns, const double *flux, double *div) {
+  LAUNCH(k_div_species, dim3(nblk(g.N), ns), dim3(TPB), 0, s, g, ns, flux, div);
+}
+
+// ------------------------------------------------------------------------
+// explicit instantiations
+// ------------------------------------------------------------------------
+#define INST(T)                                                                                              \
+  template void launch_stag_gs_colour<T>(hipStream_t, const Grid &, int, int, int, T, T *const[3],          \
+                                         const StagCoef<T> &, const T *);                                    \
+  template void launch_stag_applyop<T>(hipStream_t, const Grid &, int, int, T *const[3], T *const[3],        \
+                                       const StagCoef<T> &);                                                 \
+  template void launch_stag_jacobi<T>(hipStream_t, const Grid &, int, T, T *const[3], T *const[3],           \
+                                      T *const[3], const StagCoef<T> &);                                     \
+  template void launch_stag_residual<T>(hipStream_t, const Grid &, int, T *const[3], T *const[3],            \
+                                        T *const[3], const StagCoef<T> &);                                   \
+  template void launch_stag_restrict<T>(hipStream_t, const Grid &, T *const[3], T *const[3], int);           \
+  template void launch_stag_prolong<T>(hipStream_t, const Grid &, T *const[3], T *const[3]);                 \
+  template void launch_cc_restrict<T>(hipStream_t, const Grid &, const T *, T *);                            \
+  template void launch_edge_restrict<T>(hipStream_t, const Grid &, T *const[3], T *const[3]);                \
+  template void launch_nodal_restrict<T>(hipStream_t, const Grid &, const T *, T *);                         \
+  template void launch_cc_rb_colour<T>(hipStream_t, const Grid &, int, T *, const T *, T *const[3]);         \
+  template void launch_cc_residual<T>(hipStream_t, const Grid &, const T *, const T *, T *, T *const[3]);    \
+  template void launch_cc_prolong_add<T>(hipStream_t, const Grid &, T *, const T *);                         \
+  template void launch_fill<T>(hipStream_t, T *, long, T);                                                   \
+  template void launch_scale<T>(hipStream_t, T *, long, T);                                                  \
+  template void launch_apply_matrix<T>(hipStream_t, const Grid &, int, T *const[3], const T *, T *const[3], \
+                                       T *, const StagCoef<T> &);                                            \
+  template void launch_div_plus<T>(hipStream_t, const Grid &, T *const[3], const T *, T *, int);             \
+  template void launch_precon_final<T>(hipStream_t, const Grid &, int, T *const[3], T *, const T *,          \
+                                       const T *, T *const[3], const T *, const T *, double, double *);  \
+  template void launch_sum_faces_weighted<T>(hipStream_t, const Grid &, T *const[3], double *);              \
+  template void launch_sub_scalar<T>(hipStream_t, T *, long, const double *);                                \
+  template void launch_sub_scalar3<T>(hipStream_t, const Grid &, T *const[3], const double *);
+INST(double)
+INST(float)
+template void launch_convert<float, double>(hipStream_t, float *, const double *, long);
+template void launch_convert<double, float>(hipStream_t, double *, const float *, long);
+
+// fold per-block pressure sums (k_precon_final partials) into the 4-segment layout
+void launch_fold_cells(hipStream_t s, const double *src, int nsrc, double *partial) {
+  LAUNCH(k_fold_cells, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, src, nsrc, partial);
+}
+
+}  // namespace lmm

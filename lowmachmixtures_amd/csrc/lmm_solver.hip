@@ -1,0 +1,1682 @@
+// Context, multigrid drivers, GMRES / flexible GMRES and the C ABI of the
+// MI355X Stokes library.  Host code only orchestrates: every field operation
+// is a gfx950 kernel on the context stream; the host keeps the GMRES
+// Hessenberg scalars (gmres.f90:55-59) and the stopping logic.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cstdlib>
+#include <cmath>
+#include <cstdarg>
+#include <cstring>
+#include <algorithm>
+#include <map>
+#include <type_traits>
+#include <string>
+#include <vector>
+
+#include "../../include/lmm.h"
+#include "lmm_kernels.h"
+
+using namespace lmm;
+
+namespace lmm {
+void launch_fold_cells(hipStream_t s, const double *src, int nsrc, double *partial);
+}
+
+// ---------------------------------------------------------------------------
+// context and fields
+// ---------------------------------------------------------------------------
+
+template <typename T>
+struct SLevel {
+  Grid g;
+  T *alpha[3] = {nullptr, nullptr, nullptr};  // theta*alpha (owned)
+  T *beta = nullptr, *gamma = nullptr;        // owned on coarse levels (and level 0 in FP32)
+  T *bed[3] = {nullptr, nullptr, nullptr};
+  T *phi[3] = {nullptr, nullptr, nullptr};
+  T *rhs[3] = {nullptr, nullptr, nullptr};
+  T *res[3] = {nullptr, nullptr, nullptr};
+  StagCoef<T> K() const {
+    StagCoef<T> k;
+    for (int d = 0; d < 3; d++) {
+      k.alpha[d] = alpha[d];
+      k.bed[d] = bed[d];
+    }
+    k.beta = beta;
+    k.gamma = gamma;
+    return k;
+  }
+};
+
+template <typename T>
+struct CLevel {
+  Grid g;
+  T *a[3] = {nullptr, nullptr, nullptr};
+  T *phi = nullptr, *rhs = nullptr, *res = nullptr;
+};
+
+struct ProfFamily {
+  int64_t launches = 0;
+  double ms = 0.0;
+  double bytes = 0.0;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> pending;
+};
+
+struct lmm_field {
+  lmm_ctx *ctx;
+  int kind;
+  int ncomp;
+  double *d;
+};
+
+struct lmm_ctx {
+  Grid g;
+  lmm_params prm;
+  hipStream_t stream = nullptr;
+  int nl_s = 0, nl_c = 0;
+  std::vector<SLevel<double>> s64;
+  std::vector<CLevel<double>> c64;
+  std::vector<SLevel<float>> s32;
+  std::vector<CLevel<float>> c32;
+  bool lvl32 = false;
+  double *partial = nullptr;   // RED_BLOCKS*4
+  double *partial2 = nullptr;  // per-block cell sums
+  double *scal = nullptr;      // device scalars
+  double *hscal = nullptr;     // pinned host mirror
+  double *ainv[3] = {nullptr, nullptr, nullptr};
+  double *alpha_theta[3] = {nullptr, nullptr, nullptr};
+  double *scaled_beta = nullptr, *scaled_gamma = nullptr, *scaled_bed[3] = {nullptr, nullptr, nullptr};
+  // Krylov storage (Stokes vectors [u0|u1|(u2)|p])
+  int kry_m = -1;
+  double *V = nullptr, *r = nullptr, *w = nullptr, *tmp = nullptr, *b = nullptr, *x = nullptr;
+  float *Z32 = nullptr, *b32 = nullptr;
+  double *mac_rhs = nullptr, *phi_cc = nullptr;
+  float *mac_rhs32 = nullptr;
+  // stochastic noise held by the context (init_m_stochastic)
+  int n_rngs = 0;
+  double *Wcc = nullptr, *Wed = nullptr;
+  double *dweights = nullptr;
+  // profiling
+  bool prof = false;
+  std::map<std::string, ProfFamily> fam;
+  std::string err;
+  int vcycles = 0;
+};
+
+static int set_err(lmm_ctx *c, int code, const char *fmt, ...) {
+  if (c) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    c->err = buf;
+  }
+  return code;
+}
+
+#define HIPCHK(ctx, call)                                                                    \
+  do {                                                                                       \
+    hipError_t e_ = (call);                                                                  \
+    if (e_ != hipSuccess)                                                                    \
+      return set_err((ctx), LMM_ERR_DEVICE, "%s failed: %s", #call, hipGetErrorString(e_));  \
+  } while (0)
+
+// Device allocation, zeroed in order on the owning context's stream (a
+// blocking stream, so it is also ordered with the legacy null stream).
+template <typename T>
+static T *dalloc(hipStream_t stream, long n) {
+  void *p = nullptr;
+  size_t bytes = sizeof(T) * (size_t)(n > 0 ? n : 1);
+  if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+  if (hipMemsetAsync(p, 0, bytes, stream) != hipSuccess) {
+    hipFree(p);
+    return nullptr;
+  }
+  return (T *)p;
+}
+
+static void dfree(void *p) {
+  if (p) hipFree(p);
+}
+
+// ------------------------- profiling ------------------------------------
+
+struct ProfScope {
+  lmm_ctx *c;
+  ProfFamily *f = nullptr;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(lmm_ctx *ctx, const char *name, double bytes) : c(ctx) {
+    if (!c->prof) return;
+    f = &c->fam[name];
+    hipEventCreate(&a);
+    hipEventCreate(&b);
+    hipEventRecord(a, c->stream);
+    f->launches++;
+    f->bytes += bytes;
+  }
+  ~ProfScope() {
+    if (!f) return;
+    hipEventRecord(b, c->stream);
+    f->pending.push_back({a, b});
+  }
+};
+
+static void prof_flush(lmm_ctx *c) {
+  for (auto &kv : c->fam) {
+    for (auto &ev : kv.second.pending) {
+      hipEventSynchronize(ev.second);
+      float ms = 0.f;
+      hipEventElapsedTime(&ms, ev.first, ev.second);
+      kv.second.ms += ms;
+      hipEventDestroy(ev.first);
+      hipEventDestroy(ev.second);
+    }
+    kv.second.pending.clear();
+  }
+}
+
+// ------------------------- helpers --------------------------------------
+
+static int compute_nlevs(const Grid &g, int minwidth) {
+  int nl = -1;
+  for (int d = 0; d < g.dim; d++) {
+    int temp = g.n[d], rdir = 1;
+    while (temp % 2 == 0 && temp / minwidth != 1) {
+      temp /= 2;
+      rdir++;
+    }
+    nl = (nl == -1) ? rdir : std::min(nl, rdir);
+  }
+  return nl;
+}
+
+static long kind_count(const lmm_ctx *c) { return c->g.N; }
+
+static bool is_face(int kind) { return kind >= LMM_FACE_X && kind <= LMM_FACE_Z; }
+
+template <typename T>
+static int alloc_levels(lmm_ctx *c, std::vector<SLevel<T>> &S, std::vector<CLevel<T>> &CC) {
+  int dim = c->g.dim;
+  S.assign(c->nl_s, SLevel<T>());
+  Grid g = c->g;
+  for (int n = 0; n < c->nl_s; n++) {
+    S[n].g = g;
+    long N = g.N;
+    for (int d = 0; d < dim; d++) {
+      S[n].alpha[d] = dalloc<T>(c->stream, N);
+      S[n].res[d] = dalloc<T>(c->stream, N);
+      if (n > 0 || sizeof(T) == 4) {
+        S[n].phi[d] = dalloc<T>(c->stream, N);
+        S[n].rhs[d] = dalloc<T>(c->stream, N);
+      }
+      if (!S[n].alpha[d] || !S[n].res[d]) return LMM_ERR_NOMEM;
+      if ((n > 0 || sizeof(T) == 4) && (!S[n].phi[d] || !S[n].rhs[d])) return LMM_ERR_NOMEM;
+    }
+    if (n > 0 || sizeof(T) == 4) {
+      S[n].beta = dalloc<T>(c->stream, N);
+      S[n].gamma = dalloc<T>(c->stream, N);
+      if (!S[n].beta || !S[n].gamma) return LMM_ERR_NOMEM;
+      for (int e = 0; e < (dim == 2 ? 1 : 3); e++) {
+        S[n].bed[e] = dalloc<T>(c->stream, N);
+        if (!S[n].bed[e]) return LMM_ERR_NOMEM;
+      }
+    }
+    g = coarsen(g);
+  }
+  CC.assign(c->nl_c, CLevel<T>());
+  g = c->g;
+  for (int n = 0; n < c->nl_c; n++) {
+    CC[n].g = g;
+    long N = g.N;
+    for (int d = 0; d < dim; d++) {
+      CC[n].a[d] = dalloc<T>(c->stream, N);
+      if (!CC[n].a[d]) return LMM_ERR_NOMEM;
+    }
+    if (n > 0 || sizeof(T) == 4) {
+      CC[n].phi = dalloc<T>(c->stream, N);
+      CC[n].rhs = dalloc<T>(c->stream, N);
+      if (!CC[n].phi || !CC[n].rhs) return LMM_ERR_NOMEM;
+    }
+    CC[n].res = dalloc<T>(c->stream, N);
+    if (!CC[n].res) return LMM_ERR_NOMEM;
+    g = coarsen(g);
+  }
+  return LMM_OK;
+}
+
+template <typename T>
+static void free_levels(std::vector<SLevel<T>> &S, std::vector<CLevel<T>> &CC, bool own0) {
+  for (size_t n = 0; n < S.size(); n++) {
+    for (int d = 0; d < 3; d++) {
+      dfree(S[n].alpha[d]);
+      dfree(S[n].res[d]);
+      if (n > 0 || own0) {
+        dfree(S[n].phi[d]);
+        dfree(S[n].rhs[d]);
+        dfree(S[n].bed[d]);
+      }
+    }
+    if (n > 0 || own0) {
+      dfree(S[n].beta);
+      dfree(S[n].gamma);
+    }
+  }
+  for (size_t n = 0; n < CC.size(); n++) {
+    for (int d = 0; d < 3; d++) dfree(CC[n].a[d]);
+    if (n > 0 || own0) {
+      dfree(CC[n].phi);
+      dfree(CC[n].rhs);
+    }
+    dfree(CC[n].res);
+  }
+  S.clear();
+  CC.clear();
+}
+
+static int ensure_krylov(lmm_ctx *c, int m) {
+  if (c->kry_m >= m) return LMM_OK;
+  long L = (long)(c->g.dim + 1) * c->g.N;
+  dfree(c->V);
+  dfree(c->Z32);
+  c->V = dalloc<double>(c->stream, L * (m + 1));
+  if (!c->V) return set_err(c, LMM_ERR_NOMEM, "cannot allocate %d Krylov vectors", m + 1);
+  if (c->prm.precision != LMM_FP64) {
+    c->Z32 = dalloc<float>(c->stream, L * m);
+    if (!c->Z32) return set_err(c, LMM_ERR_NOMEM, "cannot allocate FP32 preconditioned vectors");
+  }
+  c->kry_m = m;
+  return LMM_OK;
+}
+
+// ------------------------- coefficient staging ---------------------------
+
+struct CoefIn {
+  const double *alpha[3];
+  const double *beta;
+  const double *bed[3];
+  const double *gamma;
+};
+
+static int ne_of(int dim) { return dim == 2 ? 1 : 3; }
+
+// Level-0 FP64 coefficients (alpha*theta, optional scaled copies), coarse
+// levels by the reference's restrictions (stag_mg_solver.f90:200-230), the
+// face coefficients 1/alpha of the preconditioner (apply_precon.f90:74-76)
+// and the cell-centred hierarchy; FP32 copies when requested.
+static void prepare_coefs(lmm_ctx *c, const CoefIn &in, double theta_alpha, double scale, bool need_cc,
+                          bool need32) {
+  hipStream_t s = c->stream;
+  const Grid &g = c->g;
+  int dim = g.dim;
+  auto &S = c->s64;
+  for (int d = 0; d < dim; d++) {
+    // alpha_fc_mg(1) = alpha_fc * theta_alpha
+    launch_scale_copy(s, S[0].alpha[d], in.alpha[d], g.N, theta_alpha);
+  }
+  if (scale != 1.0) {
+    launch_scale_copy(s, c->scaled_beta, in.beta, g.N, scale);
+    launch_scale_copy(s, c->scaled_gamma, in.gamma, g.N, scale);
+    for (int e = 0; e < ne_of(dim); e++) launch_scale_copy(s, c->scaled_bed[e], in.bed[e], g.N, scale);
+    S[0].beta = c->scaled_beta;
+    S[0].gamma = c->scaled_gamma;
+    for (int e = 0; e < ne_of(dim); e++) S[0].bed[e] = c->scaled_bed[e];
+  } else {
+    S[0].beta = const_cast<double *>(in.beta);
+    S[0].gamma = const_cast<double *>(in.gamma);
+    for (int e = 0; e < ne_of(dim); e++) S[0].bed[e] = const_cast<double *>(in.bed[e]);
+  }
+  for (int n = 1; n < c->nl_s; n++) {
+    const Grid &gc = S[n].g;
+    launch_cc_restrict<double>(s, gc, S[n - 1].beta, S[n].beta);
+    launch_cc_restrict<double>(s, gc, S[n - 1].gamma, S[n].gamma);
+    launch_stag_restrict<double>(s, gc, S[n - 1].alpha, S[n].alpha, 1);
+    if (dim == 2)
+      launch_nodal_restrict<double>(s, gc, S[n - 1].bed[0], S[n].bed[0]);
+    else
+      launch_edge_restrict<double>(s, gc, S[n - 1].bed, S[n].bed);
+  }
+  if (need_cc) {
+    for (int d = 0; d < dim; d++) launch_reciprocal(s, c->ainv[d], in.alpha[d], g.N);
+    auto &C = c->c64;
+    for (int d = 0; d < dim; d++) hipMemcpyAsync(C[0].a[d], c->ainv[d], sizeof(double) * g.N, hipMemcpyDeviceToDevice, s);
+    for (int n = 1; n < c->nl_c; n++) launch_stag_restrict<double>(s, C[n].g, C[n - 1].a, C[n].a, 1);
+  }
+  if (need32) {
+    auto &S32 = c->s32;
+    for (int n = 0; n < c->nl_s; n++) {
+      long N = S[n].g.N;
+      for (int d = 0; d < dim; d++) launch_convert<float, double>(s, S32[n].alpha[d], S[n].alpha[d], N);
+      launch_convert<float, double>(s, S32[n].beta, S[n].beta, N);
+      launch_convert<float, double>(s, S32[n].gamma, S[n].gamma, N);
+      for (int e = 0; e < ne_of(dim); e++) launch_convert<float, double>(s, S32[n].bed[e], S[n].bed[e], N);
+    }
+    auto &C = c->c64;
+    auto &C32 = c->c32;
+    for (int n = 0; n < c->nl_c; n++)
+      for (int d = 0; d < dim; d++) launch_convert<float, double>(s, C32[n].a[d], C[n].a[d], C[n].g.N);
+  }
+}
+
+// ------------------------- staggered multigrid ---------------------------
+
+static double sweep_bytes(const Grid &g, int visc, size_t es) {
+  // compulsory traffic of one full sweep (all colours): read phi, rhs, alpha
+  // per component, beta_cc (+gamma for |visc|=3) and the dim*(dim-1)/2 edge
+  // or node arrays, write phi per component.
+  int dim = g.dim;
+  int ne = (dim == 2) ? 1 : 3;
+  int arrays = 4 * dim + 1 + (visc < 0 ? ne : 0) + ((visc == -3) ? 1 : 0);
+  return (double)arrays * (double)g.N * (double)es;
+}
+
+template <typename T>
+static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], int nsweeps) {
+  const lmm_params &p = c->prm;
+  int dim = L.g.dim;
+  StagCoef<T> K = L.K();
+  for (int m = 0; m < nsweeps; m++) {
+    if (p.stag_mg_smoother == 0) {
+      ProfScope ps(c, "stag_smooth", sweep_bytes(L.g, p.visc_type, sizeof(T)));
+      launch_stag_jacobi<T>(c->stream, L.g, p.visc_type, (T)p.stag_mg_omega, phi, rhs, L.res, K);
+      for (int d = 0; d < dim; d++)
+        hipMemcpyAsync(phi[d], L.res[d], sizeof(T) * L.g.N, hipMemcpyDeviceToDevice, c->stream);
+      continue;
+    }
+    for (int color = 1; color <= 2 * dim; color++) {
+      ProfScope ps(c, "stag_smooth", sweep_bytes(L.g, p.visc_type, sizeof(T)) / (2 * dim));
+      launch_stag_gs_colour<T>(c->stream, L.g, p.visc_type, (color - 1) / 2, (color + 1) % 2,
+                               (T)p.stag_mg_omega, phi, K, rhs[(color - 1) / 2]);
+    }
+  }
+}
+
+// max |rhs - L phi| per component on level 0 (FP64), synchronises
+static void resid_inf(lmm_ctx *c, SLevel<double> &L, double *const phi[3], double *const rhs[3], double out[3]) {
+  launch_stag_residual<double>(c->stream, L.g, c->prm.visc_type, phi, rhs, L.res, L.K());
+  launch_face_maxabs(c->stream, L.g, L.res, c->partial);
+  launch_finalize_max(c->stream, c->partial, c->scal + 4000);
+  hipMemcpyAsync(c->hscal, c->scal + 4000, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  for (int d = 0; d < 3; d++) out[d] = c->hscal[d];
+}
+
+// One call of stag_mg_solver (stag_mg_solver.f90:37-868) on prepared levels.
+// phi0/rhs0 are the level-0 arrays; phi_zero tells that phi0 is zero on
+// entry, in which case the initial-residual exit of :276-306 cannot trigger
+// unless rhs is zero, and a V-cycle of zero data returns zero exactly.
+template <typename T>
+static void stag_vcycles(lmm_ctx *c, std::vector<SLevel<T>> &S, T *const phi0[3], T *const rhs0[3],
+                         bool phi_zero) {
+  const lmm_params &p = c->prm;
+  int dim = c->g.dim;
+  int nl = c->nl_s;
+  double resid0[3] = {1, 1, 1};
+  bool need_check = (p.stag_mg_max_vcycles > 1) || !phi_zero;
+  if constexpr (std::is_same<T, double>::value) {
+   if (need_check) {
+    resid_inf(c, S[0], phi0, rhs0, resid0);
+    bool allzero = true, anyzero = false;
+    double mx = 0;
+    for (int d = 0; d < dim; d++) {
+      if (resid0[d] != 0.0) allzero = false;
+      else anyzero = true;
+      mx = std::max(mx, resid0[d]);
+    }
+    if (allzero) return;
+    if (anyzero)
+      for (int d = 0; d < dim; d++) resid0[d] = mx;
+   }
+  }
+  for (int vc = 1; vc <= p.stag_mg_max_vcycles; vc++) {
+    for (int n = 1; n < nl; n++)
+      for (int d = 0; d < dim; d++) launch_fill<T>(c->stream, S[n].phi[d], S[n].g.N, (T)0);
+    for (int n = 0; n < nl - 1; n++) {
+      T *const *ph = (n == 0) ? phi0 : S[n].phi;
+      T *const *rh = (n == 0) ? rhs0 : S[n].rhs;
+      smooth<T>(c, S[n], ph, rh, p.stag_mg_nsmooths_down);
+      {
+        ProfScope ps(c, "stag_restrict", 0.0);
+        launch_stag_residual<T>(c->stream, S[n].g, p.visc_type, ph, rh, S[n].res, S[n].K());
+        launch_stag_restrict<T>(c->stream, S[n + 1].g, S[n].res, S[n + 1].rhs, 0);
+      }
+    }
+    {
+      int n = nl - 1;
+      T *const *ph = (n == 0) ? phi0 : S[n].phi;
+      T *const *rh = (n == 0) ? rhs0 : S[n].rhs;
+      smooth<T>(c, S[n], ph, rh, p.stag_mg_nsmooths_bottom);
+    }
+    for (int n = nl - 2; n >= 0; n--) {
+      T *const *ph = (n == 0) ? phi0 : S[n].phi;
+      T *const *rh = (n == 0) ? rhs0 : S[n].rhs;
+      {
+        ProfScope ps(c, "stag_prolong", 0.0);
+        launch_stag_prolong<T>(c->stream, S[n].g, ph, S[n + 1].phi);
+      }
+      smooth<T>(c, S[n], ph, rh, p.stag_mg_nsmooths_up);
+    }
+    c->vcycles += dim;
+    if constexpr (std::is_same<T, double>::value) {
+     if (vc < p.stag_mg_max_vcycles) {
+      double r[3];
+      resid_inf(c, S[0], phi0, rhs0, r);
+      bool done = true;
+      for (int d = 0; d < dim; d++)
+        if (!(r[d] / resid0[d] < p.stag_mg_rel_tol)) done = false;
+      if (done) break;
+     }
+    }
+  }
+}
+
+// ------------------------- cell-centred multigrid ------------------------
+
+template <typename T>
+static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *rhs0) {
+  const lmm_params &p = c->prm;
+  int nl = c->nl_c;
+  launch_fill<T>(c->stream, phi0, c->g.N, (T)0);
+  auto rb = [&](int n, int sweeps) {
+    T *ph = (n == 0) ? phi0 : C[n].phi;
+    const T *rh = (n == 0) ? rhs0 : C[n].rhs;
+    for (int m = 0; m < sweeps; m++)
+      for (int colour = 0; colour < 2; colour++) {
+        ProfScope ps(c, "cc_smooth", 0.0);
+        launch_cc_rb_colour<T>(c->stream, C[n].g, colour, ph, rh, C[n].a);
+      }
+  };
+  for (int cyc = 0; cyc < p.mg_max_vcycles; cyc++) {
+    for (int n = 1; n < nl; n++) launch_fill<T>(c->stream, C[n].phi, C[n].g.N, (T)0);
+    for (int n = 0; n < nl - 1; n++) {
+      rb(n, p.mg_nsmooths_down);
+      T *ph = (n == 0) ? phi0 : C[n].phi;
+      const T *rh = (n == 0) ? rhs0 : C[n].rhs;
+      launch_cc_residual<T>(c->stream, C[n].g, ph, rh, C[n].res, C[n].a);
+      launch_cc_restrict<T>(c->stream, C[n + 1].g, C[n].res, C[n + 1].rhs);
+    }
+    rb(nl - 1, p.mg_nsmooths_bottom);
+    for (int n = nl - 2; n >= 0; n--) {
+      T *ph = (n == 0) ? phi0 : C[n].phi;
+      launch_cc_prolong_add<T>(c->stream, C[n].g, ph, C[n + 1].phi);
+      rb(n, p.mg_nsmooths_up);
+    }
+    c->vcycles += 1;
+  }
+}
+
+// ------------------------- Stokes pieces ---------------------------------
+
+struct SV {
+  double *u[3];
+  double *p;
+};
+
+static SV sv_of(lmm_ctx *c, double *base) {
+  SV v;
+  long N = c->g.N;
+  for (int d = 0; d < 3; d++) v.u[d] = (d < c->g.dim) ? base + d * N : nullptr;
+  v.p = base + c->g.dim * N;
+  return v;
+}
+
+template <typename T>
+struct SVT {
+  T *u[3];
+  T *p;
+};
+
+template <typename T>
+static SVT<T> svt_of(lmm_ctx *c, T *base) {
+  SVT<T> v;
+  long N = c->g.N;
+  for (int d = 0; d < 3; d++) v.u[d] = (d < c->g.dim) ? base + d * N : nullptr;
+  v.p = base + c->g.dim * N;
+  return v;
+}
+
+static void apply_matrix_dev(lmm_ctx *c, const SV &x, const SV &b) {
+  ProfScope ps(c, "apply_matrix", (double)c->g.N * 8.0 * (c->g.dim == 3 ? 15 : 10));
+  launch_apply_matrix<double>(c->stream, c->g, c->prm.visc_type, x.u, x.p, b.u, b.p, c->s64[0].K());
+}
+
+// apply_precon.f90:29-195,434-460 with precon_type 1 (T = FP64 or FP32).
+template <typename T>
+static void apply_precon_t(lmm_ctx *c, std::vector<SLevel<T>> &S, std::vector<CLevel<T>> &C,
+                           T *const bu[3], const T *bp, T *const xu[3], T *xp, T *mac_rhs, T *phi,
+                           double theta_alpha) {
+  hipStream_t s = c->stream;
+  const Grid &g = c->g;
+  int dim = g.dim;
+  for (int d = 0; d < dim; d++) launch_fill<T>(s, xu[d], g.N, (T)0);
+  stag_vcycles<T>(c, S, xu, bu, true);
+  launch_div_plus<T>(s, g, xu, bp, mac_rhs, 1);
+  cc_vcycles<T>(c, C, phi, mac_rhs);
+  T *ainv[3] = {C[0].a[0], C[0].a[1], C[0].a[2]};
+  const T *beta = S[0].beta, *gamma = S[0].gamma;
+  // beta/gamma of the preconditioner are the unscaled-by-theta level-0 cell
+  // coefficients (scale_factor already folded in by prepare_coefs)
+  int nb = (int)((g.N + 255) / 256);
+  launch_precon_final<T>(s, g, c->prm.visc_type, xu, xp, phi, mac_rhs, ainv, beta, gamma, theta_alpha,
+                         c->partial2);
+  launch_fold_cells(s, c->partial2, nb, c->partial);
+  launch_finalize(s, g, c->partial, c->scal + 4010, 3, 1.0);
+  // the pressure mean (and velocity means when theta_alpha = 0) are
+  // accumulated and subtracted in FP64 also for FP32 fields
+  launch_sub_scalar<T>(s, xp, g.N, c->scal + 4010);
+  if (theta_alpha == 0.0) {
+    launch_sum_faces_weighted<T>(s, g, xu, c->partial);
+    launch_finalize(s, g, c->partial, c->scal + 4020, 4, 1.0);
+    launch_sub_scalar3<T>(s, g, xu, c->scal + 4020);
+  }
+}
+
+static void precon64(lmm_ctx *c, const SV &b, const SV &x, double theta_alpha) {
+  apply_precon_t<double>(c, c->s64, c->c64, b.u, b.p, x.u, x.p, c->mac_rhs, c->phi_cc, theta_alpha);
+}
+
+// ------------------------- reductions to host ----------------------------
+
+// dot / norm of Stokes vectors; result in device slot, optional host copy
+static void stokes_dot_dev(lmm_ctx *c, const double *a, const double *b, double *slot, int op) {
+  ProfScope ps(c, "krylov", (double)c->g.N * 8.0 * 2 * (c->g.dim + 1));
+  launch_stokes_dot(c->stream, c->g, a, b, c->partial);
+  launch_finalize(c->stream, c->g, c->partial, slot, op, c->prm.p_norm_weight);
+}
+
+static void fetch(lmm_ctx *c, const double *dev, int n, double *host) {
+  hipMemcpyAsync(c->hscal, dev, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  memcpy(host, c->hscal, sizeof(double) * n);
+}
+
+// ------------------------- GMRES host scalars (gmres.f90:517-605) --------
+
+static void rotmat(double a, double b, double &cs, double &sn) {
+  if (b == 0.0) {
+    cs = 1.0;
+    sn = 0.0;
+  } else if (std::fabs(b) > std::fabs(a)) {
+    double temp = a / b;
+    sn = 1.0 / std::sqrt(1.0 + temp * temp);
+    cs = temp * sn;
+  } else {
+    double temp = b / a;
+    cs = 1.0 / std::sqrt(1.0 + temp * temp);
+    sn = temp * cs;
+  }
+}
+
+struct Hess {
+  int m;
+  std::vector<double> H, cs, sn, s, y;
+  explicit Hess(int mm) : m(mm), H((size_t)(mm + 1) * mm, 0.0), cs(mm), sn(mm), s(mm + 1), y(mm) {}
+  double &h(int r, int col) { return H[(size_t)(r - 1) + (size_t)(m + 1) * (col - 1)]; }
+  void least_squares(int i) {
+    for (int k = 1; k <= i - 1; k++) {
+      double temp = cs[k - 1] * h(k, i) + sn[k - 1] * h(k + 1, i);
+      h(k + 1, i) = -sn[k - 1] * h(k, i) + cs[k - 1] * h(k + 1, i);
+      h(k, i) = temp;
+    }
+    rotmat(h(i, i), h(i + 1, i), cs[i - 1], sn[i - 1]);
+    double temp = cs[i - 1] * s[i - 1];
+    s[i] = -sn[i - 1] * s[i - 1];
+    s[i - 1] = temp;
+    h(i, i) = cs[i - 1] * h(i, i) + sn[i - 1] * h(i + 1, i);
+    h(i + 1, i) = 0.0;
+  }
+  void solve(int k) {
+    y[k] = s[k] / h(k + 1, k + 1);
+    for (int i = k; i >= 1; i--) {
+      double dp = 0.0;
+      for (int q = i + 1; q <= k + 1; q++) dp = dp + h(i, q) * y[q - 1];
+      y[i - 1] = (s[i - 1] - dp) / h(i, i);
+    }
+  }
+};
+
+// ------------------------- field checks ----------------------------------
+
+static bool field_ok(lmm_ctx *c, const lmm_field *f, int kind, int ncomp = 1) {
+  return f && f->ctx == c && f->kind == kind && f->ncomp >= ncomp;
+}
+
+static int check_stokes(lmm_ctx *c, lmm_field *const u[3], const lmm_field *p, const char *what) {
+  for (int d = 0; d < c->g.dim; d++)
+    if (!field_ok(c, u[d], LMM_FACE_X + d)) return set_err(c, LMM_ERR_ARG, "%s: face component %d missing or wrong kind", what, d);
+  if (p && !field_ok(c, p, LMM_CELL)) return set_err(c, LMM_ERR_ARG, "%s: cell field missing or wrong kind", what);
+  return LMM_OK;
+}
+
+static int check_coefs(lmm_ctx *c, lmm_field *const alpha[3], const lmm_field *beta, lmm_field *const bed[3],
+                       const lmm_field *gamma) {
+  int dim = c->g.dim;
+  for (int d = 0; d < dim; d++)
+    if (!field_ok(c, alpha[d], LMM_FACE_X + d)) return set_err(c, LMM_ERR_ARG, "alpha_fc[%d] missing or wrong kind", d);
+  if (!field_ok(c, beta, LMM_CELL)) return set_err(c, LMM_ERR_ARG, "beta missing or wrong kind");
+  if (!field_ok(c, gamma, LMM_CELL)) return set_err(c, LMM_ERR_ARG, "gamma missing or wrong kind");
+  if (dim == 2) {
+    if (!field_ok(c, bed[0], LMM_NODE)) return set_err(c, LMM_ERR_ARG, "beta_ed[0] must be a node field in 2D");
+  } else {
+    for (int e = 0; e < 3; e++)
+      if (!field_ok(c, bed[e], LMM_EDGE_XY + e)) return set_err(c, LMM_ERR_ARG, "beta_ed[%d] missing or wrong kind", e);
+  }
+  return LMM_OK;
+}
+
+static CoefIn coef_in(lmm_ctx *c, lmm_field *const alpha[3], lmm_field *beta, lmm_field *const bed[3],
+                      lmm_field *gamma) {
+  CoefIn ci;
+  for (int d = 0; d < 3; d++) {
+    ci.alpha[d] = (d < c->g.dim) ? alpha[d]->d : nullptr;
+    ci.bed[d] = (d < ne_of(c->g.dim)) ? bed[d]->d : nullptr;
+  }
+  ci.beta = beta->d;
+  ci.gamma = gamma->d;
+  return ci;
+}
+
+static int ensure_lvl32(lmm_ctx *c) {
+  if (c->lvl32) return LMM_OK;
+  int rc = alloc_levels<float>(c, c->s32, c->c32);
+  if (rc) return set_err(c, rc, "cannot allocate FP32 multigrid levels");
+  long L = (long)(c->g.dim + 1) * c->g.N;
+  c->b32 = dalloc<float>(c->stream, L);
+  c->mac_rhs32 = dalloc<float>(c->stream, c->g.N);
+  if (!c->b32 || !c->mac_rhs32) return set_err(c, LMM_ERR_NOMEM, "cannot allocate FP32 work vectors");
+  c->lvl32 = true;
+  return LMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// GMRES, reference algorithm (gmres.f90:30-607), FP64
+// ---------------------------------------------------------------------------
+
+static int gmres_ref(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
+  const lmm_params &p = c->prm;
+  const Grid &g = c->g;
+  int dm = g.dim, m = p.gmres_max_inner;
+  long L = (long)(dm + 1) * g.N;
+  hipStream_t s = c->stream;
+  SV b = sv_of(c, c->b), x = sv_of(c, c->x), r = sv_of(c, c->r), w = sv_of(c, c->w), tmp = sv_of(c, c->tmp);
+  auto Vk = [&](int k) { return c->V + (long)k * L; };
+  Hess hs(m);
+  double *dH = c->scal;  // (m+1)*m slots, column-major like hs.H
+  // |M^-1 b|
+  precon64(c, b, tmp, theta_alpha);
+  stokes_dot_dev(c, c->tmp, c->tmp, c->scal + 3000, 1);
+  stokes_dot_dev(c, c->b, c->b, c->scal + 3004, 1);
+  double hb[8];
+  fetch(c, c->scal + 3000, 8, hb);
+  double norm_pre_b = hb[0], norm_b = hb[4];
+  info->norm_pre_rhs = norm_pre_b;
+  info->norm_b = norm_b;
+  if (norm_b <= 0.0) {
+    launch_fill<double>(s, c->x, L, 0.0);
+    info->status = LMM_OK;
+    return LMM_OK;
+  }
+  int total_iter = 0, iter = 0, i_copy = 0;
+  double norm_resid = 0, norm_init_resid = 0, norm_resid_Stokes = 0;
+  bool converged = false;
+  for (;;) {
+    apply_matrix_dev(c, x, tmp);
+    launch_bmax_neg(s, c->tmp, c->b, L);
+    stokes_dot_dev(c, c->tmp, c->tmp, c->scal + 3008, 1);
+    precon64(c, tmp, r, theta_alpha);
+    stokes_dot_dev(c, c->r, c->r, c->scal + 3012, 1);
+    fetch(c, c->scal + 3008, 8, hb);
+    norm_resid_Stokes = hb[0];
+    norm_resid = hb[4];
+    if (iter == 0) norm_init_resid = norm_resid;
+    if (p.gmres_verbose >= 1)
+      printf("lmm gmres outer %d total %d |Pr| %.10e |r| %.10e\n", iter, total_iter, norm_resid, norm_resid_Stokes);
+    if (total_iter >= p.gmres_max_iter) break;
+    if (total_iter >= p.gmres_min_iter) {
+      double mn = std::min(norm_pre_b, norm_init_resid);
+      if (norm_resid <= p.gmres_rel_tol * mn) {
+        converged = true;
+        break;
+      }
+      if (norm_resid <= p.gmres_abs_tol) {
+        converged = true;
+        break;
+      }
+    }
+    if (iter >= p.gmres_max_outer) break;
+    iter++;
+    // V(1) = r / |r|
+    launch_div_by(s, Vk(0), c->r, L, c->scal + 3012);
+    std::fill(hs.s.begin(), hs.s.end(), 0.0);
+    hs.s[0] = norm_resid;
+    int i;
+    bool brk = false;
+    for (i = 1; i <= m; i++) {
+      total_iter++;
+      i_copy = i;
+      SV vi = sv_of(c, Vk(i - 1));
+      apply_matrix_dev(c, vi, tmp);
+      precon64(c, tmp, w, theta_alpha);
+      for (int k = 1; k <= i; k++) {
+        double *slot = dH + (k - 1) + (long)(m + 1) * (i - 1);
+        stokes_dot_dev(c, c->w, Vk(k - 1), slot, 0);
+        ProfScope ps(c, "krylov", (double)L * 8.0 * 3);
+        launch_axmy(s, c->w, Vk(k - 1), L, slot);
+      }
+      double *sl = c->scal + 3100;
+      stokes_dot_dev(c, c->w, c->w, sl, 1);
+      // H(i+1,i) slot
+      hipMemcpyAsync(dH + i + (long)(m + 1) * (i - 1), sl, sizeof(double), hipMemcpyDeviceToDevice, s);
+      launch_div_by(s, Vk(i), c->w, L, sl);
+      fetch(c, dH + (long)(m + 1) * (i - 1), i + 1, &hs.h(1, i));
+      if (hs.h(i + 1, i) == 0.0) {
+        brk = true;
+        break;
+      }
+      hs.least_squares(i);
+      double est = std::fabs(hs.s[i]);
+      if (p.gmres_verbose >= 2) printf("lmm gmres inner %d est %.10e\n", total_iter, est);
+      if (total_iter >= p.gmres_max_iter) break;
+      if (total_iter >= p.gmres_min_iter) {
+        double mn = std::min(norm_pre_b, norm_init_resid);
+        if (est <= p.gmres_rel_tol * mn || est <= p.gmres_abs_tol) break;
+      }
+    }
+    if (brk) {
+      info->total_iter = total_iter;
+      return set_err(c, LMM_ERR_BREAKDOWN, "gmres: error in orthogonalization");
+    }
+    hs.solve(i_copy - 1);
+    memcpy(c->hscal, hs.y.data(), sizeof(double) * i_copy);
+    hipMemcpyAsync(c->scal + 3200, c->hscal, sizeof(double) * i_copy, hipMemcpyHostToDevice, s);
+    std::vector<double *> vl(i_copy);
+    for (int k = 0; k < i_copy; k++) vl[k] = Vk(k);
+    launch_update_sol(s, c->x, vl.data(), c->scal + 3200, i_copy, L);
+    hipStreamSynchronize(s);  // hscal reused
+  }
+  info->total_iter = total_iter;
+  info->outer_iter = iter;
+  info->norm_resid = norm_resid;
+  info->norm_resid_true = norm_resid_Stokes;
+  info->status = converged ? LMM_OK : LMM_NOT_CONVERGED;
+  return LMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Flexible GMRES with FP32 V-cycles (right preconditioned, MGS, FP64 outer)
+// ---------------------------------------------------------------------------
+
+static void precon32(lmm_ctx *c, const double *v64, float *z32, double theta_alpha) {
+  hipStream_t s = c->stream;
+  long L = (long)(c->g.dim + 1) * c->g.N;
+  launch_convert<float, double>(s, c->b32, v64, L);
+  SVT<float> b = svt_of<float>(c, c->b32), z = svt_of<float>(c, z32);
+  apply_precon_t<float>(c, c->s32, c->c32, b.u, b.p, z.u, z.p, c->mac_rhs32, c->c32[0].phi, theta_alpha);
+}
+
+static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
+  const lmm_params &p = c->prm;
+  const Grid &g = c->g;
+  int dm = g.dim, m = p.gmres_max_inner;
+  long L = (long)(dm + 1) * g.N;
+  hipStream_t s = c->stream;
+  SV b = sv_of(c, c->b), x = sv_of(c, c->x), tmp = sv_of(c, c->tmp), w = sv_of(c, c->w);
+  auto Vk = [&](int k) { return c->V + (long)k * L; };
+  auto Zk = [&](int k) { return c->Z32 + (long)k * L; };
+  Hess hs(m);
+  double *dH = c->scal;
+  double hb[8];
+  stokes_dot_dev(c, c->b, c->b, c->scal + 3004, 1);
+  fetch(c, c->scal + 3004, 4, hb);
+  double norm_b = hb[0];
+  info->norm_b = norm_b;
+  info->norm_pre_rhs = 0.0;
+  if (norm_b <= 0.0) {
+    launch_fill<double>(s, c->x, L, 0.0);
+    return LMM_OK;
+  }
+  int total_iter = 0, iter = 0;
+  double rnorm = 0;
+  bool converged = false;
+  for (;;) {
+    apply_matrix_dev(c, x, tmp);
+    launch_bmax_neg(s, c->tmp, c->b, L);
+    stokes_dot_dev(c, c->tmp, c->tmp, c->scal + 3008, 1);
+    fetch(c, c->scal + 3008, 4, hb);
+    rnorm = hb[0];
+    if (p.gmres_verbose >= 1) printf("lmm fgmres outer %d total %d |r|/|b| %.10e\n", iter, total_iter, rnorm / norm_b);
+    if (rnorm <= p.gmres_rel_tol * norm_b || rnorm <= p.gmres_abs_tol) {
+      converged = true;
+      break;
+    }
+    if (total_iter >= p.gmres_max_iter || iter >= p.gmres_max_outer) break;
+    iter++;
+    launch_div_by(s, Vk(0), c->tmp, L, c->scal + 3008);
+    std::fill(hs.s.begin(), hs.s.end(), 0.0);
+    hs.s[0] = rnorm;
+    int i, used = 0;
+    for (i = 1; i <= m; i++) {
+      total_iter++;
+      used = i;
+      precon32(c, Vk(i - 1), Zk(i - 1), theta_alpha);
+      launch_convert<double, float>(s, c->r, Zk(i - 1), L);
+      SV z = sv_of(c, c->r);
+      apply_matrix_dev(c, z, w);
+      for (int k = 1; k <= i; k++) {
+        double *slot = dH + (k - 1) + (long)(m + 1) * (i - 1);
+        stokes_dot_dev(c, c->w, Vk(k - 1), slot, 0);
+        launch_axmy(s, c->w, Vk(k - 1), L, slot);
+      }
+      double *sl = c->scal + 3100;
+      stokes_dot_dev(c, c->w, c->w, sl, 1);
+      hipMemcpyAsync(dH + i + (long)(m + 1) * (i - 1), sl, sizeof(double), hipMemcpyDeviceToDevice, s);
+      launch_div_by(s, Vk(i), c->w, L, sl);
+      fetch(c, dH + (long)(m + 1) * (i - 1), i + 1, &hs.h(1, i));
+      if (hs.h(i + 1, i) == 0.0) break;  // happy breakdown: exact solution in the space
+      hs.least_squares(i);
+      double est = std::fabs(hs.s[i]);
+      if (est <= p.gmres_rel_tol * norm_b || total_iter >= p.gmres_max_iter) break;
+    }
+    hs.solve(used - 1);
+    // x += sum_k y_k Z_k  (Z in FP32, accumulation in FP64)
+    memcpy(c->hscal, hs.y.data(), sizeof(double) * used);
+    hipMemcpyAsync(c->scal + 3200, c->hscal, sizeof(double) * used, hipMemcpyHostToDevice, s);
+    for (int k = 0; k < used; k++) {
+      launch_convert<double, float>(s, c->r, Zk(k), L);
+      double *one = c->r;
+      launch_update_sol(s, c->x, &one, c->scal + 3200 + k, 1, L);
+    }
+    hipStreamSynchronize(s);
+  }
+  info->total_iter = total_iter;
+  info->outer_iter = iter;
+  info->norm_resid = rnorm;
+  info->norm_resid_true = rnorm;
+  info->status = converged ? LMM_OK : LMM_NOT_CONVERGED;
+  (void)tmp;
+  return LMM_OK;
+}
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+extern "C" {
+
+void lmm_params_default(lmm_params *p) {
+  memset(p, 0, sizeof(*p));
+  p->visc_type = 1;
+  p->precon_type = 1;
+  p->p_norm_weight = 1.0;
+  p->scale_factor = 1.0;
+  p->mg_max_vcycles = 1;
+  p->mg_minwidth = 2;
+  p->mg_nsmooths_down = 2;
+  p->mg_nsmooths_up = 2;
+  p->mg_nsmooths_bottom = 8;
+  p->mg_rel_tol = 1e-9;
+  p->stag_mg_max_vcycles = 1;
+  p->stag_mg_minwidth = 2;
+  p->stag_mg_nsmooths_down = 2;
+  p->stag_mg_nsmooths_up = 2;
+  p->stag_mg_nsmooths_bottom = 8;
+  p->stag_mg_smoother = 1;
+  p->stag_mg_omega = 1.0;
+  p->stag_mg_rel_tol = 1e-9;
+  p->gmres_rel_tol = 1e-9;
+  p->gmres_abs_tol = 0.0;
+  p->gmres_max_outer = 20;
+  p->gmres_max_inner = 5;
+  p->gmres_max_iter = 100;
+  p->gmres_min_iter = 1;
+  p->gmres_verbose = 0;
+  p->precision = LMM_FP64;
+}
+
+static int check_params(lmm_ctx *c, const lmm_params *p) {
+  int v = p->visc_type;
+  if (!(v == 1 || v == -1 || v == 2 || v == -2 || v == 3 || v == -3))
+    return set_err(c, LMM_ERR_ARG, "visc_type %d not in +-1,+-2,+-3", v);
+  if (p->precon_type != 1) return set_err(c, LMM_ERR_UNSUPPORTED, "precon_type %d: only 1 is implemented", p->precon_type);
+  if (p->gmres_max_inner < 1 || p->gmres_max_inner > 30) return set_err(c, LMM_ERR_ARG, "gmres_max_inner out of range 1..30");
+  if (p->stag_mg_smoother != 0 && p->stag_mg_smoother != 1) return set_err(c, LMM_ERR_UNSUPPORTED, "stag_mg_smoother %d", p->stag_mg_smoother);
+  if (p->precision < LMM_FP64 || p->precision > LMM_FP32_MG)
+    return set_err(c, LMM_ERR_UNSUPPORTED, "precision %d not available", p->precision);
+  if (p->stag_mg_minwidth < 1 || p->mg_minwidth < 1) return set_err(c, LMM_ERR_ARG, "minwidth must be >= 1");
+  return LMM_OK;
+}
+
+int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p, lmm_ctx **out) {
+  if (!out || !n_cells || (dim != 2 && dim != 3) || !(dx > 0)) return LMM_ERR_ARG;
+  *out = nullptr;
+  lmm_params dp;
+  lmm_params_default(&dp);
+  if (!p) p = &dp;
+  lmm_ctx *c = new lmm_ctx();
+  {
+    const char *dbg = getenv("LMM_DEBUG_SYNC");
+    set_debug_sync(dbg && dbg[0] == '1');
+  }
+  c->g = make_grid(dim, n_cells, dx);
+  int rc = check_params(c, p);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  c->prm = *p;
+  for (int d = 0; d < dim; d++)
+    if (c->g.n[d] < 2) {
+      delete c;
+      return LMM_ERR_ARG;
+    }
+  c->nl_s = compute_nlevs(c->g, p->stag_mg_minwidth);
+  c->nl_c = compute_nlevs(c->g, p->mg_minwidth);
+  // every staggered level needs even extents (coloured sweeps on periodic faces)
+  {
+    Grid g = c->g;
+    for (int n = 0; n < std::max(c->nl_s, c->nl_c); n++) {
+      for (int d = 0; d < dim; d++)
+        if (g.n[d] % 2 != 0) {
+          delete c;
+          return LMM_ERR_UNSUPPORTED;
+        }
+      g = coarsen(g);
+    }
+  }
+  if (hipStreamCreate(&c->stream) != hipSuccess) {
+    delete c;
+    return LMM_ERR_DEVICE;
+  }
+  rc = alloc_levels<double>(c, c->s64, c->c64);
+  long N = c->g.N, L = (long)(dim + 1) * N;
+  c->partial = dalloc<double>(c->stream, RED_BLOCKS * 4);
+  c->partial2 = dalloc<double>(c->stream, (N + 255) / 256 + 1);
+  c->scal = dalloc<double>(c->stream, 8192);
+  if (hipHostMalloc((void **)&c->hscal, 8192 * sizeof(double), hipHostMallocDefault) != hipSuccess) c->hscal = nullptr;
+  for (int d = 0; d < dim; d++) {
+    c->ainv[d] = dalloc<double>(c->stream, N);
+  }
+  c->scaled_beta = dalloc<double>(c->stream, N);
+  c->scaled_gamma = dalloc<double>(c->stream, N);
+  for (int e = 0; e < ne_of(dim); e++) c->scaled_bed[e] = dalloc<double>(c->stream, N);
+  c->r = dalloc<double>(c->stream, L);
+  c->w = dalloc<double>(c->stream, L);
+  c->tmp = dalloc<double>(c->stream, L);
+  c->b = dalloc<double>(c->stream, L);
+  c->x = dalloc<double>(c->stream, L);
+  c->mac_rhs = dalloc<double>(c->stream, N);
+  c->phi_cc = dalloc<double>(c->stream, N);
+  bool ok = !rc && c->partial && c->partial2 && c->scal && c->hscal && c->r && c->w && c->tmp && c->b && c->x &&
+            c->mac_rhs && c->phi_cc && c->scaled_beta && c->scaled_gamma;
+  for (int d = 0; d < dim; d++) ok = ok && c->ainv[d];
+  for (int e = 0; e < ne_of(dim); e++) ok = ok && c->scaled_bed[e];
+  if (ok && hipStreamSynchronize(c->stream) != hipSuccess) ok = false;
+  if (!ok) {
+    lmm_ctx_destroy(c);
+    return LMM_ERR_NOMEM;
+  }
+  *out = c;
+  return LMM_OK;
+}
+
+int lmm_ctx_destroy(lmm_ctx *c) {
+  if (!c) return LMM_ERR_ARG;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  prof_flush(c);
+  free_levels<double>(c->s64, c->c64, false);
+  free_levels<float>(c->s32, c->c32, true);
+  for (void *p : {(void *)c->partial, (void *)c->partial2, (void *)c->scal, (void *)c->scaled_beta,
+                  (void *)c->scaled_gamma, (void *)c->V, (void *)c->r, (void *)c->w, (void *)c->tmp, (void *)c->b,
+                  (void *)c->x, (void *)c->Z32, (void *)c->b32, (void *)c->mac_rhs, (void *)c->phi_cc,
+                  (void *)c->mac_rhs32, (void *)c->Wcc, (void *)c->Wed, (void *)c->dweights})
+    dfree(p);
+  for (int d = 0; d < 3; d++) {
+    dfree(c->ainv[d]);
+    dfree(c->scaled_bed[d]);
+  }
+  if (c->hscal) hipHostFree(c->hscal);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return LMM_OK;
+}
+
+int lmm_ctx_set_params(lmm_ctx *c, const lmm_params *p) {
+  if (!c || !p) return LMM_ERR_ARG;
+  int rc = check_params(c, p);
+  if (rc) return rc;
+  if (p->stag_mg_minwidth != c->prm.stag_mg_minwidth || p->mg_minwidth != c->prm.mg_minwidth)
+    return set_err(c, LMM_ERR_UNSUPPORTED, "minwidth is fixed at context creation");
+  if (p->gmres_max_inner != c->prm.gmres_max_inner) c->kry_m = std::min(c->kry_m, p->gmres_max_inner - 1);
+  if (p->precision != c->prm.precision) {
+    dfree(c->V);
+    dfree(c->Z32);
+    c->V = nullptr;
+    c->Z32 = nullptr;
+    c->kry_m = -1;
+  }
+  c->prm = *p;
+  return LMM_OK;
+}
+
+int lmm_ctx_get_params(const lmm_ctx *c, lmm_params *p) {
+  if (!c || !p) return LMM_ERR_ARG;
+  *p = c->prm;
+  return LMM_OK;
+}
+
+int lmm_ctx_synchronize(lmm_ctx *c) {
+  if (!c) return LMM_ERR_ARG;
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LMM_OK;
+}
+
+int lmm_ctx_nlevels(const lmm_ctx *c, int *ns, int *nc) {
+  if (!c) return LMM_ERR_ARG;
+  if (ns) *ns = c->nl_s;
+  if (nc) *nc = c->nl_c;
+  return LMM_OK;
+}
+
+const char *lmm_ctx_last_error(const lmm_ctx *c) { return c ? c->err.c_str() : "null context"; }
+
+int lmm_ctx_set_profiling(lmm_ctx *c, int on) {
+  if (!c) return LMM_ERR_ARG;
+  c->prof = on != 0;
+  return LMM_OK;
+}
+
+int lmm_ctx_kernel_stats(lmm_ctx *c, const char *family, int64_t *launches, double *total_ms, double *bytes) {
+  if (!c || !family) return LMM_ERR_ARG;
+  prof_flush(c);
+  auto it = c->fam.find(family);
+  if (it == c->fam.end()) {
+    if (launches) *launches = 0;
+    if (total_ms) *total_ms = 0;
+    if (bytes) *bytes = 0;
+    return LMM_OK;
+  }
+  if (launches) *launches = it->second.launches;
+  if (total_ms) *total_ms = it->second.ms;
+  if (bytes) *bytes = it->second.bytes;
+  return LMM_OK;
+}
+
+int lmm_ctx_reset_stats(lmm_ctx *c) {
+  if (!c) return LMM_ERR_ARG;
+  prof_flush(c);
+  c->fam.clear();
+  return LMM_OK;
+}
+
+// ---------------- fields ----------------
+
+int lmm_field_create(lmm_ctx *c, int kind, int ncomp, lmm_field **out) {
+  if (!c || !out || ncomp < 1) return LMM_ERR_ARG;
+  int dim = c->g.dim;
+  bool ok = kind == LMM_CELL || (kind >= LMM_FACE_X && kind < LMM_FACE_X + dim) || (kind == LMM_NODE && dim == 2) ||
+            (kind >= LMM_EDGE_XY && kind <= LMM_EDGE_YZ && dim == 3);
+  if (!ok) return set_err(c, LMM_ERR_ARG, "field kind %d invalid for dim %d", kind, dim);
+  double *d = dalloc<double>(c->stream, kind_count(c) * ncomp);
+  if (!d) return set_err(c, LMM_ERR_NOMEM, "field allocation failed");
+  lmm_field *f = new lmm_field{c, kind, ncomp, d};
+  *out = f;
+  return LMM_OK;
+}
+
+int lmm_field_destroy(lmm_field *f) {
+  if (!f) return LMM_ERR_ARG;
+  hipStreamSynchronize(f->ctx->stream);
+  dfree(f->d);
+  delete f;
+  return LMM_OK;
+}
+
+int lmm_field_info(const lmm_field *f, int *kind, int *ncomp, int64_t *count) {
+  if (!f) return LMM_ERR_ARG;
+  if (kind) *kind = f->kind;
+  if (ncomp) *ncomp = f->ncomp;
+  if (count) *count = kind_count(f->ctx);
+  return LMM_OK;
+}
+
+int lmm_field_upload(lmm_field *f, const double *host) {
+  if (!f || !host) return LMM_ERR_ARG;
+  lmm_ctx *c = f->ctx;
+  HIPCHK(c, hipMemcpyAsync(f->d, host, sizeof(double) * kind_count(c) * f->ncomp, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LMM_OK;
+}
+
+int lmm_field_download(const lmm_field *f, double *host) {
+  if (!f || !host) return LMM_ERR_ARG;
+  lmm_ctx *c = f->ctx;
+  HIPCHK(c, hipMemcpyAsync(host, f->d, sizeof(double) * kind_count(c) * f->ncomp, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LMM_OK;
+}
+
+static void nodal_flags(int kind, int nod[3]) {
+  nod[0] = nod[1] = nod[2] = 0;
+  if (kind == LMM_FACE_X) nod[0] = 1;
+  if (kind == LMM_FACE_Y) nod[1] = 1;
+  if (kind == LMM_FACE_Z) nod[2] = 1;
+  if (kind == LMM_NODE) nod[0] = nod[1] = 1;
+  if (kind == LMM_EDGE_XY) nod[0] = nod[1] = 1;
+  if (kind == LMM_EDGE_XZ) nod[0] = nod[2] = 1;
+  if (kind == LMM_EDGE_YZ) nod[1] = nod[2] = 1;
+}
+
+static int fab_check(const lmm_field *f, int comp, const int lo[3], const int hi[3], int ng, long ext[3],
+                     int nod[3]) {
+  lmm_ctx *c = f->ctx;
+  if (comp < 0 || comp >= f->ncomp || ng < 0) return set_err(c, LMM_ERR_ARG, "fab: bad comp or ng");
+  nodal_flags(f->kind, nod);
+  int dim = c->g.dim;
+  for (int d = 0; d < 3; d++) {
+    if (d < dim) {
+      if (lo[d] != 0 || hi[d] - lo[d] + 1 != c->g.n[d] + nod[d])
+        return set_err(c, LMM_ERR_ARG, "fab: box [%d,%d] in dir %d does not match the context box", lo[d], hi[d], d);
+      ext[d] = c->g.n[d] + nod[d] + 2 * ng;
+    } else {
+      ext[d] = 1;
+    }
+  }
+  return LMM_OK;
+}
+
+int lmm_field_upload_fab(lmm_field *f, int comp, const double *host, const int lo[3], const int hi[3], int ng) {
+  if (!f || !host || !lo || !hi) return LMM_ERR_ARG;
+  lmm_ctx *c = f->ctx;
+  long ext[3];
+  int nod[3];
+  int rc = fab_check(f, comp, lo, hi, ng, ext, nod);
+  if (rc) return rc;
+  const Grid &g = c->g;
+  int ngz = g.dim == 3 ? ng : 0;
+  std::vector<double> buf(g.N);
+  for (int k = 0; k < g.n[2]; k++)
+    for (int j = 0; j < g.n[1]; j++)
+      for (int i = 0; i < g.n[0]; i++) {
+        long hidx = (i + ng) + ext[0] * ((j + ng) + ext[1] * (long)(k + ngz));
+        buf[i + g.s[1] * j + g.s[2] * k] = host[hidx];
+      }
+  HIPCHK(c, hipMemcpyAsync(f->d + comp * g.N, buf.data(), sizeof(double) * g.N, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LMM_OK;
+}
+
+int lmm_field_download_fab(const lmm_field *f, int comp, double *host, const int lo[3], const int hi[3], int ng) {
+  if (!f || !host || !lo || !hi) return LMM_ERR_ARG;
+  lmm_ctx *c = f->ctx;
+  long ext[3];
+  int nod[3];
+  int rc = fab_check(f, comp, lo, hi, ng, ext, nod);
+  if (rc) return rc;
+  const Grid &g = c->g;
+  std::vector<double> buf(g.N);
+  HIPCHK(c, hipMemcpyAsync(buf.data(), f->d + comp * g.N, sizeof(double) * g.N, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  int ngz = g.dim == 3 ? ng : 0;
+  auto wrap = [](int i, int n) { return ((i % n) + n) % n; };
+  for (long k = -ngz; k < ext[2] - ngz; k++)
+    for (long j = -ng; j < ext[1] - ng; j++)
+      for (long i = -ng; i < ext[0] - ng; i++) {
+        long src = wrap((int)i, g.n[0]) + g.s[1] * wrap((int)j, g.n[1]) + g.s[2] * wrap((int)k, g.n[2]);
+        host[(i + ng) + ext[0] * ((j + ng) + ext[1] * (k + ngz))] = buf[src];
+      }
+  return LMM_OK;
+}
+
+int lmm_field_setval(lmm_field *f, double v) {
+  if (!f) return LMM_ERR_ARG;
+  launch_fill<double>(f->ctx->stream, f->d, kind_count(f->ctx) * f->ncomp, v);
+  return LMM_OK;
+}
+
+int lmm_field_copy(lmm_field *dst, const lmm_field *src) {
+  if (!dst || !src || dst->ctx != src->ctx || dst->ncomp != src->ncomp) return LMM_ERR_ARG;
+  lmm_ctx *c = dst->ctx;
+  HIPCHK(c, hipMemcpyAsync(dst->d, src->d, sizeof(double) * kind_count(c) * src->ncomp, hipMemcpyDeviceToDevice,
+                           c->stream));
+  return LMM_OK;
+}
+
+double *lmm_field_device_ptr(lmm_field *f) { return f ? f->d : nullptr; }
+
+// ---------------- operators ----------------
+
+static int finish(lmm_ctx *c) {
+  hipError_t e = hipStreamSynchronize(c->stream);
+  if (first_error()[0]) return set_err(c, LMM_ERR_DEVICE, "kernel %s", first_error());
+  if (e != hipSuccess) return set_err(c, LMM_ERR_DEVICE, "stream: %s", hipGetErrorString(e));
+  e = hipGetLastError();
+  if (e != hipSuccess) return set_err(c, LMM_ERR_DEVICE, "launch: %s", hipGetErrorString(e));
+  return LMM_OK;
+}
+
+static void copy_stokes_in(lmm_ctx *c, double *dst, lmm_field *const u[3], const lmm_field *p) {
+  long N = c->g.N;
+  for (int d = 0; d < c->g.dim; d++)
+    hipMemcpyAsync(dst + d * N, u[d]->d, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream);
+  hipMemcpyAsync(dst + c->g.dim * N, p->d, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream);
+}
+
+static void copy_stokes_out(lmm_ctx *c, const double *src, lmm_field *const u[3], lmm_field *p) {
+  long N = c->g.N;
+  for (int d = 0; d < c->g.dim; d++)
+    hipMemcpyAsync(u[d]->d, src + d * N, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream);
+  hipMemcpyAsync(p->d, src + c->g.dim * N, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream);
+}
+
+int lmm_gmres(lmm_ctx *c, lmm_field *const b_u[3], lmm_field *b_p, lmm_field *x_u[3], lmm_field *x_p,
+              lmm_field *const alpha_fc[3], lmm_field *beta, lmm_field *const beta_ed[3], lmm_field *gamma,
+              double theta_alpha, double *norm_pre_rhs, lmm_gmres_info *info) {
+  if (!c) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, b_u, b_p, "gmres b"))) return rc;
+  if ((rc = check_stokes(c, x_u, x_p, "gmres x"))) return rc;
+  if ((rc = check_coefs(c, alpha_fc, beta, beta_ed, gamma))) return rc;
+  auto t0 = std::chrono::steady_clock::now();
+  lmm_gmres_info loc;
+  memset(&loc, 0, sizeof(loc));
+  if (!info) info = &loc;
+  memset(info, 0, sizeof(*info));
+  const lmm_params &p = c->prm;
+  if ((rc = ensure_krylov(c, p.gmres_max_inner))) return rc;
+  bool mixed = p.precision != LMM_FP64;
+  if (mixed && (rc = ensure_lvl32(c))) return rc;
+  double sf = p.scale_factor;
+  double ta = theta_alpha * (sf != 1.0 ? sf : 1.0);
+  if (sf != 1.0) theta_alpha = ta;
+  c->vcycles = 0;
+  prepare_coefs(c, coef_in(c, alpha_fc, beta, beta_ed, gamma), theta_alpha, sf, true, mixed);
+  copy_stokes_in(c, c->b, b_u, b_p);
+  copy_stokes_in(c, c->x, x_u, x_p);
+  long N = c->g.N;
+  if (sf != 1.0) {
+    launch_scale<double>(c->stream, c->x + c->g.dim * N, N, sf);
+    launch_scale<double>(c->stream, c->b, c->g.dim * N, sf);
+  }
+  rc = mixed ? gmres_flex(c, theta_alpha, info) : gmres_ref(c, theta_alpha, info);
+  if (sf != 1.0) launch_scale<double>(c->stream, c->x + c->g.dim * N, N, 1.0 / sf);
+  if (rc == LMM_OK) copy_stokes_out(c, c->x, x_u, x_p);
+  int rc2 = finish(c);
+  if (norm_pre_rhs) *norm_pre_rhs = info->norm_pre_rhs;
+  info->vcycles = c->vcycles;
+  info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return rc ? rc : rc2;
+}
+
+int lmm_apply_matrix(lmm_ctx *c, lmm_field *b_u[3], lmm_field *b_p, lmm_field *const x_u[3], lmm_field *x_p,
+                     lmm_field *const alpha_fc[3], lmm_field *beta, lmm_field *const beta_ed[3],
+                     lmm_field *gamma, double theta_alpha) {
+  if (!c) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, b_u, b_p, "apply_matrix b"))) return rc;
+  if ((rc = check_stokes(c, x_u, x_p, "apply_matrix x"))) return rc;
+  if ((rc = check_coefs(c, alpha_fc, beta, beta_ed, gamma))) return rc;
+  CoefIn ci = coef_in(c, alpha_fc, beta, beta_ed, gamma);
+  for (int d = 0; d < c->g.dim; d++) launch_scale_copy(c->stream, c->s64[0].alpha[d], ci.alpha[d], c->g.N, theta_alpha);
+  c->s64[0].beta = const_cast<double *>(ci.beta);
+  c->s64[0].gamma = const_cast<double *>(ci.gamma);
+  for (int e = 0; e < ne_of(c->g.dim); e++) c->s64[0].bed[e] = const_cast<double *>(ci.bed[e]);
+  SV x, b;
+  for (int d = 0; d < 3; d++) {
+    x.u[d] = d < c->g.dim ? x_u[d]->d : nullptr;
+    b.u[d] = d < c->g.dim ? b_u[d]->d : nullptr;
+  }
+  x.p = x_p->d;
+  b.p = b_p->d;
+  apply_matrix_dev(c, x, b);
+  return finish(c);
+}
+
+int lmm_apply_precon(lmm_ctx *c, lmm_field *const b_u[3], lmm_field *b_p, lmm_field *x_u[3], lmm_field *x_p,
+                     lmm_field *const alpha_fc[3], lmm_field *beta, lmm_field *const beta_ed[3],
+                     lmm_field *gamma, double theta_alpha) {
+  if (!c) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, b_u, b_p, "apply_precon b"))) return rc;
+  if ((rc = check_stokes(c, x_u, x_p, "apply_precon x"))) return rc;
+  if ((rc = check_coefs(c, alpha_fc, beta, beta_ed, gamma))) return rc;
+  c->vcycles = 0;
+  prepare_coefs(c, coef_in(c, alpha_fc, beta, beta_ed, gamma), theta_alpha, 1.0, true, false);
+  SV x, b;
+  for (int d = 0; d < 3; d++) {
+    x.u[d] = d < c->g.dim ? x_u[d]->d : nullptr;
+    b.u[d] = d < c->g.dim ? b_u[d]->d : nullptr;
+  }
+  x.p = x_p->d;
+  b.p = b_p->d;
+  precon64(c, b, x, theta_alpha);
+  return finish(c);
+}
+
+int lmm_stag_mg_solver(lmm_ctx *c, lmm_field *const alpha_fc[3], lmm_field *beta_cc, lmm_field *const beta_ed[3],
+                       lmm_field *gamma_cc, double theta_alpha, lmm_field *phi_fc[3], lmm_field *const rhs_fc[3]) {
+  if (!c) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, phi_fc, nullptr, "stag_mg_solver phi"))) return rc;
+  if ((rc = check_stokes(c, rhs_fc, nullptr, "stag_mg_solver rhs"))) return rc;
+  if ((rc = check_coefs(c, alpha_fc, beta_cc, beta_ed, gamma_cc))) return rc;
+  c->vcycles = 0;
+  prepare_coefs(c, coef_in(c, alpha_fc, beta_cc, beta_ed, gamma_cc), theta_alpha, 1.0, false, false);
+  double *ph[3] = {nullptr, nullptr, nullptr}, *rh[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < c->g.dim; d++) {
+    ph[d] = phi_fc[d]->d;
+    rh[d] = rhs_fc[d]->d;
+  }
+  stag_vcycles<double>(c, c->s64, ph, rh, false);
+  return finish(c);
+}
+
+int lmm_macproject(lmm_ctx *c, lmm_field *phi, lmm_field *const umac[3], lmm_field *const alphainv_fc[3],
+                   lmm_field *mac_rhs) {
+  (void)umac;
+  if (!c) return LMM_ERR_ARG;
+  if (!field_ok(c, phi, LMM_CELL) || !field_ok(c, mac_rhs, LMM_CELL)) return set_err(c, LMM_ERR_ARG, "macproject: bad cell field");
+  for (int d = 0; d < c->g.dim; d++)
+    if (!field_ok(c, alphainv_fc[d], LMM_FACE_X + d)) return set_err(c, LMM_ERR_ARG, "macproject: bad alphainv_fc[%d]", d);
+  auto &C = c->c64;
+  for (int d = 0; d < c->g.dim; d++)
+    hipMemcpyAsync(C[0].a[d], alphainv_fc[d]->d, sizeof(double) * c->g.N, hipMemcpyDeviceToDevice, c->stream);
+  for (int n = 1; n < c->nl_c; n++) launch_stag_restrict<double>(c->stream, C[n].g, C[n - 1].a, C[n].a, 1);
+  c->vcycles = 0;
+  cc_vcycles<double>(c, C, phi->d, mac_rhs->d);
+  return finish(c);
+}
+
+int lmm_stag_applyop(lmm_ctx *c, lmm_field *const phi_fc[3], lmm_field *Lphi_fc[3], lmm_field *const alpha_fc[3],
+                     lmm_field *beta_cc, lmm_field *const beta_ed[3], lmm_field *gamma_cc, double theta_alpha,
+                     int color) {
+  if (!c) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, phi_fc, nullptr, "stag_applyop phi"))) return rc;
+  if ((rc = check_stokes(c, Lphi_fc, nullptr, "stag_applyop Lphi"))) return rc;
+  if ((rc = check_coefs(c, alpha_fc, beta_cc, beta_ed, gamma_cc))) return rc;
+  if (color < 0 || color > 2 * c->g.dim) return set_err(c, LMM_ERR_ARG, "color %d out of range", color);
+  CoefIn ci = coef_in(c, alpha_fc, beta_cc, beta_ed, gamma_cc);
+  auto &L = c->s64[0];
+  for (int d = 0; d < c->g.dim; d++) launch_scale_copy(c->stream, L.alpha[d], ci.alpha[d], c->g.N, theta_alpha);
+  L.beta = const_cast<double *>(ci.beta);
+  L.gamma = const_cast<double *>(ci.gamma);
+  for (int e = 0; e < ne_of(c->g.dim); e++) L.bed[e] = const_cast<double *>(ci.bed[e]);
+  double *ph[3] = {nullptr, nullptr, nullptr}, *lp[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < c->g.dim; d++) {
+    ph[d] = phi_fc[d]->d;
+    lp[d] = Lphi_fc[d]->d;
+  }
+  launch_stag_applyop<double>(c->stream, c->g, c->prm.visc_type, color, ph, lp, L.K());
+  return finish(c);
+}
+
+int lmm_stag_inner_prod(lmm_ctx *c, lmm_field *const m1[3], lmm_field *const m2[3], double prod_val[3]) {
+  if (!c || !prod_val) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, m1, nullptr, "stag_inner_prod m1")) || (rc = check_stokes(c, m2, nullptr, "stag_inner_prod m2")))
+    return rc;
+  long N = c->g.N;
+  int dim = c->g.dim;
+  // pack into the Stokes-vector layout with a zero pressure block
+  for (int d = 0; d < dim; d++) {
+    hipMemcpyAsync(c->r + d * N, m1[d]->d, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream);
+    hipMemcpyAsync(c->w + d * N, m2[d]->d, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream);
+  }
+  launch_fill<double>(c->stream, c->r + dim * N, N, 0.0);
+  launch_fill<double>(c->stream, c->w + dim * N, N, 0.0);
+  launch_stokes_dot(c->stream, c->g, c->r, c->w, c->partial);
+  launch_finalize(c->stream, c->g, c->partial, c->scal + 4100, 2, 1.0);
+  double h[4];
+  fetch(c, c->scal + 4100, 4, h);
+  for (int d = 0; d < 3; d++) prod_val[d] = d < dim ? h[d] : 0.0;
+  return finish(c);
+}
+
+int lmm_cc_inner_prod(lmm_ctx *c, lmm_field *m1, lmm_field *m2, double *prod_val) {
+  if (!c || !prod_val || !field_ok(c, m1, LMM_CELL) || !field_ok(c, m2, LMM_CELL)) return LMM_ERR_ARG;
+  long N = c->g.N;
+  int dim = c->g.dim;
+  launch_fill<double>(c->stream, c->r, dim * N, 0.0);
+  launch_fill<double>(c->stream, c->w, dim * N, 0.0);
+  hipMemcpyAsync(c->r + dim * N, m1->d, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream);
+  hipMemcpyAsync(c->w + dim * N, m2->d, sizeof(double) * N, hipMemcpyDeviceToDevice, c->stream);
+  launch_stokes_dot(c->stream, c->g, c->r, c->w, c->partial);
+  launch_finalize(c->stream, c->g, c->partial, c->scal + 4100, 2, 1.0);
+  double h[4];
+  fetch(c, c->scal + 4100, 4, h);
+  *prod_val = h[3];
+  return finish(c);
+}
+
+int lmm_compute_div(lmm_ctx *c, lmm_field *const phi_fc[3], lmm_field *div, int increment) {
+  if (!c) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, phi_fc, div, "compute_div"))) return rc;
+  double *u[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < c->g.dim; d++) u[d] = phi_fc[d]->d;
+  launch_div_plus<double>(c->stream, c->g, u, nullptr, div->d, increment ? 3 : 0);
+  return finish(c);
+}
+
+int lmm_compute_grad(lmm_ctx *c, lmm_field *phi, lmm_field *gradp[3]) {
+  if (!c) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, gradp, phi, "compute_grad"))) return rc;
+  double *gp[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < c->g.dim; d++) gp[d] = gradp[d]->d;
+  launch_grad(c->stream, c->g, phi->d, gp);
+  return finish(c);
+}
+
+int lmm_average_cc_to_face(lmm_ctx *c, lmm_field *cc, lmm_field *face[3]) {
+  if (!c) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, face, cc, "average_cc_to_face"))) return rc;
+  double *f[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < c->g.dim; d++) f[d] = face[d]->d;
+  launch_avg_cc_to_face(c->stream, c->g, cc->d, f);
+  return finish(c);
+}
+
+int lmm_average_cc_to_node(lmm_ctx *c, lmm_field *cc, lmm_field *node) {
+  if (!c || c->g.dim != 2 || !field_ok(c, cc, LMM_CELL) || !field_ok(c, node, LMM_NODE)) return LMM_ERR_ARG;
+  launch_avg_cc_to_node(c->stream, c->g, cc->d, node->d);
+  return finish(c);
+}
+
+int lmm_average_cc_to_edge(lmm_ctx *c, lmm_field *cc, lmm_field *edge[3]) {
+  if (!c || c->g.dim != 3 || !field_ok(c, cc, LMM_CELL)) return LMM_ERR_ARG;
+  for (int e = 0; e < 3; e++)
+    if (!field_ok(c, edge[e], LMM_EDGE_XY + e)) return LMM_ERR_ARG;
+  double *ed[3] = {edge[0]->d, edge[1]->d, edge[2]->d};
+  launch_avg_cc_to_edge(c->stream, c->g, cc->d, ed);
+  return finish(c);
+}
+
+// ---------------- fluctuating fluxes ----------------
+
+static void mt_seed(uint32_t *mt, uint32_t seed) {
+  mt[0] = seed;
+  for (int q = 1; q < 624; q++) mt[q] = 1812433253u * (mt[q - 1] ^ (mt[q - 1] >> 30)) + (uint32_t)q;
+}
+
+static uint32_t mt_next(uint32_t *mt, int &idx) {
+  if (idx >= 624) {
+    for (int q = 0; q < 624; q++) {
+      uint32_t y = (mt[q] & 0x80000000u) | (mt[(q + 1) % 624] & 0x7fffffffu);
+      uint32_t v = mt[(q + 397) % 624] ^ (y >> 1);
+      if (y & 1u) v ^= 0x9908b0dfu;
+      mt[q] = v;
+    }
+    idx = 0;
+  }
+  uint32_t y = mt[idx++];
+  y ^= y >> 11;
+  y ^= (y << 7) & 0x9d2c5680u;
+  y ^= (y << 15) & 0xefc60000u;
+  y ^= y >> 18;
+  return y;
+}
+
+// P. J. Acklam's inverse-normal rational approximation (published coefficients)
+static double inv_normal(double p) {
+  static const double a[6] = {-3.969683028665376e+01, 2.209460984245205e+02, -2.759285104469687e+02,
+                              1.383577518672690e+02,  -3.066479806614716e+01, 2.506628277459239e+00};
+  static const double b[5] = {-5.447609879822406e+01, 1.615858368580409e+02, -1.556989798598866e+02,
+                              6.680131188771972e+01, -1.328068155288572e+01};
+  static const double cc[6] = {-7.784894002430293e-03, -3.223964580411365e-01, -2.400758277161838e+00,
+                               -2.549732539343734e+00, 4.374664141464968e+00,  2.938163982698783e+00};
+  static const double d[4] = {7.784695709041462e-03, 3.224671290700398e-01, 2.445134137142996e+00,
+                              3.754408661907416e+00};
+  const double plow = 0.02425, phigh = 1.0 - plow;
+  if (p < plow) {
+    double q = std::sqrt(-2.0 * std::log(p));
+    return (((((cc[0] * q + cc[1]) * q + cc[2]) * q + cc[3]) * q + cc[4]) * q + cc[5]) /
+           ((((d[0] * q + d[1]) * q + d[2]) * q + d[3]) * q + 1.0);
+  }
+  if (p > phigh) {
+    double q = std::sqrt(-2.0 * std::log(1.0 - p));
+    return -(((((cc[0] * q + cc[1]) * q + cc[2]) * q + cc[3]) * q + cc[4]) * q + cc[5]) /
+           ((((d[0] * q + d[1]) * q + d[2]) * q + d[3]) * q + 1.0);
+  }
+  double q = p - 0.5, r = q * q;
+  return (((((a[0] * r + a[1]) * r + a[2]) * r + a[3]) * r + a[4]) * r + a[5]) * q /
+         (((((b[0] * r + b[1]) * r + b[2]) * r + b[3]) * r + b[4]) * r + 1.0);
+}
+
+int lmm_fill_normal_host(uint32_t seed, int64_t count, double *out) {
+  if (!out || count < 0) return LMM_ERR_ARG;
+  std::vector<uint32_t> mt(624);
+  mt_seed(mt.data(), seed);
+  int idx = 624;
+  for (int64_t q = 0; q < count; q++) {
+    double u;
+    do {
+      u = (double)mt_next(mt.data(), idx) * (1.0 / 4294967296.0);
+    } while (u == 0.0);
+    out[q] = inv_normal(u);
+  }
+  return LMM_OK;
+}
+
+int lmm_init_m_stochastic(lmm_ctx *c, int n_rngs) {
+  if (!c || n_rngs < 1 || n_rngs > 8) return LMM_ERR_ARG;
+  dfree(c->Wcc);
+  dfree(c->Wed);
+  dfree(c->dweights);
+  long N = c->g.N;
+  int dim = c->g.dim, ne = ne_of(dim);
+  c->Wcc = dalloc<double>(c->stream, (long)n_rngs * dim * N);
+  c->Wed = dalloc<double>(c->stream, (long)n_rngs * ne * 2 * N);
+  c->dweights = dalloc<double>(c->stream, 8);
+  if (!c->Wcc || !c->Wed || !c->dweights) return set_err(c, LMM_ERR_NOMEM, "noise allocation failed");
+  c->n_rngs = n_rngs;
+  return LMM_OK;
+}
+
+int lmm_fill_m_stochastic(lmm_ctx *c, uint32_t seed) {
+  if (!c || c->n_rngs < 1) return LMM_ERR_ARG;
+  long N = c->g.N;
+  int dim = c->g.dim, ne = ne_of(dim);
+  long per = (long)(dim + 2 * ne) * N;
+  std::vector<double> h((size_t)per * c->n_rngs);
+  lmm_fill_normal_host(seed, (int64_t)h.size(), h.data());
+  for (int r = 0; r < c->n_rngs; r++) {
+    const double *src = h.data() + (size_t)r * per;
+    HIPCHK(c, hipMemcpyAsync(c->Wcc + (long)r * dim * N, src, sizeof(double) * dim * N, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->Wed + (long)r * ne * 2 * N, src + dim * N, sizeof(double) * ne * 2 * N,
+                             hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LMM_OK;
+}
+
+static int stoch_common(lmm_ctx *c, lmm_field *force[3], const double *Wcc, double *const Wed[3], int nst,
+                        const double *w_dev, long scc, long sed, lmm_field *eta, lmm_field *const eta_ed[3],
+                        lmm_field *temp, lmm_field *const temp_ed[3], double variance) {
+  int dim = c->g.dim, ne = ne_of(dim);
+  int rc;
+  if ((rc = check_stokes(c, force, eta, "stochastic_m_fluxdiv"))) return rc;
+  if (!field_ok(c, temp, LMM_CELL)) return set_err(c, LMM_ERR_ARG, "temperature must be a cell field");
+  int ek = dim == 2 ? LMM_NODE : LMM_EDGE_XY;
+  for (int e = 0; e < ne; e++)
+    if (!field_ok(c, eta_ed[e], ek + e) || !field_ok(c, temp_ed[e], ek + e))
+      return set_err(c, LMM_ERR_ARG, "eta_ed/temperature_ed[%d] wrong kind", e);
+  double *f[3] = {nullptr, nullptr, nullptr}, *ee[3] = {nullptr, nullptr, nullptr}, *te[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < dim; d++) f[d] = force[d]->d;
+  for (int e = 0; e < ne; e++) {
+    ee[e] = eta_ed[e]->d;
+    te[e] = temp_ed[e]->d;
+  }
+  launch_stoch_m_force(c->stream, c->g, f, Wcc, Wed, nst, w_dev, scc, sed, eta->d, ee, temp->d, te, variance);
+  return finish(c);
+}
+
+int lmm_stochastic_m_fluxdiv(lmm_ctx *c, lmm_field *stoch_m_force[3], lmm_field *eta, lmm_field *const eta_ed[3],
+                             lmm_field *temperature, lmm_field *const temperature_ed[3], double dt,
+                             const double *weights, double variance_coef_mom, double k_B) {
+  if (!c || !weights || c->n_rngs < 1 || !(dt > 0)) return LMM_ERR_ARG;
+  long N = c->g.N;
+  int dim = c->g.dim, ne = ne_of(dim);
+  double dv = 1.0;
+  for (int d = 0; d < dim; d++) dv *= c->g.dx;
+  // stochastic_m_fluxdiv.f90:135: product(dx(n,1:dm))
+  double variance = std::sqrt(variance_coef_mom * 2.0 * k_B / (dv * dt));
+  memcpy(c->hscal, weights, sizeof(double) * c->n_rngs);
+  HIPCHK(c, hipMemcpyAsync(c->dweights, c->hscal, sizeof(double) * c->n_rngs, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  double *Wed[3] = {nullptr, nullptr, nullptr};
+  for (int e = 0; e < ne; e++) Wed[e] = c->Wed + (long)e * 2 * N;
+  return stoch_common(c, stoch_m_force, c->Wcc, Wed, c->n_rngs, c->dweights, (long)dim * N, (long)ne * 2 * N, eta,
+                      eta_ed, temperature, temperature_ed, variance);
+}
+
+int lmm_stoch_m_force(lmm_ctx *c, lmm_field *stoch_m_force[3], lmm_field *W_cc, lmm_field *const W_ed[3],
+                      lmm_field *eta, lmm_field *const eta_ed[3], lmm_field *temperature,
+                      lmm_field *const temperature_ed[3], double variance) {
+  if (!c) return LMM_ERR_ARG;
+  int dim = c->g.dim, ne = ne_of(dim);
+  if (!field_ok(c, W_cc, LMM_CELL, dim)) return set_err(c, LMM_ERR_ARG, "W_cc must be a cell field with dim components");
+  int ek = dim == 2 ? LMM_NODE : LMM_EDGE_XY;
+  double *Wed[3] = {nullptr, nullptr, nullptr};
+  for (int e = 0; e < ne; e++) {
+    if (!field_ok(c, W_ed[e], ek + e, 2)) return set_err(c, LMM_ERR_ARG, "W_ed[%d] must have 2 components", e);
+    Wed[e] = W_ed[e]->d;
+  }
+  c->hscal[0] = 1.0;
+  HIPCHK(c, hipMemcpyAsync(c->scal + 4200, c->hscal, sizeof(double), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return stoch_common(c, stoch_m_force, W_cc->d, Wed, 1, c->scal + 4200, 0, 0, eta, eta_ed, temperature,
+                      temperature_ed, variance);
+}
+
+int lmm_diffusive_mass_fluxdiv(lmm_ctx *c, int ns, lmm_field *rho, lmm_field *rhotot, lmm_field *molarconc,
+                               lmm_field *rhoWchi, lmm_field *Gama, lmm_field *diff_fluxdiv, lmm_field *Temp,
+                               lmm_field *zeta_by_Temp, lmm_field *const gradp_baro[3], lmm_field *flux_total[3],
+                               int correct_flux) {
+  if (!c) return LMM_ERR_ARG;
+  if (ns < 1 || ns > 8) return set_err(c, LMM_ERR_UNSUPPORTED, "nspecies %d outside 1..8", ns);
+  if (Temp || zeta_by_Temp || (gradp_baro && gradp_baro[0]))
+    return set_err(c, LMM_ERR_UNSUPPORTED, "non-isothermal and barodiffusion terms are not implemented");
+  (void)rho;
+  (void)rhotot;
+  if (!field_ok(c, molarconc, LMM_CELL, ns) || !field_ok(c, rhoWchi, LMM_CELL, ns * ns) ||
+      !field_ok(c, Gama, LMM_CELL, ns * ns) || !field_ok(c, diff_fluxdiv, LMM_CELL, ns))
+    return set_err(c, LMM_ERR_ARG, "diffusive_mass_fluxdiv: cell fields have wrong components");
+  int dim = c->g.dim;
+  double *ft[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < dim; d++) {
+    if (!field_ok(c, flux_total[d], LMM_FACE_X + d, ns)) return set_err(c, LMM_ERR_ARG, "flux_total[%d] wrong", d);
+    ft[d] = flux_total[d]->d;
+  }
+  double *flux = dalloc<double>(c->stream, (long)dim * ns * c->g.N);
+  if (!flux) return set_err(c, LMM_ERR_NOMEM, "flux scratch");
+  launch_diffusive_mass_flux(c->stream, c->g, ns, molarconc->d, rhoWchi->d, Gama->d, flux, ft, correct_flux);
+  launch_div_species(c->stream, c->g, ns, flux, diff_fluxdiv->d);
+  int rc = finish(c);
+  dfree(flux);
+  return rc;
+}
+
+}  // extern "C"

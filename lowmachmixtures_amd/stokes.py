@@ -1,0 +1,449 @@
+"""Host-side mirror of the reference's solver interface over the gfx950 C ABI.
+
+The functions keep the names, argument order and meaning of the reference's
+Fortran routines (``gmres``, ``apply_matrix``, ``apply_precon``,
+``stag_mg_solver``, ``macproject``, ``stag_applyop_level``,
+``stochastic_m_fluxdiv``, ``diffusive_mass_fluxdiv``, ``stag_inner_prod``,
+``cc_inner_prod``); a multifab becomes a :class:`Field` resident in HBM and
+the layout / bc_tower / probin globals become a :class:`Context`.  Every
+call goes to ``liblmm_hip.so``; there is no CPU fallback, and loading fails
+with an error when the library is missing.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblmm_hip.so")
+_lib = None
+
+CELL, FACE_X, FACE_Y, FACE_Z, NODE, EDGE_XY, EDGE_XZ, EDGE_YZ = range(8)
+FP64, FP32_MG, LEARNED = 0, 1, 2
+STATUS = {0: "ok", 1: "bad argument", 2: "unsupported", 3: "breakdown", 4: "device error",
+          5: "out of memory", 6: "not converged"}
+
+
+class LmmError(RuntimeError):
+    pass
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("visc_type", C.c_int), ("precon_type", C.c_int), ("p_norm_weight", C.c_double),
+        ("scale_factor", C.c_double), ("mg_max_vcycles", C.c_int), ("mg_minwidth", C.c_int),
+        ("mg_nsmooths_down", C.c_int), ("mg_nsmooths_up", C.c_int), ("mg_nsmooths_bottom", C.c_int),
+        ("mg_rel_tol", C.c_double), ("stag_mg_max_vcycles", C.c_int), ("stag_mg_minwidth", C.c_int),
+        ("stag_mg_nsmooths_down", C.c_int), ("stag_mg_nsmooths_up", C.c_int),
+        ("stag_mg_nsmooths_bottom", C.c_int), ("stag_mg_smoother", C.c_int),
+        ("stag_mg_omega", C.c_double), ("stag_mg_rel_tol", C.c_double), ("gmres_rel_tol", C.c_double),
+        ("gmres_abs_tol", C.c_double), ("gmres_max_outer", C.c_int), ("gmres_max_inner", C.c_int),
+        ("gmres_max_iter", C.c_int), ("gmres_min_iter", C.c_int), ("gmres_verbose", C.c_int),
+        ("precision", C.c_int),
+    ]
+
+
+class GmresInfo(C.Structure):
+    _fields_ = [
+        ("total_iter", C.c_int), ("outer_iter", C.c_int), ("norm_pre_rhs", C.c_double),
+        ("norm_resid", C.c_double), ("norm_b", C.c_double), ("norm_resid_true", C.c_double),
+        ("status", C.c_int), ("vcycles", C.c_int), ("seconds", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# (name, restype, argtypes); every entry point of include/lmm.h
+_FP = C.c_void_p
+_SIGS = {
+    "lmm_params_default": (None, [C.POINTER(Params)]),
+    "lmm_ctx_create": (C.c_int, [C.c_int, C.POINTER(C.c_int), C.c_double, C.POINTER(Params), C.POINTER(C.c_void_p)]),
+    "lmm_ctx_destroy": (C.c_int, [C.c_void_p]),
+    "lmm_ctx_set_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
+    "lmm_ctx_get_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
+    "lmm_ctx_synchronize": (C.c_int, [C.c_void_p]),
+    "lmm_ctx_nlevels": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "lmm_ctx_last_error": (C.c_char_p, [C.c_void_p]),
+    "lmm_ctx_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
+    "lmm_ctx_kernel_stats": (C.c_int, [C.c_void_p, C.c_char_p, C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                       C.POINTER(C.c_double)]),
+    "lmm_ctx_reset_stats": (C.c_int, [C.c_void_p]),
+    "lmm_field_create": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]),
+    "lmm_field_destroy": (C.c_int, [C.c_void_p]),
+    "lmm_field_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_int64)]),
+    "lmm_field_upload": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "lmm_field_download": (C.c_int, [C.c_void_p, C.POINTER(C.c_double)]),
+    "lmm_field_upload_fab": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                       C.POINTER(C.c_int), C.c_int]),
+    "lmm_field_download_fab": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_double), C.POINTER(C.c_int),
+                                         C.POINTER(C.c_int), C.c_int]),
+    "lmm_field_setval": (C.c_int, [C.c_void_p, C.c_double]),
+    "lmm_field_copy": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lmm_field_device_ptr": (C.c_void_p, [C.c_void_p]),
+    "lmm_gmres": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, _FP, C.c_double,
+                            C.POINTER(C.c_double), C.POINTER(GmresInfo)]),
+    "lmm_apply_matrix": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, _FP, C.c_double]),
+    "lmm_apply_precon": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, _FP, C.c_double]),
+    "lmm_stag_mg_solver": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP * 3, _FP, C.c_double, _FP * 3, _FP * 3]),
+    "lmm_macproject": (C.c_int, [C.c_void_p, _FP, _FP * 3, _FP * 3, _FP]),
+    "lmm_stag_applyop": (C.c_int, [C.c_void_p, _FP * 3, _FP * 3, _FP * 3, _FP, _FP * 3, _FP, C.c_double, C.c_int]),
+    "lmm_stag_inner_prod": (C.c_int, [C.c_void_p, _FP * 3, _FP * 3, C.POINTER(C.c_double)]),
+    "lmm_cc_inner_prod": (C.c_int, [C.c_void_p, _FP, _FP, C.POINTER(C.c_double)]),
+    "lmm_compute_div": (C.c_int, [C.c_void_p, _FP * 3, _FP, C.c_int]),
+    "lmm_compute_grad": (C.c_int, [C.c_void_p, _FP, _FP * 3]),
+    "lmm_average_cc_to_face": (C.c_int, [C.c_void_p, _FP, _FP * 3]),
+    "lmm_average_cc_to_node": (C.c_int, [C.c_void_p, _FP, _FP]),
+    "lmm_average_cc_to_edge": (C.c_int, [C.c_void_p, _FP, _FP * 3]),
+    "lmm_stochastic_m_fluxdiv": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, C.c_double,
+                                           C.POINTER(C.c_double), C.c_double, C.c_double]),
+    "lmm_init_m_stochastic": (C.c_int, [C.c_void_p, C.c_int]),
+    "lmm_fill_m_stochastic": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "lmm_stoch_m_force": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, C.c_double]),
+    "lmm_diffusive_mass_fluxdiv": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP * 3,
+                                             _FP * 3, C.c_int]),
+    "lmm_fill_normal_host": (C.c_int, [C.c_uint32, C.c_int64, C.POINTER(C.c_double)]),
+}
+
+
+def exported_names():
+    return list(_SIGS)
+
+
+def lib():
+    """Load liblmm_hip.so; raises LmmError when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise LmmError(f"{LIB_PATH} is missing: build it with `python -m lowmachmixtures_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def default_params(**kw) -> Params:
+    p = Params()
+    lib().lmm_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class Context:
+    """ml_layout + bc_tower + dx + probin_gmres knobs for one periodic box."""
+
+    def __init__(self, dim, n, dx=1.0, **params):
+        self.dim = dim
+        self.n = tuple(n)[:dim] + (1,) * (3 - dim)
+        self.dx = dx
+        self.params = default_params(**params)
+        arr = (C.c_int * 3)(*self.n)
+        h = C.c_void_p()
+        rc = lib().lmm_ctx_create(dim, arr, C.c_double(dx), C.byref(self.params), C.byref(h))
+        if rc != 0:
+            raise LmmError(f"lmm_ctx_create failed: {STATUS.get(rc, rc)}")
+        self.h = h
+        self._fields = []
+
+    @property
+    def shape(self):
+        return tuple(reversed(self.n[: self.dim]))
+
+    def check(self, rc, what):
+        if rc != 0:
+            msg = lib().lmm_ctx_last_error(self.h)
+            raise LmmError(f"{what}: {STATUS.get(rc, rc)}: {msg.decode() if msg else ''}")
+
+    def set_params(self, **kw):
+        for k, v in kw.items():
+            setattr(self.params, k, v)
+        self.check(lib().lmm_ctx_set_params(self.h, C.byref(self.params)), "set_params")
+
+    def nlevels(self):
+        a, b = C.c_int(), C.c_int()
+        lib().lmm_ctx_nlevels(self.h, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def synchronize(self):
+        self.check(lib().lmm_ctx_synchronize(self.h), "synchronize")
+
+    def profiling(self, on=True):
+        lib().lmm_ctx_set_profiling(self.h, int(on))
+
+    def reset_stats(self):
+        lib().lmm_ctx_reset_stats(self.h)
+
+    def kernel_stats(self, family):
+        n, ms, by = C.c_int64(), C.c_double(), C.c_double()
+        lib().lmm_ctx_kernel_stats(self.h, family.encode(), C.byref(n), C.byref(ms), C.byref(by))
+        return {"launches": n.value, "ms": ms.value, "bytes": by.value}
+
+    def field(self, kind, ncomp=1, data=None):
+        f = Field(self, kind, ncomp)
+        if data is not None:
+            f.upload(data)
+        return f
+
+    def faces(self, data=None):
+        return [self.field(FACE_X + d, 1, None if data is None else data[d]) for d in range(self.dim)]
+
+    def edges(self, data=None, ncomp=1):
+        kinds = [NODE] if self.dim == 2 else [EDGE_XY, EDGE_XZ, EDGE_YZ]
+        return [self.field(k, ncomp, None if data is None else data[e]) for e, k in enumerate(kinds)]
+
+    def close(self):
+        for f in self._fields:
+            f.free()
+        self._fields = []
+        if self.h:
+            lib().lmm_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+
+class Field:
+    """A device-resident multifab of one periodic box (deduplicated layout)."""
+
+    def __init__(self, ctx: Context, kind: int, ncomp: int = 1):
+        self.ctx = ctx
+        self.kind = kind
+        self.ncomp = ncomp
+        h = C.c_void_p()
+        ctx.check(lib().lmm_field_create(ctx.h, kind, ncomp, C.byref(h)), "field_create")
+        self.h = h
+        ctx._fields.append(self)
+
+    @property
+    def shape(self):
+        s = self.ctx.shape
+        return s if self.ncomp == 1 else (self.ncomp,) + s
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.size != int(np.prod(self.shape)):
+            raise ValueError(f"size {a.size} does not match field shape {self.shape}")
+        self.ctx.check(lib().lmm_field_upload(self.h, a.ctypes.data_as(C.POINTER(C.c_double))), "upload")
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, dtype=np.float64)
+        self.ctx.check(lib().lmm_field_download(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "download")
+        return out
+
+    def upload_fab(self, a, comp, lo, hi, ng):
+        a = np.asfortranarray(a, dtype=np.float64)
+        lo3 = (C.c_int * 3)(*(list(lo) + [0] * (3 - len(lo))))
+        hi3 = (C.c_int * 3)(*(list(hi) + [0] * (3 - len(hi))))
+        self.ctx.check(lib().lmm_field_upload_fab(self.h, comp, a.ctypes.data_as(C.POINTER(C.c_double)), lo3, hi3, ng),
+                       "upload_fab")
+
+    def download_fab(self, shape, comp, lo, hi, ng):
+        out = np.zeros(shape, dtype=np.float64, order="F")
+        lo3 = (C.c_int * 3)(*(list(lo) + [0] * (3 - len(lo))))
+        hi3 = (C.c_int * 3)(*(list(hi) + [0] * (3 - len(hi))))
+        self.ctx.check(lib().lmm_field_download_fab(self.h, comp, out.ctypes.data_as(C.POINTER(C.c_double)), lo3,
+                                                    hi3, ng), "download_fab")
+        return out
+
+    def setval(self, v):
+        self.ctx.check(lib().lmm_field_setval(self.h, C.c_double(v)), "setval")
+        return self
+
+    def device_ptr(self):
+        return lib().lmm_field_device_ptr(self.h)
+
+    def free(self):
+        if self.h:
+            lib().lmm_field_destroy(self.h)
+            self.h = None
+
+
+def _a3(fields):
+    arr = (_FP * 3)()
+    if fields is None:
+        return arr
+    for i, f in enumerate(fields):
+        if f is not None:
+            arr[i] = f.h
+    return arr
+
+
+def _h(f):
+    return None if f is None else f.h
+
+
+# ---------------------------------------------------------------------------
+# reference entry points
+# ---------------------------------------------------------------------------
+
+def gmres(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, theta_alpha):
+    """gmres.f90:30-45.  Returns (norm_pre_rhs, info dict); x_u, x_p updated in place."""
+    info = GmresInfo()
+    npr = C.c_double()
+    rc = lib().lmm_gmres(ctx.h, _a3(b_u), _h(b_p), _a3(x_u), _h(x_p), _a3(alpha_fc), _h(beta), _a3(beta_ed),
+                         _h(gamma), C.c_double(theta_alpha), C.byref(npr), C.byref(info))
+    ctx.check(rc, "gmres")
+    return npr.value, info.as_dict()
+
+
+def apply_matrix(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, theta_alpha):
+    """apply_matrix.f90:23-39."""
+    ctx.check(lib().lmm_apply_matrix(ctx.h, _a3(b_u), _h(b_p), _a3(x_u), _h(x_p), _a3(alpha_fc), _h(beta),
+                                     _a3(beta_ed), _h(gamma), C.c_double(theta_alpha)), "apply_matrix")
+
+
+def apply_precon(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, theta_alpha):
+    """apply_precon.f90:29-43."""
+    ctx.check(lib().lmm_apply_precon(ctx.h, _a3(b_u), _h(b_p), _a3(x_u), _h(x_p), _a3(alpha_fc), _h(beta),
+                                     _a3(beta_ed), _h(gamma), C.c_double(theta_alpha)), "apply_precon")
+
+
+def stag_mg_solver(ctx, alpha_fc, beta_cc, beta_ed, gamma_cc, theta_alpha, phi_fc, rhs_fc):
+    """stag_mg_solver.f90:37-50."""
+    ctx.check(lib().lmm_stag_mg_solver(ctx.h, _a3(alpha_fc), _h(beta_cc), _a3(beta_ed), _h(gamma_cc),
+                                       C.c_double(theta_alpha), _a3(phi_fc), _a3(rhs_fc)), "stag_mg_solver")
+
+
+def macproject(ctx, phi, umac, alphainv_fc, mac_rhs):
+    """macproject.f90:34-45."""
+    ctx.check(lib().lmm_macproject(ctx.h, _h(phi), _a3(umac), _a3(alphainv_fc), _h(mac_rhs)), "macproject")
+
+
+def stag_applyop_level(ctx, phi_fc, Lphi_fc, alpha_fc, beta_cc, beta_ed, gamma_cc, color=0, theta_alpha=1.0):
+    """stag_applyop.f90:70-81 (alpha_fc multiplied by theta_alpha first)."""
+    ctx.check(lib().lmm_stag_applyop(ctx.h, _a3(phi_fc), _a3(Lphi_fc), _a3(alpha_fc), _h(beta_cc), _a3(beta_ed),
+                                     _h(gamma_cc), C.c_double(theta_alpha), color), "stag_applyop")
+
+
+def stag_inner_prod(ctx, m1, m2):
+    out = (C.c_double * 3)()
+    ctx.check(lib().lmm_stag_inner_prod(ctx.h, _a3(m1), _a3(m2), out), "stag_inner_prod")
+    return [out[d] for d in range(ctx.dim)]
+
+
+def cc_inner_prod(ctx, m1, m2):
+    out = C.c_double()
+    ctx.check(lib().lmm_cc_inner_prod(ctx.h, _h(m1), _h(m2), C.byref(out)), "cc_inner_prod")
+    return out.value
+
+
+def compute_div(ctx, phi_fc, div, increment=False):
+    ctx.check(lib().lmm_compute_div(ctx.h, _a3(phi_fc), _h(div), int(increment)), "compute_div")
+
+
+def compute_grad(ctx, phi, gradp):
+    ctx.check(lib().lmm_compute_grad(ctx.h, _h(phi), _a3(gradp)), "compute_grad")
+
+
+def average_cc_to_face(ctx, cc, face):
+    ctx.check(lib().lmm_average_cc_to_face(ctx.h, _h(cc), _a3(face)), "average_cc_to_face")
+
+
+def average_cc_to_node(ctx, cc, node):
+    ctx.check(lib().lmm_average_cc_to_node(ctx.h, _h(cc), _h(node)), "average_cc_to_node")
+
+
+def average_cc_to_edge(ctx, cc, edge):
+    ctx.check(lib().lmm_average_cc_to_edge(ctx.h, _h(cc), _a3(edge)), "average_cc_to_edge")
+
+
+def init_m_stochastic(ctx, n_rngs):
+    ctx.check(lib().lmm_init_m_stochastic(ctx.h, n_rngs), "init_m_stochastic")
+
+
+def fill_m_stochastic(ctx, seed):
+    ctx.check(lib().lmm_fill_m_stochastic(ctx.h, C.c_uint32(seed)), "fill_m_stochastic")
+
+
+def stochastic_m_fluxdiv(ctx, stoch_m_force, eta, eta_ed, temperature, temperature_ed, dt, weights,
+                         variance_coef_mom=1.0, k_B=1.0):
+    """stochastic_m_fluxdiv.f90:41-51 (increments stoch_m_force)."""
+    w = (C.c_double * len(weights))(*weights)
+    ctx.check(lib().lmm_stochastic_m_fluxdiv(ctx.h, _a3(stoch_m_force), _h(eta), _a3(eta_ed), _h(temperature),
+                                             _a3(temperature_ed), C.c_double(dt), w, C.c_double(variance_coef_mom),
+                                             C.c_double(k_B)), "stochastic_m_fluxdiv")
+
+
+def stoch_m_force(ctx, force, W_cc, W_ed, eta, eta_ed, temperature, temperature_ed, variance):
+    ctx.check(lib().lmm_stoch_m_force(ctx.h, _a3(force), _h(W_cc), _a3(W_ed), _h(eta), _a3(eta_ed), _h(temperature),
+                                      _a3(temperature_ed), C.c_double(variance)), "stoch_m_force")
+
+
+def diffusive_mass_fluxdiv(ctx, nspecies, rho, rhotot, molarconc, rhoWchi, Gama, diff_fluxdiv, Temp,
+                           zeta_by_Temp, gradp_baro, flux_total, correct_flux=True):
+    """diffusive_mass_fluxdiv.f90:26-42 (isothermal, no barodiffusion)."""
+    ctx.check(lib().lmm_diffusive_mass_fluxdiv(ctx.h, nspecies, _h(rho), _h(rhotot), _h(molarconc), _h(rhoWchi),
+                                               _h(Gama), _h(diff_fluxdiv), _h(Temp), _h(zeta_by_Temp),
+                                               _a3(gradp_baro), _a3(flux_total), int(correct_flux)),
+              "diffusive_mass_fluxdiv")
+
+
+def fill_normal_host(seed, count):
+    out = np.empty(int(count))
+    rc = lib().lmm_fill_normal_host(C.c_uint32(seed), C.c_int64(int(count)), out.ctypes.data_as(C.POINTER(C.c_double)))
+    if rc:
+        raise LmmError("fill_normal_host failed")
+    return out
+
+
+class DeviceProblem:
+    """A StokesProblem uploaded to a context: coefficients, RHS and solution fields."""
+
+    def __init__(self, ctx: Context, pb, rhs=None):
+        self.ctx = ctx
+        self.pb = pb
+        self.alpha = ctx.faces(pb.alpha)
+        self.beta = ctx.field(CELL, 1, pb.beta)
+        self.gamma = ctx.field(CELL, 1, pb.gamma)
+        self.beta_ed = ctx.edges(pb.beta_ed)
+        self.b_u = ctx.faces()
+        for f in self.b_u:
+            f.setval(0.0)
+        if rhs is not None:
+            for f, a in zip(self.b_u, rhs):
+                f.upload(a)
+        else:
+            self.rhs_from_noise()
+        self.b_p = ctx.field(CELL, 1, pb.b_p)
+        self.x_u = ctx.faces()
+        self.x_p = ctx.field(CELL)
+        self.zero_solution()
+
+    def rhs_from_noise(self):
+        """b_u = stoch_m_force(W) with eta = T = 1 and variance 1, on the device."""
+        ctx, pb = self.ctx, self.pb
+        ones = np.ones(ctx.shape)
+        eta = ctx.field(CELL, 1, ones)
+        ed = ctx.edges([ones] * (1 if ctx.dim == 2 else 3))
+        Wcc = ctx.field(CELL, ctx.dim, pb.W_cc)
+        Wed = ctx.edges(pb.W_ed, ncomp=2)
+        for f in self.b_u:
+            f.setval(0.0)
+        stoch_m_force(ctx, self.b_u, Wcc, Wed, eta, ed, eta, ed, 1.0)
+        for f in [eta, Wcc] + ed + Wed:
+            f.free()
+
+    def zero_solution(self):
+        for f in self.x_u:
+            f.setval(0.0)
+        self.x_p.setval(0.0)
+
+    def solve(self):
+        return gmres(self.ctx, self.b_u, self.b_p, self.x_u, self.x_p, self.alpha, self.beta, self.beta_ed,
+                     self.gamma, self.pb.theta_alpha)
+
+    def solution(self):
+        return [f.download() for f in self.x_u], self.x_p.download()

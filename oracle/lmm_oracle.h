@@ -1,0 +1,164 @@
+/*
+ * CPU oracle for the GMRES + staggered-multigrid Stokes solver of
+ * BoxLib-Codes/LowMachMixtures.  TEST INFRASTRUCTURE ONLY: this library is
+ * the checker that tests, the smoke check and the CPU-baseline leg of the
+ * benchmark compare against.  The product (lowmachmixtures_amd) never links,
+ * loads or calls it.
+ *
+ * Every field that crosses this interface is "deduplicated": one periodic
+ * box of nx*ny*nz cells (nz = 1 in 2D), i fastest, no ghost cells, and face
+ * index i of a face component is the lower face of cell i (face nx is the
+ * periodic image of face 0 and is not stored).  Internally the oracle
+ * rebuilds the reference's own storage (ghost cells filled by periodic
+ * exchange, duplicated lo/hi+1 faces and nodes) and runs the reference
+ * algorithm on it; on the way out it checks that duplicated faces agree.
+ *
+ * Edge/node coefficient layout: 2D node (i,j) is the lower-left corner of
+ * cell (i,j); 3D edge xy(i,j,k) is nodal in x,y and cell-centred in z at the
+ * lower x,y corner of cell (i,j,k); likewise xz and yz.
+ */
+#ifndef LMM_ORACLE_H
+#define LMM_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Solver knobs; same meaning and defaults as src_gmres/probin_gmres.f90:119-157
+ * and probin_common.f90 (visc_type). */
+typedef struct {
+  int visc_type;              /* -3..3 except 0 */
+  int precon_type;            /* only 1 (projection preconditioner P1) */
+  double p_norm_weight;
+  double scale_factor;
+  int mg_max_vcycles, mg_minwidth;
+  int mg_nsmooths_down, mg_nsmooths_up, mg_nsmooths_bottom;
+  double mg_rel_tol;
+  int stag_mg_max_vcycles, stag_mg_minwidth;
+  int stag_mg_nsmooths_down, stag_mg_nsmooths_up, stag_mg_nsmooths_bottom;
+  int stag_mg_smoother;       /* 0 = Jacobi, 1 = 2*dim-colour Gauss-Seidel */
+  double stag_mg_omega, stag_mg_rel_tol;
+  double gmres_rel_tol, gmres_abs_tol;
+  int gmres_max_outer, gmres_max_inner, gmres_max_iter, gmres_min_iter;
+  int verbose;
+} orc_params;
+
+/* Geometry of the single periodic box. */
+typedef struct {
+  int dim;
+  int n[3];
+  double dx;
+} orc_geom;
+
+/* Coefficients of the Stokes operator.  alpha[d], beta, gamma, beta_ed[e]
+ * are deduplicated arrays (beta_ed has 1 entry in 2D, 3 in 3D). */
+typedef struct {
+  const double *alpha[3];
+  const double *beta;
+  const double *beta_ed[3];
+  const double *gamma;
+} orc_coef;
+
+/* Results of one GMRES call. */
+typedef struct {
+  int total_iter;
+  int outer_iter;
+  double norm_pre_rhs;
+  double norm_resid;        /* final preconditioned residual */
+  double norm_b;
+  double norm_resid_true;   /* last un-preconditioned residual */
+  int status;               /* 0 ok, 1 orthogonalisation breakdown, 2 bad args */
+  double dup_mismatch;      /* max |face(0) - face(n)| seen on output */
+} orc_gmres_info;
+
+int orc_compute_nlevs_mg(const orc_geom *g, int minwidth);
+
+/* ---- stencils on one level (stag_applyop.f90, stag_mg_solver.f90) ---- */
+int orc_stag_applyop(const orc_geom *g, int visc_type, const orc_coef *c,
+                     double theta_alpha, double *const phi[3], double *Lphi[3],
+                     int color);
+int orc_stag_mg_smooth(const orc_geom *g, const orc_params *p, const orc_coef *c,
+                       double theta_alpha, double *phi[3], double *const rhs[3],
+                       int nsweeps);
+int orc_stag_restriction(const orc_geom *gf, double *const fine[3], double *coarse[3],
+                         int simple_stencil);
+int orc_stag_prolongation(const orc_geom *gf, double *fine[3], double *const coarse[3]);
+int orc_cc_restriction(const orc_geom *gf, const double *fine, double *coarse);
+int orc_edge_restriction(const orc_geom *gf, double *const fine[3], double *coarse[3]);
+int orc_nodal_restriction(const orc_geom *gf, const double *fine, double *coarse);
+
+/* ---- divergence / gradient (div_and_grad.f90) ---- */
+int orc_compute_div(const orc_geom *g, double *const u[3], double *div, int increment);
+int orc_compute_grad(const orc_geom *g, const double *p, double *gp[3]);
+
+/* ---- inner products (norm_inner_product.f90) ---- */
+int orc_stag_inner_prod(const orc_geom *g, double *const m1[3], double *const m2[3],
+                        double out[3]);
+double orc_cc_inner_prod(const orc_geom *g, const double *m1, const double *m2);
+int orc_sum_umac_press(const orc_geom *g, const double *p, double *const u[3],
+                       double *av_p, double av_u[3]);
+
+/* ---- solvers ---- */
+int orc_stag_mg_solver(const orc_geom *g, const orc_params *p, const orc_coef *c,
+                       double theta_alpha, double *phi[3], double *const rhs[3]);
+int orc_macproject(const orc_geom *g, const orc_params *p, double *const alphainv[3],
+                   const double *mac_rhs, double *phi);
+int orc_apply_matrix(const orc_geom *g, const orc_params *p, const orc_coef *c,
+                     double theta_alpha, double *const x_u[3], const double *x_p,
+                     double *b_u[3], double *b_p);
+int orc_apply_precon(const orc_geom *g, const orc_params *p, const orc_coef *c,
+                     double theta_alpha, double *const b_u[3], const double *b_p,
+                     double *x_u[3], double *x_p);
+int orc_gmres(const orc_geom *g, const orc_params *p, const orc_coef *c,
+              double theta_alpha, double *b_u[3], const double *b_p,
+              double *x_u[3], double *x_p, orc_gmres_info *info);
+
+/* Givens least-squares kernel of gmres.f90:517-605 on host scalars, exposed
+ * so tests can check it against a dense least-squares solve.  H is
+ * (m+1) x m column-major with leading dimension m+1. */
+void orc_least_squares(int i, int m, double *H, double *cs, double *sn, double *s);
+void orc_solve_utriangular(int k, int m, const double *H, const double *s, double *y);
+
+/* ---- coefficient staging (convert_stag.f90) ---- */
+int orc_average_cc_to_face(const orc_geom *g, const double *cc, double *face[3]);
+int orc_average_cc_to_node(const orc_geom *g, const double *cc, double *node);
+int orc_average_cc_to_edge(const orc_geom *g, const double *cc, double *edge[3]);
+
+/* ---- fluctuating fluxes (stochastic_m_fluxdiv.f90) ----
+ * W_cc: dim components (cell), W_ed: 2 components per node (2D, 1 node
+ * array) or per edge type (3D, 3 edge arrays); component-major, each
+ * component nx*ny*nz.  stoch_m_force is incremented. */
+int orc_stochastic_m_fluxdiv(const orc_geom *g, const double *W_cc,
+                             double *const W_ed[3], const double *eta,
+                             double *const eta_ed[3], const double *temp,
+                             double *const temp_ed[3], double variance,
+                             double *force[3]);
+
+/* ---- species diffusion (diffusive_mass_fluxdiv.f90, isothermal, no baro) ----
+ * rho, molarconc: ns comps; rhotot: 1; rhoWchi, Gama: ns*ns comps (row i,
+ * column j stored at comp i + ns*j, matching Fortran (ns,ns) reshaping);
+ * flux_total[d]: ns comps (incremented); diff_fluxdiv: ns comps (overwritten). */
+int orc_diffusive_mass_fluxdiv(const orc_geom *g, int ns, const double *rho,
+                               const double *rhotot, const double *molarconc,
+                               const double *rhoWchi, const double *Gama,
+                               double *diff_fluxdiv, double *flux_total[3],
+                               int correct_flux);
+
+/* ---- random numbers (src_random/RNGs.c restated) ---- */
+typedef struct {
+  uint32_t mt[624];
+  int mti;
+} orc_mt;
+void orc_mt_seed(orc_mt *s, uint32_t seed);
+uint32_t orc_mt_next(orc_mt *s);
+double orc_mt_uniform(orc_mt *s);
+double orc_inv_norm_dist(double p);
+double orc_mt_normal(orc_mt *s);
+void orc_fill_normal(uint32_t seed, int64_t count, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,404 @@
+"""ctypes binding of the CPU oracle (``liblmm_oracle.so``).
+
+TEST INFRASTRUCTURE ONLY.  Only ``tests/``, ``__graft_entry__.smoke()`` and
+the CPU-baseline leg of ``bench.py`` may import this module, and only as the
+checker or the CPU baseline; the product package never does.
+
+The C sources restate the reference's Fortran (see the header comment of
+each ``orc_*.c`` file for the file:line each function follows).  Arrays
+crossing this binding are deduplicated periodic fields: numpy float64 arrays
+of shape ``(nz, ny, nx)`` (``(ny, nx)`` in 2D), x fastest.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+_REF = None
+
+DPTR = C.POINTER(C.c_double)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("visc_type", C.c_int),
+        ("precon_type", C.c_int),
+        ("p_norm_weight", C.c_double),
+        ("scale_factor", C.c_double),
+        ("mg_max_vcycles", C.c_int),
+        ("mg_minwidth", C.c_int),
+        ("mg_nsmooths_down", C.c_int),
+        ("mg_nsmooths_up", C.c_int),
+        ("mg_nsmooths_bottom", C.c_int),
+        ("mg_rel_tol", C.c_double),
+        ("stag_mg_max_vcycles", C.c_int),
+        ("stag_mg_minwidth", C.c_int),
+        ("stag_mg_nsmooths_down", C.c_int),
+        ("stag_mg_nsmooths_up", C.c_int),
+        ("stag_mg_nsmooths_bottom", C.c_int),
+        ("stag_mg_smoother", C.c_int),
+        ("stag_mg_omega", C.c_double),
+        ("stag_mg_rel_tol", C.c_double),
+        ("gmres_rel_tol", C.c_double),
+        ("gmres_abs_tol", C.c_double),
+        ("gmres_max_outer", C.c_int),
+        ("gmres_max_inner", C.c_int),
+        ("gmres_max_iter", C.c_int),
+        ("gmres_min_iter", C.c_int),
+        ("verbose", C.c_int),
+    ]
+
+
+class Geom(C.Structure):
+    _fields_ = [("dim", C.c_int), ("n", C.c_int * 3), ("dx", C.c_double)]
+
+
+class Coef(C.Structure):
+    _fields_ = [
+        ("alpha", DPTR * 3),
+        ("beta", DPTR),
+        ("beta_ed", DPTR * 3),
+        ("gamma", DPTR),
+    ]
+
+
+class GmresInfo(C.Structure):
+    _fields_ = [
+        ("total_iter", C.c_int),
+        ("outer_iter", C.c_int),
+        ("norm_pre_rhs", C.c_double),
+        ("norm_resid", C.c_double),
+        ("norm_b", C.c_double),
+        ("norm_resid_true", C.c_double),
+        ("status", C.c_int),
+        ("dup_mismatch", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
+# reference defaults: src_gmres/probin_gmres.f90:119-157
+DEFAULTS = dict(
+    visc_type=1, precon_type=1, p_norm_weight=1.0, scale_factor=1.0,
+    mg_max_vcycles=1, mg_minwidth=2, mg_nsmooths_down=2, mg_nsmooths_up=2,
+    mg_nsmooths_bottom=8, mg_rel_tol=1e-9,
+    stag_mg_max_vcycles=1, stag_mg_minwidth=2, stag_mg_nsmooths_down=2,
+    stag_mg_nsmooths_up=2, stag_mg_nsmooths_bottom=8, stag_mg_smoother=1,
+    stag_mg_omega=1.0, stag_mg_rel_tol=1e-9,
+    gmres_rel_tol=1e-9, gmres_abs_tol=0.0, gmres_max_outer=20, gmres_max_inner=5,
+    gmres_max_iter=100, gmres_min_iter=1, verbose=0,
+)
+
+
+def build():
+    """Compile liblmm_oracle.so (and the reference RNG when the tree is mounted)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+    if os.path.isdir("/root/reference/src_random"):
+        subprocess.run(["make", "-s", "-C", _HERE, "ref"], check=True)
+
+
+def lib():
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, "liblmm_oracle.so")
+        if not os.path.exists(path):
+            build()
+        _LIB = C.CDLL(path)
+        _LIB.orc_cc_inner_prod.restype = C.c_double
+        _LIB.orc_inv_norm_dist.restype = C.c_double
+        _LIB.orc_mt_next.restype = C.c_uint32
+        _LIB.orc_mt_normal.restype = C.c_double
+        _LIB.orc_mt_uniform.restype = C.c_double
+    return _LIB
+
+
+def ref_rng():
+    """The reference's own RNGs.c compiled into oracle/_ref, or None."""
+    global _REF
+    if _REF is None:
+        path = os.path.join(_HERE, "_ref", "librngs_ref.so")
+        if not os.path.exists(path):
+            return None
+        _REF = C.CDLL(path)
+        _REF.genrandbase.restype = C.c_ulong
+        _REF.InvNormDist.restype = C.c_double
+        _REF.srandgen.argtypes = [C.c_ulong]
+        _REF.InvNormDist.argtypes = [C.c_double]
+    return _REF
+
+
+def params(**kw) -> Params:
+    d = dict(DEFAULTS)
+    d.update(kw)
+    p = Params()
+    for k, v in d.items():
+        setattr(p, k, v)
+    return p
+
+
+def geom(dim, n, dx=1.0) -> Geom:
+    g = Geom()
+    g.dim = dim
+    nn = list(n) + [1] * (3 - len(n))
+    for d in range(3):
+        g.n[d] = nn[d]
+    g.dx = dx
+    return g
+
+
+def _p(a):
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"], "oracle needs C-contiguous float64"
+    return a.ctypes.data_as(DPTR)
+
+
+def _arr3(lst):
+    arr = (DPTR * 3)()
+    for i, a in enumerate(lst):
+        if a is not None:
+            arr[i] = _p(a)
+    return arr
+
+
+def coef(alpha, beta, beta_ed, gamma):
+    c = Coef()
+    for i, a in enumerate(alpha):
+        c.alpha[i] = _p(a)
+    c.beta = _p(beta)
+    for i, a in enumerate(beta_ed):
+        c.beta_ed[i] = _p(a)
+    c.gamma = _p(gamma)
+    c._keep = (alpha, beta, beta_ed, gamma)
+    return c
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"oracle {what} failed with status {rc}")
+
+
+def _copy(lst):
+    return [np.ascontiguousarray(a, dtype=np.float64).copy() for a in lst]
+
+
+def stag_applyop(g, visc_type, cf, theta_alpha, phi, color=0, Lphi=None):
+    out = _copy(Lphi) if Lphi is not None else [np.zeros_like(a) for a in phi]
+    phi = _copy(phi)
+    _check(lib().orc_stag_applyop(C.byref(g), visc_type, C.byref(cf), C.c_double(theta_alpha),
+                                  _arr3(phi), _arr3(out), color), "stag_applyop")
+    return out
+
+
+def stag_mg_smooth(g, p, cf, theta_alpha, phi, rhs, nsweeps):
+    phi = _copy(phi)
+    rhs = _copy(rhs)
+    _check(lib().orc_stag_mg_smooth(C.byref(g), C.byref(p), C.byref(cf), C.c_double(theta_alpha),
+                                    _arr3(phi), _arr3(rhs), nsweeps), "stag_mg_smooth")
+    return phi
+
+
+def _coarse_shape(a):
+    return tuple(s // 2 for s in a.shape)
+
+
+def stag_restriction(g, fine, simple=False):
+    fine = _copy(fine)
+    coarse = [np.zeros(_coarse_shape(a)) for a in fine]
+    _check(lib().orc_stag_restriction(C.byref(g), _arr3(fine), _arr3(coarse), int(simple)),
+           "stag_restriction")
+    return coarse
+
+
+def stag_prolongation(g, fine, coarse):
+    fine = _copy(fine)
+    coarse = _copy(coarse)
+    _check(lib().orc_stag_prolongation(C.byref(g), _arr3(fine), _arr3(coarse)), "stag_prolongation")
+    return fine
+
+
+def cc_restriction(g, fine):
+    fine = np.ascontiguousarray(fine, dtype=np.float64)
+    coarse = np.zeros(_coarse_shape(fine))
+    _check(lib().orc_cc_restriction(C.byref(g), _p(fine), _p(coarse)), "cc_restriction")
+    return coarse
+
+
+def edge_restriction(g, fine):
+    fine = _copy(fine)
+    coarse = [np.zeros(_coarse_shape(a)) for a in fine]
+    _check(lib().orc_edge_restriction(C.byref(g), _arr3(fine), _arr3(coarse)), "edge_restriction")
+    return coarse
+
+
+def nodal_restriction(g, fine):
+    fine = np.ascontiguousarray(fine, dtype=np.float64)
+    coarse = np.zeros(_coarse_shape(fine))
+    _check(lib().orc_nodal_restriction(C.byref(g), _p(fine), _p(coarse)), "nodal_restriction")
+    return coarse
+
+
+def compute_div(g, u, div=None, increment=False):
+    u = _copy(u)
+    out = np.zeros_like(u[0]) if div is None else np.ascontiguousarray(div, dtype=np.float64).copy()
+    _check(lib().orc_compute_div(C.byref(g), _arr3(u), _p(out), int(increment)), "compute_div")
+    return out
+
+
+def compute_grad(g, p_cc):
+    p_cc = np.ascontiguousarray(p_cc, dtype=np.float64)
+    out = [np.zeros_like(p_cc) for _ in range(g.dim)]
+    _check(lib().orc_compute_grad(C.byref(g), _p(p_cc), _arr3(out)), "compute_grad")
+    return out
+
+
+def stag_inner_prod(g, m1, m2):
+    m1, m2 = _copy(m1), _copy(m2)
+    out = (C.c_double * 3)()
+    _check(lib().orc_stag_inner_prod(C.byref(g), _arr3(m1), _arr3(m2), out), "stag_inner_prod")
+    return [out[d] for d in range(g.dim)]
+
+
+def cc_inner_prod(g, a, b):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    b = np.ascontiguousarray(b, dtype=np.float64)
+    return lib().orc_cc_inner_prod(C.byref(g), _p(a), _p(b))
+
+
+def sum_umac_press(g, p_cc, u):
+    p_cc = np.ascontiguousarray(p_cc, dtype=np.float64)
+    u = _copy(u)
+    avp = C.c_double()
+    avu = (C.c_double * 3)()
+    _check(lib().orc_sum_umac_press(C.byref(g), _p(p_cc), _arr3(u), C.byref(avp), avu), "sum_umac_press")
+    return avp.value, [avu[d] for d in range(g.dim)]
+
+
+def stag_mg_solver(g, p, cf, theta_alpha, rhs, phi=None):
+    rhs = _copy(rhs)
+    phi = _copy(phi) if phi is not None else [np.zeros_like(a) for a in rhs]
+    _check(lib().orc_stag_mg_solver(C.byref(g), C.byref(p), C.byref(cf), C.c_double(theta_alpha),
+                                    _arr3(phi), _arr3(rhs)), "stag_mg_solver")
+    return phi
+
+
+def macproject(g, p, alphainv, mac_rhs):
+    alphainv = _copy(alphainv)
+    mac_rhs = np.ascontiguousarray(mac_rhs, dtype=np.float64)
+    phi = np.zeros_like(mac_rhs)
+    _check(lib().orc_macproject(C.byref(g), C.byref(p), _arr3(alphainv), _p(mac_rhs), _p(phi)),
+           "macproject")
+    return phi
+
+
+def apply_matrix(g, p, cf, theta_alpha, x_u, x_p):
+    x_u = _copy(x_u)
+    x_p = np.ascontiguousarray(x_p, dtype=np.float64)
+    b_u = [np.zeros_like(a) for a in x_u]
+    b_p = np.zeros_like(x_p)
+    _check(lib().orc_apply_matrix(C.byref(g), C.byref(p), C.byref(cf), C.c_double(theta_alpha),
+                                  _arr3(x_u), _p(x_p), _arr3(b_u), _p(b_p)), "apply_matrix")
+    return b_u, b_p
+
+
+def apply_precon(g, p, cf, theta_alpha, b_u, b_p):
+    b_u = _copy(b_u)
+    b_p = np.ascontiguousarray(b_p, dtype=np.float64)
+    x_u = [np.zeros_like(a) for a in b_u]
+    x_p = np.zeros_like(b_p)
+    _check(lib().orc_apply_precon(C.byref(g), C.byref(p), C.byref(cf), C.c_double(theta_alpha),
+                                  _arr3(b_u), _p(b_p), _arr3(x_u), _p(x_p)), "apply_precon")
+    return x_u, x_p
+
+
+def gmres(g, p, cf, theta_alpha, b_u, b_p, x_u=None, x_p=None):
+    b_u = _copy(b_u)
+    b_p = np.ascontiguousarray(b_p, dtype=np.float64).copy()
+    x_u = _copy(x_u) if x_u is not None else [np.zeros_like(a) for a in b_u]
+    x_p = np.ascontiguousarray(x_p, dtype=np.float64).copy() if x_p is not None else np.zeros_like(b_p)
+    info = GmresInfo()
+    lib().orc_gmres(C.byref(g), C.byref(p), C.byref(cf), C.c_double(theta_alpha), _arr3(b_u),
+                    _p(b_p), _arr3(x_u), _p(x_p), C.byref(info))
+    return x_u, x_p, info.as_dict()
+
+
+def least_squares(i, m, H, cs, sn, s):
+    """In-place Givens update of column i (1-based) of the (m+1) x m matrix H."""
+    for a in (H, cs, sn, s):
+        assert a.dtype == np.float64
+    lib().orc_least_squares(i, m, H.ctypes.data_as(DPTR), cs.ctypes.data_as(DPTR),
+                            sn.ctypes.data_as(DPTR), s.ctypes.data_as(DPTR))
+
+
+def solve_utriangular(k, m, H, s):
+    y = np.zeros(m)
+    lib().orc_solve_utriangular(k, m, H.ctypes.data_as(DPTR), s.ctypes.data_as(DPTR),
+                                y.ctypes.data_as(DPTR))
+    return y
+
+
+def average_cc_to_face(g, cc):
+    cc = np.ascontiguousarray(cc, dtype=np.float64)
+    out = [np.zeros_like(cc) for _ in range(g.dim)]
+    _check(lib().orc_average_cc_to_face(C.byref(g), _p(cc), _arr3(out)), "average_cc_to_face")
+    return out
+
+
+def average_cc_to_node(g, cc):
+    cc = np.ascontiguousarray(cc, dtype=np.float64)
+    out = np.zeros_like(cc)
+    _check(lib().orc_average_cc_to_node(C.byref(g), _p(cc), _p(out)), "average_cc_to_node")
+    return out
+
+
+def average_cc_to_edge(g, cc):
+    cc = np.ascontiguousarray(cc, dtype=np.float64)
+    out = [np.zeros_like(cc) for _ in range(3)]
+    _check(lib().orc_average_cc_to_edge(C.byref(g), _p(cc), _arr3(out)), "average_cc_to_edge")
+    return out
+
+
+def stochastic_m_fluxdiv(g, W_cc, W_ed, eta, eta_ed, temp, temp_ed, variance, force):
+    """W_cc: (dim, *shape); W_ed: list of (2, *shape) (1 node array in 2D, 3 edges in 3D)."""
+    W_cc = np.ascontiguousarray(W_cc, dtype=np.float64)
+    W_ed = _copy(W_ed)
+    eta = np.ascontiguousarray(eta, dtype=np.float64)
+    temp = np.ascontiguousarray(temp, dtype=np.float64)
+    eta_ed, temp_ed = _copy(eta_ed), _copy(temp_ed)
+    force = _copy(force)
+    _check(lib().orc_stochastic_m_fluxdiv(C.byref(g), _p(W_cc), _arr3(W_ed), _p(eta), _arr3(eta_ed),
+                                          _p(temp), _arr3(temp_ed), C.c_double(variance), _arr3(force)),
+           "stochastic_m_fluxdiv")
+    return force
+
+
+def diffusive_mass_fluxdiv(g, ns, rho, rhotot, molarconc, rhoWchi, Gama, flux_total, correct_flux=True):
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    rhotot = np.ascontiguousarray(rhotot, dtype=np.float64)
+    molarconc = np.ascontiguousarray(molarconc, dtype=np.float64)
+    rhoWchi = np.ascontiguousarray(rhoWchi, dtype=np.float64)
+    Gama = np.ascontiguousarray(Gama, dtype=np.float64)
+    flux_total = _copy(flux_total)
+    div = np.zeros_like(molarconc)
+    _check(lib().orc_diffusive_mass_fluxdiv(C.byref(g), ns, _p(rho), _p(rhotot), _p(molarconc), _p(rhoWchi),
+                                            _p(Gama), _p(div), _arr3(flux_total), int(correct_flux)),
+           "diffusive_mass_fluxdiv")
+    return div, flux_total
+
+
+def fill_normal(seed, count):
+    out = np.empty(int(count), dtype=np.float64)
+    lib().orc_fill_normal(C.c_uint32(seed), C.c_int64(int(count)), out.ctypes.data_as(DPTR))
+    return out
+
+
+def inv_norm_dist(p):
+    return lib().orc_inv_norm_dist(C.c_double(p))
+
+
+def compute_nlevs_mg(g, minwidth=2):
+    return lib().orc_compute_nlevs_mg(C.byref(g), minwidth)
