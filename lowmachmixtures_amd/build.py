@@ -34,6 +34,13 @@ def flags() -> list:
         "-shared",
         "-ffp-contract=off",
         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+        # Branches on workgroup-uniform values (component from blockIdx.y,
+        # visc_type, colour) are emitted as plain scalar branches.  Without
+        # this, a three-way uniform selection such as
+        #   if (e == 0) ... else if (e == 1) ... else ...
+        # is structurized so that the final case's value is never computed
+        # on gfx950 (its VGPRs stay undefined and the load faults).
+        "-mllvm", "-structurizecfg-skip-uniform-regions=true",
         "-Wno-unused-result",
         "-Wno-unused-value",
         "-I" + os.path.join(HERE, "..", "include"),

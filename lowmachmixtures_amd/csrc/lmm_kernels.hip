@@ -327,8 +327,15 @@ void launch_cc_restrict(hipStream_t s, const Grid &gc, const T *fine, T *coarse)
   LAUNCH(k_cc_restrict<T>, dim3(nblk(gc.N)), dim3(TPB), 0, s, gc, gf, fine, coarse);
 }
 
+// Edge e is averaged along its own direction: xy edges along z, xz along y,
+// yz along x.  The fine-grid stride of that direction comes in as data
+// (along.o[e]) so the kernel has no branch on the edge type.
+struct Along3 {
+  long o[3];
+};
+
 template <typename T>
-__global__ void __launch_bounds__(256) k_edge_restrict(Grid gc, Grid gf, Ptr3<T> fi, Ptr3<T> co) {
+__global__ void __launch_bounds__(256) k_edge_restrict(Grid gc, Grid gf, Along3 along, Ptr3<T> fi, Ptr3<T> co) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= gc.N) return;
   const int e = (int)blockIdx.y;
@@ -336,21 +343,15 @@ __global__ void __launch_bounds__(256) k_edge_restrict(Grid gc, Grid gf, Ptr3<T>
   decode(gc, t, i, j, k);
   long b = 2 * i + gf.s[1] * (2 * j) + gf.s[2] * (2 * k);
   const T *f = fi.c[e];
-  T v;
-  if (e == 0)
-    v = (T)0.5 * (f[b] + f[b + gf.s[2]]);   // xy edges: average in z
-  else if (e == 1)
-    v = (T)0.5 * (f[b] + f[b + gf.s[1]]);   // xz edges: average in y
-  else
-    v = (T)0.5 * (f[b] + f[b + 1]);         // yz edges: average in x
-  co.c[e][t] = v;
+  co.c[e][t] = (T)0.5 * (f[b] + f[b + along.o[e]]);
 }
 
 template <typename T>
 void launch_edge_restrict(hipStream_t s, const Grid &gc, T *const fine[3], T *const coarse[3]) {
   int n[3] = {gc.n[0] * 2, gc.n[1] * 2, gc.n[2] * 2};
   Grid gf = make_grid(3, n, gc.dx * 0.5);
-  LAUNCH(k_edge_restrict<T>, dim3(nblk(gc.N), 3), dim3(TPB), 0, s, gc, gf, p3(fine), p3(coarse));
+  Along3 along = {{gf.s[2], gf.s[1], 1}};
+  LAUNCH(k_edge_restrict<T>, dim3(nblk(gc.N), 3), dim3(TPB), 0, s, gc, gf, along, p3(fine), p3(coarse));
 }
 
 template <typename T>

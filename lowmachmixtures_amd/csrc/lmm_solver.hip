@@ -376,16 +376,18 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
   const lmm_params &p = c->prm;
   int dim = L.g.dim;
   StagCoef<T> K = L.K();
+  // the finest level is reported on its own: it carries the bulk of the bytes
+  const char *fam = (L.g.N == c->g.N) ? "stag_smooth_l0" : "stag_smooth";
   for (int m = 0; m < nsweeps; m++) {
     if (p.stag_mg_smoother == 0) {
-      ProfScope ps(c, "stag_smooth", sweep_bytes(L.g, p.visc_type, sizeof(T)));
+      ProfScope ps(c, fam, sweep_bytes(L.g, p.visc_type, sizeof(T)));
       launch_stag_jacobi<T>(c->stream, L.g, p.visc_type, (T)p.stag_mg_omega, phi, rhs, L.res, K);
       for (int d = 0; d < dim; d++)
         hipMemcpyAsync(phi[d], L.res[d], sizeof(T) * L.g.N, hipMemcpyDeviceToDevice, c->stream);
       continue;
     }
     for (int color = 1; color <= 2 * dim; color++) {
-      ProfScope ps(c, "stag_smooth", sweep_bytes(L.g, p.visc_type, sizeof(T)) / (2 * dim));
+      ProfScope ps(c, fam, sweep_bytes(L.g, p.visc_type, sizeof(T)) / (2 * dim));
       launch_stag_gs_colour<T>(c->stream, L.g, p.visc_type, (color - 1) / 2, (color + 1) % 2,
                                (T)p.stag_mg_omega, phi, K, rhs[(color - 1) / 2]);
     }
