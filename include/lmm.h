@@ -108,7 +108,8 @@ int lmm_ctx_nlevels(const lmm_ctx *ctx, int *nlevs_stag, int *nlevs_cc);
 /* Last HIP or argument error message of this context. */
 const char *lmm_ctx_last_error(const lmm_ctx *ctx);
 /* Kernel timing: when enabled, HIP events bracket every launch of the named
- * kernel families on the context stream.  family: "stag_smooth", "stag_restrict",
+ * kernel families on the context stream.  family: "stag_smooth_l0" (finest
+ * level sweeps), "stag_smooth" (coarser levels), "stag_restrict",
  * "stag_prolong", "cc_smooth", "apply_matrix", "krylov". */
 int lmm_ctx_set_profiling(lmm_ctx *ctx, int on);
 int lmm_ctx_kernel_stats(lmm_ctx *ctx, const char *family, int64_t *launches, double *total_ms,

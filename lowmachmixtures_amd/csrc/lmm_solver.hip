@@ -875,10 +875,10 @@ static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
       hipMemcpyAsync(dH + i + (long)(m + 1) * (i - 1), sl, sizeof(double), hipMemcpyDeviceToDevice, s);
       launch_div_by(s, Vk(i), c->w, L, sl);
       fetch(c, dH + (long)(m + 1) * (i - 1), i + 1, &hs.h(1, i));
-      if (hs.h(i + 1, i) == 0.0) break;  // happy breakdown: exact solution in the space
+      bool happy = (hs.h(i + 1, i) == 0.0);  // exact solution in the space
       hs.least_squares(i);
       double est = std::fabs(hs.s[i]);
-      if (est <= p.gmres_rel_tol * norm_b || total_iter >= p.gmres_max_iter) break;
+      if (happy || est <= p.gmres_rel_tol * norm_b || total_iter >= p.gmres_max_iter) break;
     }
     hs.solve(used - 1);
     // x += sum_k y_k Z_k  (Z in FP32, accumulation in FP64)

@@ -5,6 +5,8 @@ reference's own RNGs.c compiled from /root/reference into oracle/_ref (when
 it was built), Acklam's published error bound against scipy's ndtri, and the
 numpy generator of lowmachmixtures_amd.problems against the oracle stream.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -79,3 +81,17 @@ def test_inverse_normal_symmetry():
     # 1 - p is rounded, so the two tails agree to Acklam's bound, not to ulps
     for p in (1e-6, 0.01, 0.2, 0.4999):
         assert O.inv_norm_dist(p) == pytest.approx(-O.inv_norm_dist(1 - p), rel=2.5e-9)
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "rngs_c_vectors.npz")
+
+
+@pytest.mark.parametrize("seed", [1, 5489, 12345, 2 ** 31 + 7])
+def test_oracle_mt_matches_committed_reference_vectors(seed):
+    g = np.load(GOLDEN)
+    assert np.array_equal(P.mt_raw(seed, 2000), g[f"raw_{seed}"])
+
+
+def test_oracle_normals_match_committed_reference_vectors():
+    g = np.load(GOLDEN)
+    assert np.array_equal(O.fill_normal(77, 5000), g["normal_77"])
