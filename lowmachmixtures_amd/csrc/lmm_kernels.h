@@ -28,6 +28,17 @@ struct Faces {
 };
 
 // ---- staggered operator ----
+// Fused red-black sweep of one component (3D, visc_type -1 / 1): reads the
+// old field from pin, writes the new one to pout (lmm_smooth.hip).
+int fused_rb_tz(const Grid &g, int min_tiles);
+bool fused_rb_ok(const Grid &g, int visc, int min_tiles);
+template <typename T>
+void launch_stag_rb_fused(hipStream_t s, const Grid &g, int visc, int a, T omega, const T *pin, T *pout,
+                          const T *rhs, const StagCoef<T> &K, int min_tiles);
+// The same fused sweep for the cell-centred red-black Gauss-Seidel (3D).
+template <typename T>
+void launch_cc_rb_fused(hipStream_t s, const Grid &g, const T *pin, T *pout, const T *rhs, T *const a[3],
+                        int min_tiles);
 template <typename T>
 void launch_stag_gs_colour(hipStream_t s, const Grid &g, int visc, int a, int par, T omega,
                            T *const phi[3], const StagCoef<T> &K, const T *rhs);
@@ -51,6 +62,33 @@ template <typename T>
 void launch_edge_restrict(hipStream_t s, const Grid &gc, T *const fine[3], T *const coarse[3]);
 template <typename T>
 void launch_nodal_restrict(hipStream_t s, const Grid &gc, const T *fine, T *coarse);
+
+// ---- coarse levels of a V-cycle in one single-workgroup launch (lmm_coarse.hip) ----
+constexpr int COARSE_MAX_LEVELS = 12;
+// levels with at most this many cells run inside the coarse-level launch
+// (one workgroup: above 16^3 the separate wide launches are faster)
+constexpr long COARSE_MAX_CELLS = 4096;
+template <typename T>
+struct CoarseStagLevel {
+  Grid g;
+  T *phi[3], *rhs[3], *res[3];
+  StagCoef<T> K;
+};
+template <typename T>
+struct CoarseCCLevel {
+  Grid g;
+  T *phi, *rhs, *res;
+  T *a[3];
+};
+// Levels lv[0..m-1] (finest first): zero phi on all of them, then the
+// down-leg (nd sweeps, residual, 12-point restriction), nb bottom sweeps on
+// lv[m-1] and the up-leg (prolongation, nu sweeps), as stag_mg_solver's
+// level loop does; 2*dim-colour Gauss-Seidel.
+template <typename T>
+void launch_stag_coarse_vcycle(hipStream_t s, int m, const CoarseStagLevel<T> *lv, int visc, T omega, int nd,
+                               int nu, int nb);
+template <typename T>
+void launch_cc_coarse_vcycle(hipStream_t s, int m, const CoarseCCLevel<T> *lv, int nd, int nu, int nb);
 
 // ---- cell-centred multigrid ----
 template <typename T>

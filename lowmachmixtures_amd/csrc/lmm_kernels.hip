@@ -6,6 +6,7 @@
 // the library is built with -ffp-contract=off, so the FP64 kernels reproduce
 // the reference formulas bit for bit.
 #include "lmm_kernels.h"
+#include "lmm_points.h"
 
 namespace lmm {
 
@@ -29,28 +30,9 @@ void post_launch(hipStream_t s, const char *name) {
   } while (0)
 static inline int nblk(long n, int tpb = TPB) { return (int)((n + tpb - 1) / tpb); }
 
-__device__ __forceinline__ void decode(const Grid &g, long t, int &i, int &j, int &k) {
-  i = (int)(t % g.n[0]);
-  long r = t / g.n[0];
-  j = (int)(r % g.n[1]);
-  k = (int)(r / g.n[1]);
-}
-
 // ------------------------------------------------------------------------
 // staggered smoother / operator
 // ------------------------------------------------------------------------
-
-template <typename T>
-struct Ptr3 {
-  T *c[3];
-};
-
-template <typename T>
-static Ptr3<T> p3(T *const a[3]) {
-  Ptr3<T> r;
-  for (int d = 0; d < 3; d++) r.c[d] = a[d];
-  return r;
-}
 
 // One colour of the 2*dim-colour Gauss-Seidel: applyop at the colour's points
 // followed by phi += omega*(rhs - Lphi)/fac, fused.  Points of one colour
@@ -60,21 +42,9 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_stag_gs_colour(Grid g, int visc, int a, int par, T omega,
                                                         T dxsqinv, Ptr3<T> ph, StagCoef<T> K,
                                                         const T *__restrict__ rhs) {
-  const int half = g.n[0] >> 1;
-  const long total = (long)half * g.n[1] * g.n[2];
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= total) return;
-  int ii = (int)(t % half);
-  long r = t / half;
-  int j = (int)(r % g.n[1]);
-  int k = (int)(r / g.n[1]);
-  int i = 2 * ii + ((par + j + k) & 1);
-  Pt P = make_pt(g, i, j, k);
-  const T *phr[3] = {ph.c[0], ph.c[1], ph.c[2]};
-  T L, fac;
-  stag_point<T>(visc, g, a, P, phr, K, dxsqinv, L, fac);
-  long p = i + g.s[1] * j + g.s[2] * k;
-  ph.c[a][p] = ph.c[a][p] + omega * (rhs[p] - L) / fac;
+  const long t = xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= (long)(g.n[0] >> 1) * g.n[1] * g.n[2]) return;
+  gs_colour_at<T>(g, visc, a, par, omega, dxsqinv, ph, K, rhs, t);
 }
 
 template <typename T>
@@ -91,21 +61,15 @@ void launch_stag_gs_colour(hipStream_t s, const Grid &g, int visc, int a, int pa
 template <typename T>
 __global__ void __launch_bounds__(256) k_stag_applyop(Grid g, int visc, int color, T dxsqinv, Ptr3<T> ph,
                                                       Ptr3<T> lp, StagCoef<T> K) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   int i, j, k, par = (color + 1) % 2;
   const int a = (color == 0) ? (int)blockIdx.y : (color - 1) / 2;
   const long per = (color == 0) ? g.N : g.N / 2;
   if (t >= per) return;
-  if (color == 0) {
+  if (color == 0)
     decode(g, t, i, j, k);
-  } else {
-    int half = g.n[0] >> 1;
-    int ii = (int)(t % half);
-    long r = t / half;
-    j = (int)(r % g.n[1]);
-    k = (int)(r / g.n[1]);
-    i = 2 * ii + ((par + j + k) & 1);
-  }
+  else
+    decode_half(g, t, par, i, j, k);
   Pt P = make_pt(g, i, j, k);
   const T *phr[3] = {ph.c[0], ph.c[1], ph.c[2]};
   T L, fac;
@@ -127,7 +91,7 @@ void launch_stag_applyop(hipStream_t s, const Grid &g, int visc, int color, T *c
 template <typename T>
 __global__ void __launch_bounds__(256) k_stag_jacobi(Grid g, int visc, T omega, T dxsqinv, Ptr3<T> ph,
                                                      Ptr3<T> rh, Ptr3<T> tmp, StagCoef<T> K) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   const int a = (int)blockIdx.y;
   int i, j, k;
@@ -152,17 +116,9 @@ void launch_stag_jacobi(hipStream_t s, const Grid &g, int visc, T omega, T *cons
 template <typename T>
 __global__ void __launch_bounds__(256) k_stag_residual(Grid g, int visc, T dxsqinv, Ptr3<T> ph, Ptr3<T> rh,
                                                        Ptr3<T> res, StagCoef<T> K) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
-  const int a = (int)blockIdx.y;
-  int i, j, k;
-  decode(g, t, i, j, k);
-  Pt P = make_pt(g, i, j, k);
-  const T *phr[3] = {ph.c[0], ph.c[1], ph.c[2]};
-  T L, fac;
-  stag_point<T>(visc, g, a, P, phr, K, dxsqinv, L, fac);
-  long p = i + g.s[1] * j + g.s[2] * k;
-  res.c[a][p] = rh.c[a][p] - L;
+  residual_at<T>(g, visc, dxsqinv, ph, rh, res, K, (int)blockIdx.y, t);
 }
 
 template <typename T>
@@ -177,59 +133,11 @@ void launch_stag_residual(hipStream_t s, const Grid &g, int visc, T *const phi[3
 // restriction / prolongation (stag_mg_solver.f90:873-1633)
 // ------------------------------------------------------------------------
 
-// fine index of coarse point with fine offsets (along a: oa, transverse t1: o1, t2: o2)
-__device__ __forceinline__ long fidx(const Grid &gf, int a, int t1, int t2, const int C[3], int oa, int o1,
-                                     int o2) {
-  int f[3] = {2 * C[0], 2 * C[1], 2 * C[2]};
-  if (gf.dim == 2) f[2] = 0;
-  f[a] += oa;
-  f[t1] += o1;
-  if (t2 >= 0) f[t2] += o2;
-  for (int d = 0; d < gf.dim; d++) {
-    if (f[d] < 0) f[d] += gf.n[d];
-    if (f[d] >= gf.n[d]) f[d] -= gf.n[d];
-  }
-  return f[0] + gf.s[1] * f[1] + gf.s[2] * f[2];
-}
-
 template <typename T>
 __global__ void __launch_bounds__(256) k_stag_restrict(Grid gc, Grid gf, int simple, Ptr3<T> fi, Ptr3<T> co) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= gc.N) return;
-  const int a = (int)blockIdx.y;
-  int C[3];
-  decode(gc, t, C[0], C[1], C[2]);
-  int t1, t2 = -1;
-  if (gc.dim == 2) {
-    t1 = 1 - a;
-  } else {
-    t1 = (a == 0) ? 1 : 0;
-    t2 = (a == 2) ? 1 : 2;
-  }
-  const T *f = fi.c[a];
-  T v;
-  if (gc.dim == 2) {
-    if (simple) {
-      v = (T)0.5 * (f[fidx(gf, a, t1, t2, C, 0, 0, 0)] + f[fidx(gf, a, t1, t2, C, 0, 1, 0)]);
-    } else {
-      v = (T)0.25 * (f[fidx(gf, a, t1, t2, C, 0, 0, 0)] + f[fidx(gf, a, t1, t2, C, 0, 1, 0)]) +
-          (T)0.125 * (f[fidx(gf, a, t1, t2, C, 1, 0, 0)] + f[fidx(gf, a, t1, t2, C, 1, 1, 0)] +
-                      f[fidx(gf, a, t1, t2, C, -1, 0, 0)] + f[fidx(gf, a, t1, t2, C, -1, 1, 0)]);
-    }
-  } else {
-    T g0 = f[fidx(gf, a, t1, t2, C, 0, 0, 0)] + f[fidx(gf, a, t1, t2, C, 0, 1, 0)] +
-           f[fidx(gf, a, t1, t2, C, 0, 0, 1)] + f[fidx(gf, a, t1, t2, C, 0, 1, 1)];
-    if (simple) {
-      v = (T)0.25 * g0;
-    } else {
-      T gp = f[fidx(gf, a, t1, t2, C, 1, 0, 0)] + f[fidx(gf, a, t1, t2, C, 1, 1, 0)] +
-             f[fidx(gf, a, t1, t2, C, 1, 0, 1)] + f[fidx(gf, a, t1, t2, C, 1, 1, 1)];
-      T gm = f[fidx(gf, a, t1, t2, C, -1, 0, 0)] + f[fidx(gf, a, t1, t2, C, -1, 1, 0)] +
-             f[fidx(gf, a, t1, t2, C, -1, 0, 1)] + f[fidx(gf, a, t1, t2, C, -1, 1, 1)];
-      v = (T)0.125 * g0 + (T)0.0625 * gp + (T)0.0625 * gm;
-    }
-  }
-  co.c[a][C[0] + gc.s[1] * C[1] + gc.s[2] * C[2]] = v;
+  restrict_at<T>(gc, gf, simple, fi, co, (int)blockIdx.y, t);
 }
 
 template <typename T>
@@ -241,83 +149,32 @@ void launch_stag_restrict(hipStream_t s, const Grid &gc, T *const fine[3], T *co
                      p3(coarse));
 }
 
-template <typename T>
-__global__ void __launch_bounds__(256) k_stag_prolong(Grid gf, Grid gc, Ptr3<T> fi, Ptr3<T> co) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+template <typename T, int D, int A>
+__global__ void __launch_bounds__(256) k_stag_prolong(Grid gf, Grid gc, T *fine, const T *__restrict__ C) {
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= gf.N) return;
-  const int a = (int)blockIdx.y;
-  int c[3];
-  decode(gf, t, c[0], c[1], c[2]);
-  int t1, t2 = -1;
-  if (gf.dim == 2) {
-    t1 = 1 - a;
-  } else {
-    t1 = (a == 0) ? 1 : 0;
-    t2 = (a == 2) ? 1 : 2;
-  }
-  int A[3] = {c[0] / 2, c[1] / 2, c[2] / 2};
-  int o1 = (c[t1] % 2 == 0) ? -1 : 1;
-  int o2 = (t2 >= 0 && c[t2] % 2 == 0) ? -1 : 1;
-  const T *C = co.c[a];
-  auto cidx = [&](int sa, int s1, int s2) -> long {
-    int q[3] = {A[0], A[1], A[2]};
-    q[a] += sa;
-    q[t1] += s1;
-    if (t2 >= 0) q[t2] += s2;
-    for (int d = 0; d < gc.dim; d++) {
-      if (q[d] < 0) q[d] += gc.n[d];
-      if (q[d] >= gc.n[d]) q[d] -= gc.n[d];
-    }
-    return q[0] + gc.s[1] * q[1] + gc.s[2] * q[2];
-  };
-  long p = c[0] + gf.s[1] * c[1] + gf.s[2] * c[2];
-  T v = fi.c[a][p];
-  bool even = (c[a] % 2 == 0);
-  if (gf.dim == 2) {
-    if (even) {
-      v = v + (T)0.75 * C[cidx(0, 0, 0)] + (T)0.25 * C[cidx(0, o1, 0)];
-    } else {
-      v = v + (T)0.375 * C[cidx(0, 0, 0)] + (T)0.125 * C[cidx(0, o1, 0)] + (T)0.375 * C[cidx(1, 0, 0)] +
-          (T)0.125 * C[cidx(1, o1, 0)];
-    }
-  } else {
-    const T nine16 = (T)0.5625, three16 = (T)0.1875, one16 = (T)0.0625;
-    const T nine32 = (T)0.28125, three32 = (T)0.09375, one32 = (T)0.03125;
-    if (even) {
-      v = v + nine16 * C[cidx(0, 0, 0)] + three16 * C[cidx(0, o1, 0)] + three16 * C[cidx(0, 0, o2)] +
-          one16 * C[cidx(0, o1, o2)];
-    } else {
-      v = v + nine32 * C[cidx(0, 0, 0)] + three32 * C[cidx(0, o1, 0)] + three32 * C[cidx(0, 0, o2)] +
-          one32 * C[cidx(0, o1, o2)] + nine32 * C[cidx(1, 0, 0)] + three32 * C[cidx(1, o1, 0)] +
-          three32 * C[cidx(1, 0, o2)] + one32 * C[cidx(1, o1, o2)];
-    }
-  }
-  fi.c[a][p] = v;
+  prolong_at<T, D, A>(gf, gc, fine, C, t);
 }
 
 template <typename T>
 void launch_stag_prolong(hipStream_t s, const Grid &gf, T *const fine[3], T *const coarse[3]) {
   Grid gc = coarsen(gf);
-  LAUNCH(k_stag_prolong<T>, dim3(nblk(gf.N), gf.dim), dim3(TPB), 0, s, gf, gc, p3(fine),
-                     p3(coarse));
+  dim3 grid(nblk(gf.N));
+  if (gf.dim == 2) {
+    LAUNCH((k_stag_prolong<T, 2, 0>), grid, dim3(TPB), 0, s, gf, gc, fine[0], coarse[0]);
+    LAUNCH((k_stag_prolong<T, 2, 1>), grid, dim3(TPB), 0, s, gf, gc, fine[1], coarse[1]);
+  } else {
+    LAUNCH((k_stag_prolong<T, 3, 0>), grid, dim3(TPB), 0, s, gf, gc, fine[0], coarse[0]);
+    LAUNCH((k_stag_prolong<T, 3, 1>), grid, dim3(TPB), 0, s, gf, gc, fine[1], coarse[1]);
+    LAUNCH((k_stag_prolong<T, 3, 2>), grid, dim3(TPB), 0, s, gf, gc, fine[2], coarse[2]);
+  }
 }
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_cc_restrict(Grid gc, Grid gf, const T *__restrict__ f, T *c) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= gc.N) return;
-  int i, j, k;
-  decode(gc, t, i, j, k);
-  long b = 2 * i + gf.s[1] * (2 * j) + gf.s[2] * (2 * k);
-  long sy = gf.s[1], sz = gf.s[2];
-  T v;
-  if (gc.dim == 2) {
-    v = (T)0.25 * (f[b] + f[b + 1] + f[b + sy] + f[b + 1 + sy]);
-  } else {
-    v = (T)0.125 * (f[b] + f[b + 1] + f[b + sy] + f[b + 1 + sy] + f[b + sz] + f[b + 1 + sz] + f[b + sy + sz] +
-                    f[b + 1 + sy + sz]);
-  }
-  c[t] = v;
+  cc_restrict_at<T>(gc, gf, f, c, t);
 }
 
 template <typename T>
@@ -336,7 +193,7 @@ struct Along3 {
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_edge_restrict(Grid gc, Grid gf, Along3 along, Ptr3<T> fi, Ptr3<T> co) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= gc.N) return;
   const int e = (int)blockIdx.y;
   int i, j, k;
@@ -377,39 +234,11 @@ void launch_nodal_restrict(hipStream_t s, const Grid &gc, const T *fine, T *coar
 // ------------------------------------------------------------------------
 
 template <typename T>
-__device__ __forceinline__ T cc_apply(const Grid &g, int i, int j, int k, const T *phi, const Ptr3<T> &A,
-                                      T inv, T &diag) {
-  long p = i + g.s[1] * j + g.s[2] * k;
-  long xm = wm(i, g.n[0]) + g.s[1] * j + g.s[2] * k, xp = wp(i, g.n[0]) + g.s[1] * j + g.s[2] * k;
-  long ym = i + g.s[1] * wm(j, g.n[1]) + g.s[2] * k, yp = i + g.s[1] * wp(j, g.n[1]) + g.s[2] * k;
-  T axl = A.c[0][p], axh = A.c[0][xp], ayl = A.c[1][p], ayh = A.c[1][yp];
-  T s = axl + axh + ayl + ayh;
-  T off = axl * phi[xm] + axh * phi[xp] + ayl * phi[ym] + ayh * phi[yp];
-  if (g.dim == 3) {
-    long zm = i + g.s[1] * j + g.s[2] * wm(k, g.n[2]), zp = i + g.s[1] * j + g.s[2] * wp(k, g.n[2]);
-    T azl = A.c[2][p], azh = A.c[2][zp];
-    s = s + azl + azh;
-    off = off + azl * phi[zm] + azh * phi[zp];
-  }
-  diag = -s * inv;
-  return (-s * phi[p] + off) * inv;
-}
-
-template <typename T>
 __global__ void __launch_bounds__(256) k_cc_rb(Grid g, int colour, T inv, T *phi, const T *__restrict__ rhs,
                                                Ptr3<T> A) {
-  const int half = g.n[0] >> 1;
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= (long)half * g.n[1] * g.n[2]) return;
-  int ii = (int)(t % half);
-  long r = t / half;
-  int j = (int)(r % g.n[1]);
-  int k = (int)(r / g.n[1]);
-  int i = 2 * ii + ((colour + j + k) & 1);
-  T diag;
-  T lp = cc_apply<T>(g, i, j, k, phi, A, inv, diag);
-  long p = i + g.s[1] * j + g.s[2] * k;
-  phi[p] = phi[p] + (rhs[p] - lp) / diag;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= (long)(g.n[0] >> 1) * g.n[1] * g.n[2]) return;
+  cc_rb_at<T>(g, colour, inv, phi, rhs, A, t);
 }
 
 template <typename T>
@@ -422,12 +251,9 @@ void launch_cc_rb_colour(hipStream_t s, const Grid &g, int colour, T *phi, const
 template <typename T>
 __global__ void __launch_bounds__(256) k_cc_residual(Grid g, T inv, const T *phi, const T *__restrict__ rhs,
                                                      T *res, Ptr3<T> A) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
-  int i, j, k;
-  decode(g, t, i, j, k);
-  T diag;
-  res[t] = rhs[t] - cc_apply<T>(g, i, j, k, phi, A, inv, diag);
+  cc_residual_at<T>(g, inv, phi, rhs, res, A, t);
 }
 
 template <typename T>
@@ -438,11 +264,9 @@ void launch_cc_residual(hipStream_t s, const Grid &g, const T *phi, const T *rhs
 
 template <typename T>
 __global__ void __launch_bounds__(256) k_cc_prolong_add(Grid gf, Grid gc, T *f, const T *__restrict__ c) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= gf.N) return;
-  int i, j, k;
-  decode(gf, t, i, j, k);
-  f[t] = f[t] + c[i / 2 + gc.s[1] * (j / 2) + gc.s[2] * (k / 2)];
+  cc_prolong_add_at<T>(gf, gc, f, c, t);
 }
 
 template <typename T>
@@ -510,7 +334,7 @@ template <typename T>
 __global__ void __launch_bounds__(256) k_apply_matrix(Grid g, int visc, T dxsqinv, T dxinv, Ptr3<T> xu,
                                                       const T *__restrict__ xp, Ptr3<T> bu, T *bp,
                                                       StagCoef<T> K) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   int i, j, k;
   decode(g, t, i, j, k);
@@ -529,10 +353,111 @@ __global__ void __launch_bounds__(256) k_apply_matrix(Grid g, int visc, T dxsqin
   bp[t] = dv * (T)-1.0;
 }
 
+// 3D visc_type -1 / 1: the same arithmetic with every index static (the
+// stencil policy of the fused smoother supplies L at a point).
+template <typename T, int VISC>
+__device__ __forceinline__ T am_component(const Grid &g, int i, int j, int k, const Ptr3<T> &xu,
+                                          const typename StagRB<T, 0, VISC>::Args &a0,
+                                          const typename StagRB<T, 1, VISC>::Args &a1,
+                                          const typename StagRB<T, 2, VISC>::Args &a2, int comp, T dxsqinv) {
+  const long sy = g.s[1], sz = g.s[2];
+  const long p = i + sy * j + sz * k;
+  const int ip = wrapc(i + 1, g.n[0]), im = wrapc(i - 1, g.n[0]);
+  const int jp = wrapc(j + 1, g.n[1]), jm = wrapc(j - 1, g.n[1]);
+  const int kp = wrapc(k + 1, g.n[2]), km = wrapc(k - 1, g.n[2]);
+  const long ro = sy * j + sz * k;
+  auto nb = [&](const T *F, T &xp, T &xm, T &yp, T &ym, T &zp, T &zm) {
+    xp = F[ip + ro];
+    xm = F[im + ro];
+    yp = F[i + sy * jp + sz * k];
+    ym = F[i + sy * jm + sz * k];
+    zp = F[i + sy * j + sz * kp];
+    zm = F[i + sy * j + sz * km];
+  };
+  T xp, xm, yp, ym, zp, zm;
+  if (comp == 0) {
+    nb(xu.c[0], xp, xm, yp, ym, zp, zm);
+    return StagRB<T, 0, VISC>::apply(StagRB<T, 0, VISC>::load(g, i, j, k, a0), xu.c[0][p], xp, xm, yp, ym, zp, zm,
+                                     dxsqinv);
+  } else if (comp == 1) {
+    nb(xu.c[1], xp, xm, yp, ym, zp, zm);
+    return StagRB<T, 1, VISC>::apply(StagRB<T, 1, VISC>::load(g, i, j, k, a1), xu.c[1][p], xp, xm, yp, ym, zp, zm,
+                                     dxsqinv);
+  }
+  nb(xu.c[2], xp, xm, yp, ym, zp, zm);
+  return StagRB<T, 2, VISC>::apply(StagRB<T, 2, VISC>::load(g, i, j, k, a2), xu.c[2][p], xp, xm, yp, ym, zp, zm,
+                                   dxsqinv);
+}
+
+template <typename T, int VISC>
+__global__ void __launch_bounds__(256) k_apply_matrix3(Grid g, T dxsqinv, T dxinv, Ptr3<T> xu, const T *__restrict__ xp,
+                                                       Ptr3<T> bu, T *bp, typename StagRB<T, 0, VISC>::Args a0,
+                                                       typename StagRB<T, 1, VISC>::Args a1,
+                                                       typename StagRB<T, 2, VISC>::Args a2,
+                                                       const T *__restrict__ al0, const T *__restrict__ al1,
+                                                       const T *__restrict__ al2, const T *__restrict__ be,
+                                                       const T *__restrict__ ed0, const T *__restrict__ ed1,
+                                                       const T *__restrict__ ed2) {
+  a0 = StagRB<T, 0, VISC>::bind(a0, xp, al0, be, ed0, ed1);
+  a1 = StagRB<T, 1, VISC>::bind(a1, xp, al1, be, ed0, ed2);
+  a2 = StagRB<T, 2, VISC>::bind(a2, xp, al2, be, ed1, ed2);
+  long t = xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  const long sy = g.s[1], sz = g.s[2];
+  const long pmx = wrapc(i - 1, g.n[0]) + sy * j + sz * k;
+  const long pmy = i + sy * wrapc(j - 1, g.n[1]) + sz * k;
+  const long pmz = i + sy * j + sz * wrapc(k - 1, g.n[2]);
+  const T pc = xp[t];
+  bu.c[0][t] = am_component<T, VISC>(g, i, j, k, xu, a0, a1, a2, 0, dxsqinv) + (pc - xp[pmx]) * dxinv;
+  bu.c[1][t] = am_component<T, VISC>(g, i, j, k, xu, a0, a1, a2, 1, dxsqinv) + (pc - xp[pmy]) * dxinv;
+  bu.c[2][t] = am_component<T, VISC>(g, i, j, k, xu, a0, a1, a2, 2, dxsqinv) + (pc - xp[pmz]) * dxinv;
+  const long xq = wrapc(i + 1, g.n[0]) + sy * j + sz * k;
+  const long yq = i + sy * wrapc(j + 1, g.n[1]) + sz * k;
+  const long zq = i + sy * j + sz * wrapc(k + 1, g.n[2]);
+  T dv = (xu.c[0][xq] - xu.c[0][t]) * dxinv;
+  dv = dv + (xu.c[1][yq] - xu.c[1][t]) * dxinv;
+  dv = dv + (xu.c[2][zq] - xu.c[2][t]) * dxinv;
+  bp[t] = dv * (T)-1.0;
+}
+
+template <typename T, int VISC>
+static void launch_am3(hipStream_t s, const Grid &g, T dxsqinv, T dxinv, T *const x_u[3], const T *x_p,
+                       T *const b_u[3], T *b_p, const StagCoef<T> &K) {
+  typename StagRB<T, 0, VISC>::Args a0;
+  typename StagRB<T, 1, VISC>::Args a1;
+  typename StagRB<T, 2, VISC>::Args a2;
+  a0.omega = a1.omega = a2.omega = (T)0;
+  a0.dxsqinv = a1.dxsqinv = a2.dxsqinv = dxsqinv;
+  a0.rhs = a1.rhs = a2.rhs = x_p;  // unused by apply(); any valid address
+  a0.alpha = K.alpha[0];
+  a1.alpha = K.alpha[1];
+  a2.alpha = K.alpha[2];
+  a0.beta = a1.beta = a2.beta = K.beta;
+  // edges shared with (A, t1) and (A, t2): xy = 0, xz = 1, yz = 2
+  a0.e1 = K.bed[0];
+  a0.e2 = K.bed[1];
+  a1.e1 = K.bed[0];
+  a1.e2 = K.bed[2];
+  a2.e1 = K.bed[1];
+  a2.e2 = K.bed[2];
+  LAUNCH((k_apply_matrix3<T, VISC>), dim3(nblk(g.N)), dim3(TPB), 0, s, g, dxsqinv, dxinv, p3(x_u), x_p, p3(b_u),
+         b_p, a0, a1, a2, K.alpha[0], K.alpha[1], K.alpha[2], K.beta, K.bed[0], K.bed[1], K.bed[2]);
+}
+
 template <typename T>
 void launch_apply_matrix(hipStream_t s, const Grid &g, int visc, T *const x_u[3], const T *x_p,
                          T *const b_u[3], T *b_p, const StagCoef<T> &K) {
   T dxsqinv = (T)(1.0 / (g.dx * g.dx)), dxinv = (T)(1.0 / g.dx);
+  if (g.dim == 3 && visc == -1) {
+    launch_am3<T, -1>(s, g, dxsqinv, dxinv, x_u, x_p, b_u, b_p, K);
+    return;
+  }
+  if (g.dim == 3 && visc == 1) {
+    launch_am3<T, 1>(s, g, dxsqinv, dxinv, x_u, x_p, b_u, b_p, K);
+    return;
+  }
   LAUNCH(k_apply_matrix<T>, dim3(nblk(g.N)), dim3(TPB), 0, s, g, visc, dxsqinv, dxinv, p3(x_u), x_p,
                      p3(b_u), b_p, K);
 }
@@ -540,7 +465,7 @@ void launch_apply_matrix(hipStream_t s, const Grid &g, int visc, T *const x_u[3]
 // div kernel (div_and_grad.f90:400-451) with combination modes
 template <typename T>
 __global__ void __launch_bounds__(256) k_div(Grid g, T dxinv, Ptr3<T> u, const T *b_p, T *div, int mode) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   int i, j, k;
   decode(g, t, i, j, k);
@@ -633,7 +558,7 @@ void launch_precon_final(hipStream_t s, const Grid &g, int visc, T *const x_u[3]
 }
 
 __global__ void k_grad(Grid g, double dxinv, const double *__restrict__ p, Ptr3<double> gp) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   int i, j, k;
   decode(g, t, i, j, k);
@@ -646,7 +571,7 @@ void launch_grad(hipStream_t s, const Grid &g, const double *p, double *const gp
 
 // convert_stag.f90 interior formulas
 __global__ void k_avg_face(Grid g, const double *__restrict__ cc, Ptr3<double> f) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   int i, j, k;
   decode(g, t, i, j, k);
@@ -657,7 +582,7 @@ void launch_avg_cc_to_face(hipStream_t s, const Grid &g, const double *cc, doubl
   LAUNCH(k_avg_face, dim3(nblk(g.N)), dim3(TPB), 0, s, g, cc, p3(face));
 }
 __global__ void k_avg_node(Grid g, const double *__restrict__ cc, double *nd) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   int i, j, k;
   decode(g, t, i, j, k);
@@ -668,7 +593,7 @@ void launch_avg_cc_to_node(hipStream_t s, const Grid &g, const double *cc, doubl
   LAUNCH(k_avg_node, dim3(nblk(g.N)), dim3(TPB), 0, s, g, cc, node);
 }
 __global__ void k_avg_edge(Grid g, const double *__restrict__ cc, Ptr3<double> e) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   int i, j, k;
   decode(g, t, i, j, k);
@@ -910,7 +835,7 @@ __global__ void __launch_bounds__(256) k_stoch_force(Grid g, Ptr3<double> force,
                                                      const double *__restrict__ eta, Ptr3<double> eta_ed,
                                                      const double *__restrict__ temp, Ptr3<double> temp_ed,
                                                      double variance) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   int i, j, k;
   decode(g, t, i, j, k);
@@ -966,7 +891,7 @@ __global__ void __launch_bounds__(256) k_diff_mass_flux(Grid g, int ns, const do
                                                         const double *__restrict__ rw,
                                                         const double *__restrict__ gm, double *flux,
                                                         Ptr3<double> ft, int correct) {
-  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   int i, j, k;
   decode(g, t, i, j, k);

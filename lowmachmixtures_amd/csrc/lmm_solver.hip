@@ -97,6 +97,18 @@ struct lmm_ctx {
   int n_rngs = 0;
   double *Wcc = nullptr, *Wed = nullptr;
   double *dweights = nullptr;
+  // Smoother kernels: fused red-black sweeps on levels with at least
+  // fused_min_tiles workgroups (128^3 and up), colour-by-colour kernels
+  // elsewhere.  Both
+  // give the same bits.  LMM_SMOOTHER=colour in the environment selects the
+  // colour kernels everywhere, LMM_SMOOTHER=fused the fused sweep on every
+  // level whose extents allow it.
+  bool fused = true;
+  int fused_min_tiles = 512;
+  // coarse levels (at most COARSE_MAX_CELLS cells) of both multigrids run
+  // in one single-workgroup launch; LMM_COARSE=0 in the environment keeps
+  // one launch per pass on every level (same bits)
+  bool coarse_one_launch = true;
   // profiling
   bool prof = false;
   std::map<std::string, ProfFamily> fam;
@@ -386,6 +398,22 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
         hipMemcpyAsync(phi[d], L.res[d], sizeof(T) * L.g.N, hipMemcpyDeviceToDevice, c->stream);
       continue;
     }
+    if (c->fused && fused_rb_ok(L.g, p.visc_type, c->fused_min_tiles)) {
+      // sweep m reads one buffer and writes the other (phi <-> L.res); an
+      // odd sweep count ends with a copy back
+      T *const *src = (m % 2 == 0) ? phi : L.res;
+      T *const *dst = (m % 2 == 0) ? L.res : phi;
+      for (int a = 0; a < dim; a++) {
+        ProfScope ps(c, fam, sweep_bytes(L.g, p.visc_type, sizeof(T)) / dim);
+        ProfScope pf(c, "stag_rb_fused", sweep_bytes(L.g, p.visc_type, sizeof(T)) / dim);
+        launch_stag_rb_fused<T>(c->stream, L.g, p.visc_type, a, (T)p.stag_mg_omega, src[a], dst[a], rhs[a], K,
+                                c->fused_min_tiles);
+      }
+      if (m == nsweeps - 1 && m % 2 == 0)
+        for (int a = 0; a < dim; a++)
+          hipMemcpyAsync(phi[a], L.res[a], sizeof(T) * L.g.N, hipMemcpyDeviceToDevice, c->stream);
+      continue;
+    }
     for (int color = 1; color <= 2 * dim; color++) {
       ProfScope ps(c, fam, sweep_bytes(L.g, p.visc_type, sizeof(T)) / (2 * dim));
       launch_stag_gs_colour<T>(c->stream, L.g, p.visc_type, (color - 1) / 2, (color + 1) % 2,
@@ -431,10 +459,31 @@ static void stag_vcycles(lmm_ctx *c, std::vector<SLevel<T>> &S, T *const phi0[3]
       for (int d = 0; d < dim; d++) resid0[d] = mx;
    }
   }
-  for (int vc = 1; vc <= p.stag_mg_max_vcycles; vc++) {
+  // levels nc..nl-1 run in the single-workgroup coarse launch (Gauss-Seidel
+  // smoother only); nc = nl when there is none
+  int nc = nl;
+  if (c->coarse_one_launch && p.stag_mg_smoother == 1)
     for (int n = 1; n < nl; n++)
+      if (S[n].g.N <= COARSE_MAX_CELLS && nl - n <= COARSE_MAX_LEVELS) {
+        nc = n;
+        break;
+      }
+  std::vector<CoarseStagLevel<T>> clv;
+  for (int n = nc; n < nl; n++) {
+    CoarseStagLevel<T> L;
+    L.g = S[n].g;
+    for (int d = 0; d < 3; d++) {
+      L.phi[d] = S[n].phi[d];
+      L.rhs[d] = S[n].rhs[d];
+      L.res[d] = S[n].res[d];
+    }
+    L.K = S[n].K();
+    clv.push_back(L);
+  }
+  for (int vc = 1; vc <= p.stag_mg_max_vcycles; vc++) {
+    for (int n = 1; n < nc; n++)
       for (int d = 0; d < dim; d++) launch_fill<T>(c->stream, S[n].phi[d], S[n].g.N, (T)0);
-    for (int n = 0; n < nl - 1; n++) {
+    for (int n = 0; n < std::min(nc, nl - 1); n++) {
       T *const *ph = (n == 0) ? phi0 : S[n].phi;
       T *const *rh = (n == 0) ? rhs0 : S[n].rhs;
       smooth<T>(c, S[n], ph, rh, p.stag_mg_nsmooths_down);
@@ -444,13 +493,17 @@ static void stag_vcycles(lmm_ctx *c, std::vector<SLevel<T>> &S, T *const phi0[3]
         launch_stag_restrict<T>(c->stream, S[n + 1].g, S[n].res, S[n + 1].rhs, 0);
       }
     }
-    {
+    if (nc < nl) {
+      ProfScope ps(c, "stag_coarse", 0.0);
+      launch_stag_coarse_vcycle<T>(c->stream, nl - nc, clv.data(), p.visc_type, (T)p.stag_mg_omega,
+                                   p.stag_mg_nsmooths_down, p.stag_mg_nsmooths_up, p.stag_mg_nsmooths_bottom);
+    } else {
       int n = nl - 1;
       T *const *ph = (n == 0) ? phi0 : S[n].phi;
       T *const *rh = (n == 0) ? rhs0 : S[n].rhs;
       smooth<T>(c, S[n], ph, rh, p.stag_mg_nsmooths_bottom);
     }
-    for (int n = nl - 2; n >= 0; n--) {
+    for (int n = std::min(nc, nl - 1) - 1; n >= 0; n--) {
       T *const *ph = (n == 0) ? phi0 : S[n].phi;
       T *const *rh = (n == 0) ? rhs0 : S[n].rhs;
       {
@@ -483,23 +536,57 @@ static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *
   auto rb = [&](int n, int sweeps) {
     T *ph = (n == 0) ? phi0 : C[n].phi;
     const T *rh = (n == 0) ? rhs0 : C[n].rhs;
+    if (c->fused && fused_rb_tz(C[n].g, c->fused_min_tiles) > 0) {
+      // fused red-black sweeps, ping-pong between phi and C[n].res
+      for (int m = 0; m < sweeps; m++) {
+        ProfScope ps(c, "cc_smooth", 0.0);
+        const T *src = (m % 2 == 0) ? ph : C[n].res;
+        T *dst = (m % 2 == 0) ? C[n].res : ph;
+        launch_cc_rb_fused<T>(c->stream, C[n].g, src, dst, rh, C[n].a, c->fused_min_tiles);
+      }
+      if (sweeps % 2 == 1) hipMemcpyAsync(ph, C[n].res, sizeof(T) * C[n].g.N, hipMemcpyDeviceToDevice, c->stream);
+      return;
+    }
     for (int m = 0; m < sweeps; m++)
       for (int colour = 0; colour < 2; colour++) {
         ProfScope ps(c, "cc_smooth", 0.0);
         launch_cc_rb_colour<T>(c->stream, C[n].g, colour, ph, rh, C[n].a);
       }
   };
+  int nc = nl;
+  if (c->coarse_one_launch)
+    for (int n = 1; n < nl; n++)
+      if (C[n].g.N <= COARSE_MAX_CELLS && nl - n <= COARSE_MAX_LEVELS) {
+        nc = n;
+        break;
+      }
+  std::vector<CoarseCCLevel<T>> clv;
+  for (int n = nc; n < nl; n++) {
+    CoarseCCLevel<T> L;
+    L.g = C[n].g;
+    L.phi = C[n].phi;
+    L.rhs = C[n].rhs;
+    L.res = C[n].res;
+    for (int d = 0; d < 3; d++) L.a[d] = C[n].a[d];
+    clv.push_back(L);
+  }
   for (int cyc = 0; cyc < p.mg_max_vcycles; cyc++) {
-    for (int n = 1; n < nl; n++) launch_fill<T>(c->stream, C[n].phi, C[n].g.N, (T)0);
-    for (int n = 0; n < nl - 1; n++) {
+    for (int n = 1; n < nc; n++) launch_fill<T>(c->stream, C[n].phi, C[n].g.N, (T)0);
+    for (int n = 0; n < std::min(nc, nl - 1); n++) {
       rb(n, p.mg_nsmooths_down);
       T *ph = (n == 0) ? phi0 : C[n].phi;
       const T *rh = (n == 0) ? rhs0 : C[n].rhs;
       launch_cc_residual<T>(c->stream, C[n].g, ph, rh, C[n].res, C[n].a);
       launch_cc_restrict<T>(c->stream, C[n + 1].g, C[n].res, C[n + 1].rhs);
     }
-    rb(nl - 1, p.mg_nsmooths_bottom);
-    for (int n = nl - 2; n >= 0; n--) {
+    if (nc < nl) {
+      ProfScope ps(c, "cc_coarse", 0.0);
+      launch_cc_coarse_vcycle<T>(c->stream, nl - nc, clv.data(), p.mg_nsmooths_down, p.mg_nsmooths_up,
+                                 p.mg_nsmooths_bottom);
+    } else {
+      rb(nl - 1, p.mg_nsmooths_bottom);
+    }
+    for (int n = std::min(nc, nl - 1) - 1; n >= 0; n--) {
       T *ph = (n == 0) ? phi0 : C[n].phi;
       launch_cc_prolong_add<T>(c->stream, C[n].g, ph, C[n + 1].phi);
       rb(n, p.mg_nsmooths_up);
@@ -959,6 +1046,11 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
   {
     const char *dbg = getenv("LMM_DEBUG_SYNC");
     set_debug_sync(dbg && dbg[0] == '1');
+    const char *sm = getenv("LMM_SMOOTHER");
+    c->fused = !(sm && strcmp(sm, "colour") == 0);
+    if (sm && strcmp(sm, "fused") == 0) c->fused_min_tiles = 1;
+    const char *co = getenv("LMM_COARSE");
+    c->coarse_one_launch = !(co && co[0] == '0');
   }
   c->g = make_grid(dim, n_cells, dx);
   int rc = check_params(c, p);
@@ -972,6 +1064,11 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
       delete c;
       return LMM_ERR_ARG;
     }
+  // kernels index points (and dim*N face points) with 32-bit integers
+  if ((double)c->g.N * dim >= 2147483647.0) {
+    delete c;
+    return LMM_ERR_UNSUPPORTED;
+  }
   c->nl_s = compute_nlevs(c->g, p->stag_mg_minwidth);
   c->nl_c = compute_nlevs(c->g, p->mg_minwidth);
   // every staggered level needs even extents (coloured sweeps on periodic faces)
