@@ -30,6 +30,8 @@ struct Faces {
 // ---- staggered operator ----
 // Fused red-black sweep of one component (3D, visc_type -1 / 1): reads the
 // old field from pin, writes the new one to pout (lmm_smooth.hip).
+// 0: plane-marching form, 1: block form, 2: plane marching with paired loads (the default)
+void set_rb_variant(int v);
 int fused_rb_tz(const Grid &g, int min_tiles);
 bool fused_rb_ok(const Grid &g, int visc, int min_tiles);
 template <typename T>

@@ -257,6 +257,32 @@ __device__ __forceinline__ void cc_prolong_add_at(const Grid &gf, const Grid &gc
 
 __device__ __forceinline__ int wrapc(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
 
+// Two consecutive elements a[p], a[p+1] with one vector load (p even, so the
+// pair is aligned to its size).
+template <typename T>
+struct Vec2;
+template <>
+struct Vec2<double> {
+  using type = double2;
+};
+template <>
+struct Vec2<float> {
+  using type = float2;
+};
+template <typename T>
+__device__ __forceinline__ void ld2(const T *a, long p, T &x0, T &x1) {
+  const typename Vec2<T>::type v = *reinterpret_cast<const typename Vec2<T>::type *>(a + p);
+  x0 = v.x;
+  x1 = v.y;
+}
+template <typename T>
+__device__ __forceinline__ void st2(T *a, long p, T x0, T x1) {
+  typename Vec2<T>::type v;
+  v.x = x0;
+  v.y = x1;
+  *reinterpret_cast<typename Vec2<T>::type *>(a + p) = v;
+}
+
 // A red-black stencil is described by a policy: the per-point coefficients
 // (Coef, loaded by load) and the update of one point from its old value and
 // its six neighbours in the same array (update), with the kernel's scalar
@@ -317,6 +343,43 @@ struct StagRB {
       c.B1 = c.E10 = c.E11 = c.E20 = c.E21 = (T)0;
     }
     return c;
+  }
+  // Coefficients of the points (i, j, k) and (i+1, j, k), i even, with one
+  // vector load per array where the two points' values are adjacent.
+  static __device__ __forceinline__ void load_pair(const Grid &g, int i, int j, int k, const Args &a, Coef &c0,
+                                                   Coef &c1) {
+    const long sy = g.s[1], sz = g.s[2];
+    const long p = i + sy * j + sz * k;
+    ld2(a.alpha, p, c0.av, c1.av);
+    ld2(a.rhs, p, c0.rhs, c1.rhs);
+    if (VISC == -1) {
+      ld2(a.beta, p, c0.B0, c1.B0);
+      if (A == 0) {
+        c0.B1 = a.beta[wrapc(i - 1, g.n[0]) + sy * j + sz * k];
+        c1.B1 = c0.B0;
+        ld2(a.e1, p, c0.E10, c1.E10);  // xy edges
+        ld2(a.e1, i + sy * wrapc(j + 1, g.n[1]) + sz * k, c0.E11, c1.E11);
+        ld2(a.e2, p, c0.E20, c1.E20);  // xz edges
+        ld2(a.e2, i + sy * j + sz * wrapc(k + 1, g.n[2]), c0.E21, c1.E21);
+      } else {
+        if (A == 1)
+          ld2(a.beta, i + sy * wrapc(j - 1, g.n[1]) + sz * k, c0.B1, c1.B1);
+        else
+          ld2(a.beta, i + sy * j + sz * wrapc(k - 1, g.n[2]), c0.B1, c1.B1);
+        ld2(a.e1, p, c0.E10, c1.E10);  // xy (A = 1) or xz (A = 2) edges, neighbour along x
+        c0.E11 = c1.E10;
+        c1.E11 = a.e1[wrapc(i + 2, g.n[0]) + sy * j + sz * k];
+        ld2(a.e2, p, c0.E20, c1.E20);  // yz edges
+        if (A == 1)
+          ld2(a.e2, i + sy * j + sz * wrapc(k + 1, g.n[2]), c0.E21, c1.E21);
+        else
+          ld2(a.e2, i + sy * wrapc(j + 1, g.n[1]) + sz * k, c0.E21, c1.E21);
+      }
+    } else {
+      c0.B0 = c1.B0 = a.beta[0];
+      c0.B1 = c0.E10 = c0.E11 = c0.E20 = c0.E21 = (T)0;
+      c1.B1 = c1.E10 = c1.E11 = c1.E20 = c1.E21 = (T)0;
+    }
   }
   // (theta*alpha - L) phi at the point, as stag_applyop_3d computes it
   static __device__ __forceinline__ T apply(const Coef &c, T F, T xp, T xm, T yp, T ym, T zp, T zm, T dxsqinv) {
@@ -401,6 +464,19 @@ struct CCRB {
     c.azl = a.az[p];
     c.azh = a.az[i + sy * j + sz * wrapc(k + 1, g.n[2])];
     return c;
+  }
+  static __device__ __forceinline__ void load_pair(const Grid &g, int i, int j, int k, const Args &a, Coef &c0,
+                                                   Coef &c1) {
+    const long sy = g.s[1], sz = g.s[2];
+    const long p = i + sy * j + sz * k;
+    ld2(a.rhs, p, c0.rhs, c1.rhs);
+    ld2(a.ax, p, c0.axl, c1.axl);
+    c0.axh = c1.axl;
+    c1.axh = a.ax[wrapc(i + 2, g.n[0]) + sy * j + sz * k];
+    ld2(a.ay, p, c0.ayl, c1.ayl);
+    ld2(a.ay, i + sy * wrapc(j + 1, g.n[1]) + sz * k, c0.ayh, c1.ayh);
+    ld2(a.az, p, c0.azl, c1.azl);
+    ld2(a.az, i + sy * j + sz * wrapc(k + 1, g.n[2]), c0.azh, c1.azh);
   }
   static __device__ __forceinline__ T update(const Coef &c, T F, T xp, T xm, T yp, T ym, T zp, T zm,
                                              const Args &a) {

@@ -40,6 +40,8 @@ constexpr int RX = FX + 2, RY = FY + 2; // red values, 1-point halo
 constexpr int NRED = RY * (RX / 2);     // red points per plane of the red region
 
 const char kFusedName[] = "k_rb_fused";
+const char kBlockName[] = "k_rb_block";
+const char kMarchName[] = "k_rb_march2";
 
 template <typename T, typename P>
 __global__ void __launch_bounds__(256) k_rb_fused(Grid g, int tz, int ntx, int nty, int ntiles,
@@ -183,12 +185,261 @@ __global__ void __launch_bounds__(256) k_rb_fused(Grid g, int tz, int ntx, int n
   }
 }
 
+// Plane-marching form with paired coefficient loads: thread t owns the
+// point pair (2m, 2m+1) of row r of the 64 x 8 tile in every plane.  At the
+// step that computes red on plane kk it loads the coefficients of both
+// points of its pair with one vector load per array, uses the red half at
+// once and keeps the black half in registers for the next step, which
+// computes black on plane kk.  Every coefficient line of the tile is thus
+// read once, at one time; the red points of the 1-point halo ring (and of
+// the two halo planes) use scalar loads of lines that the neighbouring
+// tiles read in the same step.
+template <typename T, typename P>
+__global__ void __launch_bounds__(256) k_rb_march2(Grid g, int tz, int ntx, int nty, int ntiles,
+                                                   const T *__restrict__ pin, T *__restrict__ pout,
+                                                   typename P::Args args0, const T *__restrict__ q0,
+                                                   const T *__restrict__ q1, const T *__restrict__ q2,
+                                                   const T *__restrict__ q3, const T *__restrict__ q4) {
+  using Coef = typename P::Coef;
+  const typename P::Args args = P::bind(args0, q0, q1, q2, q3, q4);
+  __shared__ T sp[4][HY][HX];
+  __shared__ T sr[4][RY][RX];
+  int w = blockIdx.x, tile = w;
+  if (ntiles % 8 == 0) tile = (w % 8) * (ntiles / 8) + w / 8;
+  const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
+  const int i0 = bx * FX, j0 = by * FY, k0 = bz * tz;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const long sy = g.s[1], sz = g.s[2];
+  const int tid = threadIdx.x;
+  constexpr int NLD = (HY * HX + 255) / 256;
+  // this thread's pair: columns 2m, 2m+1 of tile row r
+  const int pr = tid / (FX / 2), pm = tid - pr * (FX / 2);
+  const int pj = j0 + pr;
+
+  auto fetch_plane = [&](int kk, T *dst) {
+    const int kg = wrapc(kk, nz);
+#pragma unroll
+    for (int n = 0; n < NLD; n++) {
+      const int q = tid + 256 * n;
+      if (q < HY * HX) {
+        const int r = q / HX, c = q - r * HX;
+        dst[n] = pin[wrapc(i0 - 2 + c, nx) + sy * wrapc(j0 - 2 + r, ny) + sz * kg];
+      }
+    }
+  };
+  auto store_plane = [&](int kk, const T *src) {
+    T(*dst)[HX] = sp[kk & 3];
+#pragma unroll
+    for (int n = 0; n < NLD; n++) {
+      const int q = tid + 256 * n;
+      if (q < HY * HX) {
+        const int r = q / HX, c = q - r * HX;
+        dst[r][c] = src[n];
+      }
+    }
+  };
+  // one red point of plane kk at red-region position (row, rr), scalar loads
+  auto red_point = [&](int kk, int row, int rr) {
+    const int s = kk & 3, sm = (kk - 1) & 3, spl = (kk + 1) & 3;
+    const int hr = row + 1, hc = rr + 1;
+    const Coef c = P::load(g, wrapc(i0 + rr - 1, nx), wrapc(j0 + row - 1, ny), wrapc(kk, nz), args);
+    sr[s][row][rr] = P::update(c, sp[s][hr][hc], sp[s][hr][hc + 1], sp[s][hr][hc - 1], sp[s][hr + 1][hc],
+                               sp[s][hr - 1][hc], sp[spl][hr][hc], sp[sm][hr][hc], args);
+  };
+  // red on plane kk; for interior planes returns the black half of the pair
+  auto red_plane = [&](int kk, Coef &cb) {
+    const int s = kk & 3, sm = (kk - 1) & 3, spl = (kk + 1) & 3;
+    if (kk >= k0 && kk < k0 + tz) {
+      Coef c0, c1;
+      P::load_pair(g, i0 + 2 * pm, pj, kk, args, c0, c1);
+      const bool first_red = ((pj + kk) & 1) == 0;  // column 2m is red when j + k is even
+      const Coef cr = first_red ? c0 : c1;
+      cb = first_red ? c1 : c0;
+      const int rr = 2 * pm + (first_red ? 0 : 1) + 1, row = pr + 1;
+      const int hr = row + 1, hc = rr + 1;
+      sr[s][row][rr] = P::update(cr, sp[s][hr][hc], sp[s][hr][hc + 1], sp[s][hr][hc - 1], sp[s][hr + 1][hc],
+                                 sp[s][hr - 1][hc], sp[spl][hr][hc], sp[sm][hr][hc], args);
+      // halo ring: rows -1 and FY (33 red points each), then one end point
+      // of each interior row (8)
+      for (int h = tid; h < 2 * (RX / 2) + FY; h += 256) {
+        int row2, rr2;
+        if (h < 2 * (RX / 2)) {
+          row2 = (h < RX / 2) ? 0 : RY - 1;
+          const int jj = j0 + row2 - 1;
+          rr2 = 2 * (h % (RX / 2)) + ((jj + kk + 1) & 1);
+        } else {
+          row2 = 1 + (h - 2 * (RX / 2));
+          const int jj = j0 + row2 - 1;
+          rr2 = ((jj + kk + 1) & 1) ? RX - 1 : 0;  // red: i + j + k even, i = i0 + rr - 1
+        }
+        red_point(kk, row2, rr2);
+      }
+    } else {
+      for (int q = tid; q < NRED; q += 256) {
+        const int row = q / (RX / 2), cc = q - row * (RX / 2);
+        const int jj = j0 + row - 1;
+        red_point(kk, row, 2 * cc + ((jj + kk + 1) & 1));
+      }
+    }
+  };
+  auto black_plane = [&](int kk, const Coef &cb) {
+    const int s = kk & 3, sm = (kk - 1) & 3, spl = (kk + 1) & 3;
+    const bool first_red = ((pj + kk) & 1) == 0;
+    const int il = 2 * pm + (first_red ? 1 : 0);
+    const int rrow = pr + 1, rc0 = il + 1;
+    const T v = P::update(cb, sp[s][pr + 2][il + 2], sr[s][rrow][rc0 + 1], sr[s][rrow][rc0 - 1], sr[s][rrow + 1][rc0],
+                          sr[s][rrow - 1][rc0], sr[spl][rrow][rc0], sr[sm][rrow][rc0], args);
+    const T r = sr[s][rrow][first_red ? rc0 - 1 : rc0 + 1];
+    st2(pout, i0 + 2 * pm + sy * pj + sz * (long)kk, first_red ? r : v, first_red ? v : r);
+  };
+
+  T pva[NLD] = {}, pvb[NLD] = {};
+  Coef cba = {}, cbb = {};
+  auto step = [&](int kk, const T *pv, T *pvn, const Coef &cb_prev, Coef &cb_cur) {
+    store_plane(kk + 1, pv);
+    __syncthreads();
+    if (kk + 2 <= k0 + tz + 1) fetch_plane(kk + 2, pvn);
+    red_plane(kk, cb_cur);
+    __syncthreads();
+    if (kk - 1 >= k0) black_plane(kk - 1, cb_prev);
+  };
+  fetch_plane(k0 - 2, pva);
+  store_plane(k0 - 2, pva);
+  fetch_plane(k0 - 1, pva);
+  store_plane(k0 - 1, pva);
+  fetch_plane(k0, pva);
+  // tz + 2 steps, an even count
+  for (int kk = k0 - 1; kk <= k0 + tz; kk += 2) {
+    step(kk, pva, pvb, cba, cbb);
+    step(kk + 1, pvb, pva, cbb, cba);
+  }
+}
+
+// Block form of the same sweep: a workgroup owns a 32 x 8 x 8 block and
+// updates it in three phases with no plane-to-plane dependence -- stage the
+// old field of the block and a 2-point halo in LDS, update the red points of
+// the block and a 1-point halo in place in LDS (red points read only black,
+// i.e. old, neighbours), then update the black points of the block from the
+// red values and write the block.  There is no serial chain of plane steps,
+// so many workgroups per CU overlap their memory latency.
+constexpr int BX = 32, BY = 8, BZ = 8;
+constexpr int LX = BX + 4, LY = BY + 4, LZ = BZ + 4;      // staged region
+constexpr int RBX = BX + 2, RBY = BY + 2, RBZ = BZ + 2;   // red region
+constexpr int NBB = BZ * BY * (BX / 2);                  // black (and red) points of the block
+// red points of the 1-point halo ring: end planes, end rows, one end of each inner row
+constexpr int NRING = 2 * RBY * (RBX / 2) + 2 * BZ * (RBX / 2) + BZ * BY;
+
+template <typename T, typename P>
+__global__ void __launch_bounds__(256) k_rb_block(Grid g, int nbx, int nby, int nblocks, const T *__restrict__ pin,
+                                                  T *__restrict__ pout, typename P::Args args0,
+                                                  const T *__restrict__ q0, const T *__restrict__ q1,
+                                                  const T *__restrict__ q2, const T *__restrict__ q3,
+                                                  const T *__restrict__ q4) {
+  using Coef = typename P::Coef;
+  __shared__ T sp[LZ][LY][LX];
+  const typename P::Args args = P::bind(args0, q0, q1, q2, q3, q4);
+  int w = blockIdx.x, b = w;
+  if (nblocks % 8 == 0) b = (w % 8) * (nblocks / 8) + w / 8;
+  const int bx = b % nbx, by = (b / nbx) % nby, bz = b / (nbx * nby);
+  const int i0 = bx * BX, j0 = by * BY, k0 = bz * BZ;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const long sy = g.s[1], sz = g.s[2];
+  const int tid = threadIdx.x;
+  for (int q = tid; q < LZ * LY * LX; q += 256) {
+    const int x = q % LX, y = (q / LX) % LY, z = q / (LX * LY);
+    sp[z][y][x] = pin[wrapc(i0 - 2 + x, nx) + sy * wrapc(j0 - 2 + y, ny) + sz * wrapc(k0 - 2 + z, nz)];
+  }
+  // Each thread owns NPT point pairs (x, x+1) of the block; one point of a
+  // pair is red and the other black.  Their coefficients come with one
+  // vector load per array; the black half is kept for the black phase.
+  constexpr int NPT = NBB / 256;
+  Coef cb[NPT];
+  __syncthreads();
+#pragma unroll
+  for (int n = 0; n < NPT; n++) {
+    const int q = tid + 256 * n;
+    const int m = q % (BX / 2), yy = (q / (BX / 2)) % BY, zz = q / ((BX / 2) * BY);
+    const int j = j0 + yy, k = k0 + zz;
+    Coef c0, c1;
+    P::load_pair(g, i0 + 2 * m, j, k, args, c0, c1);
+    const bool first_red = ((j + k) & 1) == 0;  // x = 2m is red when j + k is even
+    const Coef cr = first_red ? c0 : c1;
+    cb[n] = first_red ? c1 : c0;
+    const int x = 2 * m + (first_red ? 0 : 1) + 2, y = yy + 2, z = zz + 2;
+    sp[z][y][x] = P::update(cr, sp[z][y][x], sp[z][y][x + 1], sp[z][y][x - 1], sp[z][y + 1][x], sp[z][y - 1][x],
+                            sp[z + 1][y][x], sp[z - 1][y][x], args);
+  }
+  // red points of the 1-point halo ring: the two end planes, the two end rows
+  // of the inner planes, and one end point of each inner row
+  for (int h = tid; h < NRING; h += 256) {
+    int rz, ry, rx;
+    if (h < 2 * RBY * (RBX / 2)) {
+      rz = (h < RBY * (RBX / 2)) ? 0 : RBZ - 1;
+      const int hh = h % (RBY * (RBX / 2));
+      ry = hh / (RBX / 2);
+      rx = 2 * (hh % (RBX / 2));
+    } else if (h < 2 * RBY * (RBX / 2) + 2 * BZ * (RBX / 2)) {
+      const int hh = h - 2 * RBY * (RBX / 2);
+      rz = 1 + hh / RBX;
+      const int rem = hh % RBX;
+      ry = (rem < RBX / 2) ? 0 : RBY - 1;
+      rx = 2 * (rem % (RBX / 2));
+    } else {
+      const int hh = h - 2 * RBY * (RBX / 2) - 2 * BZ * (RBX / 2);
+      rz = 1 + hh / BY;
+      ry = 1 + hh % BY;
+      rx = 0;
+    }
+    const int j = j0 - 1 + ry, k = k0 - 1 + rz;
+    const int par = (j + k + 1) & 1;  // red: i + j + k even, i = i0 - 1 + rx
+    if (rx == 0 && ry >= 1 && ry <= BY && rz >= 1 && rz <= BZ)
+      rx = par ? RBX - 1 : 0;  // inner row: the red one of its two end points
+    else
+      rx += par;
+    const int i = i0 - 1 + rx;
+    const int x = rx + 1, y = ry + 1, z = rz + 1;
+    const Coef c = P::load(g, wrapc(i, nx), wrapc(j, ny), wrapc(k, nz), args);
+    sp[z][y][x] = P::update(c, sp[z][y][x], sp[z][y][x + 1], sp[z][y][x - 1], sp[z][y + 1][x], sp[z][y - 1][x],
+                            sp[z + 1][y][x], sp[z - 1][y][x], args);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int n = 0; n < NPT; n++) {
+    const int q = tid + 256 * n;
+    const int m = q % (BX / 2), yy = (q / (BX / 2)) % BY, zz = q / ((BX / 2) * BY);
+    const int j = j0 + yy, k = k0 + zz;
+    const bool first_red = ((j + k) & 1) == 0;
+    const int xb = 2 * m + (first_red ? 1 : 0);
+    const int x = xb + 2, y = yy + 2, z = zz + 2;
+    const T v = P::update(cb[n], sp[z][y][x], sp[z][y][x + 1], sp[z][y][x - 1], sp[z][y + 1][x], sp[z][y - 1][x],
+                          sp[z + 1][y][x], sp[z - 1][y][x], args);
+    const T r = sp[z][y][first_red ? x - 1 : x + 1];
+    st2(pout, i0 + 2 * m + sy * j + sz * (long)k, first_red ? r : v, first_red ? v : r);
+  }
+}
+
+int g_rb_variant = 2;  // 0: plane marching, 1: block, 2: plane marching with paired loads
+
 template <typename T, typename P>
 void launch_rb(hipStream_t s, const Grid &g, int tz, const T *pin, T *pout, const typename P::Args &args) {
-  const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
-  const int ntiles = ntx * nty * ntz;
   const T *q[5];
   P::ptrs(args, q);
+  if (g_rb_variant == 1) {
+    const int nbx = g.n[0] / BX, nby = g.n[1] / BY, nbz = g.n[2] / BZ;
+    const int nb = nbx * nby * nbz;
+    hipLaunchKernelGGL((k_rb_block<T, P>), dim3(nb), dim3(256), 0, s, g, nbx, nby, nb, pin, pout, args, q[0], q[1],
+                       q[2], q[3], q[4]);
+    post_launch(s, kBlockName);
+    return;
+  }
+  const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
+  const int ntiles = ntx * nty * ntz;
+  if (g_rb_variant == 2) {
+    hipLaunchKernelGGL((k_rb_march2<T, P>), dim3(ntiles), dim3(256), 0, s, g, tz, ntx, nty, ntiles, pin, pout, args,
+                       q[0], q[1], q[2], q[3], q[4]);
+    post_launch(s, kMarchName);
+    return;
+  }
   hipLaunchKernelGGL((k_rb_fused<T, P>), dim3(ntiles), dim3(256), 0, s, g, tz, ntx, nty, ntiles, pin, pout, args,
                      q[0], q[1], q[2], q[3], q[4]);
   post_launch(s, kFusedName);
@@ -211,11 +462,18 @@ void launch_stag_one(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, c
 
 }  // namespace
 
+void set_rb_variant(int v) { g_rb_variant = v; }
+
 // Plane count per workgroup, or 0 when the level cannot use the fused
 // sweep.  By default a level needs enough workgroups (min_tiles) to fill the
 // 256 CUs several times over; smaller levels are faster with the colour
 // kernels.
 int fused_rb_tz(const Grid &g, int min_tiles) {
+  if (g_rb_variant == 1) {
+    if (g.dim != 3 || g.n[0] % BX != 0 || g.n[1] % BY != 0 || g.n[2] % BZ != 0) return 0;
+    const int nb = (g.n[0] / BX) * (g.n[1] / BY) * (g.n[2] / BZ);
+    return nb >= min_tiles ? BZ : 0;
+  }
   if (g.dim != 3 || g.n[0] % FX != 0 || g.n[1] % FY != 0) return 0;
   const int per_plane_set = (g.n[0] / FX) * (g.n[1] / FY);
   int tz = 32;

@@ -1049,6 +1049,11 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
     const char *sm = getenv("LMM_SMOOTHER");
     c->fused = !(sm && strcmp(sm, "colour") == 0);
     if (sm && strcmp(sm, "fused") == 0) c->fused_min_tiles = 1;
+    const char *fk = getenv("LMM_FUSED_KERNEL");
+    int variant = 2;
+    if (fk && strcmp(fk, "march") == 0) variant = 0;
+    if (fk && strcmp(fk, "block") == 0) variant = 1;
+    set_rb_variant(variant);
     const char *co = getenv("LMM_COARSE");
     c->coarse_one_launch = !(co && co[0] == '0');
   }
