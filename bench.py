@@ -14,12 +14,16 @@ FP64 flexible GMRES.  The JSON line's ``value`` is the FP32-MG solve time;
 the plain FP64 time, iteration counts, the true relative residual of each
 and the FP32-MG error against the FP64 solution sit beside it.
 
-``roofline`` is for the dominant kernel, the finest-level coloured
-Gauss-Seidel sweep of the reported form: its algorithmic bytes per launch
-(read phi, rhs, alpha, beta_cc and the three edge viscosities once, write
-phi once: 16 arrays per cell per full 6-colour sweep, one sixth per launch)
-over the launch durations that HIP events on the solver's stream measure
-inside the timed region.
+``roofline`` is for the dominant kernel, the finest-level fused red-black
+sweep of one face component (k_rb_march2) in the reported form: its
+algorithmic bytes per launch (read phi, rhs, alpha of the component,
+beta_cc and the two edge viscosities touching it, write phi: 7 arrays of
+256^3 values) over the launch durations that HIP events on the solver's
+stream measure inside the timed region.  ``traffic`` is the HBM traffic per
+launch that rocprofv3's FETCH_SIZE and WRITE_SIZE counters measured for the
+same kernel on the same workload (FETCH_SIZE scaled by 2, the factor that
+makes it read the true byte count of a plain conversion kernel); it is read
+from the committed counter summary and is null when that file is absent.
 
 ``cpu_baseline`` is the CPU oracle (the FP64 restatement of the reference
 algorithm, OpenMP) solving the same problem family at a bounded size (128^3
@@ -87,6 +91,23 @@ def rel_diff(xa, xb):
     return err_u, err_p
 
 
+COUNTERS = os.path.join(ROOT, "profiles", "smoother_l0_256cubed_counters.txt")
+
+
+def measured_traffic(form, n):
+    """HBM bytes per level-0 smoother launch from the committed counter summary
+    (lines ``traffic <form> <n> <bytes>``), or None."""
+    try:
+        with open(COUNTERS) as f:
+            for line in f:
+                w = line.split()
+                if len(w) == 4 and w[0] == "traffic" and w[1] == form and int(w[2]) == n:
+                    return float(w[3])
+    except OSError:
+        pass
+    return None
+
+
 def time_form(S, ctx, dp, form, steps, warmup, quiet):
     prec = {"fp64": S.FP64, "fp32_mg": S.FP32_MG}[form]
     ctx.set_params(precision=prec)
@@ -110,7 +131,8 @@ def time_form(S, ctx, dp, form, steps, warmup, quiet):
     ctx.profiling(False)
     sm = ctx.kernel_stats("stag_smooth_l0")
     fams = {k: ctx.kernel_stats(k) for k in ("stag_smooth_l0", "stag_smooth", "stag_restrict", "stag_prolong",
-                                              "cc_smooth", "apply_matrix", "krylov")}
+                                              "stag_coarse", "cc_smooth_l0", "cc_smooth", "cc_coarse",
+                                              "apply_matrix", "krylov")}
     ms = (t1 - t0) * 1e3 / steps
     log(f"{form}: {ms:.2f} ms/solve, iters {iters}, status {info['status']}", quiet)
     return ms, info, iters, sm, fams
@@ -209,12 +231,13 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"stag_gs_colour level 0 ({head})",
+            "kernel": f"k_rb_march2 fused red-black sweep of one face component, level 0 ({head})",
             "achieved": sm["achieved_GBs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": sm["frac_hbm"],
-            "traffic": None,
+            "algorithmic_bytes": sm["algorithmic_bytes_per_launch"],
+            "traffic": measured_traffic(head, n),
         },
         "forms": results,
     }

@@ -32,6 +32,9 @@ struct Faces {
 // old field from pin, writes the new one to pout (lmm_smooth.hip).
 // 0: plane-marching form, 1: block form, 2: plane marching with paired loads (the default)
 void set_rb_variant(int v);
+// workgroup count the plane-marching forms aim for when they choose their
+// z-slab thickness
+void set_rb_target(int t);
 int fused_rb_tz(const Grid &g, int min_tiles);
 bool fused_rb_ok(const Grid &g, int visc, int min_tiles);
 template <typename T>

@@ -419,6 +419,7 @@ __global__ void __launch_bounds__(256) k_rb_block(Grid g, int nbx, int nby, int 
 }
 
 int g_rb_variant = 2;  // 0: plane marching, 1: block, 2: plane marching with paired loads
+int g_rb_target = 1024;  // workgroups the plane-marching forms aim for
 
 template <typename T, typename P>
 void launch_rb(hipStream_t s, const Grid &g, int tz, const T *pin, T *pout, const typename P::Args &args) {
@@ -463,11 +464,13 @@ void launch_stag_one(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, c
 }  // namespace
 
 void set_rb_variant(int v) { g_rb_variant = v; }
+void set_rb_target(int t) { g_rb_target = t > 0 ? t : 1; }
 
 // Plane count per workgroup, or 0 when the level cannot use the fused
-// sweep.  By default a level needs enough workgroups (min_tiles) to fill the
-// 256 CUs several times over; smaller levels are faster with the colour
-// kernels.
+// sweep.  The plane-marching forms take the thickest z-slab (at most 32
+// planes) that still gives g_rb_target workgroups, down to 2 planes for the
+// paired form (8 for the first form); a level whose count stays below
+// min_tiles is left to the colour kernels.
 int fused_rb_tz(const Grid &g, int min_tiles) {
   if (g_rb_variant == 1) {
     if (g.dim != 3 || g.n[0] % BX != 0 || g.n[1] % BY != 0 || g.n[2] % BZ != 0) return 0;
@@ -476,8 +479,10 @@ int fused_rb_tz(const Grid &g, int min_tiles) {
   }
   if (g.dim != 3 || g.n[0] % FX != 0 || g.n[1] % FY != 0) return 0;
   const int per_plane_set = (g.n[0] / FX) * (g.n[1] / FY);
+  const int tz_min = (g_rb_variant == 2) ? 2 : 8;
+  const int target = g_rb_target > min_tiles ? g_rb_target : min_tiles;
   int tz = 32;
-  while (tz > 8 && (g.n[2] % tz != 0 || per_plane_set * (g.n[2] / tz) < min_tiles)) tz /= 2;
+  while (tz > tz_min && (g.n[2] % tz != 0 || per_plane_set * (g.n[2] / tz) < target)) tz /= 2;
   if (g.n[2] % tz != 0 || per_plane_set * (g.n[2] / tz) < min_tiles) return 0;
   return tz;
 }

@@ -98,13 +98,14 @@ struct lmm_ctx {
   double *Wcc = nullptr, *Wed = nullptr;
   double *dweights = nullptr;
   // Smoother kernels: fused red-black sweeps on levels with at least
-  // fused_min_tiles workgroups (128^3 and up), colour-by-colour kernels
-  // elsewhere.  Both
-  // give the same bits.  LMM_SMOOTHER=colour in the environment selects the
-  // colour kernels everywhere, LMM_SMOOTHER=fused the fused sweep on every
-  // level whose extents allow it.
+  // fused_min_tiles workgroups (64^3 and up), colour-by-colour kernels
+  // elsewhere.  Both give the same bits.  LMM_SMOOTHER=colour in the
+  // environment selects the colour kernels everywhere, LMM_SMOOTHER=fused
+  // the fused sweep on every level whose extents allow it;
+  // LMM_FUSED_MIN_TILES and LMM_FUSED_TARGET (workgroups aimed for when the
+  // z-slab thickness is chosen) tune the choice.
   bool fused = true;
-  int fused_min_tiles = 512;
+  int fused_min_tiles = 128;
   // coarse levels (at most COARSE_MAX_CELLS cells) of both multigrids run
   // in one single-workgroup launch; LMM_COARSE=0 in the environment keeps
   // one launch per pass on every level (same bits)
@@ -373,13 +374,14 @@ static void prepare_coefs(lmm_ctx *c, const CoefIn &in, double theta_alpha, doub
 
 // ------------------------- staggered multigrid ---------------------------
 
-static double sweep_bytes(const Grid &g, int visc, size_t es) {
-  // compulsory traffic of one full sweep (all colours): read phi, rhs, alpha
-  // per component, beta_cc (+gamma for |visc|=3) and the dim*(dim-1)/2 edge
-  // or node arrays, write phi per component.
+static double comp_sweep_bytes(const Grid &g, int visc, size_t es) {
+  // compulsory traffic of one sweep (both colours) of one face component:
+  // read phi, rhs and alpha of the component, beta_cc, the edge (3D) or node
+  // (2D) arrays that touch the component (dim-1 of them, visc_type < 0),
+  // gamma and the other components' phi where |visc_type| = 3 couples them;
+  // write phi of the component.
   int dim = g.dim;
-  int ne = (dim == 2) ? 1 : 3;
-  int arrays = 4 * dim + 1 + (visc < 0 ? ne : 0) + ((visc == -3) ? 1 : 0);
+  int arrays = 5 + (visc < 0 ? dim - 1 : 0) + ((visc == 3 || visc == -3) ? dim : 0);
   return (double)arrays * (double)g.N * (double)es;
 }
 
@@ -392,7 +394,7 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
   const char *fam = (L.g.N == c->g.N) ? "stag_smooth_l0" : "stag_smooth";
   for (int m = 0; m < nsweeps; m++) {
     if (p.stag_mg_smoother == 0) {
-      ProfScope ps(c, fam, sweep_bytes(L.g, p.visc_type, sizeof(T)));
+      ProfScope ps(c, fam, dim * comp_sweep_bytes(L.g, p.visc_type, sizeof(T)));
       launch_stag_jacobi<T>(c->stream, L.g, p.visc_type, (T)p.stag_mg_omega, phi, rhs, L.res, K);
       for (int d = 0; d < dim; d++)
         hipMemcpyAsync(phi[d], L.res[d], sizeof(T) * L.g.N, hipMemcpyDeviceToDevice, c->stream);
@@ -404,8 +406,8 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
       T *const *src = (m % 2 == 0) ? phi : L.res;
       T *const *dst = (m % 2 == 0) ? L.res : phi;
       for (int a = 0; a < dim; a++) {
-        ProfScope ps(c, fam, sweep_bytes(L.g, p.visc_type, sizeof(T)) / dim);
-        ProfScope pf(c, "stag_rb_fused", sweep_bytes(L.g, p.visc_type, sizeof(T)) / dim);
+        ProfScope ps(c, fam, comp_sweep_bytes(L.g, p.visc_type, sizeof(T)));
+        ProfScope pf(c, "stag_rb_fused", comp_sweep_bytes(L.g, p.visc_type, sizeof(T)));
         launch_stag_rb_fused<T>(c->stream, L.g, p.visc_type, a, (T)p.stag_mg_omega, src[a], dst[a], rhs[a], K,
                                 c->fused_min_tiles);
       }
@@ -415,7 +417,8 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
       continue;
     }
     for (int color = 1; color <= 2 * dim; color++) {
-      ProfScope ps(c, fam, sweep_bytes(L.g, p.visc_type, sizeof(T)) / (2 * dim));
+      // a colour pass touches every cache line of the component's arrays
+      ProfScope ps(c, fam, comp_sweep_bytes(L.g, p.visc_type, sizeof(T)));
       launch_stag_gs_colour<T>(c->stream, L.g, p.visc_type, (color - 1) / 2, (color + 1) % 2,
                                (T)p.stag_mg_omega, phi, K, rhs[(color - 1) / 2]);
     }
@@ -539,7 +542,8 @@ static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *
     if (c->fused && fused_rb_tz(C[n].g, c->fused_min_tiles) > 0) {
       // fused red-black sweeps, ping-pong between phi and C[n].res
       for (int m = 0; m < sweeps; m++) {
-        ProfScope ps(c, "cc_smooth", 0.0);
+        // read phi, rhs and the dim face coefficients, write phi
+        ProfScope ps(c, (n == 0) ? "cc_smooth_l0" : "cc_smooth", (double)(3 + C[n].g.dim) * C[n].g.N * sizeof(T));
         const T *src = (m % 2 == 0) ? ph : C[n].res;
         T *dst = (m % 2 == 0) ? C[n].res : ph;
         launch_cc_rb_fused<T>(c->stream, C[n].g, src, dst, rh, C[n].a, c->fused_min_tiles);
@@ -1049,6 +1053,10 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
     const char *sm = getenv("LMM_SMOOTHER");
     c->fused = !(sm && strcmp(sm, "colour") == 0);
     if (sm && strcmp(sm, "fused") == 0) c->fused_min_tiles = 1;
+    const char *mt = getenv("LMM_FUSED_MIN_TILES");
+    if (mt && atoi(mt) > 0) c->fused_min_tiles = atoi(mt);
+    const char *tg = getenv("LMM_FUSED_TARGET");
+    set_rb_target(tg && atoi(tg) > 0 ? atoi(tg) : 1024);
     const char *fk = getenv("LMM_FUSED_KERNEL");
     int variant = 2;
     if (fk && strcmp(fk, "march") == 0) variant = 0;

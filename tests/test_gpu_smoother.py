@@ -2,9 +2,12 @@
 extents allow it (x a multiple of 64, y of 8), against the oracle and against
 the colour-by-colour kernels.  Both must agree bit for bit: every point is
 computed with the reference's operation order.  Two forms of the fused sweep
-exist (LMM_FUSED_KERNEL=march2, the default, block and march).  LMM_SMOOTHER=fused makes the
-small test levels use the fused sweep, which by default only levels with at
-least 512 workgroups (128^3 and up) do."""
+exist (LMM_FUSED_KERNEL=march2, the default, block and march).
+LMM_SMOOTHER=fused makes the small test levels use the fused sweep, which by
+default only levels with at least 128 workgroups (64^3 and up) do; the
+plane-marching forms choose their z-slab thickness for LMM_FUSED_TARGET
+workgroups (1024 by default: 2-plane slabs on these small levels, 1 for the
+thickest slab a level allows)."""
 import numpy as np
 import pytest
 
@@ -33,12 +36,13 @@ def _mg_solve(pb, kw, rhs, guess):
         return [f.download() for f in phi], fused
 
 
-@pytest.mark.parametrize("kernel", ["march2", "block", "march"])
+@pytest.mark.parametrize("kernel,target", [("march2", "1024"), ("march2", "1"), ("block", "1024"), ("march", "1")])
 @pytest.mark.parametrize("visc", [-1, 1])
 @pytest.mark.parametrize("vcycles,sweeps", [(1, (2, 2, 8)), (2, (2, 2, 8)), (1, (1, 3, 3))])
-def test_fused_mg_solver_bit_exact(visc, vcycles, sweeps, kernel, monkeypatch):
+def test_fused_mg_solver_bit_exact(visc, vcycles, sweeps, kernel, target, monkeypatch):
     monkeypatch.setenv("LMM_SMOOTHER", "fused")
     monkeypatch.setenv("LMM_FUSED_KERNEL", kernel)
+    monkeypatch.setenv("LMM_FUSED_TARGET", target)
     pb = P.small_problem(3, N3, visc_type=visc, seed=61)
     kw = dict(visc_type=visc, stag_mg_max_vcycles=vcycles, stag_mg_nsmooths_down=sweeps[0],
               stag_mg_nsmooths_up=sweeps[1], stag_mg_nsmooths_bottom=sweeps[2])
@@ -94,10 +98,11 @@ def test_fused_gmres_parity(monkeypatch):
     assert relerr(gu, ou) < 1e-10 and relerr(gp, op) < 1e-10
 
 
-@pytest.mark.parametrize("kernel", ["march2", "block", "march"])
+@pytest.mark.parametrize("kernel,target", [("march2", "1024"), ("march2", "1"), ("block", "1024"), ("march", "1")])
 @pytest.mark.parametrize("n,vcycles,sweeps", [((64, 16, 32), 1, (2, 2)), ((128, 32, 16), 2, (3, 1))])
-def test_fused_cc_multigrid_matches_oracle_and_colour(n, vcycles, sweeps, kernel, monkeypatch):
+def test_fused_cc_multigrid_matches_oracle_and_colour(n, vcycles, sweeps, kernel, target, monkeypatch):
     monkeypatch.setenv("LMM_FUSED_KERNEL", kernel)
+    monkeypatch.setenv("LMM_FUSED_TARGET", target)
     rng = np.random.default_rng(91)
     shape = tuple(reversed(n))
     ainv = [0.5 + rng.random(shape) for _ in range(3)]
