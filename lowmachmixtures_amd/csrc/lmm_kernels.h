@@ -154,6 +154,11 @@ void launch_bmax_neg(hipStream_t s, double *tmp, const double *b, long n);      
 void launch_div_by(hipStream_t s, double *v, const double *src, long n, const double *scal);  // v = src / *scal
 void launch_axmy(hipStream_t s, double *w, const double *v, long n, const double *scal);     // w = w - v * *scal
 void launch_update_sol(hipStream_t s, double *x, double *const V[], const double *y, int i, long n);
+// x += sum_k y_k Z_k, Z in FP32 (at most 32 vectors)
+void launch_update_sol_z32(hipStream_t s, double *x, float *const Z[], const double *y, int i, long n);
+// FP64 A x of an FP32 x (3D visc_type -1 / 1); false when not available
+bool launch_apply_matrix_x32(hipStream_t s, const Grid &g, int visc, float *const x_u[3], const float *x_p,
+                             double *const b_u[3], double *b_p, const StagCoef<double> &K);
 template <typename T>
 void launch_sub_scalar(hipStream_t s, T *x, long n, const double *scal);
 template <typename T>

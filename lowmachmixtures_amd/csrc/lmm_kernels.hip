@@ -355,8 +355,8 @@ __global__ void __launch_bounds__(256) k_apply_matrix(Grid g, int visc, T dxsqin
 
 // 3D visc_type -1 / 1: the same arithmetic with every index static (the
 // stencil policy of the fused smoother supplies L at a point).
-template <typename T, int VISC>
-__device__ __forceinline__ T am_component(const Grid &g, int i, int j, int k, const Ptr3<T> &xu,
+template <typename T, int VISC, typename TX>
+__device__ __forceinline__ T am_component(const Grid &g, int i, int j, int k, const Ptr3<TX> &xu,
                                           const typename StagRB<T, 0, VISC>::Args &a0,
                                           const typename StagRB<T, 1, VISC>::Args &a1,
                                           const typename StagRB<T, 2, VISC>::Args &a2, int comp, T dxsqinv) {
@@ -366,31 +366,33 @@ __device__ __forceinline__ T am_component(const Grid &g, int i, int j, int k, co
   const int jp = wrapc(j + 1, g.n[1]), jm = wrapc(j - 1, g.n[1]);
   const int kp = wrapc(k + 1, g.n[2]), km = wrapc(k - 1, g.n[2]);
   const long ro = sy * j + sz * k;
-  auto nb = [&](const T *F, T &xp, T &xm, T &yp, T &ym, T &zp, T &zm) {
-    xp = F[ip + ro];
-    xm = F[im + ro];
-    yp = F[i + sy * jp + sz * k];
-    ym = F[i + sy * jm + sz * k];
-    zp = F[i + sy * j + sz * kp];
-    zm = F[i + sy * j + sz * km];
+  // TX: storage type of the operand (FP32 preconditioned vectors of the
+  // flexible GMRES are widened exactly on load)
+  auto nb = [&](const TX *F, T &xp, T &xm, T &yp, T &ym, T &zp, T &zm) {
+    xp = (T)F[ip + ro];
+    xm = (T)F[im + ro];
+    yp = (T)F[i + sy * jp + sz * k];
+    ym = (T)F[i + sy * jm + sz * k];
+    zp = (T)F[i + sy * j + sz * kp];
+    zm = (T)F[i + sy * j + sz * km];
   };
   T xp, xm, yp, ym, zp, zm;
   if (comp == 0) {
     nb(xu.c[0], xp, xm, yp, ym, zp, zm);
-    return StagRB<T, 0, VISC>::apply(StagRB<T, 0, VISC>::load(g, i, j, k, a0), xu.c[0][p], xp, xm, yp, ym, zp, zm,
-                                     dxsqinv);
+    return StagRB<T, 0, VISC>::apply(StagRB<T, 0, VISC>::load(g, i, j, k, a0), (T)xu.c[0][p], xp, xm, yp, ym, zp,
+                                     zm, dxsqinv);
   } else if (comp == 1) {
     nb(xu.c[1], xp, xm, yp, ym, zp, zm);
-    return StagRB<T, 1, VISC>::apply(StagRB<T, 1, VISC>::load(g, i, j, k, a1), xu.c[1][p], xp, xm, yp, ym, zp, zm,
-                                     dxsqinv);
+    return StagRB<T, 1, VISC>::apply(StagRB<T, 1, VISC>::load(g, i, j, k, a1), (T)xu.c[1][p], xp, xm, yp, ym, zp,
+                                     zm, dxsqinv);
   }
   nb(xu.c[2], xp, xm, yp, ym, zp, zm);
-  return StagRB<T, 2, VISC>::apply(StagRB<T, 2, VISC>::load(g, i, j, k, a2), xu.c[2][p], xp, xm, yp, ym, zp, zm,
+  return StagRB<T, 2, VISC>::apply(StagRB<T, 2, VISC>::load(g, i, j, k, a2), (T)xu.c[2][p], xp, xm, yp, ym, zp, zm,
                                    dxsqinv);
 }
 
-template <typename T, int VISC>
-__global__ void __launch_bounds__(256) k_apply_matrix3(Grid g, T dxsqinv, T dxinv, Ptr3<T> xu, const T *__restrict__ xp,
+template <typename T, int VISC, typename TX>
+__global__ void __launch_bounds__(256) k_apply_matrix3(Grid g, T dxsqinv, T dxinv, Ptr3<TX> xu, const TX *__restrict__ xp,
                                                        Ptr3<T> bu, T *bp, typename StagRB<T, 0, VISC>::Args a0,
                                                        typename StagRB<T, 1, VISC>::Args a1,
                                                        typename StagRB<T, 2, VISC>::Args a2,
@@ -398,9 +400,10 @@ __global__ void __launch_bounds__(256) k_apply_matrix3(Grid g, T dxsqinv, T dxin
                                                        const T *__restrict__ al2, const T *__restrict__ be,
                                                        const T *__restrict__ ed0, const T *__restrict__ ed1,
                                                        const T *__restrict__ ed2) {
-  a0 = StagRB<T, 0, VISC>::bind(a0, xp, al0, be, ed0, ed1);
-  a1 = StagRB<T, 1, VISC>::bind(a1, xp, al1, be, ed0, ed2);
-  a2 = StagRB<T, 2, VISC>::bind(a2, xp, al2, be, ed1, ed2);
+  // rhs is not read by apply(); it gets a valid address of the right type
+  a0 = StagRB<T, 0, VISC>::bind(a0, be, al0, be, ed0, ed1);
+  a1 = StagRB<T, 1, VISC>::bind(a1, be, al1, be, ed0, ed2);
+  a2 = StagRB<T, 2, VISC>::bind(a2, be, al2, be, ed1, ed2);
   long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   int i, j, k;
@@ -409,28 +412,28 @@ __global__ void __launch_bounds__(256) k_apply_matrix3(Grid g, T dxsqinv, T dxin
   const long pmx = wrapc(i - 1, g.n[0]) + sy * j + sz * k;
   const long pmy = i + sy * wrapc(j - 1, g.n[1]) + sz * k;
   const long pmz = i + sy * j + sz * wrapc(k - 1, g.n[2]);
-  const T pc = xp[t];
-  bu.c[0][t] = am_component<T, VISC>(g, i, j, k, xu, a0, a1, a2, 0, dxsqinv) + (pc - xp[pmx]) * dxinv;
-  bu.c[1][t] = am_component<T, VISC>(g, i, j, k, xu, a0, a1, a2, 1, dxsqinv) + (pc - xp[pmy]) * dxinv;
-  bu.c[2][t] = am_component<T, VISC>(g, i, j, k, xu, a0, a1, a2, 2, dxsqinv) + (pc - xp[pmz]) * dxinv;
+  const T pc = (T)xp[t];
+  bu.c[0][t] = am_component<T, VISC, TX>(g, i, j, k, xu, a0, a1, a2, 0, dxsqinv) + (pc - (T)xp[pmx]) * dxinv;
+  bu.c[1][t] = am_component<T, VISC, TX>(g, i, j, k, xu, a0, a1, a2, 1, dxsqinv) + (pc - (T)xp[pmy]) * dxinv;
+  bu.c[2][t] = am_component<T, VISC, TX>(g, i, j, k, xu, a0, a1, a2, 2, dxsqinv) + (pc - (T)xp[pmz]) * dxinv;
   const long xq = wrapc(i + 1, g.n[0]) + sy * j + sz * k;
   const long yq = i + sy * wrapc(j + 1, g.n[1]) + sz * k;
   const long zq = i + sy * j + sz * wrapc(k + 1, g.n[2]);
-  T dv = (xu.c[0][xq] - xu.c[0][t]) * dxinv;
-  dv = dv + (xu.c[1][yq] - xu.c[1][t]) * dxinv;
-  dv = dv + (xu.c[2][zq] - xu.c[2][t]) * dxinv;
+  T dv = ((T)xu.c[0][xq] - (T)xu.c[0][t]) * dxinv;
+  dv = dv + ((T)xu.c[1][yq] - (T)xu.c[1][t]) * dxinv;
+  dv = dv + ((T)xu.c[2][zq] - (T)xu.c[2][t]) * dxinv;
   bp[t] = dv * (T)-1.0;
 }
 
-template <typename T, int VISC>
-static void launch_am3(hipStream_t s, const Grid &g, T dxsqinv, T dxinv, T *const x_u[3], const T *x_p,
+template <typename T, int VISC, typename TX>
+static void launch_am3(hipStream_t s, const Grid &g, T dxsqinv, T dxinv, TX *const x_u[3], const TX *x_p,
                        T *const b_u[3], T *b_p, const StagCoef<T> &K) {
   typename StagRB<T, 0, VISC>::Args a0;
   typename StagRB<T, 1, VISC>::Args a1;
   typename StagRB<T, 2, VISC>::Args a2;
   a0.omega = a1.omega = a2.omega = (T)0;
   a0.dxsqinv = a1.dxsqinv = a2.dxsqinv = dxsqinv;
-  a0.rhs = a1.rhs = a2.rhs = x_p;  // unused by apply(); any valid address
+  a0.rhs = a1.rhs = a2.rhs = K.beta;  // unused by apply(); any valid address
   a0.alpha = K.alpha[0];
   a1.alpha = K.alpha[1];
   a2.alpha = K.alpha[2];
@@ -442,8 +445,26 @@ static void launch_am3(hipStream_t s, const Grid &g, T dxsqinv, T dxinv, T *cons
   a1.e2 = K.bed[2];
   a2.e1 = K.bed[1];
   a2.e2 = K.bed[2];
-  LAUNCH((k_apply_matrix3<T, VISC>), dim3(nblk(g.N)), dim3(TPB), 0, s, g, dxsqinv, dxinv, p3(x_u), x_p, p3(b_u),
-         b_p, a0, a1, a2, K.alpha[0], K.alpha[1], K.alpha[2], K.beta, K.bed[0], K.bed[1], K.bed[2]);
+  LAUNCH((k_apply_matrix3<T, VISC, TX>), dim3(nblk(g.N)), dim3(TPB), 0, s, g, dxsqinv, dxinv, p3(x_u), x_p,
+         p3(b_u), b_p, a0, a1, a2, K.alpha[0], K.alpha[1], K.alpha[2], K.beta, K.bed[0], K.bed[1], K.bed[2]);
+}
+
+// A x with x held in FP32 (the flexible GMRES's preconditioned vectors) and
+// the operator applied in FP64: every value is widened exactly on load, so
+// the result equals converting x to FP64 first.  3D visc_type -1 and 1 only;
+// returns false (nothing launched) elsewhere.
+bool launch_apply_matrix_x32(hipStream_t s, const Grid &g, int visc, float *const x_u[3], const float *x_p,
+                             double *const b_u[3], double *b_p, const StagCoef<double> &K) {
+  const double dxsqinv = 1.0 / (g.dx * g.dx), dxinv = 1.0 / g.dx;
+  if (g.dim == 3 && visc == -1) {
+    launch_am3<double, -1, float>(s, g, dxsqinv, dxinv, x_u, x_p, b_u, b_p, K);
+    return true;
+  }
+  if (g.dim == 3 && visc == 1) {
+    launch_am3<double, 1, float>(s, g, dxsqinv, dxinv, x_u, x_p, b_u, b_p, K);
+    return true;
+  }
+  return false;
 }
 
 template <typename T>
@@ -451,11 +472,11 @@ void launch_apply_matrix(hipStream_t s, const Grid &g, int visc, T *const x_u[3]
                          T *const b_u[3], T *b_p, const StagCoef<T> &K) {
   T dxsqinv = (T)(1.0 / (g.dx * g.dx)), dxinv = (T)(1.0 / g.dx);
   if (g.dim == 3 && visc == -1) {
-    launch_am3<T, -1>(s, g, dxsqinv, dxinv, x_u, x_p, b_u, b_p, K);
+    launch_am3<T, -1, T>(s, g, dxsqinv, dxinv, x_u, x_p, b_u, b_p, K);
     return;
   }
   if (g.dim == 3 && visc == 1) {
-    launch_am3<T, 1>(s, g, dxsqinv, dxinv, x_u, x_p, b_u, b_p, K);
+    launch_am3<T, 1, T>(s, g, dxsqinv, dxinv, x_u, x_p, b_u, b_p, K);
     return;
   }
   LAUNCH(k_apply_matrix<T>, dim3(nblk(g.N)), dim3(TPB), 0, s, g, visc, dxsqinv, dxinv, p3(x_u), x_p,
@@ -790,6 +811,23 @@ void launch_update_sol(hipStream_t s, double *x, double *const V[], const double
   VList L = {};
   for (int k = 0; k < i && k < 32; k++) L.v[k] = V[k];
   LAUNCH(k_update_sol, dim3(nblk(n)), dim3(TPB), 0, s, x, L, y, i, n);
+}
+
+// x += sum_k y_k Z_k with Z in FP32, widened exactly, added in order k = 0..i-1
+struct ZList {
+  const float *v[32];
+};
+__global__ void k_update_sol_z32(double *x, ZList Z, const double *__restrict__ y, int i, long n) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  double xv = x[t];
+  for (int k = 0; k < i; k++) xv = xv + (double)Z.v[k][t] * y[k];
+  x[t] = xv;
+}
+void launch_update_sol_z32(hipStream_t s, double *x, float *const Z[], const double *y, int i, long n) {
+  ZList L = {};
+  for (int k = 0; k < i && k < 32; k++) L.v[k] = Z[k];
+  LAUNCH(k_update_sol_z32, dim3(nblk(n)), dim3(TPB), 0, s, x, L, y, i, n);
 }
 
 template <typename T>

@@ -110,6 +110,10 @@ struct lmm_ctx {
   // in one single-workgroup launch; LMM_COARSE=0 in the environment keeps
   // one launch per pass on every level (same bits)
   bool coarse_one_launch = true;
+  // the flexible GMRES applies A to its FP32 preconditioned vectors and adds
+  // them to x without widening them into FP64 buffers first (same bits);
+  // LMM_Z32=0 in the environment widens them first
+  bool z32_direct = true;
   // profiling
   bool prof = false;
   std::map<std::string, ProfFamily> fam;
@@ -634,6 +638,18 @@ static void apply_matrix_dev(lmm_ctx *c, const SV &x, const SV &b) {
   launch_apply_matrix<double>(c->stream, c->g, c->prm.visc_type, x.u, x.p, b.u, b.p, c->s64[0].K());
 }
 
+// FP64 A z of an FP32 Stokes vector z (same bits as widening z first)
+static void apply_matrix_z32(lmm_ctx *c, float *z, const SV &b) {
+  SVT<float> zf = svt_of<float>(c, z);
+  if (c->z32_direct) {
+    ProfScope ps(c, "apply_matrix", (double)c->g.N * (c->g.dim == 3 ? 11 * 8.0 + 4 * 4.0 : 10 * 8.0));
+    if (launch_apply_matrix_x32(c->stream, c->g, c->prm.visc_type, zf.u, zf.p, b.u, b.p, c->s64[0].K())) return;
+  }
+  long L = (long)(c->g.dim + 1) * c->g.N;
+  launch_convert<double, float>(c->stream, c->r, z, L);
+  apply_matrix_dev(c, sv_of(c, c->r), b);
+}
+
 // apply_precon.f90:29-195,434-460 with precon_type 1 (T = FP64 or FP32).
 template <typename T>
 static void apply_precon_t(lmm_ctx *c, std::vector<SLevel<T>> &S, std::vector<CLevel<T>> &C,
@@ -953,9 +969,7 @@ static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
       total_iter++;
       used = i;
       precon32(c, Vk(i - 1), Zk(i - 1), theta_alpha);
-      launch_convert<double, float>(s, c->r, Zk(i - 1), L);
-      SV z = sv_of(c, c->r);
-      apply_matrix_dev(c, z, w);
+      apply_matrix_z32(c, Zk(i - 1), w);
       for (int k = 1; k <= i; k++) {
         double *slot = dH + (k - 1) + (long)(m + 1) * (i - 1);
         stokes_dot_dev(c, c->w, Vk(k - 1), slot, 0);
@@ -975,10 +989,17 @@ static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
     // x += sum_k y_k Z_k  (Z in FP32, accumulation in FP64)
     memcpy(c->hscal, hs.y.data(), sizeof(double) * used);
     hipMemcpyAsync(c->scal + 3200, c->hscal, sizeof(double) * used, hipMemcpyHostToDevice, s);
-    for (int k = 0; k < used; k++) {
-      launch_convert<double, float>(s, c->r, Zk(k), L);
-      double *one = c->r;
-      launch_update_sol(s, c->x, &one, c->scal + 3200 + k, 1, L);
+    if (c->z32_direct) {
+      std::vector<float *> zl(used);
+      for (int k = 0; k < used; k++) zl[k] = Zk(k);
+      ProfScope ps(c, "krylov", (double)L * (16.0 + 4.0 * used));
+      launch_update_sol_z32(s, c->x, zl.data(), c->scal + 3200, used, L);
+    } else {
+      for (int k = 0; k < used; k++) {
+        launch_convert<double, float>(s, c->r, Zk(k), L);
+        double *one = c->r;
+        launch_update_sol(s, c->x, &one, c->scal + 3200 + k, 1, L);
+      }
     }
     hipStreamSynchronize(s);
   }
@@ -1062,6 +1083,8 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
     if (fk && strcmp(fk, "march") == 0) variant = 0;
     if (fk && strcmp(fk, "block") == 0) variant = 1;
     set_rb_variant(variant);
+    const char *z3 = getenv("LMM_Z32");
+    c->z32_direct = !(z3 && z3[0] == '0');
     const char *co = getenv("LMM_COARSE");
     c->coarse_one_launch = !(co && co[0] == '0');
   }

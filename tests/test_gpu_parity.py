@@ -327,3 +327,22 @@ def test_host_noise_generator_matches_oracle():
     want = O.fill_normal(12345, 5000)
     got = S.fill_normal_host(12345, 5000)
     assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("visc", [-1, 1, -2])
+def test_fgmres_fp32_vectors_used_directly_equal_widened(visc, monkeypatch):
+    """The flexible GMRES applies the FP64 operator to its FP32 preconditioned
+    vectors and adds them to x without widening them into FP64 buffers first
+    (LMM_Z32=0 widens first).  Widening is exact, so both give the same bits;
+    visc_type -2 has no direct kernel and always widens."""
+    pb = P.small_problem(3, (64, 32, 32), visc_type=visc, seed=89, contrast=30.0)
+    b_u = P.rhs_from_noise_host(pb)
+    out = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("LMM_Z32", mode)
+        with S.Context(3, pb.n, pb.dx, visc_type=visc, precision=S.FP32_MG) as ctx:
+            dp = S.DeviceProblem(ctx, pb, rhs=b_u)
+            _, info = dp.solve()
+            out.append((info["total_iter"], dp.solution()))
+    assert out[0][0] == out[1][0]
+    assert bit_equal(out[0][1][0], out[1][1][0]) and bit_equal(out[0][1][1], out[1][1][1])
