@@ -130,7 +130,8 @@ def time_form(S, ctx, dp, form, steps, warmup, quiet):
     t1 = time.perf_counter()
     ctx.profiling(False)
     sm = ctx.kernel_stats("stag_smooth_l0")
-    fams = {k: ctx.kernel_stats(k) for k in ("stag_smooth_l0", "stag_smooth", "stag_restrict", "stag_prolong",
+    fams = {k: ctx.kernel_stats(k) for k in ("stag_smooth_l0", "stag_smooth", "stag_restrict_l0", "stag_restrict",
+                                              "stag_prolong",
                                               "stag_coarse", "cc_smooth_l0", "cc_smooth", "cc_coarse",
                                               "apply_matrix", "krylov")}
     ms = (t1 - t0) * 1e3 / steps

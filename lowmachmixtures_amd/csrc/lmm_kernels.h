@@ -35,11 +35,19 @@ void set_rb_variant(int v);
 // workgroup count the plane-marching forms aim for when they choose their
 // z-slab thickness
 void set_rb_target(int t);
-int fused_rb_tz(const Grid &g, int min_tiles);
+// z-slab thickness for element size es (4 or 8 bytes), 0: not eligible
+int fused_rb_tz(const Grid &g, int min_tiles, int es);
 bool fused_rb_ok(const Grid &g, int visc, int min_tiles);
 template <typename T>
 void launch_stag_rb_fused(hipStream_t s, const Grid &g, int visc, int a, T omega, const T *pin, T *pout,
                           const T *rhs, const StagCoef<T> &K, int min_tiles);
+// Residual of every face component and its full-weighting restriction into
+// the coarse right-hand side crhs in one pass per component (3D, visc_type
+// -1 / 1, x a multiple of 64, y of 8); false (nothing launched) elsewhere.
+bool stag_resid_restrict_ok(const Grid &g, const Grid &gc, int visc);
+template <typename T>
+bool launch_stag_resid_restrict(hipStream_t s, const Grid &g, const Grid &gc, int visc, T *const phi[3],
+                                T *const rhs[3], T *const crhs[3], const StagCoef<T> &K);
 // The same fused sweep for the cell-centred red-black Gauss-Seidel (3D).
 template <typename T>
 void launch_cc_rb_fused(hipStream_t s, const Grid &g, const T *pin, T *pout, const T *rhs, T *const a[3],

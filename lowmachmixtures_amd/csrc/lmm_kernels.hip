@@ -392,7 +392,8 @@ __device__ __forceinline__ T am_component(const Grid &g, int i, int j, int k, co
 }
 
 template <typename T, int VISC, typename TX>
-__global__ void __launch_bounds__(256) k_apply_matrix3(Grid g, T dxsqinv, T dxinv, Ptr3<TX> xu, const TX *__restrict__ xp,
+__global__ void __launch_bounds__(256) k_apply_matrix3(Grid g, T dxsqinv, T dxinv, Ptr3<TX> xu,
+                                                       const TX *__restrict__ xp,
                                                        Ptr3<T> bu, T *bp, typename StagRB<T, 0, VISC>::Args a0,
                                                        typename StagRB<T, 1, VISC>::Args a1,
                                                        typename StagRB<T, 2, VISC>::Args a2,
@@ -404,7 +405,7 @@ __global__ void __launch_bounds__(256) k_apply_matrix3(Grid g, T dxsqinv, T dxin
   a0 = StagRB<T, 0, VISC>::bind(a0, be, al0, be, ed0, ed1);
   a1 = StagRB<T, 1, VISC>::bind(a1, be, al1, be, ed0, ed2);
   a2 = StagRB<T, 2, VISC>::bind(a2, be, al2, be, ed1, ed2);
-  long t = xcd_block() * blockDim.x + threadIdx.x;
+  const long t = xcd_block() * blockDim.x + threadIdx.x;
   if (t >= g.N) return;
   int i, j, k;
   decode(g, t, i, j, k);

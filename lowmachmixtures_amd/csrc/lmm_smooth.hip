@@ -42,6 +42,7 @@ constexpr int NRED = RY * (RX / 2);     // red points per plane of the red regio
 const char kFusedName[] = "k_rb_fused";
 const char kBlockName[] = "k_rb_block";
 const char kMarchName[] = "k_rb_march2";
+const char kResidRestrictName[] = "k_resid_restrict";
 
 template <typename T, typename P>
 __global__ void __launch_bounds__(256) k_rb_fused(Grid g, int tz, int ntx, int nty, int ntiles,
@@ -446,6 +447,137 @@ void launch_rb(hipStream_t s, const Grid &g, int tz, const T *pin, T *pout, cons
   post_launch(s, kFusedName);
 }
 
+// Residual r = rhs - L phi of one face component and its full-weighting
+// restriction to the next coarser level in one pass: the down-leg step
+// "residual, then restrict" of stag_mg_solver.f90:384-404 (the residual of
+// stag_applyop, the restriction of stag_restriction, :971-1175).  The fine
+// residual never goes to memory.  A workgroup owns a 64 x 8 column of fine
+// points and marches through its z-slab; per plane it stages phi with a halo
+// in LDS, computes the residual of its points (and of the one-point halo on
+// the low side of the component's own direction, which the restriction's
+// i-1 / j-1 terms read) into a ring of residual planes, and after every odd
+// plane writes the 32 x 4 coarse points of the plane pair.  Component 2
+// starts one plane early for its k-1 term.  Each residual is c.rhs - L with
+// the operator of stag_applyop_3d and each coarse value sums its 12 fine
+// residuals in the order of the reference, so the coarse right-hand side is
+// bit for bit that of the two separate passes.
+template <typename T, typename P, int A>
+__global__ void __launch_bounds__(256) k_resid_restrict(Grid g, Grid gc, int tz, int ntx, int nty, int ntiles,
+                                                        const T *__restrict__ pin, T *__restrict__ cout,
+                                                        typename P::Args args0, const T *__restrict__ q0,
+                                                        const T *__restrict__ q1, const T *__restrict__ q2,
+                                                        const T *__restrict__ q3, const T *__restrict__ q4) {
+  using Coef = typename P::Coef;
+  const typename P::Args args = P::bind(args0, q0, q1, q2, q3, q4);
+  __shared__ T sp[4][HY][HX];
+  __shared__ T sres[4][FY + 1][FX + 1];  // residuals, row / column 0 = the low-side halo
+  int w = blockIdx.x, tile = w;
+  if (ntiles % 8 == 0) tile = (w % 8) * (ntiles / 8) + w / 8;
+  const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
+  const int i0 = bx * FX, j0 = by * FY, k0 = bz * tz;
+  const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+  const long sy = g.s[1], sz = g.s[2];
+  const int tid = threadIdx.x;
+  constexpr int NLD = (HY * HX + 255) / 256;
+  const int pr = tid / (FX / 2), pm = tid - pr * (FX / 2);
+  const T dxsqinv = args.dxsqinv;
+
+  auto fetch_plane = [&](int kk, T *dst) {
+    const int kg = wrapc(kk, nz);
+#pragma unroll
+    for (int n = 0; n < NLD; n++) {
+      const int q = tid + 256 * n;
+      if (q < HY * HX) {
+        const int r = q / HX, c = q - r * HX;
+        dst[n] = pin[wrapc(i0 - 2 + c, nx) + sy * wrapc(j0 - 2 + r, ny) + sz * kg];
+      }
+    }
+  };
+  auto store_plane = [&](int kk, const T *src) {
+    T(*dst)[HX] = sp[kk & 3];
+#pragma unroll
+    for (int n = 0; n < NLD; n++) {
+      const int q = tid + 256 * n;
+      if (q < HY * HX) {
+        const int r = q / HX, c = q - r * HX;
+        dst[r][c] = src[n];
+      }
+    }
+  };
+  // residual at tile-local (il, jl) of plane kk from a loaded coefficient set
+  auto resid = [&](const Coef &c, int kk, int jl, int il) -> T {
+    const int s = kk & 3, sm = (kk - 1) & 3, spl = (kk + 1) & 3;
+    const int hr = jl + 2, hc = il + 2;
+    return c.rhs - P::apply(c, sp[s][hr][hc], sp[s][hr][hc + 1], sp[s][hr][hc - 1], sp[s][hr + 1][hc],
+                            sp[s][hr - 1][hc], sp[spl][hr][hc], sp[sm][hr][hc], dxsqinv);
+  };
+  auto resid_plane = [&](int kk) {
+    const int s = kk & 3, kg = wrapc(kk, nz);
+    Coef c0, c1;
+    P::load_pair(g, i0 + 2 * pm, j0 + pr, kg, args, c0, c1);
+    sres[s][pr + 1][2 * pm + 1] = resid(c0, kk, pr, 2 * pm);
+    sres[s][pr + 1][2 * pm + 2] = resid(c1, kk, pr, 2 * pm + 1);
+    if (A == 0 && tid < FY) {
+      const Coef c = P::load(g, wrapc(i0 - 1, nx), j0 + tid, kg, args);
+      sres[s][tid + 1][0] = resid(c, kk, tid, -1);
+    }
+    if (A == 1 && tid < FX) {
+      const Coef c = P::load(g, i0 + tid, wrapc(j0 - 1, ny), kg, args);
+      sres[s][0][tid + 1] = resid(c, kk, -1, tid);
+    }
+  };
+  // coarse plane K = (kk - 1) / 2 from fine planes kk - 2 (A = 2), kk - 1, kk
+  auto restrict_plane = [&](int kk) {
+    if (tid >= (FX / 2) * (FY / 2)) return;
+    const int cj = tid / (FX / 2), ci = tid - cj * (FX / 2);
+    const int s0 = (kk - 1) & 3, s1 = kk & 3, sm = (kk - 2) & 3;
+    const int il = 2 * ci + 1, jl = 2 * cj + 1;  // position of fine (2I, 2J) in sres
+    T g0, gp, gm;
+    if (A == 0) {
+      // transverse t1 = y, t2 = z
+      g0 = sres[s0][jl][il] + sres[s0][jl + 1][il] + sres[s1][jl][il] + sres[s1][jl + 1][il];
+      gp = sres[s0][jl][il + 1] + sres[s0][jl + 1][il + 1] + sres[s1][jl][il + 1] + sres[s1][jl + 1][il + 1];
+      gm = sres[s0][jl][il - 1] + sres[s0][jl + 1][il - 1] + sres[s1][jl][il - 1] + sres[s1][jl + 1][il - 1];
+    } else if (A == 1) {
+      // t1 = x, t2 = z
+      g0 = sres[s0][jl][il] + sres[s0][jl][il + 1] + sres[s1][jl][il] + sres[s1][jl][il + 1];
+      gp = sres[s0][jl + 1][il] + sres[s0][jl + 1][il + 1] + sres[s1][jl + 1][il] + sres[s1][jl + 1][il + 1];
+      gm = sres[s0][jl - 1][il] + sres[s0][jl - 1][il + 1] + sres[s1][jl - 1][il] + sres[s1][jl - 1][il + 1];
+    } else {
+      // t1 = x, t2 = y
+      g0 = sres[s0][jl][il] + sres[s0][jl][il + 1] + sres[s0][jl + 1][il] + sres[s0][jl + 1][il + 1];
+      gp = sres[s1][jl][il] + sres[s1][jl][il + 1] + sres[s1][jl + 1][il] + sres[s1][jl + 1][il + 1];
+      gm = sres[sm][jl][il] + sres[sm][jl][il + 1] + sres[sm][jl + 1][il] + sres[sm][jl + 1][il + 1];
+    }
+    const T v = (T)0.125 * g0 + (T)0.0625 * gp + (T)0.0625 * gm;
+    cout[(i0 / 2 + ci) + gc.s[1] * (j0 / 2 + cj) + gc.s[2] * (long)((kk - 1) / 2)] = v;
+  };
+
+  const int ks = (A == 2) ? k0 - 1 : k0, ke = k0 + tz;  // fine planes ks .. ke-1
+  T pva[NLD] = {}, pvb[NLD] = {};
+  auto step = [&](int kk, const T *pv, T *pvn) {
+    store_plane(kk + 1, pv);
+    __syncthreads();
+    if (kk + 2 <= ke) fetch_plane(kk + 2, pvn);
+    resid_plane(kk);
+    // after fine planes k0+1, k0+3, ...; not after the extra plane k0-1 of
+    // component 2
+    if (kk > k0 && ((kk - k0) & 1) == 1) {
+      __syncthreads();
+      restrict_plane(kk);
+    }
+  };
+  fetch_plane(ks - 1, pva);
+  store_plane(ks - 1, pva);
+  fetch_plane(ks, pva);
+  store_plane(ks, pva);
+  fetch_plane(ks + 1, pva);
+  for (int kk = ks; kk < ke; kk += 2) {
+    step(kk, pva, pvb);
+    if (kk + 1 < ke) step(kk + 1, pvb, pva);
+  }
+}
+
 template <typename T, int A, int VISC>
 void launch_stag_one(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, const T *pin, T *pout,
                      const T *rhs, const StagCoef<T> &K) {
@@ -466,12 +598,18 @@ void launch_stag_one(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, c
 void set_rb_variant(int v) { g_rb_variant = v; }
 void set_rb_target(int t) { g_rb_target = t > 0 ? t : 1; }
 
+// Workgroups aimed for by FP32 (es = 4) and FP64 (es = 8) launches: an FP64
+// workgroup's LDS rings allow 3 resident per CU instead of 4, and twice the
+// count keeps the last round of workgroups from running on a third of the
+// machine.
+static int rb_target(int es) { return es == 8 ? 2 * g_rb_target : g_rb_target; }
+
 // Plane count per workgroup, or 0 when the level cannot use the fused
 // sweep.  The plane-marching forms take the thickest z-slab (at most 32
-// planes) that still gives g_rb_target workgroups, down to 2 planes for the
-// paired form (8 for the first form); a level whose count stays below
+// planes) that still gives rb_target(es) workgroups, down to 2 planes for
+// the paired form (8 for the first form); a level whose count stays below
 // min_tiles is left to the colour kernels.
-int fused_rb_tz(const Grid &g, int min_tiles) {
+int fused_rb_tz(const Grid &g, int min_tiles, int es) {
   if (g_rb_variant == 1) {
     if (g.dim != 3 || g.n[0] % BX != 0 || g.n[1] % BY != 0 || g.n[2] % BZ != 0) return 0;
     const int nb = (g.n[0] / BX) * (g.n[1] / BY) * (g.n[2] / BZ);
@@ -480,7 +618,8 @@ int fused_rb_tz(const Grid &g, int min_tiles) {
   if (g.dim != 3 || g.n[0] % FX != 0 || g.n[1] % FY != 0) return 0;
   const int per_plane_set = (g.n[0] / FX) * (g.n[1] / FY);
   const int tz_min = (g_rb_variant == 2) ? 2 : 8;
-  const int target = g_rb_target > min_tiles ? g_rb_target : min_tiles;
+  const int t = rb_target(es);
+  const int target = t > min_tiles ? t : min_tiles;
   int tz = 32;
   while (tz > tz_min && (g.n[2] % tz != 0 || per_plane_set * (g.n[2] / tz) < target)) tz /= 2;
   if (g.n[2] % tz != 0 || per_plane_set * (g.n[2] / tz) < min_tiles) return 0;
@@ -488,13 +627,13 @@ int fused_rb_tz(const Grid &g, int min_tiles) {
 }
 
 bool fused_rb_ok(const Grid &g, int visc, int min_tiles) {
-  return (visc == -1 || visc == 1) && fused_rb_tz(g, min_tiles) > 0;
+  return (visc == -1 || visc == 1) && fused_rb_tz(g, min_tiles, 4) > 0;
 }
 
 template <typename T>
 void launch_stag_rb_fused(hipStream_t s, const Grid &g, int visc, int a, T omega, const T *pin, T *pout,
                           const T *rhs, const StagCoef<T> &K, int min_tiles) {
-  const int tz = fused_rb_tz(g, min_tiles);
+  const int tz = fused_rb_tz(g, min_tiles, sizeof(T));
   const T dxsqinv = (T)(1.0 / (g.dx * g.dx));
   if (visc == -1) {
     if (a == 0) launch_stag_one<T, 0, -1>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K);
@@ -516,7 +655,63 @@ void launch_cc_rb_fused(hipStream_t s, const Grid &g, const T *pin, T *pout, con
   args.ax = a[0];
   args.ay = a[1];
   args.az = a[2];
-  launch_rb<T, CCRB<T>>(s, g, fused_rb_tz(g, min_tiles), pin, pout, args);
+  launch_rb<T, CCRB<T>>(s, g, fused_rb_tz(g, min_tiles, sizeof(T)), pin, pout, args);
+}
+
+// z-slab thickness of the fused residual + restriction, or 0 when the level
+// cannot use it (3D, x a multiple of 64, y of 8, z of 2, coarse = fine / 2)
+static int resid_restrict_tz(const Grid &g, const Grid &gc, int es) {
+  if (g.dim != 3 || g.n[0] % FX != 0 || g.n[1] % FY != 0 || g.n[2] % 2 != 0) return 0;
+  for (int d = 0; d < 3; d++)
+    if (gc.n[d] * 2 != g.n[d]) return 0;
+  const int per_plane_set = (g.n[0] / FX) * (g.n[1] / FY);
+  int tz = 32;
+  while (tz > 2 && (g.n[2] % tz != 0 || per_plane_set * (g.n[2] / tz) < rb_target(es))) tz /= 2;
+  return (g.n[2] % tz == 0) ? tz : 0;
+}
+
+namespace {
+template <typename T, int A, int VISC>
+void launch_rr_one(hipStream_t s, const Grid &g, const Grid &gc, int tz, const T *phi, const T *rhs, T *crhs,
+                   const StagCoef<T> &K) {
+  using P = StagRB<T, A, VISC>;
+  typename P::Args a;
+  a.omega = (T)0;
+  a.dxsqinv = (T)(1.0 / (g.dx * g.dx));
+  a.rhs = rhs;
+  a.alpha = K.alpha[A];
+  a.beta = K.beta;
+  a.e1 = (VISC == -1) ? K.bed[A == 2 ? 1 : 0] : nullptr;
+  a.e2 = (VISC == -1) ? K.bed[A == 0 ? 1 : 2] : nullptr;
+  const T *q[5];
+  P::ptrs(a, q);
+  const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
+  const int ntiles = ntx * nty * ntz;
+  hipLaunchKernelGGL((k_resid_restrict<T, P, A>), dim3(ntiles), dim3(256), 0, s, g, gc, tz, ntx, nty, ntiles, phi,
+                     crhs, a, q[0], q[1], q[2], q[3], q[4]);
+  post_launch(s, kResidRestrictName);
+}
+}  // namespace
+
+bool stag_resid_restrict_ok(const Grid &g, const Grid &gc, int visc) {
+  return (visc == -1 || visc == 1) && resid_restrict_tz(g, gc, 4) > 0;
+}
+
+template <typename T>
+bool launch_stag_resid_restrict(hipStream_t s, const Grid &g, const Grid &gc, int visc, T *const phi[3],
+                                T *const rhs[3], T *const crhs[3], const StagCoef<T> &K) {
+  if (!stag_resid_restrict_ok(g, gc, visc)) return false;
+  const int tz = resid_restrict_tz(g, gc, sizeof(T));
+  if (visc == -1) {
+    launch_rr_one<T, 0, -1>(s, g, gc, tz, phi[0], rhs[0], crhs[0], K);
+    launch_rr_one<T, 1, -1>(s, g, gc, tz, phi[1], rhs[1], crhs[1], K);
+    launch_rr_one<T, 2, -1>(s, g, gc, tz, phi[2], rhs[2], crhs[2], K);
+  } else {
+    launch_rr_one<T, 0, 1>(s, g, gc, tz, phi[0], rhs[0], crhs[0], K);
+    launch_rr_one<T, 1, 1>(s, g, gc, tz, phi[1], rhs[1], crhs[1], K);
+    launch_rr_one<T, 2, 1>(s, g, gc, tz, phi[2], rhs[2], crhs[2], K);
+  }
+  return true;
 }
 
 template void launch_stag_rb_fused<double>(hipStream_t, const Grid &, int, int, double, const double *, double *,
@@ -525,6 +720,10 @@ template void launch_stag_rb_fused<float>(hipStream_t, const Grid &, int, int, f
                                           const float *, const StagCoef<float> &, int);
 template void launch_cc_rb_fused<double>(hipStream_t, const Grid &, const double *, double *, const double *,
                                           double *const[3], int);
+template bool launch_stag_resid_restrict<double>(hipStream_t, const Grid &, const Grid &, int, double *const[3],
+                                                 double *const[3], double *const[3], const StagCoef<double> &);
+template bool launch_stag_resid_restrict<float>(hipStream_t, const Grid &, const Grid &, int, float *const[3],
+                                                float *const[3], float *const[3], const StagCoef<float> &);
 template void launch_cc_rb_fused<float>(hipStream_t, const Grid &, const float *, float *, const float *,
                                          float *const[3], int);
 

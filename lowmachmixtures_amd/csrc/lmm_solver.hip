@@ -114,6 +114,10 @@ struct lmm_ctx {
   // them to x without widening them into FP64 buffers first (same bits);
   // LMM_Z32=0 in the environment widens them first
   bool z32_direct = true;
+  // the staggered down-leg computes the residual and restricts it in one
+  // pass where the level allows (same bits); LMM_RESID_RESTRICT=0 in the
+  // environment keeps the two passes
+  bool fused_resid_restrict = true;
   // profiling
   bool prof = false;
   std::map<std::string, ProfFamily> fam;
@@ -495,9 +499,20 @@ static void stag_vcycles(lmm_ctx *c, std::vector<SLevel<T>> &S, T *const phi0[3]
       T *const *rh = (n == 0) ? rhs0 : S[n].rhs;
       smooth<T>(c, S[n], ph, rh, p.stag_mg_nsmooths_down);
       {
-        ProfScope ps(c, "stag_restrict", 0.0);
-        launch_stag_residual<T>(c->stream, S[n].g, p.visc_type, ph, rh, S[n].res, S[n].K());
-        launch_stag_restrict<T>(c->stream, S[n + 1].g, S[n].res, S[n + 1].rhs, 0);
+        // read phi, rhs and the coefficients of every component, write the
+        // coarse right-hand side
+        const double fine = dim * comp_sweep_bytes(S[n].g, p.visc_type, sizeof(T)) - (double)dim * S[n].g.N * sizeof(T);
+        const double bytes = fine + (double)dim * S[n + 1].g.N * sizeof(T);
+        ProfScope ps(c, (n == 0) ? "stag_restrict_l0" : "stag_restrict", bytes);
+        if (c->fused_resid_restrict && p.stag_mg_smoother == 1 &&
+            stag_resid_restrict_ok(S[n].g, S[n + 1].g, p.visc_type)) {
+          ProfScope pf(c, "stag_rr_fused", bytes);
+          launch_stag_resid_restrict<T>(c->stream, S[n].g, S[n + 1].g, p.visc_type, ph, rh, S[n + 1].rhs,
+                                        S[n].K());
+        } else {
+          launch_stag_residual<T>(c->stream, S[n].g, p.visc_type, ph, rh, S[n].res, S[n].K());
+          launch_stag_restrict<T>(c->stream, S[n + 1].g, S[n].res, S[n + 1].rhs, 0);
+        }
       }
     }
     if (nc < nl) {
@@ -543,7 +558,7 @@ static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *
   auto rb = [&](int n, int sweeps) {
     T *ph = (n == 0) ? phi0 : C[n].phi;
     const T *rh = (n == 0) ? rhs0 : C[n].rhs;
-    if (c->fused && fused_rb_tz(C[n].g, c->fused_min_tiles) > 0) {
+    if (c->fused && fused_rb_tz(C[n].g, c->fused_min_tiles, 4) > 0) {
       // fused red-black sweeps, ping-pong between phi and C[n].res
       for (int m = 0; m < sweeps; m++) {
         // read phi, rhs and the dim face coefficients, write phi
@@ -1083,6 +1098,8 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
     if (fk && strcmp(fk, "march") == 0) variant = 0;
     if (fk && strcmp(fk, "block") == 0) variant = 1;
     set_rb_variant(variant);
+    const char *rr = getenv("LMM_RESID_RESTRICT");
+    c->fused_resid_restrict = !(rr && rr[0] == '0');
     const char *z3 = getenv("LMM_Z32");
     c->z32_direct = !(z3 && z3[0] == '0');
     const char *co = getenv("LMM_COARSE");
