@@ -18,6 +18,13 @@ constexpr int RED_BLOCKS = 1024;
 // synchronises the stream so a failing kernel is named in first_error().
 void set_debug_sync(int on);
 void post_launch(hipStream_t s, const char *name);
+// Kernel-exact timing: a profiling scope may leave a start / stop event pair
+// here; the next fused-sweep launch then has the dispatch itself timestamp
+// them (hipExtLaunchKernelGGL) and clears the pair.
+struct LaunchEvents {
+  hipEvent_t start, stop;
+};
+LaunchEvents *launch_events();
 const char *first_error();
 void clear_first_error();
 constexpr int RED_TPB = 256;

@@ -27,6 +27,8 @@
 // the current plane is computed.
 // Workgroups are numbered so that tiles sharing halo rows are dispatched to
 // the same XCD (consecutive workgroup ids are spread over the 8 XCDs).
+#include <hip/hip_ext.h>
+
 #include "lmm_kernels.h"
 #include "lmm_points.h"
 
@@ -437,8 +439,15 @@ void launch_rb(hipStream_t s, const Grid &g, int tz, const T *pin, T *pout, cons
   const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
   const int ntiles = ntx * nty * ntz;
   if (g_rb_variant == 2) {
-    hipLaunchKernelGGL((k_rb_march2<T, P>), dim3(ntiles), dim3(256), 0, s, g, tz, ntx, nty, ntiles, pin, pout, args,
-                       q[0], q[1], q[2], q[3], q[4]);
+    LaunchEvents *ev = launch_events();
+    if (ev->start) {
+      hipExtLaunchKernelGGL((k_rb_march2<T, P>), dim3(ntiles), dim3(256), 0, s, ev->start, ev->stop, 0, g, tz, ntx,
+                            nty, ntiles, pin, pout, args, q[0], q[1], q[2], q[3], q[4]);
+      ev->start = ev->stop = nullptr;
+    } else {
+      hipLaunchKernelGGL((k_rb_march2<T, P>), dim3(ntiles), dim3(256), 0, s, g, tz, ntx, nty, ntiles, pin, pout,
+                         args, q[0], q[1], q[2], q[3], q[4]);
+    }
     post_launch(s, kMarchName);
     return;
   }
@@ -594,6 +603,11 @@ void launch_stag_one(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, c
 }
 
 }  // namespace
+
+LaunchEvents *launch_events() {
+  static thread_local LaunchEvents ev = {nullptr, nullptr};
+  return &ev;
+}
 
 void set_rb_variant(int v) { g_rb_variant = v; }
 void set_rb_target(int t) { g_rb_target = t > 0 ? t : 1; }

@@ -164,22 +164,41 @@ static void dfree(void *p) {
 
 // ------------------------- profiling ------------------------------------
 
+// Times the launches made inside its lifetime with an event pair on the
+// context's stream.  A kernel-exact scope (exact = true) instead hands the
+// pair to the next fused-sweep launch, whose dispatch timestamps it; a pair
+// no launch took is dropped, not counted.
 struct ProfScope {
   lmm_ctx *c;
   ProfFamily *f = nullptr;
   hipEvent_t a = nullptr, b = nullptr;
-  ProfScope(lmm_ctx *ctx, const char *name, double bytes) : c(ctx) {
+  bool exact = false;
+  double bytes = 0.0;
+  ProfScope(lmm_ctx *ctx, const char *name, double nbytes, bool kernel_exact = false) : c(ctx) {
     if (!c->prof) return;
     f = &c->fam[name];
+    exact = kernel_exact;
+    bytes = nbytes;
     hipEventCreate(&a);
     hipEventCreate(&b);
-    hipEventRecord(a, c->stream);
-    f->launches++;
-    f->bytes += bytes;
+    if (exact) *launch_events() = {a, b};
+    else hipEventRecord(a, c->stream);
   }
   ~ProfScope() {
     if (!f) return;
-    hipEventRecord(b, c->stream);
+    if (exact) {
+      LaunchEvents *ev = launch_events();
+      if (ev->start == a) {  // not taken by a launch
+        *ev = {nullptr, nullptr};
+        hipEventDestroy(a);
+        hipEventDestroy(b);
+        return;
+      }
+    } else {
+      hipEventRecord(b, c->stream);
+    }
+    f->launches++;
+    f->bytes += bytes;
     f->pending.push_back({a, b});
   }
 };
@@ -414,8 +433,8 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
       T *const *src = (m % 2 == 0) ? phi : L.res;
       T *const *dst = (m % 2 == 0) ? L.res : phi;
       for (int a = 0; a < dim; a++) {
-        ProfScope ps(c, fam, comp_sweep_bytes(L.g, p.visc_type, sizeof(T)));
         ProfScope pf(c, "stag_rb_fused", comp_sweep_bytes(L.g, p.visc_type, sizeof(T)));
+        ProfScope ps(c, fam, comp_sweep_bytes(L.g, p.visc_type, sizeof(T)), true);
         launch_stag_rb_fused<T>(c->stream, L.g, p.visc_type, a, (T)p.stag_mg_omega, src[a], dst[a], rhs[a], K,
                                 c->fused_min_tiles);
       }
@@ -562,7 +581,8 @@ static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *
       // fused red-black sweeps, ping-pong between phi and C[n].res
       for (int m = 0; m < sweeps; m++) {
         // read phi, rhs and the dim face coefficients, write phi
-        ProfScope ps(c, (n == 0) ? "cc_smooth_l0" : "cc_smooth", (double)(3 + C[n].g.dim) * C[n].g.N * sizeof(T));
+        ProfScope ps(c, (n == 0) ? "cc_smooth_l0" : "cc_smooth", (double)(3 + C[n].g.dim) * C[n].g.N * sizeof(T),
+                     true);
         const T *src = (m % 2 == 0) ? ph : C[n].res;
         T *dst = (m % 2 == 0) ? C[n].res : ph;
         launch_cc_rb_fused<T>(c->stream, C[n].g, src, dst, rh, C[n].a, c->fused_min_tiles);
