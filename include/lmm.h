@@ -107,10 +107,14 @@ int lmm_ctx_synchronize(lmm_ctx *ctx);
 int lmm_ctx_nlevels(const lmm_ctx *ctx, int *nlevs_stag, int *nlevs_cc);
 /* Last HIP or argument error message of this context. */
 const char *lmm_ctx_last_error(const lmm_ctx *ctx);
-/* Kernel timing: when enabled, HIP events bracket every launch of the named
- * kernel families on the context stream.  family: "stag_smooth_l0" (finest
- * level sweeps), "stag_smooth" (coarser levels), "stag_restrict",
- * "stag_prolong", "cc_smooth", "apply_matrix", "krylov". */
+/* Kernel timing (no counterpart in the reference).  on = 1: HIP events
+ * bracket every launch of the named kernel families on the context stream;
+ * on = 2: only the fused red-black sweeps are timed, with events their own
+ * dispatch timestamps (hipExtLaunchKernelGGL), which costs the solve almost
+ * nothing; on = 0: off.  family: "stag_smooth_l0" / "cc_smooth_l0" (finest
+ * level sweeps), "stag_smooth" / "cc_smooth" (coarser levels),
+ * "stag_rb_fused", "stag_restrict_l0", "stag_restrict", "stag_rr_fused",
+ * "stag_prolong", "stag_coarse", "cc_coarse", "apply_matrix", "krylov". */
 int lmm_ctx_set_profiling(lmm_ctx *ctx, int on);
 int lmm_ctx_kernel_stats(lmm_ctx *ctx, const char *family, int64_t *launches, double *total_ms,
                          double *algorithmic_bytes);

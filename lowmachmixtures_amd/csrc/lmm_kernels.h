@@ -169,6 +169,12 @@ void launch_bmax_neg(hipStream_t s, double *tmp, const double *b, long n);      
 void launch_div_by(hipStream_t s, double *v, const double *src, long n, const double *scal);  // v = src / *scal
 void launch_axmy(hipStream_t s, double *w, const double *v, long n, const double *scal);     // w = w - v * *scal
 void launch_update_sol(hipStream_t s, double *x, double *const V[], const double *y, int i, long n);
+// w -= h[0]*v, then the Stokes dot of the new w with vn (vn = nullptr: with
+// itself) into partial, bit-identical to launch_axmy + launch_stokes_dot
+void launch_axmy_dot(hipStream_t s, const Grid &g, double *w, const double *v, const double *h, const double *vn,
+                     double *partial);
+// v = src / scal[0] and v32 = (float)v in one pass
+void launch_div_by_f32(hipStream_t s, double *v, float *v32, const double *src, long n, const double *scal);
 // x += sum_k y_k Z_k, Z in FP32 (at most 32 vectors)
 void launch_update_sol_z32(hipStream_t s, double *x, float *const Z[], const double *y, int i, long n);
 // FP64 A x of an FP32 x (3D visc_type -1 / 1); false when not available

@@ -797,6 +797,61 @@ void launch_axmy(hipStream_t s, double *w, const double *v, long n, const double
   LAUNCH(k_axmy, dim3(nblk(n)), dim3(TPB), 0, s, w, v, n, scal);
 }
 
+// One modified Gram-Schmidt step fused with the next reduction: w -= h*v
+// (k_axmy's arithmetic), then the weighted dot of the new w with vn (the next
+// H entry) or, NORM, with itself (|w|^2 for H(i+1,i)).  Cells are visited in
+// k_stokes_dot's grid-stride order and summed the same way, so the partial
+// sums are bit-identical to an axmy pass followed by a dot pass, while w is
+// read and written once.
+template <bool NORM>
+__global__ void __launch_bounds__(RED_TPB) k_axmy_dot(Grid g, double *w, const double *__restrict__ v,
+                                                      const double *__restrict__ h, const double *__restrict__ vn,
+                                                      double *partial) {
+  __shared__ double sh[8];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const double hv = h[0];
+  long stride = (long)gridDim.x * blockDim.x;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < g.N; q += stride) {
+    int i, j, k;
+    decode(g, q, i, j, k);
+    int c[3] = {i, j, k};
+    for (int d = 0; d < g.dim; d++) {
+      const long p = d * g.N + q;
+      const double wn = w[p] - v[p] * hv;
+      w[p] = wn;
+      const double wt = (c[d] == 0) ? 3.0 : 1.0;
+      acc[d] += wt * (wn * (NORM ? wn : vn[p]));
+    }
+    const long p = g.dim * g.N + q;
+    const double wn = w[p] - v[p] * hv;
+    w[p] = wn;
+    acc[3] += wn * (NORM ? wn : vn[p]);
+  }
+  for (int seg = 0; seg < 4; seg++) {
+    double r = block_sum(acc[seg], sh);
+    if (threadIdx.x == 0) partial[blockIdx.x * 4 + seg] = r;
+  }
+}
+void launch_axmy_dot(hipStream_t s, const Grid &g, double *w, const double *v, const double *h, const double *vn,
+                     double *partial) {
+  if (vn) LAUNCH(k_axmy_dot<false>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, w, v, h, vn, partial);
+  else LAUNCH(k_axmy_dot<true>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, w, v, h, vn, partial);
+}
+
+// v = src / scal (k_div_by) and its FP32 copy v32 (k_convert's rounding) in
+// one pass: the next Krylov vector and the FP32 preconditioner input.
+__global__ void k_div_by_f32(double *v, float *v32, const double *__restrict__ src, long n, const double *scal) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t < n) {
+    const double x = src[t] / scal[0];
+    v[t] = x;
+    v32[t] = (float)x;
+  }
+}
+void launch_div_by_f32(hipStream_t s, double *v, float *v32, const double *src, long n, const double *scal) {
+  LAUNCH(k_div_by_f32, dim3(nblk(n)), dim3(TPB), 0, s, v, v32, src, n, scal);
+}
+
 struct VList {
   const double *v[32];
 };

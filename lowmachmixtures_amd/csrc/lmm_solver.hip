@@ -118,9 +118,14 @@ struct lmm_ctx {
   // pass where the level allows (same bits); LMM_RESID_RESTRICT=0 in the
   // environment keeps the two passes
   bool fused_resid_restrict = true;
-  // profiling
-  bool prof = false;
+  // GMRES fuses each MGS axpy with the next reduction and the basis scaling
+  // with the FP32 conversion of the preconditioner input (same bits);
+  // LMM_KRYLOV_FUSED=0 in the environment keeps separate passes
+  bool fused_krylov = true;
+  // profiling: 0 off, 1 every family, 2 only the kernel-exact fused sweeps
+  int prof = 0;
   std::map<std::string, ProfFamily> fam;
+  std::vector<hipEvent_t> evpool;  // events reused across scopes
   std::string err;
   int vcycles = 0;
 };
@@ -175,12 +180,12 @@ struct ProfScope {
   bool exact = false;
   double bytes = 0.0;
   ProfScope(lmm_ctx *ctx, const char *name, double nbytes, bool kernel_exact = false) : c(ctx) {
-    if (!c->prof) return;
+    if (!c->prof || (c->prof == 2 && !kernel_exact)) return;
     f = &c->fam[name];
     exact = kernel_exact;
     bytes = nbytes;
-    hipEventCreate(&a);
-    hipEventCreate(&b);
+    a = take();
+    b = take();
     if (exact) *launch_events() = {a, b};
     else hipEventRecord(a, c->stream);
   }
@@ -190,8 +195,8 @@ struct ProfScope {
       LaunchEvents *ev = launch_events();
       if (ev->start == a) {  // not taken by a launch
         *ev = {nullptr, nullptr};
-        hipEventDestroy(a);
-        hipEventDestroy(b);
+        c->evpool.push_back(a);
+        c->evpool.push_back(b);
         return;
       }
     } else {
@@ -200,6 +205,16 @@ struct ProfScope {
     f->launches++;
     f->bytes += bytes;
     f->pending.push_back({a, b});
+  }
+  hipEvent_t take() {
+    if (c->evpool.empty()) {
+      hipEvent_t e;
+      hipEventCreate(&e);
+      return e;
+    }
+    hipEvent_t e = c->evpool.back();
+    c->evpool.pop_back();
+    return e;
   }
 };
 
@@ -210,8 +225,8 @@ static void prof_flush(lmm_ctx *c) {
       float ms = 0.f;
       hipEventElapsedTime(&ms, ev.first, ev.second);
       kv.second.ms += ms;
-      hipEventDestroy(ev.first);
-      hipEventDestroy(ev.second);
+      c->evpool.push_back(ev.first);
+      c->evpool.push_back(ev.second);
     }
     kv.second.pending.clear();
   }
@@ -729,6 +744,33 @@ static void stokes_dot_dev(lmm_ctx *c, const double *a, const double *b, double 
   launch_finalize(c->stream, c->g, c->partial, slot, op, c->prm.p_norm_weight);
 }
 
+// Modified Gram-Schmidt of w against V_0..V_{i-1} (gmres.f90:338-358): the
+// entries H(1..i, i) into H[0..i-1] and |w| (with its velocity and pressure
+// parts) into nrm[0..2] (device slots).  Fused form: the first dot, then each
+// axmy fused with the next dot or, last, with |w|^2; the same arithmetic and
+// summation order as the separate passes, so the same bits.
+template <typename VF>
+static void mgs(lmm_ctx *c, const VF &Vk, int i, double *H, double *nrm) {
+  hipStream_t s = c->stream;
+  const long L = (long)(c->g.dim + 1) * c->g.N;
+  if (!c->fused_krylov) {
+    for (int k = 1; k <= i; k++) {
+      stokes_dot_dev(c, c->w, Vk(k - 1), H + k - 1, 0);
+      ProfScope ps(c, "krylov", (double)L * 8.0 * 3);
+      launch_axmy(s, c->w, Vk(k - 1), L, H + k - 1);
+    }
+    stokes_dot_dev(c, c->w, c->w, nrm, 1);
+    return;
+  }
+  stokes_dot_dev(c, c->w, Vk(0), H, 0);
+  for (int k = 1; k <= i; k++) {
+    const bool last = (k == i);
+    ProfScope ps(c, "krylov", (double)L * 8.0 * (last ? 3 : 4));
+    launch_axmy_dot(s, c->g, c->w, Vk(k - 1), H + k - 1, last ? nullptr : Vk(k), c->partial);
+    launch_finalize(s, c->g, c->partial, last ? nrm : H + k, last ? 1 : 0, c->prm.p_norm_weight);
+  }
+}
+
 static void fetch(lmm_ctx *c, const double *dev, int n, double *host) {
   hipMemcpyAsync(c->hscal, dev, sizeof(double) * n, hipMemcpyDeviceToHost, c->stream);
   hipStreamSynchronize(c->stream);
@@ -902,16 +944,11 @@ static int gmres_ref(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
       SV vi = sv_of(c, Vk(i - 1));
       apply_matrix_dev(c, vi, tmp);
       precon64(c, tmp, w, theta_alpha);
-      for (int k = 1; k <= i; k++) {
-        double *slot = dH + (k - 1) + (long)(m + 1) * (i - 1);
-        stokes_dot_dev(c, c->w, Vk(k - 1), slot, 0);
-        ProfScope ps(c, "krylov", (double)L * 8.0 * 3);
-        launch_axmy(s, c->w, Vk(k - 1), L, slot);
-      }
+      // H(1..i, i), then H(i+1,i) = |w|
+      double *col = dH + (long)(m + 1) * (i - 1);
       double *sl = c->scal + 3100;
-      stokes_dot_dev(c, c->w, c->w, sl, 1);
-      // H(i+1,i) slot
-      hipMemcpyAsync(dH + i + (long)(m + 1) * (i - 1), sl, sizeof(double), hipMemcpyDeviceToDevice, s);
+      mgs(c, Vk, i, col, sl);
+      hipMemcpyAsync(col + i, sl, sizeof(double), hipMemcpyDeviceToDevice, s);
       launch_div_by(s, Vk(i), c->w, L, sl);
       fetch(c, dH + (long)(m + 1) * (i - 1), i + 1, &hs.h(1, i));
       if (hs.h(i + 1, i) == 0.0) {
@@ -951,10 +988,10 @@ static int gmres_ref(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
 // Flexible GMRES with FP32 V-cycles (right preconditioned, MGS, FP64 outer)
 // ---------------------------------------------------------------------------
 
-static void precon32(lmm_ctx *c, const double *v64, float *z32, double theta_alpha) {
+static void precon32(lmm_ctx *c, const double *v64, float *z32, double theta_alpha, bool have32) {
   hipStream_t s = c->stream;
   long L = (long)(c->g.dim + 1) * c->g.N;
-  launch_convert<float, double>(s, c->b32, v64, L);
+  if (!have32) launch_convert<float, double>(s, c->b32, v64, L);
   SVT<float> b = svt_of<float>(c, c->b32), z = svt_of<float>(c, z32);
   apply_precon_t<float>(c, c->s32, c->c32, b.u, b.p, z.u, z.p, c->mac_rhs32, c->c32[0].phi, theta_alpha);
 }
@@ -996,24 +1033,24 @@ static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
     }
     if (total_iter >= p.gmres_max_iter || iter >= p.gmres_max_outer) break;
     iter++;
-    launch_div_by(s, Vk(0), c->tmp, L, c->scal + 3008);
+    if (c->fused_krylov) launch_div_by_f32(s, Vk(0), c->b32, c->tmp, L, c->scal + 3008);
+    else launch_div_by(s, Vk(0), c->tmp, L, c->scal + 3008);
     std::fill(hs.s.begin(), hs.s.end(), 0.0);
     hs.s[0] = rnorm;
     int i, used = 0;
     for (i = 1; i <= m; i++) {
       total_iter++;
       used = i;
-      precon32(c, Vk(i - 1), Zk(i - 1), theta_alpha);
+      // b32 already holds V(i-1) in FP32 when the scaling that made V(i-1)
+      // wrote it too
+      precon32(c, Vk(i - 1), Zk(i - 1), theta_alpha, c->fused_krylov);
       apply_matrix_z32(c, Zk(i - 1), w);
-      for (int k = 1; k <= i; k++) {
-        double *slot = dH + (k - 1) + (long)(m + 1) * (i - 1);
-        stokes_dot_dev(c, c->w, Vk(k - 1), slot, 0);
-        launch_axmy(s, c->w, Vk(k - 1), L, slot);
-      }
+      double *col = dH + (long)(m + 1) * (i - 1);
       double *sl = c->scal + 3100;
-      stokes_dot_dev(c, c->w, c->w, sl, 1);
-      hipMemcpyAsync(dH + i + (long)(m + 1) * (i - 1), sl, sizeof(double), hipMemcpyDeviceToDevice, s);
-      launch_div_by(s, Vk(i), c->w, L, sl);
+      mgs(c, Vk, i, col, sl);
+      hipMemcpyAsync(col + i, sl, sizeof(double), hipMemcpyDeviceToDevice, s);
+      if (c->fused_krylov) launch_div_by_f32(s, Vk(i), c->b32, c->w, L, sl);
+      else launch_div_by(s, Vk(i), c->w, L, sl);
       fetch(c, dH + (long)(m + 1) * (i - 1), i + 1, &hs.h(1, i));
       bool happy = (hs.h(i + 1, i) == 0.0);  // exact solution in the space
       hs.least_squares(i);
@@ -1120,6 +1157,8 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
     set_rb_variant(variant);
     const char *rr = getenv("LMM_RESID_RESTRICT");
     c->fused_resid_restrict = !(rr && rr[0] == '0');
+    const char *kf = getenv("LMM_KRYLOV_FUSED");
+    c->fused_krylov = !(kf && kf[0] == '0');
     const char *z3 = getenv("LMM_Z32");
     c->z32_direct = !(z3 && z3[0] == '0');
     const char *co = getenv("LMM_COARSE");
@@ -1196,6 +1235,7 @@ int lmm_ctx_destroy(lmm_ctx *c) {
   if (!c) return LMM_ERR_ARG;
   if (c->stream) hipStreamSynchronize(c->stream);
   prof_flush(c);
+  for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   free_levels<double>(c->s64, c->c64, false);
   free_levels<float>(c->s32, c->c32, true);
   for (void *p : {(void *)c->partial, (void *)c->partial2, (void *)c->scal, (void *)c->scaled_beta,
@@ -1253,8 +1293,8 @@ int lmm_ctx_nlevels(const lmm_ctx *c, int *ns, int *nc) {
 const char *lmm_ctx_last_error(const lmm_ctx *c) { return c ? c->err.c_str() : "null context"; }
 
 int lmm_ctx_set_profiling(lmm_ctx *c, int on) {
-  if (!c) return LMM_ERR_ARG;
-  c->prof = on != 0;
+  if (!c || on < 0 || on > 2) return LMM_ERR_ARG;
+  c->prof = on;
   return LMM_OK;
 }
 

@@ -176,7 +176,9 @@ class Context:
         self.check(lib().lmm_ctx_synchronize(self.h), "synchronize")
 
     def profiling(self, on=True):
-        lib().lmm_ctx_set_profiling(self.h, int(on))
+        """on: False/0 off, True/1 every kernel family, 2 only the fused sweeps
+        (kernel-exact events, negligible cost)."""
+        self.check(lib().lmm_ctx_set_profiling(self.h, int(on)), "lmm_ctx_set_profiling")
 
     def reset_stats(self):
         lib().lmm_ctx_reset_stats(self.h)

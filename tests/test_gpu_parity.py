@@ -346,3 +346,26 @@ def test_fgmres_fp32_vectors_used_directly_equal_widened(visc, monkeypatch):
             out.append((info["total_iter"], dp.solution()))
     assert out[0][0] == out[1][0]
     assert bit_equal(out[0][1][0], out[1][1][0]) and bit_equal(out[0][1][1], out[1][1][1])
+
+
+@pytest.mark.parametrize("dim,n,prec", [(3, (64, 32, 32), "fp64"), (3, (64, 32, 32), "fp32_mg"),
+                                        (2, (128, 64), "fp64"), (2, (128, 64), "fp32_mg")])
+def test_fused_krylov_equals_separate_passes(dim, n, prec, monkeypatch):
+    """GMRES fuses each modified Gram-Schmidt axpy with the next dot (or the
+    norm) and the basis scaling with the FP32 conversion of the next
+    preconditioner input; LMM_KRYLOV_FUSED=0 keeps the separate passes.  The
+    arithmetic and the summation order are the same, so the solutions, the
+    iteration counts and the residual estimates agree bit for bit."""
+    pb = P.small_problem(dim, n, visc_type=-1, seed=97, contrast=30.0)
+    b_u = P.rhs_from_noise_host(pb)
+    precision = {"fp64": S.FP64, "fp32_mg": S.FP32_MG}[prec]
+    out = []
+    for mode in ("0", "1"):
+        monkeypatch.setenv("LMM_KRYLOV_FUSED", mode)
+        with S.Context(dim, pb.n, pb.dx, visc_type=-1, precision=precision, gmres_max_inner=4) as ctx:
+            dp = S.DeviceProblem(ctx, pb, rhs=b_u)
+            _, info = dp.solve()
+            out.append((info["total_iter"], info["norm_resid"], dp.solution()))
+    assert out[0][0] == out[1][0] and out[0][0] > 4
+    assert out[0][1] == out[1][1]
+    assert bit_equal(out[0][2][0], out[1][2][0]) and bit_equal(out[0][2][1], out[1][2][1])
