@@ -225,6 +225,17 @@ int lmm_init_m_stochastic(lmm_ctx *ctx, int n_rngs);
  * inverse normal), in the order: stage, W_cc components, then node/edge
  * arrays and their 2 components, x fastest. */
 int lmm_fill_m_stochastic(lmm_ctx *ctx, uint32_t seed);
+/* No counterpart in the reference, whose noise comes from the host MT19937
+ * stream above: fills every stage of the context's noise on the device, in
+ * the same value order, with N(0,1) values of the counter-based stream
+ * (seed, step) -- Philox4x32-10 (Salmon et al., SC'11) of the counter
+ * (value/2, step) under the key seed, Box-Muller pairs.  For callers that
+ * draw noise every time step without a host round trip; each (seed, step)
+ * gives an independent, reproducible field. */
+int lmm_fill_m_stochastic_device(lmm_ctx *ctx, uint64_t seed, uint64_t step);
+/* The same stream into a field: values v0, v0+1, ... in storage order
+ * (component, then x fastest).  No counterpart in the reference. */
+int lmm_field_fill_normal(lmm_field *f, uint64_t seed, uint64_t step, int64_t v0);
 /* Same divergence with caller-provided noise: W_cc has dim components, W_ed
  * has 2 components per node (2D, W_ed[0]) or edge array (3D). */
 int lmm_stoch_m_force(lmm_ctx *ctx, lmm_field *stoch_m_force[3], lmm_field *W_cc,

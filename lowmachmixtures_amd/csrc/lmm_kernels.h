@@ -175,6 +175,9 @@ void launch_axmy_dot(hipStream_t s, const Grid &g, double *w, const double *v, c
                      double *partial);
 // v = src / scal[0] and v32 = (float)v in one pass
 void launch_div_by_f32(hipStream_t s, double *v, float *v32, const double *src, long n, const double *scal);
+// values v0 .. v0+count-1 of the counter-based Gaussian stream (seed, step)
+// into out (lmm_rng.hip)
+void launch_fill_normal_philox(hipStream_t s, double *out, long count, long v0, uint64_t seed, uint64_t step);
 // x += sum_k y_k Z_k, Z in FP32 (at most 32 vectors)
 void launch_update_sol_z32(hipStream_t s, double *x, float *const Z[], const double *y, int i, long n);
 // FP64 A x of an FP32 x (3D visc_type -1 / 1); false when not available

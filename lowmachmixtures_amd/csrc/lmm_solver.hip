@@ -1804,6 +1804,27 @@ int lmm_fill_m_stochastic(lmm_ctx *c, uint32_t seed) {
   return LMM_OK;
 }
 
+int lmm_fill_m_stochastic_device(lmm_ctx *c, uint64_t seed, uint64_t step) {
+  if (!c || c->n_rngs < 1) return LMM_ERR_ARG;
+  const long N = c->g.N;
+  const int dim = c->g.dim, ne = ne_of(dim);
+  const long per = (long)(dim + 2 * ne) * N;
+  // the value order of lmm_fill_m_stochastic: stage, W_cc components, then
+  // the node / edge arrays with their 2 components
+  for (int r = 0; r < c->n_rngs; r++) {
+    launch_fill_normal_philox(c->stream, c->Wcc + (long)r * dim * N, dim * N, r * per, seed, step);
+    launch_fill_normal_philox(c->stream, c->Wed + (long)r * ne * 2 * N, ne * 2 * N, r * per + dim * N, seed, step);
+  }
+  return finish(c);
+}
+
+int lmm_field_fill_normal(lmm_field *f, uint64_t seed, uint64_t step, int64_t v0) {
+  if (!f || v0 < 0) return LMM_ERR_ARG;
+  lmm_ctx *c = f->ctx;
+  launch_fill_normal_philox(c->stream, f->d, kind_count(c) * f->ncomp, (long)v0, seed, step);
+  return finish(c);
+}
+
 static int stoch_common(lmm_ctx *c, lmm_field *force[3], const double *Wcc, double *const Wed[3], int nst,
                         const double *w_dev, long scc, long sed, lmm_field *eta, lmm_field *const eta_ed[3],
                         lmm_field *temp, lmm_field *const temp_ed[3], double variance) {

@@ -101,6 +101,8 @@ _SIGS = {
                                            C.POINTER(C.c_double), C.c_double, C.c_double]),
     "lmm_init_m_stochastic": (C.c_int, [C.c_void_p, C.c_int]),
     "lmm_fill_m_stochastic": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "lmm_fill_m_stochastic_device": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
+    "lmm_field_fill_normal": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int64]),
     "lmm_stoch_m_force": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, C.c_double]),
     "lmm_diffusive_mass_fluxdiv": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP * 3,
                                              _FP * 3, C.c_int]),
@@ -245,6 +247,13 @@ class Field:
         self.ctx.check(lib().lmm_field_download(self.h, out.ctypes.data_as(C.POINTER(C.c_double))), "download")
         return out
 
+    def fill_normal(self, seed, step=0, v0=0):
+        """Values v0, v0+1, ... of the counter-based Gaussian stream (seed, step)
+        in storage order."""
+        self.ctx.check(lib().lmm_field_fill_normal(self.h, C.c_uint64(seed), C.c_uint64(step), C.c_int64(v0)),
+                       "field_fill_normal")
+        return self
+
     def upload_fab(self, a, comp, lo, hi, ng):
         a = np.asfortranarray(a, dtype=np.float64)
         lo3 = (C.c_int * 3)(*(list(lo) + [0] * (3 - len(lo))))
@@ -368,6 +377,13 @@ def init_m_stochastic(ctx, n_rngs):
 
 def fill_m_stochastic(ctx, seed):
     ctx.check(lib().lmm_fill_m_stochastic(ctx.h, C.c_uint32(seed)), "fill_m_stochastic")
+
+
+def fill_m_stochastic_device(ctx, seed, step=0):
+    """Device fill of the context's noise from the counter-based stream (no
+    counterpart in the reference, whose noise is the host MT19937 stream)."""
+    ctx.check(lib().lmm_fill_m_stochastic_device(ctx.h, C.c_uint64(seed), C.c_uint64(step)),
+              "fill_m_stochastic_device")
 
 
 def stochastic_m_fluxdiv(ctx, stoch_m_force, eta, eta_ed, temperature, temperature_ed, dt, weights,

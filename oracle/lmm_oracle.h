@@ -157,6 +157,9 @@ double orc_mt_uniform(orc_mt *s);
 double orc_inv_norm_dist(double p);
 double orc_mt_normal(orc_mt *s);
 void orc_fill_normal(uint32_t seed, int64_t count, double *out);
+/* Philox4x32-10 block and the counter-based Gaussian stream (orc_rng.c) */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
+void orc_fill_normal_philox(uint64_t seed, uint64_t step, int64_t v0, int64_t count, double *out);
 
 #ifdef __cplusplus
 }

@@ -396,6 +396,23 @@ def fill_normal(seed, count):
     return out
 
 
+def philox4x32_10(ctr, key):
+    """One Philox4x32-10 block (orc_rng.c): four 32-bit words."""
+    c = (C.c_uint32 * 4)(*[int(x) & 0xFFFFFFFF for x in ctr])
+    k = (C.c_uint32 * 2)(*[int(x) & 0xFFFFFFFF for x in key])
+    o = (C.c_uint32 * 4)()
+    lib().orc_philox4x32_10(c, k, o)
+    return [int(x) for x in o]
+
+
+def fill_normal_philox(seed, step, v0, count):
+    """Values v0 .. v0+count-1 of the counter-based Gaussian stream (orc_rng.c)."""
+    out = np.empty(int(count), dtype=np.float64)
+    lib().orc_fill_normal_philox(C.c_uint64(int(seed)), C.c_uint64(int(step)), C.c_int64(int(v0)),
+                                 C.c_int64(int(count)), out.ctypes.data_as(DPTR))
+    return out
+
+
 def inv_norm_dist(p):
     return lib().orc_inv_norm_dist(C.c_double(p))
 

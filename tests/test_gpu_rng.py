@@ -1,0 +1,70 @@
+"""The device fill of the noise fields from the counter-based Gaussian stream
+(Philox4x32-10 + Box-Muller), which has no counterpart in the reference (its
+noise is the host MT19937 stream, checked in test_oracle_rng.py and
+test_gpu_parity.py).  The device values follow the oracle's restatement of
+the same stream: the Philox words are integers and agree exactly, only the
+log / cos / sin of the Box-Muller transform may differ in the last bits, so
+the tolerance is |device - oracle| <= 1e-14 (1 + |value|)."""
+import numpy as np
+import pytest
+from scipy import stats
+
+from lowmachmixtures_amd import stokes as S
+from oracle import oracle as O
+from tests.helpers import bit_equal
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-14
+
+
+@pytest.mark.parametrize("seed,step,v0", [(0, 0, 0), (2024, 7, 1), (0xDEADBEEF12345678, 1 << 40, 12345)])
+def test_field_fill_normal_matches_oracle(seed, step, v0):
+    n = (32, 16, 8)
+    with S.Context(3, n, 1.0) as ctx:
+        f = ctx.field(S.CELL, 3).fill_normal(seed, step, v0)
+        got = f.download().ravel()
+    want = O.fill_normal_philox(seed, step, v0, got.size)
+    assert np.all(np.abs(got - want) <= TOL * (1.0 + np.abs(want)))
+    assert np.mean(got == want) > 0.5  # most values are bit-equal
+
+
+def test_device_stream_is_standard_normal():
+    with S.Context(3, (128, 128, 64), 1.0) as ctx:
+        z = ctx.field(S.CELL).fill_normal(99, 3).download().ravel()
+    n = z.size
+    assert abs(z.mean()) < 5.0 / np.sqrt(n)
+    assert abs(z.var() - 1.0) < 5.0 * np.sqrt(2.0 / n)
+    assert stats.kstest(z[: 1 << 20], "norm").pvalue > 1e-3
+
+
+@pytest.mark.parametrize("dim,n", [(3, (16, 8, 8)), (2, (32, 16))])
+def test_fill_m_stochastic_device_layout(dim, n):
+    """The context's noise after the device fill equals explicit noise fields
+    filled from the same stream at the value offsets of the documented order
+    (stage, W_cc components, then node / edge arrays with 2 components)."""
+    shape = tuple(reversed(n))
+    N = int(np.prod(n))
+    ne = 1 if dim == 2 else 3
+    per = (dim + 2 * ne) * N
+    ones = np.ones(shape)
+    with S.Context(dim, n, 0.5) as ctx:
+        S.init_m_stochastic(ctx, 2)
+        S.fill_m_stochastic_device(ctx, 5, 11)
+        eta = ctx.field(S.CELL, 1, ones)
+        ed = ctx.edges([ones] * ne)
+        variance = np.sqrt(1.0 * 2.0 * 1.0 / (0.5 ** dim * 0.1))
+        for r in range(2):
+            w = [0.0, 0.0]
+            w[r] = 1.0
+            f1 = ctx.faces([np.zeros(shape)] * dim)
+            S.stochastic_m_fluxdiv(ctx, f1, eta, ed, eta, ed, dt=0.1, weights=w, variance_coef_mom=1.0, k_B=1.0)
+            got = [f.download() for f in f1]
+            wcc = ctx.field(S.CELL, dim).fill_normal(5, 11, r * per)
+            wed = ctx.edges(ncomp=2)
+            for e in range(ne):
+                wed[e].fill_normal(5, 11, r * per + dim * N + 2 * N * e)
+            f2 = ctx.faces([np.zeros(shape)] * dim)
+            S.stoch_m_force(ctx, f2, wcc, wed, eta, ed, eta, ed, variance)
+            want = [f.download() for f in f2]
+            assert bit_equal(got, want)
