@@ -154,6 +154,27 @@ int lmm_gmres(lmm_ctx *ctx, lmm_field *const b_u[3], lmm_field *b_p, lmm_field *
               lmm_field *const beta_ed[3], lmm_field *gamma, double theta_alpha,
               double *norm_pre_rhs, lmm_gmres_info *info);
 
+/* One member of a batched solve: the arguments of lmm_gmres for one
+ * context, and its results. */
+typedef struct {
+  lmm_ctx *ctx;
+  lmm_field *b_u[3], *b_p, *x_u[3], *x_p;
+  lmm_field *alpha_fc[3], *beta, *beta_ed[3], *gamma;
+  double theta_alpha;
+  double norm_pre_rhs;   /* out */
+  lmm_gmres_info info;   /* out */
+  int status;            /* out: what lmm_gmres returned for this member */
+} lmm_gmres_member;
+
+/* Ensembles of independent solves (the reference runs one gmres call per
+ * ensemble member, each on its own MPI ranks): n members, each with its own
+ * context (contexts must be distinct), solved concurrently by up to
+ * nthreads host threads (0: one per member, at most 16), each driving its
+ * context's stream, so the members' kernels and host round trips overlap on
+ * the GPU.  Every member's result equals lmm_gmres on it alone.  Returns
+ * LMM_OK when every member did, else the first member's nonzero status. */
+int lmm_gmres_batched(int n, lmm_gmres_member *members, int nthreads);
+
 /* apply_matrix.f90:23-39  apply_matrix(mla,b_u,b_p,x_u,x_p,alpha_fc,beta,beta_ed,gamma,
  *                                       theta_alpha,dx,the_bc_tower)
  * [b_u; b_p] = [A G; -D 0][x_u; x_p], A = theta_alpha*alpha - L_beta. */

@@ -13,7 +13,8 @@ namespace lmm {
 constexpr int TPB = 256;
 
 static int g_debug_sync = 0;
-static char g_first_error[256] = {0};
+// per host thread: each thread drives its own contexts (lmm_gmres_batched)
+static thread_local char g_first_error[256] = {0};
 void set_debug_sync(int on) { g_debug_sync = on; }
 const char *first_error() { return g_first_error; }
 void clear_first_error() { g_first_error[0] = 0; }

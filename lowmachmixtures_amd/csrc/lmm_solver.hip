@@ -5,7 +5,9 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <atomic>
 #include <cstdlib>
+#include <thread>
 #include <cmath>
 #include <cstdarg>
 #include <cstring>
@@ -1518,6 +1520,35 @@ int lmm_gmres(lmm_ctx *c, lmm_field *const b_u[3], lmm_field *b_p, lmm_field *x_
   info->vcycles = c->vcycles;
   info->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   return rc ? rc : rc2;
+}
+
+int lmm_gmres_batched(int n, lmm_gmres_member *m, int nthreads) {
+  if (n < 0 || (n > 0 && !m) || nthreads < 0) return LMM_ERR_ARG;
+  for (int i = 0; i < n; i++) {
+    if (!m[i].ctx) return LMM_ERR_ARG;
+    for (int j = 0; j < i; j++)
+      if (m[j].ctx == m[i].ctx) return LMM_ERR_ARG;  // a context is not re-entrant
+  }
+  int nt = nthreads > 0 ? nthreads : std::min(n, 16);
+  nt = std::max(1, std::min(nt, n));
+  std::atomic<int> next(0);
+  int device = 0;
+  hipGetDevice(&device);
+  auto work = [&]() {
+    hipSetDevice(device);  // the caller's device, where the contexts live
+    for (int i; (i = next.fetch_add(1)) < n;) {
+      lmm_gmres_member &a = m[i];
+      a.status = lmm_gmres(a.ctx, a.b_u, a.b_p, a.x_u, a.x_p, a.alpha_fc, a.beta, a.beta_ed, a.gamma, a.theta_alpha,
+                           &a.norm_pre_rhs, &a.info);
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; t++) pool.emplace_back(work);
+  work();
+  for (auto &th : pool) th.join();
+  for (int i = 0; i < n; i++)
+    if (m[i].status != LMM_OK) return m[i].status;
+  return LMM_OK;
 }
 
 int lmm_apply_matrix(lmm_ctx *c, lmm_field *b_u[3], lmm_field *b_p, lmm_field *const x_u[3], lmm_field *x_p,

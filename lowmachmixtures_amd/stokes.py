@@ -56,6 +56,16 @@ class GmresInfo(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class GmresMember(C.Structure):
+    """lmm_gmres_member: one solve of a batch."""
+    _fields_ = [
+        ("ctx", C.c_void_p), ("b_u", C.c_void_p * 3), ("b_p", C.c_void_p), ("x_u", C.c_void_p * 3),
+        ("x_p", C.c_void_p), ("alpha_fc", C.c_void_p * 3), ("beta", C.c_void_p), ("beta_ed", C.c_void_p * 3),
+        ("gamma", C.c_void_p), ("theta_alpha", C.c_double), ("norm_pre_rhs", C.c_double), ("info", GmresInfo),
+        ("status", C.c_int),
+    ]
+
+
 # (name, restype, argtypes); every entry point of include/lmm.h
 _FP = C.c_void_p
 _SIGS = {
@@ -101,6 +111,7 @@ _SIGS = {
                                            C.POINTER(C.c_double), C.c_double, C.c_double]),
     "lmm_init_m_stochastic": (C.c_int, [C.c_void_p, C.c_int]),
     "lmm_fill_m_stochastic": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "lmm_gmres_batched": (C.c_int, [C.c_int, C.POINTER(GmresMember), C.c_int]),
     "lmm_fill_m_stochastic_device": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
     "lmm_field_fill_normal": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64, C.c_int64]),
     "lmm_stoch_m_force": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, C.c_double]),
@@ -308,6 +319,32 @@ def gmres(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, theta_alpha):
                          _h(gamma), C.c_double(theta_alpha), C.byref(npr), C.byref(info))
     ctx.check(rc, "gmres")
     return npr.value, info.as_dict()
+
+
+def gmres_batched(problems, nthreads=0):
+    """Independent solves of several DeviceProblems (each with its own
+    context) at once: lmm_gmres_batched.  Returns [(status, norm_pre_rhs,
+    info dict)] in order; raises LmmError when any member failed."""
+    n = len(problems)
+    arr = (GmresMember * n)()
+    for m, dp in zip(arr, problems):
+        m.ctx = dp.ctx.h
+        for d in range(3):
+            m.b_u[d] = dp.b_u[d].h if d < len(dp.b_u) else None
+            m.x_u[d] = dp.x_u[d].h if d < len(dp.x_u) else None
+            m.alpha_fc[d] = dp.alpha[d].h if d < len(dp.alpha) else None
+            m.beta_ed[d] = dp.beta_ed[d].h if d < len(dp.beta_ed) else None
+        m.b_p, m.x_p, m.beta = dp.b_p.h, dp.x_p.h, dp.beta.h
+        m.gamma = _h(dp.gamma)
+        m.theta_alpha = dp.pb.theta_alpha
+    rc = lib().lmm_gmres_batched(n, arr, int(nthreads))
+    out = [(m.status, m.norm_pre_rhs, m.info.as_dict()) for m in arr]
+    if rc != 0:
+        bad = [i for i, m in enumerate(arr) if m.status != 0]
+        if not bad:  # rejected before any member ran (e.g. a shared context)
+            raise LmmError(f"gmres_batched: {STATUS.get(rc, rc)}")
+        problems[bad[0]].ctx.check(arr[bad[0]].status, f"gmres_batched member {bad[0]}")
+    return out
 
 
 def apply_matrix(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, theta_alpha):
