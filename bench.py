@@ -15,15 +15,16 @@ the plain FP64 time, iteration counts, the true relative residual of each
 and the FP32-MG error against the FP64 solution sit beside it.
 
 ``roofline`` is for the dominant kernel, the finest-level fused red-black
-sweep of one face component (k_rb_march2) in the reported form: its
-algorithmic bytes per launch (read phi, rhs, alpha of the component,
-beta_cc and the two edge viscosities touching it, write phi: 7 arrays of
-256^3 values) over the launch durations that HIP events on the solver's
-stream measure inside the timed region.  ``traffic`` is the HBM traffic per
-launch that rocprofv3's FETCH_SIZE and WRITE_SIZE counters measured for the
-same kernel on the same workload (FETCH_SIZE scaled by 2, the factor that
-makes it read the true byte count of a plain conversion kernel); it is read
-from the committed counter summary and is null when that file is absent.
+sweep (k_rb_march2x3: the three face components in one launch) in the
+reported form: its algorithmic bytes per launch (read phi, rhs and alpha of
+each component, beta_cc and the three edge viscosities once, write phi of
+each component: 16 arrays of 256^3 values) over the launch durations that
+the dispatch itself timestamps on the solver's stream inside the timed
+region.  ``traffic`` is the HBM traffic per launch that rocprofv3's
+FETCH_SIZE and WRITE_SIZE counters measured for the same kernel on the same
+workload (FETCH_SIZE scaled by 2, the factor that makes it read the true byte
+count of a plain conversion kernel); it is read from the committed counter
+summary and is null when that file is absent.
 
 ``cpu_baseline`` is the CPU oracle (the FP64 restatement of the reference
 algorithm, OpenMP) solving the same problem family at a bounded size (128^3
@@ -242,7 +243,7 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"k_rb_march2 fused red-black sweep of one face component, level 0 ({head})",
+            "kernel": f"k_rb_march2x3 fused red-black sweep of the three face components, level 0 ({head})",
             "achieved": sm["achieved_GBs"],
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",

@@ -27,6 +27,8 @@ struct LaunchEvents {
 LaunchEvents *launch_events();
 const char *first_error();
 void clear_first_error();
+// records a host-side launch error (first one wins) for the next status check
+void report_error(const char *msg);
 constexpr int RED_TPB = 256;
 
 template <typename T>
@@ -42,12 +44,19 @@ void set_rb_variant(int v);
 // workgroup count the plane-marching forms aim for when they choose their
 // z-slab thickness
 void set_rb_target(int t);
-// z-slab thickness for element size es (4 or 8 bytes), 0: not eligible
-int fused_rb_tz(const Grid &g, int min_tiles, int es);
+// z-slab thickness for element size es (4 or 8 bytes) when one launch
+// sweeps ncomp components, 0: not eligible
+int fused_rb_tz(const Grid &g, int min_tiles, int es, int ncomp);
 bool fused_rb_ok(const Grid &g, int visc, int min_tiles);
 template <typename T>
 void launch_stag_rb_fused(hipStream_t s, const Grid &g, int visc, int a, T omega, const T *pin, T *pout,
                           const T *rhs, const StagCoef<T> &K, int min_tiles);
+// All three components of one fused sweep in one launch (3D, visc -1 / 1,
+// the paired plane-marching form); reads pin[a], writes pout[a].
+bool fused_rb3_ok(const Grid &g, int visc, int min_tiles);
+template <typename T>
+void launch_stag_rb_fused3(hipStream_t s, const Grid &g, int visc, T omega, T *const pin[3], T *const pout[3],
+                           T *const rhs[3], const StagCoef<T> &K, int min_tiles);
 // Residual of every face component and its full-weighting restriction into
 // the coarse right-hand side crhs in one pass per component (3D, visc_type
 // -1 / 1, x a multiple of 64, y of 8); false (nothing launched) elsewhere.

@@ -17,6 +17,9 @@ static int g_debug_sync = 0;
 static thread_local char g_first_error[256] = {0};
 void set_debug_sync(int on) { g_debug_sync = on; }
 const char *first_error() { return g_first_error; }
+void report_error(const char *msg) {
+  if (g_first_error[0] == 0) snprintf(g_first_error, sizeof(g_first_error), "%s", msg);
+}
 void clear_first_error() { g_first_error[0] = 0; }
 void post_launch(hipStream_t s, const char *name) {
   hipError_t e = hipGetLastError();

@@ -44,6 +44,7 @@ constexpr int NRED = RY * (RX / 2);     // red points per plane of the red regio
 const char kFusedName[] = "k_rb_fused";
 const char kBlockName[] = "k_rb_block";
 const char kMarchName[] = "k_rb_march2";
+const char kMarch3Name[] = "k_rb_march2x3";
 const char kResidRestrictName[] = "k_resid_restrict";
 
 template <typename T, typename P>
@@ -198,17 +199,10 @@ __global__ void __launch_bounds__(256) k_rb_fused(Grid g, int tz, int ntx, int n
 // the two halo planes) use scalar loads of lines that the neighbouring
 // tiles read in the same step.
 template <typename T, typename P>
-__global__ void __launch_bounds__(256) k_rb_march2(Grid g, int tz, int ntx, int nty, int ntiles,
-                                                   const T *__restrict__ pin, T *__restrict__ pout,
-                                                   typename P::Args args0, const T *__restrict__ q0,
-                                                   const T *__restrict__ q1, const T *__restrict__ q2,
-                                                   const T *__restrict__ q3, const T *__restrict__ q4) {
+__device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int nty, int tile,
+                                            const T *__restrict__ pin, T *__restrict__ pout,
+                                            const typename P::Args &args, T (*sp)[HY][HX], T (*sr)[RY][RX]) {
   using Coef = typename P::Coef;
-  const typename P::Args args = P::bind(args0, q0, q1, q2, q3, q4);
-  __shared__ T sp[4][HY][HX];
-  __shared__ T sr[4][RY][RX];
-  int w = blockIdx.x, tile = w;
-  if (ntiles % 8 == 0) tile = (w % 8) * (ntiles / 8) + w / 8;
   const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
   const int i0 = bx * FX, j0 = by * FY, k0 = bz * tz;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
@@ -318,6 +312,53 @@ __global__ void __launch_bounds__(256) k_rb_march2(Grid g, int tz, int ntx, int 
   }
 }
 
+template <typename T, typename P>
+__global__ void __launch_bounds__(256) k_rb_march2(Grid g, int tz, int ntx, int nty, int ntiles,
+                                                   const T *__restrict__ pin, T *__restrict__ pout,
+                                                   typename P::Args args0, const T *__restrict__ q0,
+                                                   const T *__restrict__ q1, const T *__restrict__ q2,
+                                                   const T *__restrict__ q3, const T *__restrict__ q4) {
+  const typename P::Args args = P::bind(args0, q0, q1, q2, q3, q4);
+  __shared__ T sp[4][HY][HX];
+  __shared__ T sr[4][RY][RX];
+  int w = blockIdx.x, tile = w;
+  if (ntiles % 8 == 0) tile = (w % 8) * (ntiles / 8) + w / 8;
+  march2_tile<T, P>(g, tz, ntx, nty, tile, pin, pout, args, sp, sr);
+}
+
+// The three face components of one sweep in one launch (visc_type -1 / 1:
+// the components are independent).  Workgroup slots (tile, component) are
+// numbered so that the three components of a tile go to the same XCD one
+// after another: they run side by side and share the lines of beta_cc and
+// of the edge arrays each of them reads in that XCD's L2, where three
+// separate launches stream those arrays from memory once per component.
+// Each workgroup runs the single-component tile code, so every point is
+// computed exactly as in k_rb_march2.
+template <typename T, typename P0, typename P1, typename P2>
+__global__ void __launch_bounds__(256)
+    k_rb_march2x3(Grid g, int tz, int ntx, int nty, int ntiles, const T *__restrict__ pin0, T *__restrict__ pout0,
+                  const T *__restrict__ pin1, T *__restrict__ pout1, const T *__restrict__ pin2,
+                  T *__restrict__ pout2, typename P0::Args a0, typename P1::Args a1, typename P2::Args a2,
+                  const T *__restrict__ q00, const T *__restrict__ q01, const T *__restrict__ q02,
+                  const T *__restrict__ q03, const T *__restrict__ q04, const T *__restrict__ q10,
+                  const T *__restrict__ q11, const T *__restrict__ q12, const T *__restrict__ q13,
+                  const T *__restrict__ q14, const T *__restrict__ q20, const T *__restrict__ q21,
+                  const T *__restrict__ q22, const T *__restrict__ q23, const T *__restrict__ q24) {
+  __shared__ T sp[4][HY][HX];
+  __shared__ T sr[4][RY][RX];
+  const int nslots = 3 * ntiles, w = blockIdx.x;
+  int slot = w;
+  if (nslots % 8 == 0) slot = (w % 8) * (nslots / 8) + w / 8;
+  const int tile = slot / 3, comp = slot - 3 * tile;
+  if (comp == 0) {
+    march2_tile<T, P0>(g, tz, ntx, nty, tile, pin0, pout0, P0::bind(a0, q00, q01, q02, q03, q04), sp, sr);
+  } else if (comp == 1) {
+    march2_tile<T, P1>(g, tz, ntx, nty, tile, pin1, pout1, P1::bind(a1, q10, q11, q12, q13, q14), sp, sr);
+  } else {
+    march2_tile<T, P2>(g, tz, ntx, nty, tile, pin2, pout2, P2::bind(a2, q20, q21, q22, q23, q24), sp, sr);
+  }
+}
+
 // Block form of the same sweep: a workgroup owns a 32 x 8 x 8 block and
 // updates it in three phases with no plane-to-plane dependence -- stage the
 // old field of the block and a 2-point halo in LDS, update the red points of
@@ -422,7 +463,7 @@ __global__ void __launch_bounds__(256) k_rb_block(Grid g, int nbx, int nby, int 
 }
 
 int g_rb_variant = 2;  // 0: plane marching, 1: block, 2: plane marching with paired loads
-int g_rb_target = 1024;  // workgroups the plane-marching forms aim for
+int g_rb_target = 768;  // workgroups the plane-marching forms aim for (FP32; FP64 twice)
 
 template <typename T, typename P>
 void launch_rb(hipStream_t s, const Grid &g, int tz, const T *pin, T *pout, const typename P::Args &args) {
@@ -588,6 +629,44 @@ __global__ void __launch_bounds__(256) k_resid_restrict(Grid g, Grid gc, int tz,
 }
 
 template <typename T, int A, int VISC>
+typename StagRB<T, A, VISC>::Args stag_args(T omega, T dxsqinv, const T *rhs, const StagCoef<T> &K) {
+  typename StagRB<T, A, VISC>::Args a;
+  a.omega = omega;
+  a.dxsqinv = dxsqinv;
+  a.rhs = rhs;
+  a.alpha = K.alpha[A];
+  a.beta = K.beta;
+  // xy = 0, xz = 1, yz = 2; edges shared with (A, t1) and (A, t2)
+  a.e1 = (VISC == -1) ? K.bed[A == 2 ? 1 : 0] : nullptr;
+  a.e2 = (VISC == -1) ? K.bed[A == 0 ? 1 : 2] : nullptr;
+  return a;
+}
+
+template <typename T, int VISC>
+void launch_stag_three(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, T *const pin[3], T *const pout[3],
+                       T *const rhs[3], const StagCoef<T> &K) {
+  using P0 = StagRB<T, 0, VISC>;
+  using P1 = StagRB<T, 1, VISC>;
+  using P2 = StagRB<T, 2, VISC>;
+  const auto a0 = stag_args<T, 0, VISC>(omega, dxsqinv, rhs[0], K);
+  const auto a1 = stag_args<T, 1, VISC>(omega, dxsqinv, rhs[1], K);
+  const auto a2 = stag_args<T, 2, VISC>(omega, dxsqinv, rhs[2], K);
+  const T *q0[5], *q1[5], *q2[5];
+  P0::ptrs(a0, q0);
+  P1::ptrs(a1, q1);
+  P2::ptrs(a2, q2);
+  const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
+  const int ntiles = ntx * nty * ntz;
+  LaunchEvents *ev = launch_events();
+  hipExtLaunchKernelGGL((k_rb_march2x3<T, P0, P1, P2>), dim3(3 * ntiles), dim3(256), 0, s, ev->start, ev->stop, 0, g,
+                        tz, ntx, nty, ntiles, pin[0], pout[0], pin[1], pout[1], pin[2], pout[2], a0, a1, a2, q0[0],
+                        q0[1], q0[2], q0[3], q0[4], q1[0], q1[1], q1[2], q1[3], q1[4], q2[0], q2[1], q2[2], q2[3],
+                        q2[4]);
+  ev->start = ev->stop = nullptr;
+  post_launch(s, kMarch3Name);
+}
+
+template <typename T, int A, int VISC>
 void launch_stag_one(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, const T *pin, T *pout,
                      const T *rhs, const StagCoef<T> &K) {
   typename StagRB<T, A, VISC>::Args a;
@@ -620,10 +699,11 @@ static int rb_target(int es) { return es == 8 ? 2 * g_rb_target : g_rb_target; }
 
 // Plane count per workgroup, or 0 when the level cannot use the fused
 // sweep.  The plane-marching forms take the thickest z-slab (at most 32
-// planes) that still gives rb_target(es) workgroups, down to 2 planes for
-// the paired form (8 for the first form); a level whose count stays below
-// min_tiles is left to the colour kernels.
-int fused_rb_tz(const Grid &g, int min_tiles, int es) {
+// planes) that still gives rb_target(es) workgroups over the ncomp
+// components one launch sweeps, down to 2 planes for the paired form (8 for
+// the first form); a level with fewer than min_tiles tiles is left to the
+// colour kernels.
+int fused_rb_tz(const Grid &g, int min_tiles, int es, int ncomp) {
   if (g_rb_variant == 1) {
     if (g.dim != 3 || g.n[0] % BX != 0 || g.n[1] % BY != 0 || g.n[2] % BZ != 0) return 0;
     const int nb = (g.n[0] / BX) * (g.n[1] / BY) * (g.n[2] / BZ);
@@ -635,19 +715,28 @@ int fused_rb_tz(const Grid &g, int min_tiles, int es) {
   const int t = rb_target(es);
   const int target = t > min_tiles ? t : min_tiles;
   int tz = 32;
-  while (tz > tz_min && (g.n[2] % tz != 0 || per_plane_set * (g.n[2] / tz) < target)) tz /= 2;
+  auto fits = [&](int z) {
+    const int tiles = per_plane_set * (g.n[2] / z);
+    return g.n[2] % z == 0 && tiles >= min_tiles && ncomp * tiles >= target;
+  };
+  while (tz > tz_min && !fits(tz)) tz /= 2;
+  // at the thinnest slab only the minimum (and divisibility) must hold
   if (g.n[2] % tz != 0 || per_plane_set * (g.n[2] / tz) < min_tiles) return 0;
   return tz;
 }
 
 bool fused_rb_ok(const Grid &g, int visc, int min_tiles) {
-  return (visc == -1 || visc == 1) && fused_rb_tz(g, min_tiles, 4) > 0;
+  return (visc == -1 || visc == 1) && fused_rb_tz(g, min_tiles, 4, 1) > 0;
 }
 
 template <typename T>
 void launch_stag_rb_fused(hipStream_t s, const Grid &g, int visc, int a, T omega, const T *pin, T *pout,
                           const T *rhs, const StagCoef<T> &K, int min_tiles) {
-  const int tz = fused_rb_tz(g, min_tiles, sizeof(T));
+  const int tz = fused_rb_tz(g, min_tiles, sizeof(T), 1);
+  if (tz <= 0) {
+    report_error("launch_stag_rb_fused: level not eligible");
+    return;
+  }
   const T dxsqinv = (T)(1.0 / (g.dx * g.dx));
   if (visc == -1) {
     if (a == 0) launch_stag_one<T, 0, -1>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K);
@@ -660,6 +749,23 @@ void launch_stag_rb_fused(hipStream_t s, const Grid &g, int visc, int a, T omega
   }
 }
 
+bool fused_rb3_ok(const Grid &g, int visc, int min_tiles) {
+  return g_rb_variant == 2 && g.dim == 3 && fused_rb_ok(g, visc, min_tiles);
+}
+
+template <typename T>
+void launch_stag_rb_fused3(hipStream_t s, const Grid &g, int visc, T omega, T *const pin[3], T *const pout[3],
+                           T *const rhs[3], const StagCoef<T> &K, int min_tiles) {
+  const int tz = fused_rb_tz(g, min_tiles, sizeof(T), 3);
+  if (tz <= 0) {
+    report_error("launch_stag_rb_fused3: level not eligible");
+    return;
+  }
+  const T dxsqinv = (T)(1.0 / (g.dx * g.dx));
+  if (visc == -1) launch_stag_three<T, -1>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K);
+  else launch_stag_three<T, 1>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K);
+}
+
 template <typename T>
 void launch_cc_rb_fused(hipStream_t s, const Grid &g, const T *pin, T *pout, const T *rhs, T *const a[3],
                         int min_tiles) {
@@ -669,7 +775,12 @@ void launch_cc_rb_fused(hipStream_t s, const Grid &g, const T *pin, T *pout, con
   args.ax = a[0];
   args.ay = a[1];
   args.az = a[2];
-  launch_rb<T, CCRB<T>>(s, g, fused_rb_tz(g, min_tiles, sizeof(T)), pin, pout, args);
+  const int tz = fused_rb_tz(g, min_tiles, sizeof(T), 1);
+  if (tz <= 0) {
+    report_error("launch_cc_rb_fused: level not eligible");
+    return;
+  }
+  launch_rb<T, CCRB<T>>(s, g, tz, pin, pout, args);
 }
 
 // z-slab thickness of the fused residual + restriction, or 0 when the level
@@ -728,6 +839,10 @@ bool launch_stag_resid_restrict(hipStream_t s, const Grid &g, const Grid &gc, in
   return true;
 }
 
+template void launch_stag_rb_fused3<double>(hipStream_t, const Grid &, int, double, double *const[3],
+                                            double *const[3], double *const[3], const StagCoef<double> &, int);
+template void launch_stag_rb_fused3<float>(hipStream_t, const Grid &, int, float, float *const[3], float *const[3],
+                                           float *const[3], const StagCoef<float> &, int);
 template void launch_stag_rb_fused<double>(hipStream_t, const Grid &, int, int, double, const double *, double *,
                                            const double *, const StagCoef<double> &, int);
 template void launch_stag_rb_fused<float>(hipStream_t, const Grid &, int, int, float, const float *, float *,

@@ -120,6 +120,9 @@ struct lmm_ctx {
   // pass where the level allows (same bits); LMM_RESID_RESTRICT=0 in the
   // environment keeps the two passes
   bool fused_resid_restrict = true;
+  // the fused sweep runs the three face components in one launch
+  // (LMM_FUSED_X3=0 in the environment: one launch per component; same bits)
+  bool fused3 = true;
   // GMRES fuses each MGS axpy with the next reduction and the basis scaling
   // with the FP32 conversion of the preconditioner input (same bits);
   // LMM_KRYLOV_FUSED=0 in the environment keeps separate passes
@@ -449,6 +452,15 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
       // odd sweep count ends with a copy back
       T *const *src = (m % 2 == 0) ? phi : L.res;
       T *const *dst = (m % 2 == 0) ? L.res : phi;
+      if (c->fused3 && fused_rb3_ok(L.g, p.visc_type, c->fused_min_tiles)) {
+        // all components in one launch: each array is compulsory once per
+        // sweep (beta_cc and the edges shared by the components)
+        const double bytes = (double)(4 * dim + 1 + (p.visc_type < 0 ? 3 : 0)) * L.g.N * sizeof(T);
+        ProfScope pf(c, "stag_rb_fused", bytes);
+        ProfScope ps(c, fam, bytes, true);
+        launch_stag_rb_fused3<T>(c->stream, L.g, p.visc_type, (T)p.stag_mg_omega, src, dst, rhs, K,
+                                 c->fused_min_tiles);
+      } else
       for (int a = 0; a < dim; a++) {
         ProfScope pf(c, "stag_rb_fused", comp_sweep_bytes(L.g, p.visc_type, sizeof(T)));
         ProfScope ps(c, fam, comp_sweep_bytes(L.g, p.visc_type, sizeof(T)), true);
@@ -594,7 +606,7 @@ static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *
   auto rb = [&](int n, int sweeps) {
     T *ph = (n == 0) ? phi0 : C[n].phi;
     const T *rh = (n == 0) ? rhs0 : C[n].rhs;
-    if (c->fused && fused_rb_tz(C[n].g, c->fused_min_tiles, 4) > 0) {
+    if (c->fused && fused_rb_tz(C[n].g, c->fused_min_tiles, 4, 1) > 0) {
       // fused red-black sweeps, ping-pong between phi and C[n].res
       for (int m = 0; m < sweeps; m++) {
         // read phi, rhs and the dim face coefficients, write phi
@@ -1151,12 +1163,14 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
     const char *mt = getenv("LMM_FUSED_MIN_TILES");
     if (mt && atoi(mt) > 0) c->fused_min_tiles = atoi(mt);
     const char *tg = getenv("LMM_FUSED_TARGET");
-    set_rb_target(tg && atoi(tg) > 0 ? atoi(tg) : 1024);
+    set_rb_target(tg && atoi(tg) > 0 ? atoi(tg) : 768);
     const char *fk = getenv("LMM_FUSED_KERNEL");
     int variant = 2;
     if (fk && strcmp(fk, "march") == 0) variant = 0;
     if (fk && strcmp(fk, "block") == 0) variant = 1;
     set_rb_variant(variant);
+    const char *x3 = getenv("LMM_FUSED_X3");
+    c->fused3 = !(x3 && x3[0] == '0');
     const char *rr = getenv("LMM_RESID_RESTRICT");
     c->fused_resid_restrict = !(rr && rr[0] == '0');
     const char *kf = getenv("LMM_KRYLOV_FUSED");

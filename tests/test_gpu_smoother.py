@@ -6,7 +6,7 @@ exist (LMM_FUSED_KERNEL=march2, the default, block and march).
 LMM_SMOOTHER=fused makes the small test levels use the fused sweep, which by
 default only levels with at least 128 workgroups (64^3 and up) do; the
 plane-marching forms choose their z-slab thickness for LMM_FUSED_TARGET
-workgroups (1024 by default: 2-plane slabs on these small levels, 1 for the
+workgroups (768 by default: 2-plane slabs on these small levels, 1 for the
 thickest slab a level allows)."""
 import numpy as np
 import pytest
@@ -121,3 +121,22 @@ def test_fused_cc_multigrid_matches_oracle_and_colour(n, vcycles, sweeps, kernel
             out[mode] = phi.download()
     assert bit_equal(out["fused"], out["colour"])
     assert bit_equal(out["fused"], want)
+
+
+@pytest.mark.parametrize("visc", [-1, 1])
+@pytest.mark.parametrize("n,target", [((64, 16, 32), "1024"), ((128, 32, 16), "1")])
+def test_three_components_in_one_launch_equal_separate(visc, n, target, monkeypatch):
+    """The default fused sweep runs the three face components in one launch;
+    LMM_FUSED_X3=0 launches them one by one.  Same tile code per point, so the
+    same bits, and three times fewer launches."""
+    monkeypatch.setenv("LMM_SMOOTHER", "fused")
+    monkeypatch.setenv("LMM_FUSED_TARGET", target)
+    pb = P.small_problem(3, n, visc_type=visc, seed=101)
+    kw = dict(visc_type=visc, stag_mg_max_vcycles=2)
+    rhs, guess = _rand_faces(pb.shape, 8), _rand_faces(pb.shape, 9)
+    monkeypatch.setenv("LMM_FUSED_X3", "0")
+    ref, n0 = _mg_solve(pb, kw, rhs, guess)
+    monkeypatch.setenv("LMM_FUSED_X3", "1")
+    got, n1 = _mg_solve(pb, kw, rhs, guess)
+    assert n1 > 0 and n0 == 3 * n1
+    assert bit_equal(got, ref)
