@@ -630,7 +630,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(Grid g, Grid gc, int tz,
 
 template <typename T, int A, int VISC>
 typename StagRB<T, A, VISC>::Args stag_args(T omega, T dxsqinv, const T *rhs, const StagCoef<T> &K) {
-  typename StagRB<T, A, VISC>::Args a;
+  typename StagRB<T, A, VISC>::Args a = {};
   a.omega = omega;
   a.dxsqinv = dxsqinv;
   a.rhs = rhs;
@@ -669,7 +669,7 @@ void launch_stag_three(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz,
 template <typename T, int A, int VISC>
 void launch_stag_one(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, const T *pin, T *pout,
                      const T *rhs, const StagCoef<T> &K) {
-  typename StagRB<T, A, VISC>::Args a;
+  typename StagRB<T, A, VISC>::Args a = {};
   a.omega = omega;
   a.dxsqinv = dxsqinv;
   a.rhs = rhs;
@@ -769,7 +769,7 @@ void launch_stag_rb_fused3(hipStream_t s, const Grid &g, int visc, T omega, T *c
 template <typename T>
 void launch_cc_rb_fused(hipStream_t s, const Grid &g, const T *pin, T *pout, const T *rhs, T *const a[3],
                         int min_tiles) {
-  typename CCRB<T>::Args args;
+  typename CCRB<T>::Args args = {};  // zeroed padding: kernel arguments are copied bytewise
   args.inv = (T)(1.0 / (g.dx * g.dx));
   args.rhs = rhs;
   args.ax = a[0];
