@@ -160,10 +160,36 @@ __global__ void __launch_bounds__(256) k_stag_prolong(Grid gf, Grid gc, T *fine,
   prolong_at<T, D, A>(gf, gc, fine, C, t);
 }
 
+// Two x-neighbours (2m, 2m+1) per thread: one vector load and store of the
+// fine pair; each point gets prolong_value as in the one-point kernel, and
+// for the x-face component the even and odd point no longer sit in
+// alternate lanes of a wave.
+template <typename T, int A>
+__global__ void __launch_bounds__(256) k_stag_prolong2(Grid gf, Grid gc, T *fine, const T *__restrict__ C) {
+  const long t = xcd_block() * blockDim.x + threadIdx.x;
+  if (2 * t >= gf.N) return;
+  int c0[3];
+  decode(gf, 2 * t, c0[0], c0[1], c0[2]);
+  const int c1[3] = {c0[0] + 1, c0[1], c0[2]};
+  const long p = 2 * t;
+  T f0, f1;
+  ld2(fine, p, f0, f1);
+  f0 = prolong_value<T, 3, A>(gf, gc, C, c0, f0);
+  f1 = prolong_value<T, 3, A>(gf, gc, C, c1, f1);
+  st2(fine, p, f0, f1);
+}
+
 template <typename T>
 void launch_stag_prolong(hipStream_t s, const Grid &gf, T *const fine[3], T *const coarse[3]) {
   Grid gc = coarsen(gf);
   dim3 grid(nblk(gf.N));
+  if (gf.dim == 3 && gf.n[0] % 2 == 0) {
+    dim3 grid2(nblk(gf.N / 2));
+    LAUNCH((k_stag_prolong2<T, 0>), grid2, dim3(TPB), 0, s, gf, gc, fine[0], coarse[0]);
+    LAUNCH((k_stag_prolong2<T, 1>), grid2, dim3(TPB), 0, s, gf, gc, fine[1], coarse[1]);
+    LAUNCH((k_stag_prolong2<T, 2>), grid2, dim3(TPB), 0, s, gf, gc, fine[2], coarse[2]);
+    return;
+  }
   if (gf.dim == 2) {
     LAUNCH((k_stag_prolong<T, 2, 0>), grid, dim3(TPB), 0, s, gf, gc, fine[0], coarse[0]);
     LAUNCH((k_stag_prolong<T, 2, 1>), grid, dim3(TPB), 0, s, gf, gc, fine[1], coarse[1]);

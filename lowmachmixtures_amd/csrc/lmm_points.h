@@ -145,11 +145,9 @@ __device__ __forceinline__ void restrict_at(const Grid &gc, const Grid &gf, int 
 // stag_mg_solver.f90:1370-1633): bilinear in the transverse directions,
 // linear along A.  The component is a template argument so every index is static.
 template <typename T, int D, int A>
-__device__ __forceinline__ void prolong_at(const Grid &gf, const Grid &gc, T *fine, const T *C, long t) {
+__device__ __forceinline__ T prolong_value(const Grid &gf, const Grid &gc, const T *C, const int c[3], T v) {
   constexpr int T1 = (D == 2) ? 1 - A : (A == 0 ? 1 : 0);
   constexpr int T2 = (D == 2) ? -1 : (A == 2 ? 1 : 2);
-  int c[3];
-  decode(gf, t, c[0], c[1], c[2]);
   const int Q[3] = {c[0] >> 1, c[1] >> 1, c[2] >> 1};
   const int o1 = (c[T1] & 1) ? 1 : -1;
   const int o2 = (T2 >= 0 && (c[T2 >= 0 ? T2 : 0] & 1)) ? 1 : -1;
@@ -165,8 +163,6 @@ __device__ __forceinline__ void prolong_at(const Grid &gf, const Grid &gc, T *fi
     }
     return q[0] + gc.s[1] * q[1] + gc.s[2] * q[2];
   };
-  const long p = c[0] + gf.s[1] * c[1] + gf.s[2] * c[2];
-  T v = fine[p];
   const bool even = (c[A] & 1) == 0;
   if (D == 2) {
     if (even) {
@@ -187,7 +183,15 @@ __device__ __forceinline__ void prolong_at(const Grid &gf, const Grid &gc, T *fi
           three32 * C[cidx(1, 0, o2)] + one32 * C[cidx(1, o1, o2)];
     }
   }
-  fine[p] = v;
+  return v;
+}
+
+template <typename T, int D, int A>
+__device__ __forceinline__ void prolong_at(const Grid &gf, const Grid &gc, T *fine, const T *C, long t) {
+  int c[3];
+  decode(gf, t, c[0], c[1], c[2]);
+  const long p = c[0] + gf.s[1] * c[1] + gf.s[2] * c[2];
+  fine[p] = prolong_value<T, D, A>(gf, gc, C, c, fine[p]);
 }
 
 template <typename T>
