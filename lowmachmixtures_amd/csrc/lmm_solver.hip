@@ -460,12 +460,13 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
         ProfScope ps(c, fam, bytes, true);
         launch_stag_rb_fused3<T>(c->stream, L.g, p.visc_type, (T)p.stag_mg_omega, src, dst, rhs, K,
                                  c->fused_min_tiles);
-      } else
-      for (int a = 0; a < dim; a++) {
-        ProfScope pf(c, "stag_rb_fused", comp_sweep_bytes(L.g, p.visc_type, sizeof(T)));
-        ProfScope ps(c, fam, comp_sweep_bytes(L.g, p.visc_type, sizeof(T)), true);
-        launch_stag_rb_fused<T>(c->stream, L.g, p.visc_type, a, (T)p.stag_mg_omega, src[a], dst[a], rhs[a], K,
-                                c->fused_min_tiles);
+      } else {
+        for (int a = 0; a < dim; a++) {
+          ProfScope pf(c, "stag_rb_fused", comp_sweep_bytes(L.g, p.visc_type, sizeof(T)));
+          ProfScope ps(c, fam, comp_sweep_bytes(L.g, p.visc_type, sizeof(T)), true);
+          launch_stag_rb_fused<T>(c->stream, L.g, p.visc_type, a, (T)p.stag_mg_omega, src[a], dst[a], rhs[a], K,
+                                  c->fused_min_tiles);
+        }
       }
       if (m == nsweeps - 1 && m % 2 == 0)
         for (int a = 0; a < dim; a++)
