@@ -40,6 +40,7 @@ constexpr int FX = 64, FY = 8;          // interior tile
 constexpr int HX = FX + 4, HY = FY + 4; // old field, 2-point halo
 constexpr int RX = FX + 2, RY = FY + 2; // red values, 1-point halo
 constexpr int NRED = RY * (RX / 2);     // red points per plane of the red region
+constexpr int FX2 = 32, FY2 = 16;       // narrow tile of the three-component sweep (x = 32 levels)
 
 const char kFusedName[] = "k_rb_fused";
 const char kBlockName[] = "k_rb_block";
@@ -198,11 +199,15 @@ __global__ void __launch_bounds__(256) k_rb_fused(Grid g, int tz, int ntx, int n
 // read once, at one time; the red points of the 1-point halo ring (and of
 // the two halo planes) use scalar loads of lines that the neighbouring
 // tiles read in the same step.
-template <typename T, typename P>
+template <typename T, typename P, int TX>
 __device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int nty, int tile,
                                             const T *__restrict__ pin, T *__restrict__ pout,
-                                            const typename P::Args &args, T (*sp)[HY][HX], T (*sr)[RY][RX]) {
+                                            const typename P::Args &args, T (*sp)[512 / TX + 4][TX + 4],
+                                            T (*sr)[512 / TX + 2][TX + 2]) {
   using Coef = typename P::Coef;
+  // tile TX x (512 / TX): one point pair per thread of the 256
+  constexpr int FX = TX, FY = 512 / TX, HX = FX + 4, HY = FY + 4, RX = FX + 2, RY = FY + 2;
+  constexpr int NRED = RY * (RX / 2);
   const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
   const int i0 = bx * FX, j0 = by * FY, k0 = bz * tz;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
@@ -256,8 +261,8 @@ __device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int 
       const int hr = row + 1, hc = rr + 1;
       sr[s][row][rr] = P::update(cr, sp[s][hr][hc], sp[s][hr][hc + 1], sp[s][hr][hc - 1], sp[s][hr + 1][hc],
                                  sp[s][hr - 1][hc], sp[spl][hr][hc], sp[sm][hr][hc], args);
-      // halo ring: rows -1 and FY (33 red points each), then one end point
-      // of each interior row (8)
+      // halo ring: rows -1 and FY (RX / 2 red points each), then one end
+      // point of each interior row
       for (int h = tid; h < 2 * (RX / 2) + FY; h += 256) {
         int row2, rr2;
         if (h < 2 * (RX / 2)) {
@@ -323,7 +328,7 @@ __global__ void __launch_bounds__(256) k_rb_march2(Grid g, int tz, int ntx, int 
   __shared__ T sr[4][RY][RX];
   int w = blockIdx.x, tile = w;
   if (ntiles % 8 == 0) tile = (w % 8) * (ntiles / 8) + w / 8;
-  march2_tile<T, P>(g, tz, ntx, nty, tile, pin, pout, args, sp, sr);
+  march2_tile<T, P, FX>(g, tz, ntx, nty, tile, pin, pout, args, sp, sr);
 }
 
 // The three face components of one sweep in one launch (visc_type -1 / 1:
@@ -334,7 +339,7 @@ __global__ void __launch_bounds__(256) k_rb_march2(Grid g, int tz, int ntx, int 
 // separate launches stream those arrays from memory once per component.
 // Each workgroup runs the single-component tile code, so every point is
 // computed exactly as in k_rb_march2.
-template <typename T, typename P0, typename P1, typename P2>
+template <typename T, typename P0, typename P1, typename P2, int TX>
 __global__ void __launch_bounds__(256)
     k_rb_march2x3(Grid g, int tz, int ntx, int nty, int ntiles, const T *__restrict__ pin0, T *__restrict__ pout0,
                   const T *__restrict__ pin1, T *__restrict__ pout1, const T *__restrict__ pin2,
@@ -344,18 +349,18 @@ __global__ void __launch_bounds__(256)
                   const T *__restrict__ q11, const T *__restrict__ q12, const T *__restrict__ q13,
                   const T *__restrict__ q14, const T *__restrict__ q20, const T *__restrict__ q21,
                   const T *__restrict__ q22, const T *__restrict__ q23, const T *__restrict__ q24) {
-  __shared__ T sp[4][HY][HX];
-  __shared__ T sr[4][RY][RX];
+  __shared__ T sp[4][512 / TX + 4][TX + 4];
+  __shared__ T sr[4][512 / TX + 2][TX + 2];
   const int nslots = 3 * ntiles, w = blockIdx.x;
   int slot = w;
   if (nslots % 8 == 0) slot = (w % 8) * (nslots / 8) + w / 8;
   const int tile = slot / 3, comp = slot - 3 * tile;
   if (comp == 0) {
-    march2_tile<T, P0>(g, tz, ntx, nty, tile, pin0, pout0, P0::bind(a0, q00, q01, q02, q03, q04), sp, sr);
+    march2_tile<T, P0, TX>(g, tz, ntx, nty, tile, pin0, pout0, P0::bind(a0, q00, q01, q02, q03, q04), sp, sr);
   } else if (comp == 1) {
-    march2_tile<T, P1>(g, tz, ntx, nty, tile, pin1, pout1, P1::bind(a1, q10, q11, q12, q13, q14), sp, sr);
+    march2_tile<T, P1, TX>(g, tz, ntx, nty, tile, pin1, pout1, P1::bind(a1, q10, q11, q12, q13, q14), sp, sr);
   } else {
-    march2_tile<T, P2>(g, tz, ntx, nty, tile, pin2, pout2, P2::bind(a2, q20, q21, q22, q23, q24), sp, sr);
+    march2_tile<T, P2, TX>(g, tz, ntx, nty, tile, pin2, pout2, P2::bind(a2, q20, q21, q22, q23, q24), sp, sr);
   }
 }
 
@@ -643,8 +648,8 @@ typename StagRB<T, A, VISC>::Args stag_args(T omega, T dxsqinv, const T *rhs, co
 }
 
 template <typename T, int VISC>
-void launch_stag_three(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, T *const pin[3], T *const pout[3],
-                       T *const rhs[3], const StagCoef<T> &K) {
+void launch_stag_three(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, int tx, T *const pin[3],
+                       T *const pout[3], T *const rhs[3], const StagCoef<T> &K) {
   using P0 = StagRB<T, 0, VISC>;
   using P1 = StagRB<T, 1, VISC>;
   using P2 = StagRB<T, 2, VISC>;
@@ -655,13 +660,22 @@ void launch_stag_three(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz,
   P0::ptrs(a0, q0);
   P1::ptrs(a1, q1);
   P2::ptrs(a2, q2);
-  const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
-  const int ntiles = ntx * nty * ntz;
   LaunchEvents *ev = launch_events();
-  hipExtLaunchKernelGGL((k_rb_march2x3<T, P0, P1, P2>), dim3(3 * ntiles), dim3(256), 0, s, ev->start, ev->stop, 0, g,
-                        tz, ntx, nty, ntiles, pin[0], pout[0], pin[1], pout[1], pin[2], pout[2], a0, a1, a2, q0[0],
-                        q0[1], q0[2], q0[3], q0[4], q1[0], q1[1], q1[2], q1[3], q1[4], q2[0], q2[1], q2[2], q2[3],
-                        q2[4]);
+  if (tx == FX) {
+    const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
+    const int ntiles = ntx * nty * ntz;
+    hipExtLaunchKernelGGL((k_rb_march2x3<T, P0, P1, P2, FX>), dim3(3 * ntiles), dim3(256), 0, s, ev->start, ev->stop,
+                          0, g, tz, ntx, nty, ntiles, pin[0], pout[0], pin[1], pout[1], pin[2], pout[2], a0, a1, a2,
+                          q0[0], q0[1], q0[2], q0[3], q0[4], q1[0], q1[1], q1[2], q1[3], q1[4], q2[0], q2[1], q2[2],
+                          q2[3], q2[4]);
+  } else {
+    const int ntx = g.n[0] / FX2, nty = g.n[1] / FY2, ntz = g.n[2] / tz;
+    const int ntiles = ntx * nty * ntz;
+    hipExtLaunchKernelGGL((k_rb_march2x3<T, P0, P1, P2, FX2>), dim3(3 * ntiles), dim3(256), 0, s, ev->start,
+                          ev->stop, 0, g, tz, ntx, nty, ntiles, pin[0], pout[0], pin[1], pout[1], pin[2], pout[2], a0,
+                          a1, a2, q0[0], q0[1], q0[2], q0[3], q0[4], q1[0], q1[1], q1[2], q1[3], q1[4], q2[0], q2[1],
+                          q2[2], q2[3], q2[4]);
+  }
   ev->start = ev->stop = nullptr;
   post_launch(s, kMarch3Name);
 }
@@ -701,28 +715,29 @@ static int rb_target(int es) { return es == 8 ? 2 * g_rb_target : g_rb_target; }
 // sweep.  The plane-marching forms take the thickest z-slab (at most 32
 // planes) that still gives rb_target(es) workgroups over the ncomp
 // components one launch sweeps, down to 2 planes for the paired form (8 for
-// the first form); a level with fewer than min_tiles tiles is left to the
-// colour kernels.
+// the first form); a launch with fewer than min_tiles workgroups is left to
+// the colour kernels.  rb_tz is the rule for fx x fy tiles.
+static int rb_tz(const Grid &g, int min_tiles, int es, int ncomp, int fx, int fy) {
+  if (g.dim != 3 || g.n[0] % fx != 0 || g.n[1] % fy != 0) return 0;
+  const int per_plane_set = (g.n[0] / fx) * (g.n[1] / fy);
+  const int tz_min = (g_rb_variant == 2) ? 2 : 8;
+  const int t = rb_target(es);
+  const int target = t > min_tiles ? t : min_tiles;
+  int tz = 32;
+  auto fits = [&](int z) { return g.n[2] % z == 0 && ncomp * per_plane_set * (g.n[2] / z) >= target; };
+  while (tz > tz_min && !fits(tz)) tz /= 2;
+  // at the thinnest slab only the minimum (and divisibility) must hold
+  if (g.n[2] % tz != 0 || ncomp * per_plane_set * (g.n[2] / tz) < min_tiles) return 0;
+  return tz;
+}
+
 int fused_rb_tz(const Grid &g, int min_tiles, int es, int ncomp) {
   if (g_rb_variant == 1) {
     if (g.dim != 3 || g.n[0] % BX != 0 || g.n[1] % BY != 0 || g.n[2] % BZ != 0) return 0;
     const int nb = (g.n[0] / BX) * (g.n[1] / BY) * (g.n[2] / BZ);
     return nb >= min_tiles ? BZ : 0;
   }
-  if (g.dim != 3 || g.n[0] % FX != 0 || g.n[1] % FY != 0) return 0;
-  const int per_plane_set = (g.n[0] / FX) * (g.n[1] / FY);
-  const int tz_min = (g_rb_variant == 2) ? 2 : 8;
-  const int t = rb_target(es);
-  const int target = t > min_tiles ? t : min_tiles;
-  int tz = 32;
-  auto fits = [&](int z) {
-    const int tiles = per_plane_set * (g.n[2] / z);
-    return g.n[2] % z == 0 && tiles >= min_tiles && ncomp * tiles >= target;
-  };
-  while (tz > tz_min && !fits(tz)) tz /= 2;
-  // at the thinnest slab only the minimum (and divisibility) must hold
-  if (g.n[2] % tz != 0 || per_plane_set * (g.n[2] / tz) < min_tiles) return 0;
-  return tz;
+  return rb_tz(g, min_tiles, es, ncomp, FX, FY);
 }
 
 bool fused_rb_ok(const Grid &g, int visc, int min_tiles) {
@@ -749,21 +764,33 @@ void launch_stag_rb_fused(hipStream_t s, const Grid &g, int visc, int a, T omega
   }
 }
 
+// tile width (64, or 32 on levels 32 points wide) and z-slab of the
+// three-component sweep; 0 when the level cannot use it
+static int rb3_shape(const Grid &g, int min_tiles, int es, int &tx) {
+  tx = FX;
+  const int tz = rb_tz(g, min_tiles, es, 3, FX, FY);
+  if (tz > 0) return tz;
+  tx = FX2;
+  return rb_tz(g, min_tiles, es, 3, FX2, FY2);
+}
+
 bool fused_rb3_ok(const Grid &g, int visc, int min_tiles) {
-  return g_rb_variant == 2 && g.dim == 3 && fused_rb_ok(g, visc, min_tiles);
+  int tx;
+  return g_rb_variant == 2 && (visc == -1 || visc == 1) && rb3_shape(g, min_tiles, 4, tx) > 0;
 }
 
 template <typename T>
 void launch_stag_rb_fused3(hipStream_t s, const Grid &g, int visc, T omega, T *const pin[3], T *const pout[3],
                            T *const rhs[3], const StagCoef<T> &K, int min_tiles) {
-  const int tz = fused_rb_tz(g, min_tiles, sizeof(T), 3);
+  int tx = FX;
+  const int tz = rb3_shape(g, min_tiles, sizeof(T), tx);
   if (tz <= 0) {
     report_error("launch_stag_rb_fused3: level not eligible");
     return;
   }
   const T dxsqinv = (T)(1.0 / (g.dx * g.dx));
-  if (visc == -1) launch_stag_three<T, -1>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K);
-  else launch_stag_three<T, 1>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K);
+  if (visc == -1) launch_stag_three<T, -1>(s, g, omega, dxsqinv, tz, tx, pin, pout, rhs, K);
+  else launch_stag_three<T, 1>(s, g, omega, dxsqinv, tz, tx, pin, pout, rhs, K);
 }
 
 template <typename T>

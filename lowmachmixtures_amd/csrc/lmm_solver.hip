@@ -100,14 +100,14 @@ struct lmm_ctx {
   double *Wcc = nullptr, *Wed = nullptr;
   double *dweights = nullptr;
   // Smoother kernels: fused red-black sweeps on levels with at least
-  // fused_min_tiles workgroups (64^3 and up), colour-by-colour kernels
+  // fused_min_tiles workgroups in a launch (32^3 and up), colour-by-colour kernels
   // elsewhere.  Both give the same bits.  LMM_SMOOTHER=colour in the
   // environment selects the colour kernels everywhere, LMM_SMOOTHER=fused
   // the fused sweep on every level whose extents allow it;
   // LMM_FUSED_MIN_TILES and LMM_FUSED_TARGET (workgroups aimed for when the
   // z-slab thickness is chosen) tune the choice.
   bool fused = true;
-  int fused_min_tiles = 128;
+  int fused_min_tiles = 64;
   // coarse levels (at most COARSE_MAX_CELLS cells) of both multigrids run
   // in one single-workgroup launch; LMM_COARSE=0 in the environment keeps
   // one launch per pass on every level (same bits)
@@ -447,12 +447,13 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
         hipMemcpyAsync(phi[d], L.res[d], sizeof(T) * L.g.N, hipMemcpyDeviceToDevice, c->stream);
       continue;
     }
-    if (c->fused && fused_rb_ok(L.g, p.visc_type, c->fused_min_tiles)) {
+    const bool x3 = c->fused && c->fused3 && fused_rb3_ok(L.g, p.visc_type, c->fused_min_tiles);
+    if (x3 || (c->fused && fused_rb_ok(L.g, p.visc_type, c->fused_min_tiles))) {
       // sweep m reads one buffer and writes the other (phi <-> L.res); an
       // odd sweep count ends with a copy back
       T *const *src = (m % 2 == 0) ? phi : L.res;
       T *const *dst = (m % 2 == 0) ? L.res : phi;
-      if (c->fused3 && fused_rb3_ok(L.g, p.visc_type, c->fused_min_tiles)) {
+      if (x3) {
         // all components in one launch: each array is compulsory once per
         // sweep (beta_cc and the edges shared by the components)
         const double bytes = (double)(4 * dim + 1 + (p.visc_type < 0 ? 3 : 0)) * L.g.N * sizeof(T);

@@ -4,7 +4,7 @@ the colour-by-colour kernels.  Both must agree bit for bit: every point is
 computed with the reference's operation order.  Two forms of the fused sweep
 exist (LMM_FUSED_KERNEL=march2, the default, block and march).
 LMM_SMOOTHER=fused makes the small test levels use the fused sweep, which by
-default only levels with at least 128 workgroups (64^3 and up) do; the
+default only launches of at least 64 workgroups (32^3 and up) do; the
 plane-marching forms choose their z-slab thickness for LMM_FUSED_TARGET
 workgroups (768 by default: 2-plane slabs on these small levels, 1 for the
 thickest slab a level allows)."""
@@ -124,11 +124,13 @@ def test_fused_cc_multigrid_matches_oracle_and_colour(n, vcycles, sweeps, kernel
 
 
 @pytest.mark.parametrize("visc", [-1, 1])
-@pytest.mark.parametrize("n,target", [((64, 16, 32), "1024"), ((128, 32, 16), "1")])
+@pytest.mark.parametrize("n,target", [((64, 16, 32), "1024"), ((128, 32, 16), "1"), ((64, 32, 32), "1"),
+                                      ((32, 32, 16), "768")])
 def test_three_components_in_one_launch_equal_separate(visc, n, target, monkeypatch):
-    """The default fused sweep runs the three face components in one launch;
-    LMM_FUSED_X3=0 launches them one by one.  Same tile code per point, so the
-    same bits, and three times fewer launches."""
+    """The default fused sweep runs the three face components in one launch
+    (64 x 8 tiles, or 32 x 16 on levels 32 points wide, which the
+    one-component launches leave to the colour kernels); LMM_FUSED_X3=0
+    launches them one by one.  Same point code, so the same bits."""
     monkeypatch.setenv("LMM_SMOOTHER", "fused")
     monkeypatch.setenv("LMM_FUSED_TARGET", target)
     pb = P.small_problem(3, n, visc_type=visc, seed=101)
@@ -138,5 +140,21 @@ def test_three_components_in_one_launch_equal_separate(visc, n, target, monkeypa
     ref, n0 = _mg_solve(pb, kw, rhs, guess)
     monkeypatch.setenv("LMM_FUSED_X3", "1")
     got, n1 = _mg_solve(pb, kw, rhs, guess)
-    assert n1 > 0 and n0 == 3 * n1
+    assert n1 > 0
+    if n[0] % 64 == 0 and n[0] // 2 % 64 == 0:  # no 32-wide fused level
+        assert n0 == 3 * n1
     assert bit_equal(got, ref)
+
+
+@pytest.mark.parametrize("visc", [-1, 1])
+def test_narrow_tile_sweep_matches_oracle(visc, monkeypatch):
+    """Levels 32 points wide run the three-component sweep on 32 x 16 tiles."""
+    monkeypatch.setenv("LMM_SMOOTHER", "fused")
+    pb = P.small_problem(3, (32, 32, 16), visc_type=visc, seed=107)
+    kw = dict(visc_type=visc)
+    g, p, cf = oracle_args(pb, **kw)
+    rhs, guess = _rand_faces(pb.shape, 10), _rand_faces(pb.shape, 11)
+    want = O.stag_mg_solver(g, p, cf, 0.9, rhs, phi=guess)
+    got, fused = _mg_solve(pb, kw, rhs, guess)
+    assert fused > 0
+    assert bit_equal(got, want), relerr(got, want)
