@@ -42,18 +42,28 @@ __device__ void blk_fill(T *x, long n, T v) {
   for (long t = threadIdx.x; t < n; t += CTPB) x[t] = v;
 }
 
-template <typename T>
-__device__ void blk_stag_smooth(const SLevelDesc<T> &L, int visc, T omega, int nsweeps) {
-  const int dim = L.g.dim;
+// the colour loop is unrolled for each dimension so that the component a of
+// every colour is a constant after inlining (no per-lane stack for the
+// per-component pointer tables)
+template <typename T, int DIM>
+__device__ void blk_stag_smooth_d(const SLevelDesc<T> &L, int visc, T omega, int nsweeps) {
   const long half = (long)(L.g.n[0] >> 1) * L.g.n[1] * L.g.n[2];
   const T dxsqinv = (T)(1.0 / (L.g.dx * L.g.dx));
-  for (int m = 0; m < nsweeps; m++)
-    for (int color = 1; color <= 2 * dim; color++) {
+  for (int m = 0; m < nsweeps; m++) {
+#pragma unroll
+    for (int color = 1; color <= 2 * DIM; color++) {
       const int a = (color - 1) / 2, par = (color + 1) % 2;
       for (long t = threadIdx.x; t < half; t += CTPB)
         gs_colour_at<T>(L.g, visc, a, par, omega, dxsqinv, L.phi, L.K, L.rhs.c[a], t);
       __syncthreads();
     }
+  }
+}
+
+template <typename T>
+__device__ void blk_stag_smooth(const SLevelDesc<T> &L, int visc, T omega, int nsweeps) {
+  if (L.g.dim == 3) blk_stag_smooth_d<T, 3>(L, visc, omega, nsweeps);
+  else blk_stag_smooth_d<T, 2>(L, visc, omega, nsweeps);
 }
 
 template <typename T>
