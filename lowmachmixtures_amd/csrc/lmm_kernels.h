@@ -52,11 +52,12 @@ template <typename T>
 void launch_stag_rb_fused(hipStream_t s, const Grid &g, int visc, int a, T omega, const T *pin, T *pout,
                           const T *rhs, const StagCoef<T> &K, int min_tiles);
 // All three components of one fused sweep in one launch (3D, visc -1 / 1,
-// the paired plane-marching form); reads pin[a], writes pout[a].
+// the paired plane-marching form); reads pin[a] (zero_in: a field of zeros,
+// pin not read), writes pout[a].
 bool fused_rb3_ok(const Grid &g, int visc, int min_tiles);
 template <typename T>
 void launch_stag_rb_fused3(hipStream_t s, const Grid &g, int visc, T omega, T *const pin[3], T *const pout[3],
-                           T *const rhs[3], const StagCoef<T> &K, int min_tiles);
+                           T *const rhs[3], const StagCoef<T> &K, int min_tiles, bool zero_in);
 // Residual of every face component and its full-weighting restriction into
 // the coarse right-hand side crhs in one pass per component (3D, visc_type
 // -1 / 1, x a multiple of 64, y of 8); false (nothing launched) elsewhere.

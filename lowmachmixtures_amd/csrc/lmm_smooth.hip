@@ -225,7 +225,8 @@ __device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int 
       const int q = tid + 256 * n;
       if (q < HY * HX) {
         const int r = q / HX, c = q - r * HX;
-        dst[n] = pin[wrapc(i0 - 2 + c, nx) + sy * wrapc(j0 - 2 + r, ny) + sz * kg];
+        // a null input stands for a field of zeros (the first sweep from a zero guess)
+        dst[n] = pin ? pin[wrapc(i0 - 2 + c, nx) + sy * wrapc(j0 - 2 + r, ny) + sz * kg] : (T)0;
       }
     }
   };
@@ -648,8 +649,10 @@ typename StagRB<T, A, VISC>::Args stag_args(T omega, T dxsqinv, const T *rhs, co
 }
 
 template <typename T, int VISC>
-void launch_stag_three(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, int tx, T *const pin[3],
-                       T *const pout[3], T *const rhs[3], const StagCoef<T> &K) {
+void launch_stag_three(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, int tx, T *const pin_[3],
+                       T *const pout[3], T *const rhs[3], const StagCoef<T> &K, bool zero_in) {
+  T *const zeros[3] = {nullptr, nullptr, nullptr};
+  T *const *pin = zero_in ? zeros : pin_;
   using P0 = StagRB<T, 0, VISC>;
   using P1 = StagRB<T, 1, VISC>;
   using P2 = StagRB<T, 2, VISC>;
@@ -781,7 +784,7 @@ bool fused_rb3_ok(const Grid &g, int visc, int min_tiles) {
 
 template <typename T>
 void launch_stag_rb_fused3(hipStream_t s, const Grid &g, int visc, T omega, T *const pin[3], T *const pout[3],
-                           T *const rhs[3], const StagCoef<T> &K, int min_tiles) {
+                           T *const rhs[3], const StagCoef<T> &K, int min_tiles, bool zero_in) {
   int tx = FX;
   const int tz = rb3_shape(g, min_tiles, sizeof(T), tx);
   if (tz <= 0) {
@@ -789,8 +792,8 @@ void launch_stag_rb_fused3(hipStream_t s, const Grid &g, int visc, T omega, T *c
     return;
   }
   const T dxsqinv = (T)(1.0 / (g.dx * g.dx));
-  if (visc == -1) launch_stag_three<T, -1>(s, g, omega, dxsqinv, tz, tx, pin, pout, rhs, K);
-  else launch_stag_three<T, 1>(s, g, omega, dxsqinv, tz, tx, pin, pout, rhs, K);
+  if (visc == -1) launch_stag_three<T, -1>(s, g, omega, dxsqinv, tz, tx, pin, pout, rhs, K, zero_in);
+  else launch_stag_three<T, 1>(s, g, omega, dxsqinv, tz, tx, pin, pout, rhs, K, zero_in);
 }
 
 template <typename T>
@@ -867,9 +870,10 @@ bool launch_stag_resid_restrict(hipStream_t s, const Grid &g, const Grid &gc, in
 }
 
 template void launch_stag_rb_fused3<double>(hipStream_t, const Grid &, int, double, double *const[3],
-                                            double *const[3], double *const[3], const StagCoef<double> &, int);
+                                            double *const[3], double *const[3], const StagCoef<double> &, int,
+                                            bool);
 template void launch_stag_rb_fused3<float>(hipStream_t, const Grid &, int, float, float *const[3], float *const[3],
-                                           float *const[3], const StagCoef<float> &, int);
+                                           float *const[3], const StagCoef<float> &, int, bool);
 template void launch_stag_rb_fused<double>(hipStream_t, const Grid &, int, int, double, const double *, double *,
                                            const double *, const StagCoef<double> &, int);
 template void launch_stag_rb_fused<float>(hipStream_t, const Grid &, int, int, float, const float *, float *,

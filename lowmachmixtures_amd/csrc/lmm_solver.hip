@@ -433,7 +433,7 @@ static double comp_sweep_bytes(const Grid &g, int visc, size_t es) {
 }
 
 template <typename T>
-static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], int nsweeps) {
+static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], int nsweeps, bool zero_in = false) {
   const lmm_params &p = c->prm;
   int dim = L.g.dim;
   StagCoef<T> K = L.K();
@@ -460,7 +460,7 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
         ProfScope pf(c, "stag_rb_fused", bytes);
         ProfScope ps(c, fam, bytes, true);
         launch_stag_rb_fused3<T>(c->stream, L.g, p.visc_type, (T)p.stag_mg_omega, src, dst, rhs, K,
-                                 c->fused_min_tiles);
+                                 c->fused_min_tiles, zero_in && m == 0);
       } else {
         for (int a = 0; a < dim; a++) {
           ProfScope pf(c, "stag_rb_fused", comp_sweep_bytes(L.g, p.visc_type, sizeof(T)));
@@ -497,6 +497,16 @@ static void resid_inf(lmm_ctx *c, SLevel<double> &L, double *const phi[3], doubl
 // phi0/rhs0 are the level-0 arrays; phi_zero tells that phi0 is zero on
 // entry, in which case the initial-residual exit of :276-306 cannot trigger
 // unless rhs is zero, and a V-cycle of zero data returns zero exactly.
+// The first down-leg sweep of a level runs as a three-component sweep that
+// can start from a zero field without reading it: then the level's phi need
+// not be zeroed first (every point of the sweep's output is written).
+template <typename T>
+static bool sweep_takes_zeros(lmm_ctx *c, const SLevel<T> &L) {
+  const lmm_params &p = c->prm;
+  return c->fused && c->fused3 && p.stag_mg_smoother == 1 && p.stag_mg_nsmooths_down >= 1 &&
+         fused_rb3_ok(L.g, p.visc_type, c->fused_min_tiles);
+}
+
 template <typename T>
 static void stag_vcycles(lmm_ctx *c, std::vector<SLevel<T>> &S, T *const phi0[3], T *const rhs0[3],
                          bool phi_zero) {
@@ -541,13 +551,19 @@ static void stag_vcycles(lmm_ctx *c, std::vector<SLevel<T>> &S, T *const phi0[3]
     L.K = S[n].K();
     clv.push_back(L);
   }
+  // phi0 from a zero guess: zeroed here unless the first sweep takes zeros
+  const bool zero0 = phi_zero && !need_check && sweep_takes_zeros(c, S[0]);
+  if (phi_zero && !zero0)
+    for (int d = 0; d < dim; d++) launch_fill<T>(c->stream, phi0[d], S[0].g.N, (T)0);
   for (int vc = 1; vc <= p.stag_mg_max_vcycles; vc++) {
     for (int n = 1; n < nc; n++)
-      for (int d = 0; d < dim; d++) launch_fill<T>(c->stream, S[n].phi[d], S[n].g.N, (T)0);
+      if (!sweep_takes_zeros(c, S[n]))
+        for (int d = 0; d < dim; d++) launch_fill<T>(c->stream, S[n].phi[d], S[n].g.N, (T)0);
     for (int n = 0; n < std::min(nc, nl - 1); n++) {
       T *const *ph = (n == 0) ? phi0 : S[n].phi;
       T *const *rh = (n == 0) ? rhs0 : S[n].rhs;
-      smooth<T>(c, S[n], ph, rh, p.stag_mg_nsmooths_down);
+      const bool zin = (n == 0) ? (zero0 && vc == 1) : sweep_takes_zeros(c, S[n]);
+      smooth<T>(c, S[n], ph, rh, p.stag_mg_nsmooths_down, zin);
       {
         // read phi, rhs and the coefficients of every component, write the
         // coarse right-hand side
@@ -724,8 +740,7 @@ static void apply_precon_t(lmm_ctx *c, std::vector<SLevel<T>> &S, std::vector<CL
   hipStream_t s = c->stream;
   const Grid &g = c->g;
   int dim = g.dim;
-  for (int d = 0; d < dim; d++) launch_fill<T>(s, xu[d], g.N, (T)0);
-  stag_vcycles<T>(c, S, xu, bu, true);
+  stag_vcycles<T>(c, S, xu, bu, true);  // x_u from a zero guess
   launch_div_plus<T>(s, g, xu, bp, mac_rhs, 1);
   cc_vcycles<T>(c, C, phi, mac_rhs);
   T *ainv[3] = {C[0].a[0], C[0].a[1], C[0].a[2]};
