@@ -65,6 +65,9 @@ bool stag_resid_restrict_ok(const Grid &g, const Grid &gc, int visc);
 template <typename T>
 bool launch_stag_resid_restrict(hipStream_t s, const Grid &g, const Grid &gc, int visc, T *const phi[3],
                                 T *const rhs[3], T *const crhs[3], const StagCoef<T> &K);
+// whether the fused sweeps take a null input as a field of zeros (the
+// paired plane-marching form does)
+bool fused_rb_takes_zeros();
 // The same fused sweep for the cell-centred red-black Gauss-Seidel (3D).
 template <typename T>
 void launch_cc_rb_fused(hipStream_t s, const Grid &g, const T *pin, T *pout, const T *rhs, T *const a[3],

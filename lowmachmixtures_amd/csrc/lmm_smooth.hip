@@ -706,6 +706,7 @@ LaunchEvents *launch_events() {
 }
 
 void set_rb_variant(int v) { g_rb_variant = v; }
+bool fused_rb_takes_zeros() { return g_rb_variant == 2; }
 void set_rb_target(int t) { g_rb_target = t > 0 ? t : 1; }
 
 // Workgroups aimed for by FP32 (es = 4) and FP64 (es = 8) launches: an FP64

@@ -497,14 +497,13 @@ static void resid_inf(lmm_ctx *c, SLevel<double> &L, double *const phi[3], doubl
 // phi0/rhs0 are the level-0 arrays; phi_zero tells that phi0 is zero on
 // entry, in which case the initial-residual exit of :276-306 cannot trigger
 // unless rhs is zero, and a V-cycle of zero data returns zero exactly.
-// The first down-leg sweep of a level runs as a three-component sweep that
-// can start from a zero field without reading it: then the level's phi need
-// not be zeroed first (every point of the sweep's output is written).
+// A level whose sweeps run as three-component sweeps can start its first
+// smoothing of a cycle from a zero field without reading it: then its phi
+// need not be zeroed first (every point of the sweep's output is written).
 template <typename T>
 static bool sweep_takes_zeros(lmm_ctx *c, const SLevel<T> &L) {
   const lmm_params &p = c->prm;
-  return c->fused && c->fused3 && p.stag_mg_smoother == 1 && p.stag_mg_nsmooths_down >= 1 &&
-         fused_rb3_ok(L.g, p.visc_type, c->fused_min_tiles);
+  return c->fused && c->fused3 && p.stag_mg_smoother == 1 && fused_rb3_ok(L.g, p.visc_type, c->fused_min_tiles);
 }
 
 template <typename T>
@@ -551,19 +550,27 @@ static void stag_vcycles(lmm_ctx *c, std::vector<SLevel<T>> &S, T *const phi0[3]
     L.K = S[n].K();
     clv.push_back(L);
   }
-  // phi0 from a zero guess: zeroed here unless the first sweep takes zeros
-  const bool zero0 = phi_zero && !need_check && sweep_takes_zeros(c, S[0]);
+  // Levels 0..nc-1 are smoothed here; a level's first smoothing in a cycle
+  // is the down-leg one, or the bottom one for the last level (nc = nl).
+  // When that smoothing has at least one sweep and takes zeros, the level's
+  // phi is not zeroed: phi0 (zero guess, first cycle, no initial residual
+  // check) and levels 1..nc-1 (zeroed every cycle).
+  auto takes_zeros = [&](int n) {
+    const int first = (n < nl - 1) ? p.stag_mg_nsmooths_down : p.stag_mg_nsmooths_bottom;
+    return first >= 1 && sweep_takes_zeros(c, S[n]);
+  };
+  const bool zero0 = phi_zero && !need_check && takes_zeros(0);
   if (phi_zero && !zero0)
     for (int d = 0; d < dim; d++) launch_fill<T>(c->stream, phi0[d], S[0].g.N, (T)0);
+  auto zin = [&](int n, int vc) { return (n == 0) ? (zero0 && vc == 1) : takes_zeros(n); };
   for (int vc = 1; vc <= p.stag_mg_max_vcycles; vc++) {
     for (int n = 1; n < nc; n++)
-      if (!sweep_takes_zeros(c, S[n]))
+      if (!takes_zeros(n))
         for (int d = 0; d < dim; d++) launch_fill<T>(c->stream, S[n].phi[d], S[n].g.N, (T)0);
     for (int n = 0; n < std::min(nc, nl - 1); n++) {
       T *const *ph = (n == 0) ? phi0 : S[n].phi;
       T *const *rh = (n == 0) ? rhs0 : S[n].rhs;
-      const bool zin = (n == 0) ? (zero0 && vc == 1) : sweep_takes_zeros(c, S[n]);
-      smooth<T>(c, S[n], ph, rh, p.stag_mg_nsmooths_down, zin);
+      smooth<T>(c, S[n], ph, rh, p.stag_mg_nsmooths_down, zin(n, vc));
       {
         // read phi, rhs and the coefficients of every component, write the
         // coarse right-hand side
@@ -589,7 +596,7 @@ static void stag_vcycles(lmm_ctx *c, std::vector<SLevel<T>> &S, T *const phi0[3]
       int n = nl - 1;
       T *const *ph = (n == 0) ? phi0 : S[n].phi;
       T *const *rh = (n == 0) ? rhs0 : S[n].rhs;
-      smooth<T>(c, S[n], ph, rh, p.stag_mg_nsmooths_bottom);
+      smooth<T>(c, S[n], ph, rh, p.stag_mg_nsmooths_bottom, zin(n, vc));
     }
     for (int n = std::min(nc, nl - 1) - 1; n >= 0; n--) {
       T *const *ph = (n == 0) ? phi0 : S[n].phi;
@@ -620,17 +627,25 @@ template <typename T>
 static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *rhs0) {
   const lmm_params &p = c->prm;
   int nl = c->nl_c;
-  launch_fill<T>(c->stream, phi0, c->g.N, (T)0);
-  auto rb = [&](int n, int sweeps) {
+  auto fused_at = [&](int n) { return c->fused && fused_rb_tz(C[n].g, c->fused_min_tiles, 4, 1) > 0; };
+  // a level's first down-leg sweep from its zero guess can take the zeros
+  // without reading them (every point of its output is written): then phi is
+  // not zeroed first
+  auto takes_zeros = [&](int n) {
+    const int first = (n < nl - 1) ? p.mg_nsmooths_down : p.mg_nsmooths_bottom;
+    return first >= 1 && fused_at(n) && fused_rb_takes_zeros();
+  };
+  if (!takes_zeros(0)) launch_fill<T>(c->stream, phi0, c->g.N, (T)0);
+  auto rb = [&](int n, int sweeps, bool zero_in) {
     T *ph = (n == 0) ? phi0 : C[n].phi;
     const T *rh = (n == 0) ? rhs0 : C[n].rhs;
-    if (c->fused && fused_rb_tz(C[n].g, c->fused_min_tiles, 4, 1) > 0) {
+    if (fused_at(n)) {
       // fused red-black sweeps, ping-pong between phi and C[n].res
       for (int m = 0; m < sweeps; m++) {
         // read phi, rhs and the dim face coefficients, write phi
         ProfScope ps(c, (n == 0) ? "cc_smooth_l0" : "cc_smooth", (double)(3 + C[n].g.dim) * C[n].g.N * sizeof(T),
                      true);
-        const T *src = (m % 2 == 0) ? ph : C[n].res;
+        const T *src = (m == 0 && zero_in) ? nullptr : (m % 2 == 0) ? ph : C[n].res;
         T *dst = (m % 2 == 0) ? C[n].res : ph;
         launch_cc_rb_fused<T>(c->stream, C[n].g, src, dst, rh, C[n].a, c->fused_min_tiles);
       }
@@ -661,9 +676,10 @@ static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *
     clv.push_back(L);
   }
   for (int cyc = 0; cyc < p.mg_max_vcycles; cyc++) {
-    for (int n = 1; n < nc; n++) launch_fill<T>(c->stream, C[n].phi, C[n].g.N, (T)0);
+    for (int n = 1; n < nc; n++)
+      if (!takes_zeros(n)) launch_fill<T>(c->stream, C[n].phi, C[n].g.N, (T)0);
     for (int n = 0; n < std::min(nc, nl - 1); n++) {
-      rb(n, p.mg_nsmooths_down);
+      rb(n, p.mg_nsmooths_down, takes_zeros(n) && (n > 0 || cyc == 0));  // first smoothing of the level
       T *ph = (n == 0) ? phi0 : C[n].phi;
       const T *rh = (n == 0) ? rhs0 : C[n].rhs;
       launch_cc_residual<T>(c->stream, C[n].g, ph, rh, C[n].res, C[n].a);
@@ -674,12 +690,12 @@ static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *
       launch_cc_coarse_vcycle<T>(c->stream, nl - nc, clv.data(), p.mg_nsmooths_down, p.mg_nsmooths_up,
                                  p.mg_nsmooths_bottom);
     } else {
-      rb(nl - 1, p.mg_nsmooths_bottom);
+      rb(nl - 1, p.mg_nsmooths_bottom, takes_zeros(nl - 1) && (nl - 1 > 0 || cyc == 0));
     }
     for (int n = std::min(nc, nl - 1) - 1; n >= 0; n--) {
       T *ph = (n == 0) ? phi0 : C[n].phi;
       launch_cc_prolong_add<T>(c->stream, C[n].g, ph, C[n + 1].phi);
-      rb(n, p.mg_nsmooths_up);
+      rb(n, p.mg_nsmooths_up, false);
     }
     c->vcycles += 1;
   }

@@ -158,3 +158,26 @@ def test_narrow_tile_sweep_matches_oracle(visc, monkeypatch):
     got, fused = _mg_solve(pb, kw, rhs, guess)
     assert fused > 0
     assert bit_equal(got, want), relerr(got, want)
+
+
+@pytest.mark.parametrize("n,coarse", [((64, 32, 2), "1"), ((128, 64, 4), "0"), ((128, 64, 4), "1")])
+@pytest.mark.parametrize("prec", ["fp64", "fp32_mg"])
+def test_first_sweep_from_zeros_equals_filled_guess(n, coarse, prec, monkeypatch):
+    """A level whose first smoothing in a cycle is a fused sweep starts from
+    its zero guess without the guess being written or read.  Covers a single
+    level (z = 2 does not coarsen) and a bottom level smoothed in the level
+    loop (LMM_COARSE=0), in both multigrids, against the colour kernels,
+    which zero the guess in memory."""
+    monkeypatch.setenv("LMM_COARSE", coarse)
+    pb = P.small_problem(3, n, visc_type=-1, seed=109, contrast=30.0)
+    b_u = P.rhs_from_noise_host(pb)
+    precision = {"fp64": S.FP64, "fp32_mg": S.FP32_MG}[prec]
+    out = {}
+    for mode in ("fused", "colour"):
+        monkeypatch.setenv("LMM_SMOOTHER", mode)
+        with S.Context(3, pb.n, pb.dx, visc_type=-1, precision=precision, gmres_max_iter=12) as ctx:
+            dp = S.DeviceProblem(ctx, pb, rhs=b_u)
+            _, info = dp.solve()
+            out[mode] = (info["total_iter"], dp.solution())
+    assert out["fused"][0] == out["colour"][0]
+    assert bit_equal(out["fused"][1][0], out["colour"][1][0]) and bit_equal(out["fused"][1][1], out["colour"][1][1])
