@@ -456,11 +456,13 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
       if (x3) {
         // all components in one launch: each array is compulsory once per
         // sweep (beta_cc and the edges shared by the components)
-        const double bytes = (double)(4 * dim + 1 + (p.visc_type < 0 ? 3 : 0)) * L.g.N * sizeof(T);
+        // (a sweep from a zero guess does not read phi)
+        const bool zin = zero_in && m == 0;
+        const double bytes = (double)(4 * dim + 1 + (p.visc_type < 0 ? 3 : 0) - (zin ? dim : 0)) * L.g.N * sizeof(T);
         ProfScope pf(c, "stag_rb_fused", bytes);
         ProfScope ps(c, fam, bytes, true);
         launch_stag_rb_fused3<T>(c->stream, L.g, p.visc_type, (T)p.stag_mg_omega, src, dst, rhs, K,
-                                 c->fused_min_tiles, zero_in && m == 0);
+                                 c->fused_min_tiles, zin);
       } else {
         for (int a = 0; a < dim; a++) {
           ProfScope pf(c, "stag_rb_fused", comp_sweep_bytes(L.g, p.visc_type, sizeof(T)));
@@ -643,9 +645,10 @@ static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *
       // fused red-black sweeps, ping-pong between phi and C[n].res
       for (int m = 0; m < sweeps; m++) {
         // read phi, rhs and the dim face coefficients, write phi
-        ProfScope ps(c, (n == 0) ? "cc_smooth_l0" : "cc_smooth", (double)(3 + C[n].g.dim) * C[n].g.N * sizeof(T),
-                     true);
-        const T *src = (m == 0 && zero_in) ? nullptr : (m % 2 == 0) ? ph : C[n].res;
+        const bool zin = (m == 0 && zero_in);  // from a zero guess: phi not read
+        ProfScope ps(c, (n == 0) ? "cc_smooth_l0" : "cc_smooth",
+                     (double)(3 + C[n].g.dim - (zin ? 1 : 0)) * C[n].g.N * sizeof(T), true);
+        const T *src = zin ? nullptr : (m % 2 == 0) ? ph : C[n].res;
         T *dst = (m % 2 == 0) ? C[n].res : ph;
         launch_cc_rb_fused<T>(c->stream, C[n].g, src, dst, rh, C[n].a, c->fused_min_tiles);
       }
