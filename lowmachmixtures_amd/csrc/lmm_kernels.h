@@ -193,6 +193,10 @@ void launch_div_by_f32(hipStream_t s, double *v, float *v32, const double *src, 
 void launch_fill_normal_philox(hipStream_t s, double *out, long count, long v0, uint64_t seed, uint64_t step);
 // x += sum_k y_k Z_k, Z in FP32 (at most 32 vectors)
 void launch_update_sol_z32(hipStream_t s, double *x, float *const Z[], const double *y, int i, long n);
+// r = b - A x in one pass (FP64, 3D visc_type -1 / 1); false when not available
+bool launch_apply_matrix_resid(hipStream_t s, const Grid &g, int visc, double *const x_u[3], const double *x_p,
+                               double *const b_u[3], const double *b_p, double *const r_u[3], double *r_p,
+                               const StagCoef<double> &K);
 // FP64 A x of an FP32 x (3D visc_type -1 / 1); false when not available
 bool launch_apply_matrix_x32(hipStream_t s, const Grid &g, int visc, float *const x_u[3], const float *x_p,
                              double *const b_u[3], double *b_p, const StagCoef<double> &K);

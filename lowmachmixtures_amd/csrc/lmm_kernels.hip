@@ -421,9 +421,9 @@ __device__ __forceinline__ T am_component(const Grid &g, int i, int j, int k, co
                                    dxsqinv);
 }
 
-template <typename T, int VISC, typename TX>
+template <typename T, int VISC, typename TX, bool RES>
 __global__ void __launch_bounds__(256) k_apply_matrix3(Grid g, T dxsqinv, T dxinv, Ptr3<TX> xu,
-                                                       const TX *__restrict__ xp,
+                                                       const TX *__restrict__ xp, Ptr3<T> rbu, const T *rbp,
                                                        Ptr3<T> bu, T *bp, typename StagRB<T, 0, VISC>::Args a0,
                                                        typename StagRB<T, 1, VISC>::Args a1,
                                                        typename StagRB<T, 2, VISC>::Args a2,
@@ -444,21 +444,28 @@ __global__ void __launch_bounds__(256) k_apply_matrix3(Grid g, T dxsqinv, T dxin
   const long pmy = i + sy * wrapc(j - 1, g.n[1]) + sz * k;
   const long pmz = i + sy * j + sz * wrapc(k - 1, g.n[2]);
   const T pc = (T)xp[t];
-  bu.c[0][t] = am_component<T, VISC, TX>(g, i, j, k, xu, a0, a1, a2, 0, dxsqinv) + (pc - (T)xp[pmx]) * dxinv;
-  bu.c[1][t] = am_component<T, VISC, TX>(g, i, j, k, xu, a0, a1, a2, 1, dxsqinv) + (pc - (T)xp[pmy]) * dxinv;
-  bu.c[2][t] = am_component<T, VISC, TX>(g, i, j, k, xu, a0, a1, a2, 2, dxsqinv) + (pc - (T)xp[pmz]) * dxinv;
+  // RES: write (Ax - b) * -1 = b - Ax instead of Ax, as k_bmax_neg does to a
+  // stored Ax, so the same bits
+  auto out = [&](T v, T bv) { return RES ? (v - bv) * (T)-1.0 : v; };
+  bu.c[0][t] = out(am_component<T, VISC, TX>(g, i, j, k, xu, a0, a1, a2, 0, dxsqinv) + (pc - (T)xp[pmx]) * dxinv,
+                   RES ? rbu.c[0][t] : (T)0);
+  bu.c[1][t] = out(am_component<T, VISC, TX>(g, i, j, k, xu, a0, a1, a2, 1, dxsqinv) + (pc - (T)xp[pmy]) * dxinv,
+                   RES ? rbu.c[1][t] : (T)0);
+  bu.c[2][t] = out(am_component<T, VISC, TX>(g, i, j, k, xu, a0, a1, a2, 2, dxsqinv) + (pc - (T)xp[pmz]) * dxinv,
+                   RES ? rbu.c[2][t] : (T)0);
   const long xq = wrapc(i + 1, g.n[0]) + sy * j + sz * k;
   const long yq = i + sy * wrapc(j + 1, g.n[1]) + sz * k;
   const long zq = i + sy * j + sz * wrapc(k + 1, g.n[2]);
   T dv = ((T)xu.c[0][xq] - (T)xu.c[0][t]) * dxinv;
   dv = dv + ((T)xu.c[1][yq] - (T)xu.c[1][t]) * dxinv;
   dv = dv + ((T)xu.c[2][zq] - (T)xu.c[2][t]) * dxinv;
-  bp[t] = dv * (T)-1.0;
+  bp[t] = out(dv * (T)-1.0, RES ? rbp[t] : (T)0);
 }
 
 template <typename T, int VISC, typename TX>
 static void launch_am3(hipStream_t s, const Grid &g, T dxsqinv, T dxinv, TX *const x_u[3], const TX *x_p,
-                       T *const b_u[3], T *b_p, const StagCoef<T> &K) {
+                       T *const b_u[3], T *b_p, const StagCoef<T> &K, T *const rb_u[3] = nullptr,
+                       const T *rb_p = nullptr) {
   typename StagRB<T, 0, VISC>::Args a0;
   typename StagRB<T, 1, VISC>::Args a1;
   typename StagRB<T, 2, VISC>::Args a2;
@@ -476,8 +483,32 @@ static void launch_am3(hipStream_t s, const Grid &g, T dxsqinv, T dxinv, TX *con
   a1.e2 = K.bed[2];
   a2.e1 = K.bed[1];
   a2.e2 = K.bed[2];
-  LAUNCH((k_apply_matrix3<T, VISC, TX>), dim3(nblk(g.N)), dim3(TPB), 0, s, g, dxsqinv, dxinv, p3(x_u), x_p,
-         p3(b_u), b_p, a0, a1, a2, K.alpha[0], K.alpha[1], K.alpha[2], K.beta, K.bed[0], K.bed[1], K.bed[2]);
+  if (rb_u) {
+    LAUNCH((k_apply_matrix3<T, VISC, TX, true>), dim3(nblk(g.N)), dim3(TPB), 0, s, g, dxsqinv, dxinv, p3(x_u), x_p,
+           p3(rb_u), rb_p, p3(b_u), b_p, a0, a1, a2, K.alpha[0], K.alpha[1], K.alpha[2], K.beta, K.bed[0], K.bed[1],
+           K.bed[2]);
+    return;
+  }
+  LAUNCH((k_apply_matrix3<T, VISC, TX, false>), dim3(nblk(g.N)), dim3(TPB), 0, s, g, dxsqinv, dxinv, p3(x_u), x_p,
+         p3(b_u), (const T *)nullptr, p3(b_u), b_p, a0, a1, a2, K.alpha[0], K.alpha[1], K.alpha[2], K.beta, K.bed[0],
+         K.bed[1], K.bed[2]);
+}
+
+// r = b - A x (FP64, 3D visc_type -1 / 1) in one pass: the operator's output
+// goes through k_bmax_neg's arithmetic before it is stored; false elsewhere
+bool launch_apply_matrix_resid(hipStream_t s, const Grid &g, int visc, double *const x_u[3], const double *x_p,
+                               double *const b_u[3], const double *b_p, double *const r_u[3], double *r_p,
+                               const StagCoef<double> &K) {
+  const double dxsqinv = 1.0 / (g.dx * g.dx), dxinv = 1.0 / g.dx;
+  if (g.dim == 3 && visc == -1) {
+    launch_am3<double, -1, double>(s, g, dxsqinv, dxinv, x_u, x_p, r_u, r_p, K, b_u, b_p);
+    return true;
+  }
+  if (g.dim == 3 && visc == 1) {
+    launch_am3<double, 1, double>(s, g, dxsqinv, dxinv, x_u, x_p, r_u, r_p, K, b_u, b_p);
+    return true;
+  }
+  return false;
 }
 
 // A x with x held in FP32 (the flexible GMRES's preconditioned vectors) and

@@ -123,9 +123,10 @@ struct lmm_ctx {
   // the fused sweep runs the three face components in one launch
   // (LMM_FUSED_X3=0 in the environment: one launch per component; same bits)
   bool fused3 = true;
-  // GMRES fuses each MGS axpy with the next reduction and the basis scaling
-  // with the FP32 conversion of the preconditioner input (same bits);
-  // LMM_KRYLOV_FUSED=0 in the environment keeps separate passes
+  // GMRES fuses each MGS axpy with the next reduction, the basis scaling
+  // with the FP32 conversion of the preconditioner input and the restart
+  // residual b - Ax into the operator (same bits); LMM_KRYLOV_FUSED=0 in the
+  // environment keeps separate passes
   bool fused_krylov = true;
   // profiling: 0 off, 1 every family, 2 only the kernel-exact fused sweeps
   int prof = 0;
@@ -739,6 +740,21 @@ static void apply_matrix_dev(lmm_ctx *c, const SV &x, const SV &b) {
   launch_apply_matrix<double>(c->stream, c->g, c->prm.visc_type, x.u, x.p, b.u, b.p, c->s64[0].K());
 }
 
+// r = b - A x with b the context's right-hand side (k_bmax_neg's
+// arithmetic on A x; one pass where the fused kernel applies)
+static void residual_dev(lmm_ctx *c, const SV &x, double *r) {
+  SV b = sv_of(c, c->b), rv = sv_of(c, r);
+  const int visc = c->prm.visc_type;
+  if (c->fused_krylov && c->g.dim == 3 && (visc == -1 || visc == 1)) {
+    // read x, b and the coefficients, write r
+    ProfScope ps(c, "apply_matrix", (double)c->g.N * 8.0 * 19);
+    launch_apply_matrix_resid(c->stream, c->g, visc, x.u, x.p, b.u, b.p, rv.u, rv.p, c->s64[0].K());
+    return;
+  }
+  apply_matrix_dev(c, x, rv);
+  launch_bmax_neg(c->stream, r, c->b, (long)(c->g.dim + 1) * c->g.N);
+}
+
 // FP64 A z of an FP32 Stokes vector z (same bits as widening z first)
 static void apply_matrix_z32(lmm_ctx *c, float *z, const SV &b) {
   SVT<float> zf = svt_of<float>(c, z);
@@ -957,8 +973,7 @@ static int gmres_ref(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
   double norm_resid = 0, norm_init_resid = 0, norm_resid_Stokes = 0;
   bool converged = false;
   for (;;) {
-    apply_matrix_dev(c, x, tmp);
-    launch_bmax_neg(s, c->tmp, c->b, L);
+    residual_dev(c, x, c->tmp);
     stokes_dot_dev(c, c->tmp, c->tmp, c->scal + 3008, 1);
     precon64(c, tmp, r, theta_alpha);
     stokes_dot_dev(c, c->r, c->r, c->scal + 3012, 1);
@@ -1071,8 +1086,7 @@ static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
   double rnorm = 0;
   bool converged = false;
   for (;;) {
-    apply_matrix_dev(c, x, tmp);
-    launch_bmax_neg(s, c->tmp, c->b, L);
+    residual_dev(c, x, c->tmp);
     stokes_dot_dev(c, c->tmp, c->tmp, c->scal + 3008, 1);
     fetch(c, c->scal + 3008, 4, hb);
     rnorm = hb[0];
