@@ -18,7 +18,9 @@ and the FP32-MG error against the FP64 solution sit beside it.
 sweep (k_rb_march2x3: the three face components in one launch) in the
 reported form: its algorithmic bytes per launch (read phi, rhs and alpha of
 each component, beta_cc and the three edge viscosities once, write phi of
-each component: 16 arrays of 256^3 values) over the launch durations that
+each component: 16 arrays of 256^3 values, 13 for a V-cycle's first sweep,
+which starts from the zero guess without reading phi; averaged over the
+launches) over the launch durations that
 the dispatch itself timestamps on the solver's stream inside the timed
 region.  ``traffic`` is the HBM traffic per launch that rocprofv3's
 FETCH_SIZE and WRITE_SIZE counters measured for the same kernel on the same
@@ -143,7 +145,7 @@ def time_form(S, ctx, dp, form, steps, warmup, quiet):
     ctx.profiling(False)
     fams = {k: ctx.kernel_stats(k) for k in ("stag_smooth_l0", "stag_smooth", "stag_restrict_l0", "stag_restrict",
                                               "stag_prolong",
-                                              "stag_coarse", "cc_smooth_l0", "cc_smooth", "cc_coarse",
+                                              "stag_coarse", "cc_smooth_l0", "cc_smooth", "cc_restrict_l0", "cc_coarse",
                                               "apply_matrix", "krylov")}
     ms = (t1 - t0) * 1e3 / steps
     log(f"{form}: {ms:.2f} ms/solve, iters {iters}, status {info['status']}", quiet)

@@ -131,6 +131,9 @@ template <typename T>
 void launch_cc_residual(hipStream_t s, const Grid &g, const T *phi, const T *rhs, T *res,
                         T *const a[3]);
 template <typename T>
+void launch_cc_resid_restrict(hipStream_t s, const Grid &gf, const T *phi, const T *rhs, T *const a[3],
+                              const Grid &gc, T *coarse_rhs);
+template <typename T>
 void launch_cc_prolong_add(hipStream_t s, const Grid &gf, T *fine, const T *coarse);
 
 // ---- elementwise / conversions ----

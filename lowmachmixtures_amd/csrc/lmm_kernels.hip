@@ -292,6 +292,63 @@ void launch_cc_residual(hipStream_t s, const Grid &g, const T *phi, const T *rhs
   LAUNCH(k_cc_residual<T>, dim3(nblk(g.N)), dim3(TPB), 0, s, g, inv, phi, rhs, res, p3(a));
 }
 
+// Residual of the cell-centred level and its average onto the next coarser
+// level in one pass; the fine residual is never stored.  A workgroup covers
+// a run of 64 fine cells in x on the 2 (2D) or 2x2 (3D) fine rows of one
+// coarse row: each thread forms one fine residual (64-wide coalesced rows, as
+// the residual pass alone reads them) into LDS, then one thread per coarse
+// cell sums its 2^dim children.  The fine residuals and the sum are the ones
+// cc_residual_at / cc_restrict_at form, in the same order, so the coarse
+// right-hand side has the same bits as the two-pass form
+// (stag_mg_solver.f90:873-968 restriction of the residual).
+template <typename T, int DIM>
+__global__ void __launch_bounds__(256) k_cc_resid_restrict(Grid gc, Grid gf, T inv, const T *phi,
+                                                           const T *__restrict__ rhs, Ptr3<T> A, T *c) {
+  __shared__ T sr[DIM == 3 ? 2 : 1][2][64];
+  const int tiles_x = (gf.n[0] + 63) >> 6;
+  const long w = xcd_block();
+  const int tx = (int)(w % tiles_x);
+  const long row = w / tiles_x;  // coarse row: cj + ny_c * ck
+  const int cj = (int)(row % gc.n[1]), ck = (int)(row / gc.n[1]);
+  const int x = threadIdx.x & 63, dy = (threadIdx.x >> 6) & 1, dz = (DIM == 3) ? (threadIdx.x >> 7) : 0;
+  const int fi = tx * 64 + x, fj = 2 * cj + dy, fk = (DIM == 3) ? 2 * ck + dz : 0;
+  if (fi < gf.n[0]) {
+    const long p = fi + gf.s[1] * fj + gf.s[2] * fk;
+    T diag;
+    sr[dz][dy][x] = rhs[p] - cc_apply<T>(gf, fi, fj, fk, phi, A, inv, diag);
+  }
+  __syncthreads();
+  const int ci = tx * 32 + (int)threadIdx.x;
+  if (threadIdx.x >= 32 || ci >= gc.n[0]) return;
+  const int l = 2 * (int)threadIdx.x;
+  const long t = ci + gc.s[1] * cj + gc.s[2] * ck;
+  if (DIM == 2) {
+    c[t] = (T)0.25 * (sr[0][0][l] + sr[0][0][l + 1] + sr[0][1][l] + sr[0][1][l + 1]);
+  } else {
+    c[t] = (T)0.125 * (sr[0][0][l] + sr[0][0][l + 1] + sr[0][1][l] + sr[0][1][l + 1] + sr[1][0][l] +
+                       sr[1][0][l + 1] + sr[1][1][l] + sr[1][1][l + 1]);
+  }
+}
+
+template <typename T>
+void launch_cc_resid_restrict(hipStream_t s, const Grid &gf, const T *phi, const T *rhs, T *const a[3],
+                              const Grid &gc, T *coarse_rhs) {
+  for (int d = 0; d < gf.dim; d++)
+    if (gf.n[d] != 2 * gc.n[d]) {
+      report_error("cc residual-restriction: fine extent is not twice the coarse one");
+      return;
+    }
+  T inv = (T)(1.0 / (gf.dx * gf.dx));
+  // one workgroup per 64-cell run of each coarse row
+  const long nwg = (long)((gf.n[0] + 63) / 64) * gc.n[1] * (gf.dim == 3 ? gc.n[2] : 1);
+  if (gf.dim == 2)
+    LAUNCH((k_cc_resid_restrict<T, 2>), dim3((unsigned)nwg), dim3(128), 0, s, gc, gf, inv, phi, rhs, p3(a),
+           coarse_rhs);
+  else
+    LAUNCH((k_cc_resid_restrict<T, 3>), dim3((unsigned)nwg), dim3(256), 0, s, gc, gf, inv, phi, rhs, p3(a),
+           coarse_rhs);
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) k_cc_prolong_add(Grid gf, Grid gc, T *f, const T *__restrict__ c) {
   long t = xcd_block() * blockDim.x + threadIdx.x;
@@ -1134,6 +1191,8 @@ void launch_div_species(hipStream_t s, const Grid &g, int ns, const double *flux
   template void launch_nodal_restrict<T>(hipStream_t, const Grid &, const T *, T *);                         \
   template void launch_cc_rb_colour<T>(hipStream_t, const Grid &, int, T *, const T *, T *const[3]);         \
   template void launch_cc_residual<T>(hipStream_t, const Grid &, const T *, const T *, T *, T *const[3]);    \
+  template void launch_cc_resid_restrict<T>(hipStream_t, const Grid &, const T *, const T *, T *const[3],   \
+                                            const Grid &, T *);                                            \
   template void launch_cc_prolong_add<T>(hipStream_t, const Grid &, T *, const T *);                         \
   template void launch_fill<T>(hipStream_t, T *, long, T);                                                   \
   template void launch_scale<T>(hipStream_t, T *, long, T);                                                  \

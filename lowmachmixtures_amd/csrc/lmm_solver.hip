@@ -686,8 +686,15 @@ static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *
       rb(n, p.mg_nsmooths_down, takes_zeros(n) && (n > 0 || cyc == 0));  // first smoothing of the level
       T *ph = (n == 0) ? phi0 : C[n].phi;
       const T *rh = (n == 0) ? rhs0 : C[n].rhs;
-      launch_cc_residual<T>(c->stream, C[n].g, ph, rh, C[n].res, C[n].a);
-      launch_cc_restrict<T>(c->stream, C[n + 1].g, C[n].res, C[n + 1].rhs);
+      if (c->fused_resid_restrict) {
+        // read phi, rhs and the dim face coefficients, write the coarse rhs
+        ProfScope ps(c, (n == 0) ? "cc_restrict_l0" : "cc_restrict",
+                     ((double)(2 + C[n].g.dim) * C[n].g.N + (double)C[n + 1].g.N) * sizeof(T));
+        launch_cc_resid_restrict<T>(c->stream, C[n].g, ph, rh, C[n].a, C[n + 1].g, C[n + 1].rhs);
+      } else {
+        launch_cc_residual<T>(c->stream, C[n].g, ph, rh, C[n].res, C[n].a);
+        launch_cc_restrict<T>(c->stream, C[n + 1].g, C[n].res, C[n + 1].rhs);
+      }
     }
     if (nc < nl) {
       ProfScope ps(c, "cc_coarse", 0.0);
