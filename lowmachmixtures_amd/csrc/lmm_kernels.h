@@ -17,6 +17,8 @@ constexpr int RED_BLOCKS = 1024;
 // hipGetLastError() and, when debug_sync is set (environment LMM_DEBUG_SYNC=1),
 // synchronises the stream so a failing kernel is named in first_error().
 void set_debug_sync(int on);
+// z depth of the operator's brick order (0: row order)
+void set_am_brick_depth(int dz);
 void post_launch(hipStream_t s, const char *name);
 // Kernel-exact timing: a profiling scope may leave a start / stop event pair
 // here; the next fused-sweep launch then has the dispatch itself timestamp

@@ -478,6 +478,18 @@ __device__ __forceinline__ T am_component(const Grid &g, int i, int j, int k, co
                                    dxsqinv);
 }
 
+// Slab depth (z planes) of the operator's brick order: the largest of the
+// configured depth, 8, 4, 2, 1 that divides nz; 0 (row order) when x is not a
+// multiple of 64 or y of 4, or when brick order is switched off.
+static int g_am_dz = 16;
+void set_am_brick_depth(int dz) { g_am_dz = dz; }
+static int am_brick_depth(const Grid &g) {
+  if (g_am_dz <= 0 || g.dim != 3 || g.n[0] % 64 != 0 || g.n[1] % 4 != 0) return 0;
+  for (int d = g_am_dz; d >= 1; d /= 2)
+    if (g.n[2] % d == 0) return d;
+  return 1;
+}
+
 template <typename T, int VISC, typename TX, bool RES>
 __global__ void __launch_bounds__(256) k_apply_matrix3(Grid g, T dxsqinv, T dxinv, Ptr3<TX> xu,
                                                        const TX *__restrict__ xp, Ptr3<T> rbu, const T *rbp,
@@ -487,16 +499,31 @@ __global__ void __launch_bounds__(256) k_apply_matrix3(Grid g, T dxsqinv, T dxin
                                                        const T *__restrict__ al0, const T *__restrict__ al1,
                                                        const T *__restrict__ al2, const T *__restrict__ be,
                                                        const T *__restrict__ ed0, const T *__restrict__ ed1,
-                                                       const T *__restrict__ ed2) {
+                                                       const T *__restrict__ ed2, int dz) {
   // rhs is not read by apply(); it gets a valid address of the right type
   a0 = StagRB<T, 0, VISC>::bind(a0, be, al0, be, ed0, ed1);
   a1 = StagRB<T, 1, VISC>::bind(a1, be, al1, be, ed0, ed2);
   a2 = StagRB<T, 2, VISC>::bind(a2, be, al2, be, ed1, ed2);
-  const long t = xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= g.N) return;
-  int i, j, k;
-  decode(g, t, i, j, k);
   const long sy = g.s[1], sz = g.s[2];
+  long t;
+  int i, j, k;
+  if (dz > 0) {
+    // bricks of 64 x 4 x 1 cells; consecutive workgroups go x-brick, then z
+    // within a slab of dz planes, then y, then slab, so the workgroups in
+    // flight on one XCD cover a compact box whose neighbours are in flight
+    // too (row order keeps a whole plane of every array in flight, more than
+    // that XCD's L2 holds, and re-reads the z-neighbour planes from HBM)
+    const unsigned w = (unsigned)xcd_block(), bx = (unsigned)g.n[0] >> 6, by = (unsigned)g.n[1] >> 2;
+    const unsigned r = w / bx, r2 = r / (unsigned)dz, zs = r2 / by;
+    i = (int)((w - r * bx) << 6) + (int)(threadIdx.x & 63);
+    j = (int)((r2 - zs * by) << 2) + (int)(threadIdx.x >> 6);
+    k = (int)(zs * (unsigned)dz + (r - r2 * (unsigned)dz));
+    t = i + sy * j + sz * k;
+  } else {
+    t = xcd_block() * blockDim.x + threadIdx.x;
+    if (t >= g.N) return;
+    decode(g, t, i, j, k);
+  }
   const long pmx = wrapc(i - 1, g.n[0]) + sy * j + sz * k;
   const long pmy = i + sy * wrapc(j - 1, g.n[1]) + sz * k;
   const long pmz = i + sy * j + sz * wrapc(k - 1, g.n[2]);
@@ -540,15 +567,16 @@ static void launch_am3(hipStream_t s, const Grid &g, T dxsqinv, T dxinv, TX *con
   a1.e2 = K.bed[2];
   a2.e1 = K.bed[1];
   a2.e2 = K.bed[2];
+  const int dz = am_brick_depth(g);
   if (rb_u) {
     LAUNCH((k_apply_matrix3<T, VISC, TX, true>), dim3(nblk(g.N)), dim3(TPB), 0, s, g, dxsqinv, dxinv, p3(x_u), x_p,
            p3(rb_u), rb_p, p3(b_u), b_p, a0, a1, a2, K.alpha[0], K.alpha[1], K.alpha[2], K.beta, K.bed[0], K.bed[1],
-           K.bed[2]);
+           K.bed[2], dz);
     return;
   }
   LAUNCH((k_apply_matrix3<T, VISC, TX, false>), dim3(nblk(g.N)), dim3(TPB), 0, s, g, dxsqinv, dxinv, p3(x_u), x_p,
          p3(b_u), (const T *)nullptr, p3(b_u), b_p, a0, a1, a2, K.alpha[0], K.alpha[1], K.alpha[2], K.beta, K.bed[0],
-         K.bed[1], K.bed[2]);
+         K.bed[1], K.bed[2], dz);
 }
 
 // r = b - A x (FP64, 3D visc_type -1 / 1) in one pass: the operator's output
