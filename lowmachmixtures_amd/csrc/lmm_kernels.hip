@@ -479,9 +479,9 @@ __device__ __forceinline__ T am_component(const Grid &g, int i, int j, int k, co
 }
 
 // Slab depth (z planes) of the operator's brick order: the largest of the
-// configured depth, 8, 4, 2, 1 that divides nz; 0 (row order) when x is not a
+// configured depth and its halvings that divides nz; 0 (row order) when x is not a
 // multiple of 64 or y of 4, or when brick order is switched off.
-static int g_am_dz = 16;
+static int g_am_dz = 32;
 void set_am_brick_depth(int dz) { g_am_dz = dz; }
 static int am_brick_depth(const Grid &g) {
   if (g_am_dz <= 0 || g.dim != 3 || g.n[0] % 64 != 0 || g.n[1] % 4 != 0) return 0;

@@ -1235,7 +1235,7 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
     const char *z3 = getenv("LMM_Z32");
     c->z32_direct = !(z3 && z3[0] == '0');
     const char *ab = getenv("LMM_AM_BRICK");
-    set_am_brick_depth(ab ? atoi(ab) : 16);
+    set_am_brick_depth(ab ? atoi(ab) : 32);
     const char *co = getenv("LMM_COARSE");
     c->coarse_one_launch = !(co && co[0] == '0');
   }

@@ -22,7 +22,7 @@ def _faces(shape, seed):
 
 @pytest.mark.parametrize("n", [(64, 8, 8), (128, 16, 32), (192, 16, 8)])
 @pytest.mark.parametrize("visc", [-1, 1])
-@pytest.mark.parametrize("depth", ["16", "1"])
+@pytest.mark.parametrize("depth", ["32", "16", "1"])
 def test_brick_order_operator_matches_oracle(n, visc, depth, monkeypatch):
     monkeypatch.setenv("LMM_AM_BRICK", depth)
     pb = P.small_problem(3, n, visc_type=visc, seed=31)
@@ -47,7 +47,7 @@ def test_brick_order_solve_equals_row_order(precision, monkeypatch):
     pb = P.small_problem(3, (128, 32, 32), visc_type=-1, seed=35, contrast=100.0)
     b_u = P.rhs_from_noise_host(pb)
     out = []
-    for mode in ("0", "16"):
+    for mode in ("0", "32"):
         monkeypatch.setenv("LMM_AM_BRICK", mode)
         with S.Context(3, pb.n, pb.dx, visc_type=-1, precision=precision) as ctx:
             dp = S.DeviceProblem(ctx, pb, rhs=b_u)
