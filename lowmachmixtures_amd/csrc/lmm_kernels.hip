@@ -651,9 +651,61 @@ __global__ void __launch_bounds__(256) k_div(Grid g, T dxinv, Ptr3<T> u, const T
   div[t] = v;
 }
 
+// k_div on an x-pair of cells per thread (x extents are even on every level
+// the library builds, so a pair never leaves its row): the pair's own face
+// values, the y / z neighbour rows, b_p and div move as 2-wide loads and
+// stores. Each cell's value is the same expression in the same order as
+// k_div's, so the same bits.
+template <typename T>
+__global__ void __launch_bounds__(256) k_div2(Grid g, T dxinv, Ptr3<T> u, const T *b_p, T *div, int mode) {
+  const long t = 2 * (xcd_block() * blockDim.x + threadIdx.x);
+  if (t >= g.N) return;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  // x neighbour of cell i is i + 1 (i is even); of cell i + 1 it wraps
+  const long x1 = wp(i + 1, g.n[0]) + g.s[1] * j + g.s[2] * k;
+  const long yr = i + g.s[1] * wp(j, g.n[1]) + g.s[2] * k;
+  T a0, a1, n0, n1;
+  ld2(u.c[0], t, a0, a1);
+  T v0 = (a1 - a0) * dxinv;
+  T v1 = (u.c[0][x1] - a1) * dxinv;
+  if (mode == 3) {
+    T d0, d1;
+    ld2((const T *)div, t, d0, d1);
+    v0 = d0 + v0;
+    v1 = d1 + v1;
+  }
+  ld2(u.c[1], t, a0, a1);
+  ld2(u.c[1], yr, n0, n1);
+  v0 = v0 + (n0 - a0) * dxinv;
+  v1 = v1 + (n1 - a1) * dxinv;
+  if (g.dim == 3) {
+    const long zr = i + g.s[1] * j + g.s[2] * wp(k, g.n[2]);
+    ld2(u.c[2], t, a0, a1);
+    ld2(u.c[2], zr, n0, n1);
+    v0 = v0 + (n0 - a0) * dxinv;
+    v1 = v1 + (n1 - a1) * dxinv;
+  }
+  if (mode == 1) {
+    T b0, b1;
+    ld2(b_p, t, b0, b1);
+    v0 = v0 + b0;
+    v1 = v1 + b1;
+  }
+  if (mode == 2) {
+    v0 = v0 * (T)-1.0;
+    v1 = v1 * (T)-1.0;
+  }
+  st2(div, t, v0, v1);
+}
+
 template <typename T>
 void launch_div_plus(hipStream_t s, const Grid &g, T *const u[3], const T *b_p, T *div, int mode) {
   T dxinv = (T)(1.0 / g.dx);
+  if (g.n[0] % 2 == 0) {
+    LAUNCH(k_div2<T>, dim3(nblk(g.N / 2)), dim3(TPB), 0, s, g, dxinv, p3(u), b_p, div, mode);
+    return;
+  }
   LAUNCH(k_div<T>, dim3(nblk(g.N)), dim3(TPB), 0, s, g, dxinv, p3(u), b_p, div, mode);
 }
 
