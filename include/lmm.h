@@ -90,6 +90,23 @@ typedef struct {
   double seconds;              /* wall time of the call */
 } lmm_gmres_info;
 
+/* Kernel forms (no counterpart in the reference).  Every choice gives the
+ * same bits; they exist for measurement, tuning and the tests that compare
+ * kernel forms.  A new context takes the defaults, overridden by the
+ * LMM_* environment switches present at its creation. */
+typedef struct {
+  int fused_sweeps;        /* 1: fused red-black sweeps where a level allows; 0: colour kernels */
+  int fused_min_tiles;     /* a level below this many workgroups per launch uses the colour kernels */
+  int fused_kernel;        /* fused-sweep form: 0 plane marching, 1 block, 2 paired plane marching */
+  int fused_target;        /* workgroups the z-slab choice aims for (FP32; FP64 twice) */
+  int fused_x3;            /* 1: the three face components in one launch */
+  int resid_restrict;      /* 1: residual and restriction in one pass */
+  int coarse_one_launch;   /* 1: levels of at most 16^3 cells in one single-workgroup launch */
+  int am_brick;            /* z depth of the operator's brick order (0: row order) */
+  int z32_direct;          /* 1: FGMRES reads its FP32 preconditioned vectors directly */
+  int krylov_fused;        /* 1: fused Gram-Schmidt and basis-scaling passes */
+} lmm_kernel_options;
+
 /* ---------------- context ---------------- */
 
 /* Fills p with the reference defaults (probin_gmres.f90:119-157, visc_type 1). */
@@ -103,6 +120,9 @@ int lmm_ctx_destroy(lmm_ctx *ctx);
 int lmm_ctx_set_params(lmm_ctx *ctx, const lmm_params *p);
 int lmm_ctx_get_params(const lmm_ctx *ctx, lmm_params *p);
 int lmm_ctx_synchronize(lmm_ctx *ctx);
+void lmm_kernel_options_default(lmm_kernel_options *o);
+int lmm_ctx_get_kernel_options(const lmm_ctx *ctx, lmm_kernel_options *o);
+int lmm_ctx_set_kernel_options(lmm_ctx *ctx, const lmm_kernel_options *o);
 /* Number of staggered multigrid levels (compute_nlevs_mg, stag_mg_layout.f90:179-222). */
 int lmm_ctx_nlevels(const lmm_ctx *ctx, int *nlevs_stag, int *nlevs_cc);
 /* Last HIP or argument error message of this context. */

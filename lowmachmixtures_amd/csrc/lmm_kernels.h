@@ -17,8 +17,24 @@ constexpr int RED_BLOCKS = 1024;
 // hipGetLastError() and, when debug_sync is set (environment LMM_DEBUG_SYNC=1),
 // synchronises the stream so a failing kernel is named in first_error().
 void set_debug_sync(int on);
-// z depth of the operator's brick order (0: row order)
-void set_am_brick_depth(int dz);
+
+// Kernel-choice settings of one context.  Every setting gives the same bits;
+// they exist for measurement and for the tests that compare kernel forms.
+struct Tuning {
+  int rb_variant = 2;   // fused sweep: 0 plane marching, 1 block, 2 paired plane marching
+  int rb_target = 768;  // workgroups the z-slab choice aims for (FP32; FP64 twice)
+  int am_dz = 32;       // z depth of the operator's brick order (0: row order)
+};
+// The settings the launchers on this thread use: those of the context whose
+// library call is running on it (TuningScope), else the defaults.
+const Tuning &tuning();
+struct TuningScope {
+  const Tuning *prev;
+  explicit TuningScope(const Tuning *t);
+  ~TuningScope();
+  TuningScope(const TuningScope &) = delete;
+  TuningScope &operator=(const TuningScope &) = delete;
+};
 void post_launch(hipStream_t s, const char *name);
 // Kernel-exact timing: a profiling scope may leave a start / stop event pair
 // here; the next fused-sweep launch then has the dispatch itself timestamp
@@ -40,12 +56,8 @@ struct Faces {
 
 // ---- staggered operator ----
 // Fused red-black sweep of one component (3D, visc_type -1 / 1): reads the
-// old field from pin, writes the new one to pout (lmm_smooth.hip).
-// 0: plane-marching form, 1: block form, 2: plane marching with paired loads (the default)
-void set_rb_variant(int v);
-// workgroup count the plane-marching forms aim for when they choose their
-// z-slab thickness
-void set_rb_target(int t);
+// old field from pin, writes the new one to pout (lmm_smooth.hip); the form
+// and the z-slab target come from tuning().
 // z-slab thickness for element size es (4 or 8 bytes) when one launch
 // sweeps ncomp components, 0: not eligible
 int fused_rb_tz(const Grid &g, int min_tiles, int es, int ncomp);

@@ -16,6 +16,14 @@ static int g_debug_sync = 0;
 // per host thread: each thread drives its own contexts (lmm_gmres_batched)
 static thread_local char g_first_error[256] = {0};
 void set_debug_sync(int on) { g_debug_sync = on; }
+// the context settings in force on this thread (null: the defaults)
+static thread_local const Tuning *t_tuning = nullptr;
+const Tuning &tuning() {
+  static const Tuning defaults;
+  return t_tuning ? *t_tuning : defaults;
+}
+TuningScope::TuningScope(const Tuning *t) : prev(t_tuning) { t_tuning = t; }
+TuningScope::~TuningScope() { t_tuning = prev; }
 const char *first_error() { return g_first_error; }
 void report_error(const char *msg) {
   if (g_first_error[0] == 0) snprintf(g_first_error, sizeof(g_first_error), "%s", msg);
@@ -481,11 +489,9 @@ __device__ __forceinline__ T am_component(const Grid &g, int i, int j, int k, co
 // Slab depth (z planes) of the operator's brick order: the largest of the
 // configured depth and its halvings that divides nz; 0 (row order) when x is not a
 // multiple of 64 or y of 4, or when brick order is switched off.
-static int g_am_dz = 32;
-void set_am_brick_depth(int dz) { g_am_dz = dz; }
 static int am_brick_depth(const Grid &g) {
-  if (g_am_dz <= 0 || g.dim != 3 || g.n[0] % 64 != 0 || g.n[1] % 4 != 0) return 0;
-  for (int d = g_am_dz; d >= 1; d /= 2)
+  if (tuning().am_dz <= 0 || g.dim != 3 || g.n[0] % 64 != 0 || g.n[1] % 4 != 0) return 0;
+  for (int d = tuning().am_dz; d >= 1; d /= 2)
     if (g.n[2] % d == 0) return d;
   return 1;
 }
@@ -944,8 +950,12 @@ void launch_finalize_max(hipStream_t s, const double *partial, double *out) {
   LAUNCH(k_finalize_max, dim3(1), dim3(64), 0, s, partial, out);
 }
 
-// sum_umac_press velocity part for a tile spanning the box: every face once,
-// plus 0.5*u on the lo and hi+1 copies -> face index 0 weighted 3.
+// sum_umac_press velocity part (norm_inner_product.f90:579-721): every face
+// of the reference layout once (face 0 and its periodic copy hi+1: weight 2),
+// plus 0.5*u on the lo and hi+1 copies in a direction where the reference's
+// tile spans the whole box (weight 3).  Its tiles are BoxLib's default
+// 1024000 x 8 x 8: x always spans, y and z only when the box is at most 8
+// cells long there.
 template <typename T>
 __global__ void __launch_bounds__(RED_TPB) k_sum_faces_w(Grid g, Ptr3<T> u, double *partial) {
   __shared__ double sh[8];
@@ -955,7 +965,10 @@ __global__ void __launch_bounds__(RED_TPB) k_sum_faces_w(Grid g, Ptr3<T> u, doub
     int i, j, k;
     decode(g, q, i, j, k);
     int c[3] = {i, j, k};
-    for (int d = 0; d < g.dim; d++) acc[d] += (c[d] == 0 ? 3.0 : 1.0) * (double)u.c[d][q];
+    for (int d = 0; d < g.dim; d++) {
+      const double w0 = (d == 0 || g.n[d] <= 8) ? 3.0 : 2.0;
+      acc[d] += (c[d] == 0 ? w0 : 1.0) * (double)u.c[d][q];
+    }
   }
   for (int seg = 0; seg < 4; seg++) {
     double r = block_sum(seg < 3 ? acc[seg] : 0.0, sh);

@@ -468,14 +468,11 @@ __global__ void __launch_bounds__(256) k_rb_block(Grid g, int nbx, int nby, int 
   }
 }
 
-int g_rb_variant = 2;  // 0: plane marching, 1: block, 2: plane marching with paired loads
-int g_rb_target = 768;  // workgroups the plane-marching forms aim for (FP32; FP64 twice)
-
 template <typename T, typename P>
 void launch_rb(hipStream_t s, const Grid &g, int tz, const T *pin, T *pout, const typename P::Args &args) {
   const T *q[5];
   P::ptrs(args, q);
-  if (g_rb_variant == 1) {
+  if (tuning().rb_variant == 1) {
     const int nbx = g.n[0] / BX, nby = g.n[1] / BY, nbz = g.n[2] / BZ;
     const int nb = nbx * nby * nbz;
     hipLaunchKernelGGL((k_rb_block<T, P>), dim3(nb), dim3(256), 0, s, g, nbx, nby, nb, pin, pout, args, q[0], q[1],
@@ -485,7 +482,7 @@ void launch_rb(hipStream_t s, const Grid &g, int tz, const T *pin, T *pout, cons
   }
   const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
   const int ntiles = ntx * nty * ntz;
-  if (g_rb_variant == 2) {
+  if (tuning().rb_variant == 2) {
     LaunchEvents *ev = launch_events();
     if (ev->start) {
       hipExtLaunchKernelGGL((k_rb_march2<T, P>), dim3(ntiles), dim3(256), 0, s, ev->start, ev->stop, 0, g, tz, ntx,
@@ -705,15 +702,16 @@ LaunchEvents *launch_events() {
   return &ev;
 }
 
-void set_rb_variant(int v) { g_rb_variant = v; }
-bool fused_rb_takes_zeros() { return g_rb_variant == 2; }
-void set_rb_target(int t) { g_rb_target = t > 0 ? t : 1; }
+bool fused_rb_takes_zeros() { return tuning().rb_variant == 2; }
 
 // Workgroups aimed for by FP32 (es = 4) and FP64 (es = 8) launches: an FP64
 // workgroup's LDS rings allow 3 resident per CU instead of 4, and twice the
 // count keeps the last round of workgroups from running on a third of the
 // machine.
-static int rb_target(int es) { return es == 8 ? 2 * g_rb_target : g_rb_target; }
+static int rb_target(int es) {
+  const int t = tuning().rb_target > 0 ? tuning().rb_target : 1;
+  return es == 8 ? 2 * t : t;
+}
 
 // Plane count per workgroup, or 0 when the level cannot use the fused
 // sweep.  The plane-marching forms take the thickest z-slab (at most 32
@@ -724,7 +722,7 @@ static int rb_target(int es) { return es == 8 ? 2 * g_rb_target : g_rb_target; }
 static int rb_tz(const Grid &g, int min_tiles, int es, int ncomp, int fx, int fy) {
   if (g.dim != 3 || g.n[0] % fx != 0 || g.n[1] % fy != 0) return 0;
   const int per_plane_set = (g.n[0] / fx) * (g.n[1] / fy);
-  const int tz_min = (g_rb_variant == 2) ? 2 : 8;
+  const int tz_min = (tuning().rb_variant == 2) ? 2 : 8;
   const int t = rb_target(es);
   const int target = t > min_tiles ? t : min_tiles;
   int tz = 32;
@@ -736,7 +734,7 @@ static int rb_tz(const Grid &g, int min_tiles, int es, int ncomp, int fx, int fy
 }
 
 int fused_rb_tz(const Grid &g, int min_tiles, int es, int ncomp) {
-  if (g_rb_variant == 1) {
+  if (tuning().rb_variant == 1) {
     if (g.dim != 3 || g.n[0] % BX != 0 || g.n[1] % BY != 0 || g.n[2] % BZ != 0) return 0;
     const int nb = (g.n[0] / BX) * (g.n[1] / BY) * (g.n[2] / BZ);
     return nb >= min_tiles ? BZ : 0;
@@ -780,7 +778,7 @@ static int rb3_shape(const Grid &g, int min_tiles, int es, int &tx) {
 
 bool fused_rb3_ok(const Grid &g, int visc, int min_tiles) {
   int tx;
-  return g_rb_variant == 2 && (visc == -1 || visc == 1) && rb3_shape(g, min_tiles, 4, tx) > 0;
+  return tuning().rb_variant == 2 && (visc == -1 || visc == 1) && rb3_shape(g, min_tiles, 4, tx) > 0;
 }
 
 template <typename T>

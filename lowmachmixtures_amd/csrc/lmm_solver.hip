@@ -6,6 +6,7 @@
 
 #include <chrono>
 #include <atomic>
+#include <mutex>
 #include <cstdlib>
 #include <thread>
 #include <cmath>
@@ -128,6 +129,11 @@ struct lmm_ctx {
   // residual b - Ax into the operator (same bits); LMM_KRYLOV_FUSED=0 in the
   // environment keeps separate passes
   bool fused_krylov = true;
+  // fused-sweep form, z-slab target and operator brick depth of this
+  // context's launches (from the environment at creation;
+  // lmm_ctx_set_kernel_options changes them)
+  Tuning tune;
+  lmm_kernel_options kopt;
   // profiling: 0 off, 1 every family, 2 only the kernel-exact fused sweeps
   int prof = 0;
   std::map<std::string, ProfFamily> fam;
@@ -496,6 +502,15 @@ static void resid_inf(lmm_ctx *c, SLevel<double> &L, double *const phi[3], doubl
   for (int d = 0; d < 3; d++) out[d] = c->hscal[d];
 }
 
+// max |rhs| per component on level 0 (FP64), synchronises
+static void rhs_inf(lmm_ctx *c, SLevel<double> &L, double *const rhs[3], double out[3]) {
+  launch_face_maxabs(c->stream, L.g, rhs, c->partial);
+  launch_finalize_max(c->stream, c->partial, c->scal + 4000);
+  hipMemcpyAsync(c->hscal, c->scal + 4000, 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  for (int d = 0; d < 3; d++) out[d] = c->hscal[d];
+}
+
 // One call of stag_mg_solver (stag_mg_solver.f90:37-868) on prepared levels.
 // phi0/rhs0 are the level-0 arrays; phi_zero tells that phi0 is zero on
 // entry, in which case the initial-residual exit of :276-306 cannot trigger
@@ -519,7 +534,11 @@ static void stag_vcycles(lmm_ctx *c, std::vector<SLevel<T>> &S, T *const phi0[3]
   bool need_check = (p.stag_mg_max_vcycles > 1) || !phi_zero;
   if constexpr (std::is_same<T, double>::value) {
    if (need_check) {
-    resid_inf(c, S[0], phi0, rhs0, resid0);
+    // With a zero guess (apply_precon zeroes x_u first, apply_precon.f90:88-90)
+    // the initial residual L 0 - rhs is -rhs bit for bit, so max |rhs| is
+    // taken without reading phi0, whose contents are not the guess.
+    if (phi_zero) rhs_inf(c, S[0], rhs0, resid0);
+    else resid_inf(c, S[0], phi0, rhs0, resid0);
     bool allzero = true, anyzero = false;
     double mx = 0;
     for (int d = 0; d < dim; d++) {
@@ -527,7 +546,12 @@ static void stag_vcycles(lmm_ctx *c, std::vector<SLevel<T>> &S, T *const phi0[3]
       else anyzero = true;
       mx = std::max(mx, resid0[d]);
     }
-    if (allzero) return;
+    if (allzero) {
+      // stag_mg_solver.f90:276-306 returns the guess: zero here
+      if (phi_zero)
+        for (int d = 0; d < dim; d++) launch_fill<T>(c->stream, phi0[d], S[0].g.N, (T)0);
+      return;
+    }
     if (anyzero)
       for (int d = 0; d < dim; d++) resid0[d] = mx;
    }
@@ -626,6 +650,21 @@ static void stag_vcycles(lmm_ctx *c, std::vector<SLevel<T>> &S, T *const phi0[3]
 
 // ------------------------- cell-centred multigrid ------------------------
 
+// abs_eps of the cell-centred multigrid's stopping test (macproject.f90:423)
+static constexpr double CC_ABS_EPS = 1e-16;
+
+// max |a| over a cell array of grid g (FP64), synchronises
+static double cell_inf(lmm_ctx *c, const Grid &g, const double *a) {
+  Grid g1 = g;
+  g1.dim = 1;  // the face reduction over one array
+  const double *r[3] = {a, nullptr, nullptr};
+  launch_face_maxabs(c->stream, g1, r, c->partial);
+  launch_finalize_max(c->stream, c->partial, c->scal + 4030);
+  hipMemcpyAsync(c->hscal, c->scal + 4030, sizeof(double), hipMemcpyDeviceToHost, c->stream);
+  hipStreamSynchronize(c->stream);
+  return c->hscal[0];
+}
+
 template <typename T>
 static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *rhs0) {
   const lmm_params &p = c->prm;
@@ -638,6 +677,23 @@ static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *
     const int first = (n < nl - 1) ? p.mg_nsmooths_down : p.mg_nsmooths_bottom;
     return first >= 1 && fused_at(n) && fused_rb_takes_zeros();
   };
+  // The stopping test of the cell-centred multigrid: max |rhs - L phi| <=
+  // mg_rel_tol * max |rhs| or <= 1e-16, the eps and abs_eps that
+  // mgt_macproj_precon_build gives the tower (macproject.f90:402-425),
+  // checked before the first cycle and after each cycle but the last.  It is
+  // evaluated (FP64 only) when more than one cycle may run; with one cycle
+  // the cycle always runs.
+  const bool check = std::is_same<T, double>::value && p.mg_max_vcycles > 1;
+  double bnorm = 0.0;
+  if constexpr (std::is_same<T, double>::value) {
+    if (check) {
+      bnorm = cell_inf(c, c->g, rhs0);
+      if (bnorm <= p.mg_rel_tol * bnorm || bnorm <= CC_ABS_EPS) {
+        launch_fill<T>(c->stream, phi0, c->g.N, (T)0);
+        return;
+      }
+    }
+  }
   if (!takes_zeros(0)) launch_fill<T>(c->stream, phi0, c->g.N, (T)0);
   auto rb = [&](int n, int sweeps, bool zero_in) {
     T *ph = (n == 0) ? phi0 : C[n].phi;
@@ -709,6 +765,13 @@ static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *
       rb(n, p.mg_nsmooths_up, false);
     }
     c->vcycles += 1;
+    if constexpr (std::is_same<T, double>::value) {
+      if (check && cyc + 1 < p.mg_max_vcycles) {
+        launch_cc_residual<T>(c->stream, C[0].g, phi0, rhs0, C[0].res, C[0].a);
+        const double r = cell_inf(c, c->g, C[0].res);
+        if (r <= p.mg_rel_tol * bnorm || r <= CC_ABS_EPS) break;
+      }
+    }
   }
 }
 
@@ -1159,6 +1222,57 @@ static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
 // C ABI
 // ---------------------------------------------------------------------------
 
+// Kernel-choice switches of a new context, from the environment (read, not
+// written: each context keeps its own copy, so a context created later, or
+// on another thread, changes nothing in the others).  The debug switch is
+// process-wide and read once.
+static lmm_kernel_options env_options() {
+  static std::once_flag once;
+  std::call_once(once, [] {
+    const char *dbg = getenv("LMM_DEBUG_SYNC");
+    set_debug_sync(dbg && dbg[0] == '1');
+  });
+  lmm_kernel_options o;
+  lmm_kernel_options_default(&o);
+  const char *sm = getenv("LMM_SMOOTHER");
+  if (sm && strcmp(sm, "colour") == 0) o.fused_sweeps = 0;
+  if (sm && strcmp(sm, "fused") == 0) o.fused_min_tiles = 1;
+  const char *mt = getenv("LMM_FUSED_MIN_TILES");
+  if (mt && atoi(mt) > 0) o.fused_min_tiles = atoi(mt);
+  const char *tg = getenv("LMM_FUSED_TARGET");
+  if (tg && atoi(tg) > 0) o.fused_target = atoi(tg);
+  const char *fk = getenv("LMM_FUSED_KERNEL");
+  if (fk && strcmp(fk, "march") == 0) o.fused_kernel = 0;
+  if (fk && strcmp(fk, "block") == 0) o.fused_kernel = 1;
+  const char *x3 = getenv("LMM_FUSED_X3");
+  if (x3 && x3[0] == '0') o.fused_x3 = 0;
+  const char *rr = getenv("LMM_RESID_RESTRICT");
+  if (rr && rr[0] == '0') o.resid_restrict = 0;
+  const char *kf = getenv("LMM_KRYLOV_FUSED");
+  if (kf && kf[0] == '0') o.krylov_fused = 0;
+  const char *z3 = getenv("LMM_Z32");
+  if (z3 && z3[0] == '0') o.z32_direct = 0;
+  const char *ab = getenv("LMM_AM_BRICK");
+  if (ab) o.am_brick = atoi(ab);
+  const char *co = getenv("LMM_COARSE");
+  if (co && co[0] == '0') o.coarse_one_launch = 0;
+  return o;
+}
+
+static void apply_options(lmm_ctx *c, const lmm_kernel_options &o) {
+  c->fused = o.fused_sweeps != 0;
+  c->fused_min_tiles = o.fused_min_tiles;
+  c->fused3 = o.fused_x3 != 0;
+  c->fused_resid_restrict = o.resid_restrict != 0;
+  c->fused_krylov = o.krylov_fused != 0;
+  c->z32_direct = o.z32_direct != 0;
+  c->coarse_one_launch = o.coarse_one_launch != 0;
+  c->tune.rb_variant = o.fused_kernel;
+  c->tune.rb_target = o.fused_target;
+  c->tune.am_dz = o.am_brick;
+  c->kopt = o;
+}
+
 extern "C" {
 
 void lmm_params_default(lmm_params *p) {
@@ -1211,34 +1325,7 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
   lmm_params_default(&dp);
   if (!p) p = &dp;
   lmm_ctx *c = new lmm_ctx();
-  {
-    const char *dbg = getenv("LMM_DEBUG_SYNC");
-    set_debug_sync(dbg && dbg[0] == '1');
-    const char *sm = getenv("LMM_SMOOTHER");
-    c->fused = !(sm && strcmp(sm, "colour") == 0);
-    if (sm && strcmp(sm, "fused") == 0) c->fused_min_tiles = 1;
-    const char *mt = getenv("LMM_FUSED_MIN_TILES");
-    if (mt && atoi(mt) > 0) c->fused_min_tiles = atoi(mt);
-    const char *tg = getenv("LMM_FUSED_TARGET");
-    set_rb_target(tg && atoi(tg) > 0 ? atoi(tg) : 768);
-    const char *fk = getenv("LMM_FUSED_KERNEL");
-    int variant = 2;
-    if (fk && strcmp(fk, "march") == 0) variant = 0;
-    if (fk && strcmp(fk, "block") == 0) variant = 1;
-    set_rb_variant(variant);
-    const char *x3 = getenv("LMM_FUSED_X3");
-    c->fused3 = !(x3 && x3[0] == '0');
-    const char *rr = getenv("LMM_RESID_RESTRICT");
-    c->fused_resid_restrict = !(rr && rr[0] == '0');
-    const char *kf = getenv("LMM_KRYLOV_FUSED");
-    c->fused_krylov = !(kf && kf[0] == '0');
-    const char *z3 = getenv("LMM_Z32");
-    c->z32_direct = !(z3 && z3[0] == '0');
-    const char *ab = getenv("LMM_AM_BRICK");
-    set_am_brick_depth(ab ? atoi(ab) : 32);
-    const char *co = getenv("LMM_COARSE");
-    c->coarse_one_launch = !(co && co[0] == '0');
-  }
+  apply_options(c, env_options());
   c->g = make_grid(dim, n_cells, dx);
   int rc = check_params(c, p);
   if (rc) {
@@ -1349,6 +1436,36 @@ int lmm_ctx_set_params(lmm_ctx *c, const lmm_params *p) {
 int lmm_ctx_get_params(const lmm_ctx *c, lmm_params *p) {
   if (!c || !p) return LMM_ERR_ARG;
   *p = c->prm;
+  return LMM_OK;
+}
+
+void lmm_kernel_options_default(lmm_kernel_options *o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->fused_sweeps = 1;
+  o->fused_min_tiles = 64;
+  o->fused_kernel = 2;
+  o->fused_target = 768;
+  o->fused_x3 = 1;
+  o->resid_restrict = 1;
+  o->coarse_one_launch = 1;
+  o->am_brick = 32;
+  o->z32_direct = 1;
+  o->krylov_fused = 1;
+}
+
+int lmm_ctx_get_kernel_options(const lmm_ctx *c, lmm_kernel_options *o) {
+  if (!c || !o) return LMM_ERR_ARG;
+  *o = c->kopt;
+  return LMM_OK;
+}
+
+int lmm_ctx_set_kernel_options(lmm_ctx *c, const lmm_kernel_options *o) {
+  if (!c || !o) return LMM_ERR_ARG;
+  if (o->fused_min_tiles < 1 || o->fused_target < 1 || o->fused_kernel < 0 || o->fused_kernel > 2 || o->am_brick < 0)
+    return set_err(c, LMM_ERR_ARG, "kernel options out of range");
+  hipStreamSynchronize(c->stream);
+  apply_options(c, *o);
   return LMM_OK;
 }
 
@@ -1560,6 +1677,7 @@ int lmm_gmres(lmm_ctx *c, lmm_field *const b_u[3], lmm_field *b_p, lmm_field *x_
               lmm_field *const alpha_fc[3], lmm_field *beta, lmm_field *const beta_ed[3], lmm_field *gamma,
               double theta_alpha, double *norm_pre_rhs, lmm_gmres_info *info) {
   if (!c) return LMM_ERR_ARG;
+  TuningScope ts(&c->tune);
   int rc;
   if ((rc = check_stokes(c, b_u, b_p, "gmres b"))) return rc;
   if ((rc = check_stokes(c, x_u, x_p, "gmres x"))) return rc;
@@ -1628,6 +1746,7 @@ int lmm_apply_matrix(lmm_ctx *c, lmm_field *b_u[3], lmm_field *b_p, lmm_field *c
                      lmm_field *const alpha_fc[3], lmm_field *beta, lmm_field *const beta_ed[3],
                      lmm_field *gamma, double theta_alpha) {
   if (!c) return LMM_ERR_ARG;
+  TuningScope ts(&c->tune);
   int rc;
   if ((rc = check_stokes(c, b_u, b_p, "apply_matrix b"))) return rc;
   if ((rc = check_stokes(c, x_u, x_p, "apply_matrix x"))) return rc;
@@ -1652,6 +1771,7 @@ int lmm_apply_precon(lmm_ctx *c, lmm_field *const b_u[3], lmm_field *b_p, lmm_fi
                      lmm_field *const alpha_fc[3], lmm_field *beta, lmm_field *const beta_ed[3],
                      lmm_field *gamma, double theta_alpha) {
   if (!c) return LMM_ERR_ARG;
+  TuningScope ts(&c->tune);
   int rc;
   if ((rc = check_stokes(c, b_u, b_p, "apply_precon b"))) return rc;
   if ((rc = check_stokes(c, x_u, x_p, "apply_precon x"))) return rc;
@@ -1672,6 +1792,7 @@ int lmm_apply_precon(lmm_ctx *c, lmm_field *const b_u[3], lmm_field *b_p, lmm_fi
 int lmm_stag_mg_solver(lmm_ctx *c, lmm_field *const alpha_fc[3], lmm_field *beta_cc, lmm_field *const beta_ed[3],
                        lmm_field *gamma_cc, double theta_alpha, lmm_field *phi_fc[3], lmm_field *const rhs_fc[3]) {
   if (!c) return LMM_ERR_ARG;
+  TuningScope ts(&c->tune);
   int rc;
   if ((rc = check_stokes(c, phi_fc, nullptr, "stag_mg_solver phi"))) return rc;
   if ((rc = check_stokes(c, rhs_fc, nullptr, "stag_mg_solver rhs"))) return rc;
@@ -1691,6 +1812,7 @@ int lmm_macproject(lmm_ctx *c, lmm_field *phi, lmm_field *const umac[3], lmm_fie
                    lmm_field *mac_rhs) {
   (void)umac;
   if (!c) return LMM_ERR_ARG;
+  TuningScope ts(&c->tune);
   if (!field_ok(c, phi, LMM_CELL) || !field_ok(c, mac_rhs, LMM_CELL)) return set_err(c, LMM_ERR_ARG, "macproject: bad cell field");
   for (int d = 0; d < c->g.dim; d++)
     if (!field_ok(c, alphainv_fc[d], LMM_FACE_X + d)) return set_err(c, LMM_ERR_ARG, "macproject: bad alphainv_fc[%d]", d);
@@ -1707,6 +1829,7 @@ int lmm_stag_applyop(lmm_ctx *c, lmm_field *const phi_fc[3], lmm_field *Lphi_fc[
                      lmm_field *beta_cc, lmm_field *const beta_ed[3], lmm_field *gamma_cc, double theta_alpha,
                      int color) {
   if (!c) return LMM_ERR_ARG;
+  TuningScope ts(&c->tune);
   int rc;
   if ((rc = check_stokes(c, phi_fc, nullptr, "stag_applyop phi"))) return rc;
   if ((rc = check_stokes(c, Lphi_fc, nullptr, "stag_applyop Lphi"))) return rc;

@@ -56,6 +56,19 @@ class GmresInfo(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class KernelOptions(C.Structure):
+    """lmm_kernel_options: kernel forms of a context (every choice gives the same bits)."""
+    _fields_ = [
+        ("fused_sweeps", C.c_int), ("fused_min_tiles", C.c_int), ("fused_kernel", C.c_int),
+        ("fused_target", C.c_int), ("fused_x3", C.c_int), ("resid_restrict", C.c_int),
+        ("coarse_one_launch", C.c_int), ("am_brick", C.c_int), ("z32_direct", C.c_int),
+        ("krylov_fused", C.c_int),
+    ]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class GmresMember(C.Structure):
     """lmm_gmres_member: one solve of a batch."""
     _fields_ = [
@@ -75,6 +88,9 @@ _SIGS = {
     "lmm_ctx_set_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
     "lmm_ctx_get_params": (C.c_int, [C.c_void_p, C.POINTER(Params)]),
     "lmm_ctx_synchronize": (C.c_int, [C.c_void_p]),
+    "lmm_kernel_options_default": (None, [C.POINTER(KernelOptions)]),
+    "lmm_ctx_get_kernel_options": (C.c_int, [C.c_void_p, C.POINTER(KernelOptions)]),
+    "lmm_ctx_set_kernel_options": (C.c_int, [C.c_void_p, C.POINTER(KernelOptions)]),
     "lmm_ctx_nlevels": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "lmm_ctx_last_error": (C.c_char_p, [C.c_void_p]),
     "lmm_ctx_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
@@ -179,6 +195,18 @@ class Context:
         for k, v in kw.items():
             setattr(self.params, k, v)
         self.check(lib().lmm_ctx_set_params(self.h, C.byref(self.params)), "set_params")
+
+    def kernel_options(self, **kw):
+        """Current kernel forms of this context; keyword arguments change them first."""
+        o = KernelOptions()
+        self.check(lib().lmm_ctx_get_kernel_options(self.h, C.byref(o)), "get_kernel_options")
+        if kw:
+            for k, v in kw.items():
+                if not hasattr(o, k):
+                    raise KeyError(k)
+                setattr(o, k, int(v))
+            self.check(lib().lmm_ctx_set_kernel_options(self.h, C.byref(o)), "set_kernel_options")
+        return o.as_dict()
 
     def nlevels(self):
         a, b = C.c_int(), C.c_int()

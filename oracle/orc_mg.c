@@ -15,7 +15,8 @@
  * injection, mg_nsmooths_bottom relaxations on the coarsest level, face
  * coefficients coarsened by averaging the overlying fine faces, and the
  * tower settings of mgt_macproj_precon_build (macproject.f90:366-430):
- * nu1 = mg_nsmooths_down, nu2 = mg_nsmooths_up, mg_max_vcycles cycles.
+ * nu1 = mg_nsmooths_down, nu2 = mg_nsmooths_up, at most mg_max_vcycles cycles
+ * with F_MG's stopping test (max|r| <= mg_rel_tol*max|rhs| or <= 1e-16).
  * Bit-level agreement with F_MG itself is not pinned by anything in the tree.
  */
 #include "orc_fab.h"
@@ -276,7 +277,15 @@ void o_macproject(const orc_geom *g, const orc_params *p, ofab *phi, ofab *alpha
   for (int n = 1; n < nl; n++) o_stag_restriction(&lev[n].g, lev[n - 1].a, lev[n].a, 1);
   fab_copy_valid(&lev[0].rhs, mac_rhs);
   int nk0 = (dm == 3) ? g->n[2] : 1;
-  for (int cyc = 0; cyc < p->mg_max_vcycles; cyc++) {
+  /* F_MG's stopping test with the tower's eps = mg_rel_tol and abs_eps =
+   * 1e-16 (macproject.f90:402-425): max|r| <= eps*max|rhs| or <= abs_eps,
+   * before the first cycle (phi = 0, so r = rhs) and after each cycle but the
+   * last; evaluated when more than one cycle may run. */
+  int check = p->mg_max_vcycles > 1;
+  double bnorm = check ? fab_norm_inf(&lev[0].rhs) : 0.0;
+  int ncyc = p->mg_max_vcycles;
+  if (check && (bnorm <= p->mg_rel_tol * bnorm || bnorm <= 1e-16)) ncyc = 0;
+  for (int cyc = 0; cyc < ncyc; cyc++) {
     for (int n = 1; n < nl; n++) fab_setval(&lev[n].phi, 0.0);
     for (int n = 0; n < nl - 1; n++) {
       cc_smooth(&lev[n], p->mg_nsmooths_down);
@@ -293,6 +302,11 @@ void o_macproject(const orc_geom *g, const orc_params *p, ofab *phi, ofab *alpha
             F(&L->phi, i, j, k) = F(&L->phi, i, j, k) + F(&C->phi, i / 2, j / 2, (dm == 3) ? k / 2 : 0);
       fab_fill_boundary(&L->phi);
       cc_smooth(L, p->mg_nsmooths_up);
+    }
+    if (check && cyc + 1 < ncyc) {
+      cc_residual(&lev[0]);
+      double r = fab_norm_inf(&lev[0].res);
+      if (r <= p->mg_rel_tol * bnorm || r <= 1e-16) break;
     }
   }
   for (int k = 0; k < nk0; k++)

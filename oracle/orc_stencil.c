@@ -765,6 +765,14 @@ double o_cc_inner_prod(const orc_geom *g, ofab *a, ofab *b) {
 /* sum_umac_press (norm_inner_product.f90:579-794) for one tile spanning the
  * whole box: pressure sum over valid cells; each velocity component sums
  * all valid faces and adds 0.5*u on the lo and hi+1 faces. */
+/* sum_umac_2d/3d add the 0.5 boundary correction in direction d only on a
+ * tile whose nodal box spans the whole box in d (norm_inner_product.f90:
+ * 701-705 and the 3D analogue).  sum_umac_press iterates tiles of BoxLib's
+ * default tile size, 1024000 x 8 x 8 (F_BaseLib layout; the BoxLib version
+ * is not pinned): the x tiles always span the box, y and z tiles only when
+ * the box is at most 8 cells long in that direction. */
+static const int o_tile_size[3] = {1024000, 8, 8};
+
 void o_sum_umac_press(const orc_geom *g, ofab *p, ofab *u, double *av_p, double av_u[3]) {
   double total_volume = (double)g->n[0] * g->n[1] * ((g->dim == 3) ? g->n[2] : 1);
   double sp = 0.0;
@@ -781,6 +789,10 @@ void o_sum_umac_press(const orc_geom *g, ofab *p, ofab *u, double *av_p, double 
       for (int j = 0; j <= vhi(a, 1); j++)
         for (int i = 0; i <= vhi(a, 0); i++) s += F(a, i, j, k);
     int n = g->n[d];
+    if (n > o_tile_size[d]) {
+      av_u[d] = s / total_volume;
+      continue;
+    }
     for (int k = 0; k <= vhi(a, 2); k++)
       for (int j = 0; j <= vhi(a, 1); j++)
         for (int i = 0; i <= vhi(a, 0); i++) {

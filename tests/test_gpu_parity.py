@@ -369,3 +369,68 @@ def test_fused_krylov_equals_separate_passes(dim, n, prec, monkeypatch):
     assert out[0][0] == out[1][0] and out[0][0] > 4
     assert out[0][1] == out[1][1]
     assert bit_equal(out[0][2][0], out[1][2][0]) and bit_equal(out[0][2][1], out[1][2][1])
+
+
+@pytest.mark.parametrize("dim,n", [(2, (32, 32)), (3, (64, 16, 16))])
+@pytest.mark.parametrize("vcycles,tol", [(4, 0.5), (4, 1e-30), (3, 1e-3)])
+def test_apply_precon_ignores_stale_output(dim, n, vcycles, tol):
+    """apply_precon zeroes x_u before its staggered V-cycles
+    (apply_precon.f90:88-90): the initial residual of stag_mg_solver's
+    stopping test (stag_mg_solver.f90:261-310) is rhs, whatever x_u held.
+    x_u is filled with random values first; several V-cycles with a loose,
+    an unreachable and a middling tolerance must give the oracle's bits."""
+    pb = P.small_problem(dim, n, visc_type=-1, seed=37)
+    kw = dict(visc_type=-1, stag_mg_max_vcycles=vcycles, stag_mg_rel_tol=tol, mg_max_vcycles=vcycles,
+              mg_rel_tol=tol)
+    g, p, cf = oracle_args(pb, **kw)
+    bu = _rand_faces(dim, pb.shape, 33)
+    bp = np.random.default_rng(34).standard_normal(pb.shape)
+    bp -= bp.mean()
+    want_u, want_p = O.apply_precon(g, p, cf, 1.0, bu, bp)
+    with S.Context(dim, pb.n, pb.dx, **kw) as ctx:
+        dp = S.DeviceProblem(ctx, pb, rhs=bu)
+        b_p = ctx.field(S.CELL, 1, bp)
+        for f, a in zip(dp.x_u, _rand_faces(dim, pb.shape, 35)):
+            f.upload(a)
+        dp.x_p.upload(np.random.default_rng(36).standard_normal(pb.shape))
+        S.apply_precon(ctx, dp.b_u, b_p, dp.x_u, dp.x_p, dp.alpha, dp.beta, dp.beta_ed, dp.gamma, 1.0)
+        got_u, got_p = dp.solution()
+    assert bit_equal(got_u, want_u)
+    assert relerr(got_p, want_p) < 1e-13
+
+
+def test_apply_precon_zero_rhs_clears_stale_output():
+    """b = 0: every V-cycle returns at once (stag_mg_solver.f90:276-306) and
+    the result is zero even where x_u held other values."""
+    pb = P.small_problem(3, (16, 16, 16), visc_type=-1, seed=38)
+    with S.Context(3, pb.n, pb.dx, visc_type=-1, stag_mg_max_vcycles=3, mg_max_vcycles=3) as ctx:
+        dp = S.DeviceProblem(ctx, pb, rhs=[np.zeros(pb.shape)] * 3)
+        b_p = ctx.field(S.CELL, 1, np.zeros(pb.shape))
+        for f in dp.x_u:
+            f.setval(5.0)
+        dp.x_p.setval(-2.0)
+        S.apply_precon(ctx, dp.b_u, b_p, dp.x_u, dp.x_p, dp.alpha, dp.beta, dp.beta_ed, dp.gamma, 1.0)
+        got_u, got_p = dp.solution()
+    assert all(np.all(u == 0) for u in got_u) and np.all(got_p == 0)
+
+
+@pytest.mark.parametrize("dim,n", [(2, (32, 32)), (3, (64, 16, 16))])
+@pytest.mark.parametrize("tol", [0.9, 1e-2, 1e-30])
+def test_macproject_stopping_test_bit_exact(dim, n, tol):
+    """Up to 6 cell-centred V-cycles with the stopping test max|r| <=
+    mg_rel_tol*max|rhs| (macproject.f90:402-425): the GPU stops after the
+    same cycle as the oracle and gives its bits."""
+    pb = P.small_problem(dim, n, seed=39)
+    g, p, cf = oracle_args(pb, mg_max_vcycles=6, mg_rel_tol=tol)
+    ainv = [1.0 / a for a in pb.alpha]
+    rhs = np.random.default_rng(40).standard_normal(pb.shape)
+    rhs -= rhs.mean()
+    want = O.macproject(g, p, ainv, rhs)
+    with S.Context(dim, pb.n, pb.dx, mg_max_vcycles=6, mg_rel_tol=tol) as ctx:
+        ai = ctx.faces(ainv)
+        r = ctx.field(S.CELL, 1, rhs)
+        phi = ctx.field(S.CELL)
+        phi.setval(7.0)
+        S.macproject(ctx, phi, None, ai, r)
+        got = phi.download()
+    assert bit_equal(got, want), relerr(got, want)
