@@ -76,6 +76,9 @@ typedef struct {
   int gmres_max_outer, gmres_max_inner, gmres_max_iter, gmres_min_iter;
   int gmres_verbose;
   int precision;               /* enum lmm_precision */
+  /* probin_common.f90 knobs of the stochastic momentum flux */
+  int stoch_stress_form;       /* 0 non-symmetric, else symmetric (default 1, probin_common.f90:230) */
+  int filtering_width;         /* 0, 1, 2 or 4 (multifab_filter.f90) */
 } lmm_params;
 
 typedef struct {
@@ -88,11 +91,15 @@ typedef struct {
   int status;                  /* LMM_OK or LMM_NOT_CONVERGED */
   int vcycles;                 /* V-cycles run (stag + cell-centred) */
   double seconds;              /* wall time of the call */
+  int learned_applied;         /* LMM_LEARNED: preconditioner applications with the learned stage */
+  int learned_fallback;        /* LMM_LEARNED: 1 if the solve switched to the plain V-cycle */
 } lmm_gmres_info;
 
-/* Kernel forms (no counterpart in the reference).  Every choice gives the
- * same bits; they exist for measurement, tuning and the tests that compare
- * kernel forms.  A new context takes the defaults, overridden by the
+/* Kernel forms (no counterpart in the reference).  Every choice but
+ * flex_cgs gives the same bits; they exist for measurement, tuning and the
+ * tests that compare kernel forms.  flex_cgs changes the rounding of the
+ * flexible GMRES (LMM_FP32_MG, LMM_LEARNED) only, never of the LMM_FP64
+ * reference algorithm.  A new context takes the defaults, overridden by the
  * LMM_* environment switches present at its creation. */
 typedef struct {
   int fused_sweeps;        /* 1: fused red-black sweeps where a level allows; 0: colour kernels */
@@ -105,6 +112,9 @@ typedef struct {
   int am_brick;            /* z depth of the operator's brick order (0: row order) */
   int z32_direct;          /* 1: FGMRES reads its FP32 preconditioned vectors directly */
   int krylov_fused;        /* 1: fused Gram-Schmidt and basis-scaling passes */
+  int flex_cgs;            /* flexible GMRES: 1 classical Gram-Schmidt (two passes per
+                              step over the basis, unnormalised stored vectors),
+                              0 modified Gram-Schmidt */
 } lmm_kernel_options;
 
 /* ---------------- context ---------------- */
@@ -232,6 +242,15 @@ int lmm_stag_applyop(lmm_ctx *ctx, lmm_field *const phi_fc[3], lmm_field *Lphi_f
                      lmm_field *const beta_ed[3], lmm_field *gamma_cc, double theta_alpha,
                      int color);
 
+/* diffusive_m_fluxdiv.f90:16-17  diffusive_m_fluxdiv(mla,m_update,umac,eta,eta_ed,kappa,dx,
+ *                                                  the_bc_level)
+ * m_update = m_update + L_eta(umac): stag_applyop_level with alpha_fc = 0
+ * (visc_type of the context, kappa in the role of gamma), subtracted from
+ * m_update (the explicit viscous term of a Crank-Nicolson step).  m_update
+ * and umac must be distinct fields. */
+int lmm_diffusive_m_fluxdiv(lmm_ctx *ctx, lmm_field *m_update[3], lmm_field *const umac[3], lmm_field *eta,
+                            lmm_field *const eta_ed[3], lmm_field *kappa);
+
 /* norm_inner_product.f90:16-21 stag_inner_prod / :266-271 cc_inner_prod
  * prod_val[d] per face component; box-boundary faces weighted 1.5 as in the reference. */
 int lmm_stag_inner_prod(lmm_ctx *ctx, lmm_field *const m1[3], lmm_field *const m2[3],
@@ -253,8 +272,11 @@ int lmm_average_cc_to_edge(lmm_ctx *ctx, lmm_field *cc, lmm_field *edge[3]);
  *        eta_ed,temperature,temperature_ed,dx,dt,weights)
  * Increments stoch_m_force by div(sqrt(variance_coef_mom*2*k_B/(dV*dt)) sqrt(eta*T) W)
  * where W = sum_r weights[r] * W_r is the noise held by the context
- * (lmm_init_m_stochastic / lmm_fill_m_stochastic).  Non-symmetric stress form,
- * filtering_width 0. */
+ * (lmm_init_m_stochastic / lmm_fill_m_stochastic).  With filtering_width > 0
+ * the scaled cell and node/edge fluxes pass through multifab_filter
+ * (multifab_filter.f90:13-200: x, then y, then z) before the divergence
+ * (stochastic_m_fluxdiv.f90:166-174, :224-230).  The stress form enters only
+ * through the fill. */
 int lmm_stochastic_m_fluxdiv(lmm_ctx *ctx, lmm_field *stoch_m_force[3], lmm_field *eta,
                              lmm_field *const eta_ed[3], lmm_field *temperature,
                              lmm_field *const temperature_ed[3], double dt,
@@ -264,7 +286,10 @@ int lmm_init_m_stochastic(lmm_ctx *ctx, int n_rngs);
 /* stochastic_m_fluxdiv.f90:556 fill_m_stochastic(mla): fills every stage with
  * N(0,1) numbers from the host generator (MT19937 seeded with seed, Acklam
  * inverse normal), in the order: stage, W_cc components, then node/edge
- * arrays and their 2 components, x fastest. */
+ * arrays and their 2 components, x fastest.  With the symmetric stress form
+ * (stoch_stress_form != 0, :569-604) W_cc is scaled by sqrt(2) (variance 2)
+ * and only component 1 of each node/edge array is drawn; component 2 is a
+ * copy of it. */
 int lmm_fill_m_stochastic(lmm_ctx *ctx, uint32_t seed);
 /* No counterpart in the reference, whose noise comes from the host MT19937
  * stream above: fills every stage of the context's noise on the device, in
@@ -278,7 +303,8 @@ int lmm_fill_m_stochastic_device(lmm_ctx *ctx, uint64_t seed, uint64_t step);
  * (component, then x fastest).  No counterpart in the reference. */
 int lmm_field_fill_normal(lmm_field *f, uint64_t seed, uint64_t step, int64_t v0);
 /* Same divergence with caller-provided noise: W_cc has dim components, W_ed
- * has 2 components per node (2D, W_ed[0]) or edge array (3D). */
+ * has 2 components per node (2D, W_ed[0]) or edge array (3D); filtered as
+ * above when the context's filtering_width > 0. */
 int lmm_stoch_m_force(lmm_ctx *ctx, lmm_field *stoch_m_force[3], lmm_field *W_cc,
                       lmm_field *const W_ed[3], lmm_field *eta, lmm_field *const eta_ed[3],
                       lmm_field *temperature, lmm_field *const temperature_ed[3], double variance);
@@ -293,6 +319,66 @@ int lmm_diffusive_mass_fluxdiv(lmm_ctx *ctx, int nspecies, lmm_field *rho, lmm_f
                                lmm_field *diff_fluxdiv, lmm_field *Temp, lmm_field *zeta_by_Temp,
                                lmm_field *const gradp_baro[3], lmm_field *flux_total[3],
                                int correct_flux);
+
+/* ---------------- learned coarse-grid correction ----------------
+ * No counterpart in the reference.  A small bias-free 3-level U-Net
+ * (3x3x3 periodic convolutions, channels c1/c2/c3 = 16/32/64 by default,
+ * BF16 weights and activations, FP32 accumulation on the gfx950 matrix
+ * cores) predicts the error left by the staggered V-cycle from its
+ * residual.  Precision LMM_LEARNED inserts it into the flexible GMRES's FP32
+ * preconditioner: after the staggered V-cycle of apply_precon.f90:103-195
+ * (the stage gmres.f90:335 reaches through apply_precon), the residual of
+ * x_u is restricted (12-point weights) to multigrid level `level`; the
+ * network maps [r/rms(r), log beta, log theta*alpha] there to the error in
+ * units of rms(r)*dx_level^2; the prediction is prolongated back to the
+ * finest level and added to x_u.  The plain V-cycle is the fallback: a step
+ * whose Hessenberg column is not finite is redone without the stage, and an
+ * inner step that cuts the residual estimate by less than 10 % ends the
+ * stage for the rest of the solve (lmm_gmres_info.learned_fallback).  The
+ * FGMRES stopping test on the true residual is unchanged, so the solution
+ * meets the same tolerance.
+ *
+ * Training pairs come from the plain FP32 V-cycle of the context on given
+ * coefficients: phi* = one V-cycle of counter-based white noise (seed, pair
+ * index), b = A phi*, phi1 = one V-cycle of b; the input is the residual
+ * b - A phi1 restricted to `level`, the target the error phi* - phi1
+ * restricted with face averages.  Pairs stay on the device. */
+typedef struct lmm_cnn lmm_cnn;
+typedef struct {
+  int cin, c1, c2, c3, cout;  /* 8, 16, 32, 64, 3; c1..c3 multiples of 16, cin of 4 */
+  int level;                  /* staggered multigrid level of the correction (default 3) */
+} lmm_cnn_config;
+void lmm_cnn_config_default(lmm_cnn_config *cfg);
+/* He-initialised network (seeded) on the context's stream; cfg may be NULL. */
+int lmm_cnn_create(lmm_ctx *ctx, const lmm_cnn_config *cfg, uint64_t seed, lmm_cnn **out);
+int lmm_cnn_destroy(lmm_cnn *net);
+/* FP32 master weights, layers L0..L10 in order, each W[co][tap][ci] with
+ * tap = (dz+1)*9 + (dy+1)*3 + (dx+1). */
+int lmm_cnn_num_params(const lmm_cnn *net, int64_t *n);
+int lmm_cnn_get_params(lmm_cnn *net, float *host);
+int lmm_cnn_set_params(lmm_cnn *net, const float *host);
+/* One forward pass on host data: x [nb][nz][ny][nx][cin] (rounded to BF16),
+ * y [..][cout] FP32; n = (nx, ny, nz), multiples of 4, nb*(n/4)^3 a multiple of 16. */
+int lmm_cnn_forward_host(lmm_cnn *net, int nb, const int n[3], const float *x, float *y);
+/* loss = sum((y - t)^2) / (2 * nb * voxels * cout) and its gradient (the
+ * flat parameter layout) by the device's backward kernels. */
+int lmm_cnn_loss_grad_host(lmm_cnn *net, int nb, const int n[3], const float *x, const float *t, double *loss,
+                           float *grad);
+/* Appends npairs training pairs generated on the coefficients given
+ * (the arguments of lmm_gmres; 3D, the network's level must exist). */
+int lmm_cnn_generate_pairs(lmm_cnn *net, lmm_field *const alpha_fc[3], lmm_field *beta,
+                           lmm_field *const beta_ed[3], lmm_field *gamma, double theta_alpha, int npairs,
+                           uint64_t seed);
+int lmm_cnn_num_pairs(const lmm_cnn *net, int64_t *count);
+/* steps of Adam (lr) on random batches of `batch` pairs, the last held_out
+ * pairs left out; loss_history (steps values) may be NULL. */
+int lmm_cnn_train(lmm_cnn *net, int steps, int batch, double lr, uint64_t seed, int64_t held_out,
+                  float *loss_history);
+/* Mean over pairs [first, last) of |t - y| / |t| (1 = no better than a zero
+ * prediction). */
+int lmm_cnn_evaluate(lmm_cnn *net, int64_t first, int64_t last, double *rel_err);
+/* Network used by precision LMM_LEARNED on this context (NULL detaches). */
+int lmm_ctx_set_cnn(lmm_ctx *ctx, lmm_cnn *net);
 
 /* Host Gaussian generator of the noise fields (RNGs.c:247 genrandn stream:
  * MT19937 init_genrand(seed) + Acklam inverse normal). */

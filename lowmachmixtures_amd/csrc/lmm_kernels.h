@@ -89,9 +89,10 @@ void launch_cc_rb_fused(hipStream_t s, const Grid &g, const T *pin, T *pout, con
 template <typename T>
 void launch_stag_gs_colour(hipStream_t s, const Grid &g, int visc, int a, int par, T omega,
                            T *const phi[3], const StagCoef<T> &K, const T *rhs);
+// Lphi = (alpha - L)phi; sub: Lphi = Lphi - (alpha - L)phi instead
 template <typename T>
 void launch_stag_applyop(hipStream_t s, const Grid &g, int visc, int color, T *const phi[3],
-                         T *const Lphi[3], const StagCoef<T> &K);
+                         T *const Lphi[3], const StagCoef<T> &K, bool sub = false);
 template <typename T>
 void launch_stag_jacobi(hipStream_t s, const Grid &g, int visc, T omega, T *const phi[3],
                         T *const rhs[3], T *const tmp[3], const StagCoef<T> &K);
@@ -210,6 +211,21 @@ void launch_div_by_f32(hipStream_t s, double *v, float *v32, const double *src, 
 void launch_fill_normal_philox(hipStream_t s, double *out, long count, long v0, uint64_t seed, uint64_t step);
 // x += sum_k y_k Z_k, Z in FP32 (at most 32 vectors)
 void launch_update_sol_z32(hipStream_t s, double *x, float *const Z[], const double *y, int i, long n);
+// Classical Gram-Schmidt passes of the flexible GMRES (1..CGS_MAXV basis
+// vectors per pass).  launch_cgs_dots: segment sums of <w, V_k> into
+// partial[blk*CGS_STRIDE + 4k + seg]; launch_cgs_finalize: d_k = <w, V_k>
+// (Stokes product, box-face weights, pressure weighted by pnw^2) and
+// c_k = sig2[k]*d_k; launch_cgs_update: vn = src - sum_k c_k V_k, its FP32
+// copy (v32 may be null) and the segment sums of <vn, vn> for
+// launch_finalize.  Every level extent must be even (cell pairs).
+constexpr int CGS_MAXV = 8;
+constexpr int CGS_STRIDE = 4 * CGS_MAXV;
+void launch_cgs_dots(hipStream_t s, const Grid &g, const double *w, const double *const V[], int nv,
+                     double *partial);
+void launch_cgs_finalize(hipStream_t s, const Grid &g, const double *partial, int nv, const double *sig2,
+                         double pnw, double *d_out, double *c_out);
+void launch_cgs_update(hipStream_t s, const Grid &g, const double *src, const double *const V[], int nv,
+                       const double *c, double *vn, float *v32, double *partial);
 // r = b - A x in one pass (FP64, 3D visc_type -1 / 1); false when not available
 bool launch_apply_matrix_resid(hipStream_t s, const Grid &g, int visc, double *const x_u[3], const double *x_p,
                                double *const b_u[3], const double *b_p, double *const r_u[3], double *r_p,
@@ -223,10 +239,13 @@ template <typename T>
 void launch_sub_scalar3(hipStream_t s, const Grid &g, T *const u[3], const double *scal);
 
 // ---- fluctuating and diffusive fluxes ----
+// width > 0: the scaled fluxes are written to work (2*(dim + 2*ne)*N doubles,
+// ne = 1 in 2D, 3 in 3D) and filtered (multifab_filter) before the divergence
 void launch_stoch_m_force(hipStream_t s, const Grid &g, double *const force[3], const double *Wcc,
                           double *const Wed[3], int nstages, const double *weights, long stage_stride_cc,
                           long stage_stride_ed, const double *eta, double *const eta_ed[3],
-                          const double *temp, double *const temp_ed[3], double variance);
+                          const double *temp, double *const temp_ed[3], double variance, int width = 0,
+                          double *work = nullptr);
 void launch_diffusive_mass_flux(hipStream_t s, const Grid &g, int ns, const double *molarconc,
                                 const double *rhoWchi, const double *Gama, double *flux,
                                 double *const flux_total[3], int correct_flux);

@@ -5,6 +5,8 @@
 // Arithmetic follows the reference expression order (see lmm_common.h) and
 // the library is built with -ffp-contract=off, so the FP64 kernels reproduce
 // the reference formulas bit for bit.
+#include <utility>
+
 #include "lmm_kernels.h"
 #include "lmm_points.h"
 
@@ -72,7 +74,7 @@ void launch_stag_gs_colour(hipStream_t s, const Grid &g, int visc, int a, int pa
 // component is blockIdx.y, uniform across a workgroup.
 template <typename T>
 __global__ void __launch_bounds__(256) k_stag_applyop(Grid g, int visc, int color, T dxsqinv, Ptr3<T> ph,
-                                                      Ptr3<T> lp, StagCoef<T> K) {
+                                                      Ptr3<T> lp, StagCoef<T> K, int sub) {
   long t = xcd_block() * blockDim.x + threadIdx.x;
   int i, j, k, par = (color + 1) % 2;
   const int a = (color == 0) ? (int)blockIdx.y : (color - 1) / 2;
@@ -86,16 +88,18 @@ __global__ void __launch_bounds__(256) k_stag_applyop(Grid g, int visc, int colo
   const T *phr[3] = {ph.c[0], ph.c[1], ph.c[2]};
   T L, fac;
   stag_point<T>(visc, g, a, P, phr, K, dxsqinv, L, fac);
-  lp.c[a][i + g.s[1] * j + g.s[2] * k] = L;
+  const long p = i + g.s[1] * j + g.s[2] * k;
+  if (sub) lp.c[a][p] = lp.c[a][p] - L;  // multifab_sub_sub_c(m_update, Lphi)
+  else lp.c[a][p] = L;
 }
 
 template <typename T>
 void launch_stag_applyop(hipStream_t s, const Grid &g, int visc, int color, T *const phi[3],
-                         T *const Lphi[3], const StagCoef<T> &K) {
+                         T *const Lphi[3], const StagCoef<T> &K, bool sub) {
   long per = (color == 0) ? g.N : g.N / 2;
   dim3 grid(nblk(per), color == 0 ? g.dim : 1);
   T dxsqinv = (T)(1.0 / (g.dx * g.dx));
-  LAUNCH(k_stag_applyop<T>, grid, dim3(TPB), 0, s, g, visc, color, dxsqinv, p3(phi), p3(Lphi), K);
+  LAUNCH(k_stag_applyop<T>, grid, dim3(TPB), 0, s, g, visc, color, dxsqinv, p3(phi), p3(Lphi), K, sub ? 1 : 0);
 }
 
 // Jacobi sweep (stag_mg_smoother = 0): tmp = phi + omega*(rhs - L phi)/fac with
@@ -1097,6 +1101,190 @@ void launch_update_sol_z32(hipStream_t s, double *x, float *const Z[], const dou
   LAUNCH(k_update_sol_z32, dim3(nblk(n)), dim3(TPB), 0, s, x, L, y, i, n);
 }
 
+// ------------------------------------------------------------------------
+// Classical Gram-Schmidt of the flexible GMRES (no counterpart in the
+// reference, whose gmres.f90:338-358 is modified Gram-Schmidt)
+// ------------------------------------------------------------------------
+
+// Weights of the Stokes inner product for the cell pair (q, q+1) of one
+// x-row (nx even): face component d counts 3x where the pair's coordinate
+// along d is 0 (only the first cell of the pair for d = 0).
+struct PairW {
+  double w0a, w0b, w1, w2;
+};
+__device__ __forceinline__ PairW pair_weights(const Grid &g, long q) {
+  const long row = q / g.n[0];
+  const int i = (int)(q - row * g.n[0]);
+  const int j = (int)(row % g.n[1]);
+  const int k = (int)(row / g.n[1]);
+  PairW w;
+  w.w0a = (i == 0) ? 3.0 : 1.0;
+  w.w0b = 1.0;
+  w.w1 = (j == 0) ? 3.0 : 1.0;
+  w.w2 = (k == 0) ? 3.0 : 1.0;
+  return w;
+}
+
+// Sums acc[0..n-1] over the workgroup (RED_TPB threads) into out[0..n-1]
+// (thread 0 writes).
+template <int NACC>
+__device__ __forceinline__ void block_sum_multi(double (&acc)[NACC], double *sh, double *out) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < NACC; a++) {
+    double v = acc[a];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if (lane == 0) sh[wid * NACC + a] = v;
+  }
+  __syncthreads();
+  if (threadIdx.x < NACC) {
+    double r = 0.0;
+    for (int w = 0; w < RED_TPB / 64; w++) r += sh[w * NACC + threadIdx.x];
+    out[threadIdx.x] = r;
+  }
+}
+
+// Segment sums of <w, V_k> for k < NV: partial[blk*CGS_STRIDE + 4k + seg]
+// (seg d < dim: face component d with the box-face weights, seg 3: cells).
+template <int NV>
+__global__ void __launch_bounds__(RED_TPB) k_cgs_dots(Grid g, const double *__restrict__ w, VList V, double *partial) {
+  __shared__ double sh[(RED_TPB / 64) * 4 * NV];
+  double acc[4 * NV];
+#pragma unroll
+  for (int a = 0; a < 4 * NV; a++) acc[a] = 0.0;
+  const long npair = g.N / 2;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < npair; t += stride) {
+    const long q = 2 * t;
+    const PairW pw = pair_weights(g, q);
+#pragma unroll
+    for (int seg = 0; seg < 4; seg++) {
+      if (seg == 2 && g.dim == 2) continue;
+      const long p = (long)(seg == 3 ? g.dim : seg) * g.N + q;
+      const double2 wv = *reinterpret_cast<const double2 *>(w + p);
+      const double wa = (seg == 0) ? pw.w0a : (seg == 1) ? pw.w1 : (seg == 2) ? pw.w2 : 1.0;
+      const double wb = (seg == 0) ? pw.w0b : wa;
+#pragma unroll
+      for (int k = 0; k < NV; k++) {
+        const double2 vv = *reinterpret_cast<const double2 *>(V.v[k] + p);
+        acc[4 * k + seg] += wa * (wv.x * vv.x) + wb * (wv.y * vv.y);
+      }
+    }
+  }
+  block_sum_multi<4 * NV>(acc, sh, partial + (long)blockIdx.x * CGS_STRIDE);
+}
+
+// d_k = <w, V_k> (Stokes product, pressure weighted by pnw^2) and
+// c_k = sig2[k] * d_k for k < nv.
+struct Sig2 {
+  double s[CGS_MAXV];
+};
+__global__ void __launch_bounds__(256) k_cgs_finalize(const double *__restrict__ partial, int nparts, int nv, int dim,
+                                                      Sig2 sig2, double pnw, double *d_out, double *c_out) {
+  __shared__ double sh[4 * 4 * CGS_MAXV];
+  __shared__ double tot[4 * CGS_MAXV];
+  double acc[4 * CGS_MAXV];
+#pragma unroll
+  for (int a = 0; a < 4 * CGS_MAXV; a++) acc[a] = 0.0;
+  for (int b = threadIdx.x; b < nparts; b += blockDim.x) {
+#pragma unroll
+    for (int a = 0; a < 4 * CGS_MAXV; a++)
+      if (a < 4 * nv) acc[a] += partial[(long)b * CGS_STRIDE + a];
+  }
+  block_sum_multi<4 * CGS_MAXV>(acc, sh, tot);
+  __syncthreads();
+  if (threadIdx.x < nv) {
+    const double *s = tot + 4 * threadIdx.x;
+    double su = s[0] + s[1];
+    if (dim == 3) su = su + s[2];
+    const double dk = su + (pnw * pnw) * s[3];
+    d_out[threadIdx.x] = dk;
+    c_out[threadIdx.x] = sig2.s[threadIdx.x] * dk;
+  }
+}
+
+// vn = src - sum_k c_k V_k (k < NV, in order), its FP32 copy v32 (when not
+// null) and the segment sums of <vn, vn> into partial[blk*4 + seg].
+template <int NV>
+__global__ void __launch_bounds__(RED_TPB) k_cgs_update(Grid g, const double *src, VList V,
+                                                        const double *__restrict__ cdev, double *vn, float *v32,
+                                                        double *partial) {
+  __shared__ double sh[(RED_TPB / 64) * 4];
+  double c[NV];
+#pragma unroll
+  for (int k = 0; k < NV; k++) c[k] = cdev[k];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const long npair = g.N / 2;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < npair; t += stride) {
+    const long q = 2 * t;
+    const PairW pw = pair_weights(g, q);
+#pragma unroll
+    for (int seg = 0; seg < 4; seg++) {
+      if (seg == 2 && g.dim == 2) continue;
+      const long p = (long)(seg == 3 ? g.dim : seg) * g.N + q;
+      double2 x = *reinterpret_cast<const double2 *>(src + p);
+#pragma unroll
+      for (int k = 0; k < NV; k++) {
+        const double2 vv = *reinterpret_cast<const double2 *>(V.v[k] + p);
+        x.x = x.x - c[k] * vv.x;
+        x.y = x.y - c[k] * vv.y;
+      }
+      *reinterpret_cast<double2 *>(vn + p) = x;
+      if (v32) *reinterpret_cast<float2 *>(v32 + p) = make_float2((float)x.x, (float)x.y);
+      const double wa = (seg == 0) ? pw.w0a : (seg == 1) ? pw.w1 : (seg == 2) ? pw.w2 : 1.0;
+      const double wb = (seg == 0) ? pw.w0b : wa;
+      acc[seg] += wa * (x.x * x.x) + wb * (x.y * x.y);
+    }
+  }
+  block_sum_multi<4>(acc, sh, partial + (long)blockIdx.x * 4);
+}
+
+void launch_cgs_dots(hipStream_t s, const Grid &g, const double *w, const double *const V[], int nv,
+                     double *partial) {
+  VList L = {};
+  for (int k = 0; k < nv && k < CGS_MAXV; k++) L.v[k] = V[k];
+  switch (nv) {
+    case 1: LAUNCH(k_cgs_dots<1>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, w, L, partial); break;
+    case 2: LAUNCH(k_cgs_dots<2>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, w, L, partial); break;
+    case 3: LAUNCH(k_cgs_dots<3>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, w, L, partial); break;
+    case 4: LAUNCH(k_cgs_dots<4>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, w, L, partial); break;
+    case 5: LAUNCH(k_cgs_dots<5>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, w, L, partial); break;
+    case 6: LAUNCH(k_cgs_dots<6>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, w, L, partial); break;
+    case 7: LAUNCH(k_cgs_dots<7>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, w, L, partial); break;
+    case 8: LAUNCH(k_cgs_dots<8>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, w, L, partial); break;
+    default: report_error("launch_cgs_dots: 1..8 vectors per pass");
+  }
+}
+
+void launch_cgs_finalize(hipStream_t s, const Grid &g, const double *partial, int nv, const double *sig2,
+                         double pnw, double *d_out, double *c_out) {
+  if (nv < 1 || nv > CGS_MAXV) {
+    report_error("launch_cgs_finalize: 1..8 vectors per pass");
+    return;
+  }
+  Sig2 q = {};
+  for (int k = 0; k < nv; k++) q.s[k] = sig2[k];
+  LAUNCH(k_cgs_finalize, dim3(1), dim3(256), 0, s, partial, RED_BLOCKS, nv, g.dim, q, pnw, d_out, c_out);
+}
+
+void launch_cgs_update(hipStream_t s, const Grid &g, const double *src, const double *const V[], int nv,
+                       const double *c, double *vn, float *v32, double *partial) {
+  VList L = {};
+  for (int k = 0; k < nv && k < CGS_MAXV; k++) L.v[k] = V[k];
+  switch (nv) {
+    case 1: LAUNCH(k_cgs_update<1>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, src, L, c, vn, v32, partial); break;
+    case 2: LAUNCH(k_cgs_update<2>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, src, L, c, vn, v32, partial); break;
+    case 3: LAUNCH(k_cgs_update<3>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, src, L, c, vn, v32, partial); break;
+    case 4: LAUNCH(k_cgs_update<4>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, src, L, c, vn, v32, partial); break;
+    case 5: LAUNCH(k_cgs_update<5>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, src, L, c, vn, v32, partial); break;
+    case 6: LAUNCH(k_cgs_update<6>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, src, L, c, vn, v32, partial); break;
+    case 7: LAUNCH(k_cgs_update<7>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, src, L, c, vn, v32, partial); break;
+    case 8: LAUNCH(k_cgs_update<8>, dim3(RED_BLOCKS), dim3(RED_TPB), 0, s, g, src, L, c, vn, v32, partial); break;
+    default: report_error("launch_cgs_update: 1..8 vectors per pass");
+  }
+}
+
 template <typename T>
 __global__ void k_sub_scalar(T *x, long n, const double *scal) {
   long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1127,6 +1315,9 @@ struct Noise {
   int nstages;
   long scc, sed;  // stage strides
   const double *w;
+  // materialized scaled fluxes (filtered form): cell components [dim][N],
+  // node/edge arrays [ne][2][N]; when set, cc/ed/w/variance/eta/T are not read
+  const double *mcc, *med;
 };
 
 // sum_r w_r W_r (saxpy from zero), times variance, times sqrt(eta*T)
@@ -1148,10 +1339,12 @@ __global__ void __launch_bounds__(256) k_stoch_force(Grid g, Ptr3<double> force,
   const long N = g.N;
   double dxinv = 1.0 / g.dx;
   auto WC = [&](int comp, long q) -> double {
+    if (nz.mcc) return nz.mcc[comp * N + q];
     double v = variance * wsum(nz, nz.cc + comp * N, nz.scc, q);
     return v * sqrt(eta[q] * temp[q]);
   };
   auto WE = [&](int e, int comp, long q) -> double {
+    if (nz.med) return nz.med[((long)e * 2 + comp) * N + q];
     double v = variance * wsum(nz, nz.ed[e] + comp * N, nz.sed, q);
     return v * sqrt(eta_ed.c[e][q] * temp_ed.c[e][q]);
   };
@@ -1170,10 +1363,67 @@ __global__ void __launch_bounds__(256) k_stoch_force(Grid g, Ptr3<double> force,
                   (WC(2, t) - WC(2, zm)) * dxinv;
 }
 
+// The scaled fluxes of k_stoch_force's WC / WE (same arithmetic) written out:
+// Scc[comp][N] (dim components), Sed[e][comp][N] (2 components per node / edge array).
+__global__ void __launch_bounds__(256) k_stoch_scale(Grid g, Noise nz, const double *__restrict__ eta,
+                                                     Ptr3<double> eta_ed, const double *__restrict__ temp,
+                                                     Ptr3<double> temp_ed, double variance, double *Scc,
+                                                     double *Sed) {
+  long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  const long N = g.N;
+  const int ne = (g.dim == 2) ? 1 : 3;
+  const double se = sqrt(eta[t] * temp[t]);
+  for (int comp = 0; comp < g.dim; comp++) {
+    const double v = variance * wsum(nz, nz.cc + comp * N, nz.scc, t);
+    Scc[comp * N + t] = v * se;
+  }
+  for (int e = 0; e < ne; e++) {
+    const double sq = sqrt(eta_ed.c[e][t] * temp_ed.c[e][t]);
+    for (int comp = 0; comp < 2; comp++) {
+      const double v = variance * wsum(nz, nz.ed[e] + comp * N, nz.sed, t);
+      Sed[((long)e * 2 + comp) * N + t] = v * sq;
+    }
+  }
+}
+
+// One direction of multifab_filter (multifab_filter.f90:13-200) on narr
+// arrays of N values each: out = the width-W filter of in along dir, terms
+// in the reference's order; periodic neighbours.
+template <int W>
+__global__ void __launch_bounds__(256) k_filter_dir(Grid g, int dir, const double *__restrict__ in, double *out,
+                                                    int narr) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= g.N * narr) return;
+  const long arr = t / g.N, q = t - arr * g.N;
+  int i, j, k;
+  decode(g, q, i, j, k);
+  const double *f = in + arr * g.N;
+  const int cd = (dir == 0) ? i : (dir == 1) ? j : k, nd = g.n[dir];
+  const long base = q - (long)cd * g.s[dir];
+  // periodic neighbour at offset o (|o| <= 4) along dir
+  auto at = [&](int o) -> double {
+    int c = (cd + o) % nd;
+    if (c < 0) c += nd;
+    return f[base + (long)c * g.s[dir]];
+  };
+  const double f0 = f[q];
+  double v;
+  if (W == 1) {
+    v = f0 / 2 + at(-1) / 4 + at(1) / 4;
+  } else if (W == 2) {
+    v = 5 * f0 / 8 - at(-2) / 16 + at(-1) / 4 - at(2) / 16 + at(1) / 4;
+  } else {
+    v = 93 * f0 / 128 + 7 * at(-1) / 32 - 7 * at(-2) / 64 + at(-3) / 32 - at(-4) / 256 + 7 * at(1) / 32 -
+        7 * at(2) / 64 + at(3) / 32 - at(4) / 256;
+  }
+  out[t] = v;
+}
+
 void launch_stoch_m_force(hipStream_t s, const Grid &g, double *const force[3], const double *Wcc,
                           double *const Wed[3], int nstages, const double *weights, long scc, long sed,
                           const double *eta, double *const eta_ed[3], const double *temp,
-                          double *const temp_ed[3], double variance) {
+                          double *const temp_ed[3], double variance, int width, double *work) {
   Noise nz = {};
   nz.cc = Wcc;
   for (int e = 0; e < 3; e++) nz.ed[e] = Wed[e];
@@ -1181,6 +1431,23 @@ void launch_stoch_m_force(hipStream_t s, const Grid &g, double *const force[3], 
   nz.scc = scc;
   nz.sed = sed;
   nz.w = weights;
+  if (width > 0) {
+    // scaled fluxes, then x, y (, z) filter passes between two work halves
+    const int ne = (g.dim == 2) ? 1 : 3;
+    const int narr = g.dim + 2 * ne;
+    double *a = work, *b = work + (long)narr * g.N;
+    LAUNCH(k_stoch_scale, dim3(nblk(g.N)), dim3(TPB), 0, s, g, nz, eta, p3(eta_ed), temp, p3(temp_ed), variance,
+           a, a + (long)g.dim * g.N);
+    for (int dir = 0; dir < g.dim; dir++) {
+      const long n = g.N * narr;
+      if (width == 1) LAUNCH(k_filter_dir<1>, dim3(nblk(n)), dim3(TPB), 0, s, g, dir, a, b, narr);
+      else if (width == 2) LAUNCH(k_filter_dir<2>, dim3(nblk(n)), dim3(TPB), 0, s, g, dir, a, b, narr);
+      else LAUNCH(k_filter_dir<4>, dim3(nblk(n)), dim3(TPB), 0, s, g, dir, a, b, narr);
+      std::swap(a, b);
+    }
+    nz.mcc = a;
+    nz.med = a + (long)g.dim * g.N;
+  }
   LAUNCH(k_stoch_force, dim3(nblk(g.N)), dim3(TPB), 0, s, g, p3(force), nz, eta, p3(eta_ed), temp,
                      p3(temp_ed), variance);
 }
@@ -1272,7 +1539,7 @@ void launch_div_species(hipStream_t s, const Grid &g, int ns, const double *flux
   template void launch_stag_gs_colour<T>(hipStream_t, const Grid &, int, int, int, T, T *const[3],          \
                                          const StagCoef<T> &, const T *);                                    \
   template void launch_stag_applyop<T>(hipStream_t, const Grid &, int, int, T *const[3], T *const[3],        \
-                                       const StagCoef<T> &);                                                 \
+                                       const StagCoef<T> &, bool);                                           \
   template void launch_stag_jacobi<T>(hipStream_t, const Grid &, int, T, T *const[3], T *const[3],           \
                                       T *const[3], const StagCoef<T> &);                                     \
   template void launch_stag_residual<T>(hipStream_t, const Grid &, int, T *const[3], T *const[3],            \

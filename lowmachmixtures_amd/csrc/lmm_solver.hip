@@ -19,6 +19,7 @@
 #include <vector>
 
 #include "../../include/lmm.h"
+#include "lmm_cnn.h"
 #include "lmm_kernels.h"
 
 using namespace lmm;
@@ -100,6 +101,7 @@ struct lmm_ctx {
   int n_rngs = 0;
   double *Wcc = nullptr, *Wed = nullptr;
   double *dweights = nullptr;
+  double *stoch_work = nullptr;  // filtered stochastic fluxes (filtering_width > 0)
   // Smoother kernels: fused red-black sweeps on levels with at least
   // fused_min_tiles workgroups in a launch (32^3 and up), colour-by-colour kernels
   // elsewhere.  Both give the same bits.  LMM_SMOOTHER=colour in the
@@ -129,6 +131,11 @@ struct lmm_ctx {
   // residual b - Ax into the operator (same bits); LMM_KRYLOV_FUSED=0 in the
   // environment keeps separate passes
   bool fused_krylov = true;
+  // the flexible GMRES orthogonalises by classical Gram-Schmidt (one
+  // multi-dot and one update pass per step); LMM_FLEX_GS=mgs in the
+  // environment keeps modified Gram-Schmidt (different rounding)
+  bool flex_cgs = true;
+  double *partial_cgs = nullptr;  // RED_BLOCKS*CGS_STRIDE
   // fused-sweep form, z-slab target and operator brick depth of this
   // context's launches (from the environment at creation;
   // lmm_ctx_set_kernel_options changes them)
@@ -140,6 +147,22 @@ struct lmm_ctx {
   std::vector<hipEvent_t> evpool;  // events reused across scopes
   std::string err;
   int vcycles = 0;
+  // learned coarse-grid correction (precision LMM_LEARNED)
+  lmm_cnn *cnn = nullptr;
+  bool learned_on = false;    // the FP32 preconditioner applies the learned stage
+  int learned_applied = 0;
+  float *gen32 = nullptr;     // training-pair work: phi* and e (6 face arrays)
+};
+
+// A network bound to a context: the correction acts on staggered level
+// `level` of the context's hierarchy.
+struct lmm_cnn {
+  lmm_ctx *ctx = nullptr;
+  CnnNet *net = nullptr;
+  lmm_cnn_config cfg;
+  uint16_t *xin = nullptr;  // BF16 input of one application
+  float *yout = nullptr;    // FP32 output of one application
+  double *ss = nullptr;     // sum of squares of the coarse residual
 };
 
 static int set_err(lmm_ctx *c, int code, const char *fmt, ...) {
@@ -837,6 +860,35 @@ static void apply_matrix_z32(lmm_ctx *c, float *z, const SV &b) {
   apply_matrix_dev(c, sv_of(c, c->r), b);
 }
 
+// The learned coarse-grid correction (no counterpart in the reference): the
+// residual of x_u after the staggered V-cycle, restricted to level lc with
+// the 12-point weights, goes through the network with the level's
+// coefficients; its prediction of the error, prolongated back to level 0,
+// is added to x_u.
+static void learned_stage(lmm_ctx *c, std::vector<SLevel<float>> &S, float *const bu[3], float *const xu[3]) {
+  hipStream_t s = c->stream;
+  lmm_cnn *nn = c->cnn;
+  const int lc = nn->cfg.level, visc = c->prm.visc_type;
+  {
+    ProfScope ps(c, "learned", 0.0);
+    launch_stag_residual<float>(s, S[0].g, visc, xu, bu, S[0].res, S[0].K());
+    launch_stag_restrict<float>(s, S[1].g, S[0].res, S[1].rhs, 0);
+    for (int n = 1; n < lc; n++) launch_stag_restrict<float>(s, S[n + 1].g, S[n].rhs, S[n + 1].rhs, 0);
+    const SLevel<float> &C = S[lc];
+    launch_sumsq3(s, C.g, C.rhs, c->partial, nn->ss);
+    launch_cnn_pack(s, C.g, C.rhs, C.beta, C.alpha, nn->ss, nn->xin);
+    const int gn[3] = {C.g.n[0], C.g.n[1], C.g.n[2]};
+    cnn_forward(nn->net, 1, gn, nn->xin, nn->yout);
+    launch_cnn_unpack(s, C.g, nn->yout, nn->ss, S[lc].phi);
+    for (int n = lc - 1; n >= 1; n--) {
+      for (int d = 0; d < 3; d++) launch_fill<float>(s, S[n].phi[d], S[n].g.N, 0.f);
+      launch_stag_prolong<float>(s, S[n].g, S[n].phi, S[n + 1].phi);
+    }
+    launch_stag_prolong<float>(s, S[0].g, xu, S[1].phi);
+  }
+  c->learned_applied++;
+}
+
 // apply_precon.f90:29-195,434-460 with precon_type 1 (T = FP64 or FP32).
 template <typename T>
 static void apply_precon_t(lmm_ctx *c, std::vector<SLevel<T>> &S, std::vector<CLevel<T>> &C,
@@ -846,6 +898,9 @@ static void apply_precon_t(lmm_ctx *c, std::vector<SLevel<T>> &S, std::vector<CL
   const Grid &g = c->g;
   int dim = g.dim;
   stag_vcycles<T>(c, S, xu, bu, true);  // x_u from a zero guess
+  if constexpr (std::is_same<T, float>::value) {
+    if (c->learned_on && c->cnn) learned_stage(c, S, bu, xu);
+  }
   launch_div_plus<T>(s, g, xu, bp, mac_rhs, 1);
   cc_vcycles<T>(c, C, phi, mac_rhs);
   T *ainv[3] = {C[0].a[0], C[0].a[1], C[0].a[2]};
@@ -1131,6 +1186,48 @@ static void precon32(lmm_ctx *c, const double *v64, float *z32, double theta_alp
   apply_precon_t<float>(c, c->s32, c->c32, b.u, b.p, z.u, z.p, c->mac_rhs32, c->c32[0].phi, theta_alpha);
 }
 
+// One classical Gram-Schmidt step of the flexible GMRES.  The stored basis
+// vectors V_k are not normalised: the unit vector is v_k = sig[k] V_k
+// (sig[0] = 1, V_0 = r/|r|).  On entry w = A Z_{i-1}, where Z_{i-1} = M(V_{i-1})
+// stands for z_{i-1} = sig[i-1] Z_{i-1}, so A z_{i-1} = sig[i-1] w.  One pass
+// forms d_k = <w, V_k> for every k < i, giving H(k+1, i) = sig[i-1] sig[k] d_k;
+// a second writes V_i = w - sum_k sig[k]^2 d_k V_k (= the orthogonalised
+// vector / sig[i-1]) with its FP32 copy (the next preconditioner input) and
+// |V_i|, so H(i+1, i) = sig[i-1] |V_i| and sig[i] = 1/|V_i|.  Two passes over
+// the basis per step instead of 2i, and no separate normalisation pass.
+template <typename VF>
+static void cgs_step(lmm_ctx *c, const VF &Vk, int i, std::vector<double> &sig, Hess &hs) {
+  hipStream_t s = c->stream;
+  const Grid &g = c->g;
+  const long L = (long)(g.dim + 1) * g.N;
+  double *dslot = c->scal + 3300, *cslot = c->scal + 3340, *nslot = c->scal + 3380;
+  std::vector<double> sig2(i);
+  for (int k = 0; k < i; k++) sig2[k] = sig[k] * sig[k];
+  for (int k0 = 0; k0 < i; k0 += CGS_MAXV) {
+    const int nv = std::min(CGS_MAXV, i - k0);
+    const double *vl[CGS_MAXV];
+    for (int k = 0; k < nv; k++) vl[k] = Vk(k0 + k);
+    ProfScope ps(c, "krylov", (double)L * 8.0 * (nv + 1));
+    launch_cgs_dots(s, g, c->w, vl, nv, c->partial_cgs);
+    launch_cgs_finalize(s, g, c->partial_cgs, nv, sig2.data() + k0, c->prm.p_norm_weight, dslot + k0, cslot + k0);
+  }
+  for (int k0 = 0; k0 < i; k0 += CGS_MAXV) {
+    const int nv = std::min(CGS_MAXV, i - k0);
+    const bool last = (k0 + nv == i);
+    const double *vl[CGS_MAXV];
+    for (int k = 0; k < nv; k++) vl[k] = Vk(k0 + k);
+    ProfScope ps(c, "krylov", (double)L * (8.0 * (nv + 2) + (last ? 4.0 : 0.0)));
+    launch_cgs_update(s, g, (k0 == 0) ? c->w : Vk(i), vl, nv, cslot + k0, Vk(i), last ? c->b32 : nullptr, c->partial);
+  }
+  launch_finalize(s, g, c->partial, nslot, 1, c->prm.p_norm_weight);
+  double hb[3 * 32 + 3];
+  fetch(c, dslot, 83, hb);
+  for (int k = 0; k < i; k++) hs.h(k + 1, i) = sig[i - 1] * sig[k] * hb[k];
+  const double nv = hb[80];
+  hs.h(i + 1, i) = sig[i - 1] * nv;
+  sig[i] = (nv > 0.0) ? 1.0 / nv : 0.0;
+}
+
 static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
   const lmm_params &p = c->prm;
   const Grid &g = c->g;
@@ -1141,6 +1238,7 @@ static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
   auto Vk = [&](int k) { return c->V + (long)k * L; };
   auto Zk = [&](int k) { return c->Z32 + (long)k * L; };
   Hess hs(m);
+  std::vector<double> sig(m + 1, 1.0);  // scale of each stored basis vector
   double *dH = c->scal;
   double hb[8];
   stokes_dot_dev(c, c->b, c->b, c->scal + 3004, 1);
@@ -1155,6 +1253,13 @@ static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
   int total_iter = 0, iter = 0;
   double rnorm = 0;
   bool converged = false;
+  // the learned stage runs until it fails: a non-finite Hessenberg column
+  // (the step is redone with the plain V-cycle) or an inner step that cuts
+  // the residual estimate by less than 10 % (later steps use the plain
+  // V-cycle); either way the flexible GMRES keeps its basis
+  c->learned_on = (p.precision == LMM_LEARNED && c->cnn != nullptr);
+  c->learned_applied = 0;
+  info->learned_fallback = 0;
   for (;;) {
     residual_dev(c, x, c->tmp);
     stokes_dot_dev(c, c->tmp, c->tmp, c->scal + 3008, 1);
@@ -1170,29 +1275,48 @@ static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
     if (c->fused_krylov) launch_div_by_f32(s, Vk(0), c->b32, c->tmp, L, c->scal + 3008);
     else launch_div_by(s, Vk(0), c->tmp, L, c->scal + 3008);
     std::fill(hs.s.begin(), hs.s.end(), 0.0);
+    std::fill(sig.begin(), sig.end(), 1.0);
     hs.s[0] = rnorm;
     int i, used = 0;
+    double prev_est = rnorm;
     for (i = 1; i <= m; i++) {
       total_iter++;
       used = i;
-      // b32 already holds V(i-1) in FP32 when the scaling that made V(i-1)
-      // wrote it too
-      precon32(c, Vk(i - 1), Zk(i - 1), theta_alpha, c->fused_krylov);
-      apply_matrix_z32(c, Zk(i - 1), w);
-      double *col = dH + (long)(m + 1) * (i - 1);
-      double *sl = c->scal + 3100;
-      mgs(c, Vk, i, col, sl);
-      hipMemcpyAsync(col + i, sl, sizeof(double), hipMemcpyDeviceToDevice, s);
-      if (c->fused_krylov) launch_div_by_f32(s, Vk(i), c->b32, c->w, L, sl);
-      else launch_div_by(s, Vk(i), c->w, L, sl);
-      fetch(c, dH + (long)(m + 1) * (i - 1), i + 1, &hs.h(1, i));
+      for (int attempt = 0; attempt < 2; attempt++) {
+        // b32 already holds V(i-1) in FP32 when the step that made V(i-1)
+        // wrote it too (not on a redone step: that step overwrote it)
+        precon32(c, Vk(i - 1), Zk(i - 1), theta_alpha, attempt == 0 && (c->fused_krylov || c->flex_cgs));
+        apply_matrix_z32(c, Zk(i - 1), w);
+        if (c->flex_cgs) {
+          cgs_step(c, Vk, i, sig, hs);
+        } else {
+          double *col = dH + (long)(m + 1) * (i - 1);
+          double *sl = c->scal + 3100;
+          mgs(c, Vk, i, col, sl);
+          hipMemcpyAsync(col + i, sl, sizeof(double), hipMemcpyDeviceToDevice, s);
+          if (c->fused_krylov) launch_div_by_f32(s, Vk(i), c->b32, c->w, L, sl);
+          else launch_div_by(s, Vk(i), c->w, L, sl);
+          fetch(c, dH + (long)(m + 1) * (i - 1), i + 1, &hs.h(1, i));
+        }
+        bool finite = true;
+        for (int k = 1; k <= i + 1; k++) finite = finite && std::isfinite(hs.h(k, i));
+        if (finite || !c->learned_on) break;
+        c->learned_on = false;  // redo the step with the plain V-cycle
+        info->learned_fallback = 1;
+      }
       bool happy = (hs.h(i + 1, i) == 0.0);  // exact solution in the space
       hs.least_squares(i);
       double est = std::fabs(hs.s[i]);
+      if (c->learned_on && !(est < 0.9 * prev_est)) {
+        c->learned_on = false;
+        info->learned_fallback = 1;
+      }
+      prev_est = est;
       if (happy || est <= p.gmres_rel_tol * norm_b || total_iter >= p.gmres_max_iter) break;
     }
     hs.solve(used - 1);
-    // x += sum_k y_k Z_k  (Z in FP32, accumulation in FP64)
+    // x += sum_k y_k z_k with z_k = sig_k Z_k  (Z in FP32, accumulation in FP64)
+    for (int k = 0; k < used; k++) hs.y[k] *= sig[k];
     memcpy(c->hscal, hs.y.data(), sizeof(double) * used);
     hipMemcpyAsync(c->scal + 3200, c->hscal, sizeof(double) * used, hipMemcpyHostToDevice, s);
     if (c->z32_direct) {
@@ -1214,6 +1338,8 @@ static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
   info->norm_resid = rnorm;
   info->norm_resid_true = rnorm;
   info->status = converged ? LMM_OK : LMM_NOT_CONVERGED;
+  info->learned_applied = c->learned_applied;
+  c->learned_on = false;
   (void)tmp;
   return LMM_OK;
 }
@@ -1256,6 +1382,8 @@ static lmm_kernel_options env_options() {
   if (ab) o.am_brick = atoi(ab);
   const char *co = getenv("LMM_COARSE");
   if (co && co[0] == '0') o.coarse_one_launch = 0;
+  const char *gs = getenv("LMM_FLEX_GS");
+  if (gs && strcmp(gs, "mgs") == 0) o.flex_cgs = 0;
   return o;
 }
 
@@ -1267,6 +1395,7 @@ static void apply_options(lmm_ctx *c, const lmm_kernel_options &o) {
   c->fused_krylov = o.krylov_fused != 0;
   c->z32_direct = o.z32_direct != 0;
   c->coarse_one_launch = o.coarse_one_launch != 0;
+  c->flex_cgs = o.flex_cgs != 0;
   c->tune.rb_variant = o.fused_kernel;
   c->tune.rb_target = o.fused_target;
   c->tune.am_dz = o.am_brick;
@@ -1303,6 +1432,8 @@ void lmm_params_default(lmm_params *p) {
   p->gmres_min_iter = 1;
   p->gmres_verbose = 0;
   p->precision = LMM_FP64;
+  p->stoch_stress_form = 1;
+  p->filtering_width = 0;
 }
 
 static int check_params(lmm_ctx *c, const lmm_params *p) {
@@ -1312,9 +1443,12 @@ static int check_params(lmm_ctx *c, const lmm_params *p) {
   if (p->precon_type != 1) return set_err(c, LMM_ERR_UNSUPPORTED, "precon_type %d: only 1 is implemented", p->precon_type);
   if (p->gmres_max_inner < 1 || p->gmres_max_inner > 30) return set_err(c, LMM_ERR_ARG, "gmres_max_inner out of range 1..30");
   if (p->stag_mg_smoother != 0 && p->stag_mg_smoother != 1) return set_err(c, LMM_ERR_UNSUPPORTED, "stag_mg_smoother %d", p->stag_mg_smoother);
-  if (p->precision < LMM_FP64 || p->precision > LMM_FP32_MG)
+  if (p->precision < LMM_FP64 || p->precision > LMM_LEARNED)
     return set_err(c, LMM_ERR_UNSUPPORTED, "precision %d not available", p->precision);
   if (p->stag_mg_minwidth < 1 || p->mg_minwidth < 1) return set_err(c, LMM_ERR_ARG, "minwidth must be >= 1");
+  const int fw = p->filtering_width;
+  if (!(fw == 0 || fw == 1 || fw == 2 || fw == 4))
+    return set_err(c, LMM_ERR_ARG, "filtering_width %d: must be 0, 1, 2 or 4 (multifab_filter.f90)", fw);
   return LMM_OK;
 }
 
@@ -1364,6 +1498,7 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
   rc = alloc_levels<double>(c, c->s64, c->c64);
   long N = c->g.N, L = (long)(dim + 1) * N;
   c->partial = dalloc<double>(c->stream, RED_BLOCKS * 4);
+  c->partial_cgs = dalloc<double>(c->stream, (long)RED_BLOCKS * CGS_STRIDE);
   c->partial2 = dalloc<double>(c->stream, (N + 255) / 256 + 1);
   c->scal = dalloc<double>(c->stream, 8192);
   if (hipHostMalloc((void **)&c->hscal, 8192 * sizeof(double), hipHostMallocDefault) != hipSuccess) c->hscal = nullptr;
@@ -1380,7 +1515,7 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
   c->x = dalloc<double>(c->stream, L);
   c->mac_rhs = dalloc<double>(c->stream, N);
   c->phi_cc = dalloc<double>(c->stream, N);
-  bool ok = !rc && c->partial && c->partial2 && c->scal && c->hscal && c->r && c->w && c->tmp && c->b && c->x &&
+  bool ok = !rc && c->partial && c->partial_cgs && c->partial2 && c->scal && c->hscal && c->r && c->w && c->tmp && c->b && c->x &&
             c->mac_rhs && c->phi_cc && c->scaled_beta && c->scaled_gamma;
   for (int d = 0; d < dim; d++) ok = ok && c->ainv[d];
   for (int e = 0; e < ne_of(dim); e++) ok = ok && c->scaled_bed[e];
@@ -1400,10 +1535,11 @@ int lmm_ctx_destroy(lmm_ctx *c) {
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   free_levels<double>(c->s64, c->c64, false);
   free_levels<float>(c->s32, c->c32, true);
-  for (void *p : {(void *)c->partial, (void *)c->partial2, (void *)c->scal, (void *)c->scaled_beta,
+  dfree(c->gen32);
+  for (void *p : {(void *)c->partial, (void *)c->partial_cgs, (void *)c->partial2, (void *)c->scal, (void *)c->scaled_beta,
                   (void *)c->scaled_gamma, (void *)c->V, (void *)c->r, (void *)c->w, (void *)c->tmp, (void *)c->b,
                   (void *)c->x, (void *)c->Z32, (void *)c->b32, (void *)c->mac_rhs, (void *)c->phi_cc,
-                  (void *)c->mac_rhs32, (void *)c->Wcc, (void *)c->Wed, (void *)c->dweights})
+                  (void *)c->mac_rhs32, (void *)c->Wcc, (void *)c->Wed, (void *)c->dweights, (void *)c->stoch_work})
     dfree(p);
   for (int d = 0; d < 3; d++) {
     dfree(c->ainv[d]);
@@ -1452,6 +1588,7 @@ void lmm_kernel_options_default(lmm_kernel_options *o) {
   o->am_brick = 32;
   o->z32_direct = 1;
   o->krylov_fused = 1;
+  o->flex_cgs = 1;
 }
 
 int lmm_ctx_get_kernel_options(const lmm_ctx *c, lmm_kernel_options *o) {
@@ -1690,6 +1827,8 @@ int lmm_gmres(lmm_ctx *c, lmm_field *const b_u[3], lmm_field *b_p, lmm_field *x_
   const lmm_params &p = c->prm;
   if ((rc = ensure_krylov(c, p.gmres_max_inner))) return rc;
   bool mixed = p.precision != LMM_FP64;
+  if (p.precision == LMM_LEARNED && !c->cnn)
+    return set_err(c, LMM_ERR_ARG, "precision LMM_LEARNED needs a network (lmm_ctx_set_cnn)");
   if (mixed && (rc = ensure_lvl32(c))) return rc;
   double sf = p.scale_factor;
   double ta = theta_alpha * (sf != 1.0 ? sf : 1.0);
@@ -1847,6 +1986,40 @@ int lmm_stag_applyop(lmm_ctx *c, lmm_field *const phi_fc[3], lmm_field *Lphi_fc[
     lp[d] = Lphi_fc[d]->d;
   }
   launch_stag_applyop<double>(c->stream, c->g, c->prm.visc_type, color, ph, lp, L.K());
+  return finish(c);
+}
+
+// diffusive_m_fluxdiv.f90:16-73: m_update = m_update - (0*I - L_eta)umac,
+// stag_applyop_level with alpha_fc = 0 (kappa is the gamma of the operator)
+int lmm_diffusive_m_fluxdiv(lmm_ctx *c, lmm_field *m_update[3], lmm_field *const umac[3], lmm_field *eta,
+                            lmm_field *const eta_ed[3], lmm_field *kappa) {
+  if (!c) return LMM_ERR_ARG;
+  TuningScope ts(&c->tune);
+  int rc;
+  if ((rc = check_stokes(c, m_update, nullptr, "diffusive_m_fluxdiv m_update"))) return rc;
+  if ((rc = check_stokes(c, umac, nullptr, "diffusive_m_fluxdiv umac"))) return rc;
+  if (!field_ok(c, eta, LMM_CELL)) return set_err(c, LMM_ERR_ARG, "eta missing or wrong kind");
+  if (!field_ok(c, kappa, LMM_CELL)) return set_err(c, LMM_ERR_ARG, "kappa missing or wrong kind");
+  const int dim = c->g.dim;
+  if (dim == 2) {
+    if (!field_ok(c, eta_ed[0], LMM_NODE)) return set_err(c, LMM_ERR_ARG, "eta_ed[0] must be a node field in 2D");
+  } else {
+    for (int e = 0; e < 3; e++)
+      if (!field_ok(c, eta_ed[e], LMM_EDGE_XY + e)) return set_err(c, LMM_ERR_ARG, "eta_ed[%d] missing or wrong kind", e);
+  }
+  for (int d = 0; d < dim; d++)
+    if (m_update[d]->d == umac[d]->d) return set_err(c, LMM_ERR_ARG, "m_update and umac must be distinct fields");
+  auto &L = c->s64[0];
+  for (int d = 0; d < dim; d++) launch_fill<double>(c->stream, L.alpha[d], c->g.N, 0.0);
+  L.beta = eta->d;
+  L.gamma = kappa->d;
+  for (int e = 0; e < ne_of(dim); e++) L.bed[e] = eta_ed[e]->d;
+  double *ph[3] = {nullptr, nullptr, nullptr}, *mu[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < dim; d++) {
+    ph[d] = umac[d]->d;
+    mu[d] = m_update[d]->d;
+  }
+  launch_stag_applyop<double>(c->stream, c->g, c->prm.visc_type, 0, ph, mu, L.K(), true);
   return finish(c);
 }
 
@@ -2014,20 +2187,38 @@ int lmm_init_m_stochastic(lmm_ctx *c, int n_rngs) {
   return LMM_OK;
 }
 
+// fill_m_stochastic (stochastic_m_fluxdiv.f90:556-609): per stage, W_cc,
+// then each node/edge array.  Symmetric form (stoch_stress_form != 0): W_cc
+// = sqrt(2)*N(0,1) (multifab_fill_random with variance 2, its :58-60 order
+// sqrt(variance)*fp) and only component 1 of a node/edge array is drawn,
+// component 2 being its copy (multifab_copy_c).
+static bool symmetric_stress(const lmm_ctx *c) { return c->prm.stoch_stress_form != 0; }
+
 int lmm_fill_m_stochastic(lmm_ctx *c, uint32_t seed) {
   if (!c || c->n_rngs < 1) return LMM_ERR_ARG;
   long N = c->g.N;
   int dim = c->g.dim, ne = ne_of(dim);
-  long per = (long)(dim + 2 * ne) * N;
+  const bool sym = symmetric_stress(c);
+  const int edraw = sym ? 1 : 2;  // node/edge components drawn per array
+  long per = (long)(dim + edraw * ne) * N;
   std::vector<double> h((size_t)per * c->n_rngs);
   lmm_fill_normal_host(seed, (int64_t)h.size(), h.data());
+  const double s2 = std::sqrt(2.0);
+  std::vector<double> ed((size_t)ne * 2 * N);
   for (int r = 0; r < c->n_rngs; r++) {
-    const double *src = h.data() + (size_t)r * per;
+    double *src = h.data() + (size_t)r * per;
+    if (sym)
+      for (long q = 0; q < dim * N; q++) src[q] = s2 * src[q];
+    for (int e = 0; e < ne; e++)
+      for (int comp = 0; comp < 2; comp++) {
+        const double *from = src + dim * N + ((long)e * edraw + (sym ? 0 : comp)) * N;
+        memcpy(ed.data() + ((size_t)e * 2 + comp) * N, from, sizeof(double) * N);
+      }
     HIPCHK(c, hipMemcpyAsync(c->Wcc + (long)r * dim * N, src, sizeof(double) * dim * N, hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, hipMemcpyAsync(c->Wed + (long)r * ne * 2 * N, src + dim * N, sizeof(double) * ne * 2 * N,
+    HIPCHK(c, hipMemcpyAsync(c->Wed + (long)r * ne * 2 * N, ed.data(), sizeof(double) * ne * 2 * N,
                              hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // ed is reused by the next stage
   }
-  HIPCHK(c, hipStreamSynchronize(c->stream));
   return LMM_OK;
 }
 
@@ -2035,12 +2226,24 @@ int lmm_fill_m_stochastic_device(lmm_ctx *c, uint64_t seed, uint64_t step) {
   if (!c || c->n_rngs < 1) return LMM_ERR_ARG;
   const long N = c->g.N;
   const int dim = c->g.dim, ne = ne_of(dim);
-  const long per = (long)(dim + 2 * ne) * N;
+  const bool sym = symmetric_stress(c);
+  const int edraw = sym ? 1 : 2;
+  const long per = (long)(dim + edraw * ne) * N;
   // the value order of lmm_fill_m_stochastic: stage, W_cc components, then
-  // the node / edge arrays with their 2 components
+  // the node / edge arrays with their drawn components
   for (int r = 0; r < c->n_rngs; r++) {
-    launch_fill_normal_philox(c->stream, c->Wcc + (long)r * dim * N, dim * N, r * per, seed, step);
-    launch_fill_normal_philox(c->stream, c->Wed + (long)r * ne * 2 * N, ne * 2 * N, r * per + dim * N, seed, step);
+    double *wcc = c->Wcc + (long)r * dim * N, *wed = c->Wed + (long)r * ne * 2 * N;
+    launch_fill_normal_philox(c->stream, wcc, dim * N, r * per, seed, step);
+    if (!sym) {
+      launch_fill_normal_philox(c->stream, wed, ne * 2 * N, r * per + dim * N, seed, step);
+      continue;
+    }
+    launch_scale_copy(c->stream, wcc, wcc, (long)dim * N, std::sqrt(2.0));
+    for (int e = 0; e < ne; e++) {
+      launch_fill_normal_philox(c->stream, wed + (long)e * 2 * N, N, r * per + (long)(dim + e) * N, seed, step);
+      HIPCHK(c, hipMemcpyAsync(wed + ((long)e * 2 + 1) * N, wed + (long)e * 2 * N, sizeof(double) * N,
+                               hipMemcpyDeviceToDevice, c->stream));
+    }
   }
   return finish(c);
 }
@@ -2069,7 +2272,13 @@ static int stoch_common(lmm_ctx *c, lmm_field *force[3], const double *Wcc, doub
     ee[e] = eta_ed[e]->d;
     te[e] = temp_ed[e]->d;
   }
-  launch_stoch_m_force(c->stream, c->g, f, Wcc, Wed, nst, w_dev, scc, sed, eta->d, ee, temp->d, te, variance);
+  const int width = c->prm.filtering_width;
+  if (width > 0 && !c->stoch_work) {
+    c->stoch_work = dalloc<double>(c->stream, 2L * (dim + 2 * ne) * c->g.N);
+    if (!c->stoch_work) return set_err(c, LMM_ERR_NOMEM, "cannot allocate the filtering work arrays");
+  }
+  launch_stoch_m_force(c->stream, c->g, f, Wcc, Wed, nst, w_dev, scc, sed, eta->d, ee, temp->d, te, variance, width,
+                       c->stoch_work);
   return finish(c);
 }
 
@@ -2137,6 +2346,197 @@ int lmm_diffusive_mass_fluxdiv(lmm_ctx *c, int ns, lmm_field *rho, lmm_field *rh
   int rc = finish(c);
   dfree(flux);
   return rc;
+}
+
+// ---------------- learned coarse-grid correction ----------------
+
+void lmm_cnn_config_default(lmm_cnn_config *cfg) {
+  if (!cfg) return;
+  cfg->cin = 8;
+  cfg->c1 = 16;
+  cfg->c2 = 32;
+  cfg->c3 = 64;
+  cfg->cout = 3;
+  cfg->level = 3;
+}
+
+int lmm_cnn_create(lmm_ctx *c, const lmm_cnn_config *cfg, uint64_t seed, lmm_cnn **out) {
+  if (!c || !out) return LMM_ERR_ARG;
+  *out = nullptr;
+  lmm_cnn_config d;
+  lmm_cnn_config_default(&d);
+  if (!cfg) cfg = &d;
+  CnnConfig cc;
+  cc.cin = cfg->cin;
+  cc.c1 = cfg->c1;
+  cc.c2 = cfg->c2;
+  cc.c3 = cfg->c3;
+  cc.cout = cfg->cout;
+  std::string err;
+  CnnNet *net = cnn_new(cc, seed, c->stream, &err);
+  if (!net) return set_err(c, LMM_ERR_ARG, "%s", err.c_str());
+  lmm_cnn *nn = new lmm_cnn();
+  nn->ctx = c;
+  nn->net = net;
+  nn->cfg = *cfg;
+  *out = nn;
+  return LMM_OK;
+}
+
+int lmm_cnn_destroy(lmm_cnn *nn) {
+  if (!nn) return LMM_ERR_ARG;
+  if (nn->ctx && nn->ctx->cnn == nn) nn->ctx->cnn = nullptr;
+  cnn_free(nn->net);
+  dfree(nn->xin);
+  dfree(nn->yout);
+  dfree(nn->ss);
+  delete nn;
+  return LMM_OK;
+}
+
+int lmm_cnn_num_params(const lmm_cnn *nn, int64_t *n) {
+  if (!nn || !n) return LMM_ERR_ARG;
+  *n = cnn_nparams(nn->net);
+  return LMM_OK;
+}
+
+int lmm_cnn_get_params(lmm_cnn *nn, float *host) {
+  if (!nn || !host) return LMM_ERR_ARG;
+  return cnn_get_params(nn->net, host) ? set_err(nn->ctx, LMM_ERR_DEVICE, "cnn_get_params") : LMM_OK;
+}
+
+int lmm_cnn_set_params(lmm_cnn *nn, const float *host) {
+  if (!nn || !host) return LMM_ERR_ARG;
+  return cnn_set_params(nn->net, host) ? set_err(nn->ctx, LMM_ERR_DEVICE, "cnn_set_params") : LMM_OK;
+}
+
+int lmm_cnn_forward_host(lmm_cnn *nn, int nb, const int n[3], const float *x, float *y) {
+  if (!nn || !n || !x || !y) return LMM_ERR_ARG;
+  int rc = cnn_forward_host(nn->net, nb, n, x, y);
+  if (rc) return set_err(nn->ctx, rc, "cnn forward: grid extents must be multiples of 4 and nb*(n/4)^3 of 16");
+  return finish(nn->ctx);
+}
+
+int lmm_cnn_loss_grad_host(lmm_cnn *nn, int nb, const int n[3], const float *x, const float *t, double *loss,
+                           float *grad) {
+  if (!nn || !n || !x || !t || !loss || !grad) return LMM_ERR_ARG;
+  int rc = cnn_loss_grad_host(nn->net, nb, n, x, t, loss, grad);
+  if (rc) return set_err(nn->ctx, rc, "cnn loss/gradient: grid extents must be multiples of 4 and nb*(n/4)^3 of 16");
+  return finish(nn->ctx);
+}
+
+// the network's level must exist and its grid suit the network (3D, 8 input
+// and 3 output channels: residual, log beta, log theta*alpha)
+static int cnn_level_ok(lmm_ctx *c, lmm_cnn *nn) {
+  const lmm_cnn_config &f = nn->cfg;
+  if (c->g.dim != 3) return set_err(c, LMM_ERR_UNSUPPORTED, "the learned correction is 3D only");
+  if (f.cin != 8 || f.cout != 3) return set_err(c, LMM_ERR_ARG, "the Stokes correction needs cin 8 and cout 3");
+  if (f.level < 1 || f.level >= c->nl_s) return set_err(c, LMM_ERR_ARG, "cnn level %d outside 1..%d", f.level, c->nl_s - 1);
+  Grid gc = c->g;
+  for (int n = 0; n < f.level; n++) gc = coarsen(gc);
+  for (int d = 0; d < 3; d++)
+    if (gc.n[d] % 4 != 0 || gc.n[d] < 4) return set_err(c, LMM_ERR_ARG, "cnn level grid must be a multiple of 4");
+  if (((long)(gc.n[0] / 4) * (gc.n[1] / 4) * (gc.n[2] / 4)) % 16 != 0)
+    return set_err(c, LMM_ERR_ARG, "cnn level grid: (n/4)^3 must be a multiple of 16");
+  if (!nn->xin) {
+    nn->xin = dalloc<uint16_t>(c->stream, gc.N * 8);
+    nn->yout = dalloc<float>(c->stream, gc.N * CNN_OUT_STRIDE);
+    nn->ss = dalloc<double>(c->stream, 4);
+    if (!nn->xin || !nn->yout || !nn->ss) return set_err(c, LMM_ERR_NOMEM, "cnn application buffers");
+  }
+  return LMM_OK;
+}
+
+int lmm_ctx_set_cnn(lmm_ctx *c, lmm_cnn *nn) {
+  if (!c) return LMM_ERR_ARG;
+  if (!nn) {
+    c->cnn = nullptr;
+    return LMM_OK;
+  }
+  if (nn->ctx != c) return set_err(c, LMM_ERR_ARG, "the network belongs to another context");
+  int rc = cnn_level_ok(c, nn);
+  if (rc) return rc;
+  c->cnn = nn;
+  return LMM_OK;
+}
+
+int lmm_cnn_generate_pairs(lmm_cnn *nn, lmm_field *const alpha_fc[3], lmm_field *beta, lmm_field *const beta_ed[3],
+                           lmm_field *gamma, double theta_alpha, int npairs, uint64_t seed) {
+  if (!nn || npairs < 1) return LMM_ERR_ARG;
+  lmm_ctx *c = nn->ctx;
+  TuningScope ts(&c->tune);
+  int rc;
+  if ((rc = check_coefs(c, alpha_fc, beta, beta_ed, gamma))) return rc;
+  if ((rc = cnn_level_ok(c, nn))) return rc;
+  if ((rc = ensure_lvl32(c))) return rc;
+  const Grid &g = c->g;
+  const long N = g.N;
+  if (!c->gen32) {
+    c->gen32 = dalloc<float>(c->stream, 6 * N);
+    if (!c->gen32) return set_err(c, LMM_ERR_NOMEM, "training-pair work arrays");
+  }
+  const int lc = nn->cfg.level;
+  auto &S = c->s32;
+  const int gn[3] = {S[lc].g.n[0], S[lc].g.n[1], S[lc].g.n[2]};
+  const long have = cnn_dataset_count(nn->net);
+  if ((rc = cnn_dataset_reserve(nn->net, have + npairs, gn)))
+    return set_err(c, rc, "training pairs: out of memory or a grid other than the stored pairs'");
+  prepare_coefs(c, coef_in(c, alpha_fc, beta, beta_ed, gamma), theta_alpha, 1.0, false, true);
+  hipStream_t s = c->stream;
+  const int visc = c->prm.visc_type;
+  float *ps[3], *e[3], *b[3];
+  for (int d = 0; d < 3; d++) {
+    ps[d] = c->gen32 + d * N;
+    e[d] = c->gen32 + (3 + d) * N;
+    b[d] = c->b32 + d * N;
+  }
+  for (int q = 0; q < npairs; q++) {
+    // phi* = one V-cycle of white noise (a sample of the solutions), b = A phi*
+    launch_fill_normal_philox(s, c->r, 3 * N, 0, seed, (uint64_t)q);
+    launch_convert<float, double>(s, c->b32, c->r, 3 * N);
+    stag_vcycles<float>(c, S, ps, b, true);
+    launch_stag_applyop<float>(s, g, visc, 0, ps, b, S[0].K());
+    // the plain V-cycle's answer phi1 (in e), its residual r = b - A phi1 and error e = phi* - phi1
+    stag_vcycles<float>(c, S, e, b, true);
+    launch_stag_residual<float>(s, g, visc, e, b, S[0].res, S[0].K());
+    for (int d = 0; d < 3; d++) launch_sub_f32(s, e[d], ps[d], e[d], N);
+    // the residual with the 12-point weights, the error with face averages
+    launch_stag_restrict<float>(s, S[1].g, S[0].res, S[1].rhs, 0);
+    launch_stag_restrict<float>(s, S[1].g, e, S[1].phi, 1);
+    for (int n = 1; n < lc; n++) {
+      launch_stag_restrict<float>(s, S[n + 1].g, S[n].rhs, S[n + 1].rhs, 0);
+      launch_stag_restrict<float>(s, S[n + 1].g, S[n].phi, S[n + 1].phi, 1);
+    }
+    const long slot = have + q;
+    launch_sumsq3(s, S[lc].g, S[lc].rhs, c->partial, nn->ss);
+    launch_cnn_pack(s, S[lc].g, S[lc].rhs, S[lc].beta, S[lc].alpha, nn->ss, cnn_dataset_x(nn->net, slot));
+    launch_cnn_pack_target(s, S[lc].g, S[lc].phi, nn->ss, cnn_dataset_t(nn->net, slot));
+    cnn_dataset_push(nn->net);
+  }
+  return finish(c);
+}
+
+int lmm_cnn_num_pairs(const lmm_cnn *nn, int64_t *count) {
+  if (!nn || !count) return LMM_ERR_ARG;
+  *count = cnn_dataset_count(nn->net);
+  return LMM_OK;
+}
+
+int lmm_cnn_train(lmm_cnn *nn, int steps, int batch, double lr, uint64_t seed, int64_t held_out, float *loss_history) {
+  if (!nn || held_out < 0) return LMM_ERR_ARG;
+  lmm_ctx *c = nn->ctx;
+  const long cnt = cnn_dataset_count(nn->net);
+  if (held_out >= cnt) return set_err(c, LMM_ERR_ARG, "no training pairs left after %lld held out", (long long)held_out);
+  int rc = cnn_train(nn->net, steps, batch, lr, seed, cnt - held_out, loss_history);
+  if (rc) return set_err(c, rc, "cnn training: bad steps/batch or out of memory");
+  return finish(c);
+}
+
+int lmm_cnn_evaluate(lmm_cnn *nn, int64_t first, int64_t last, double *rel_err) {
+  if (!nn || !rel_err) return LMM_ERR_ARG;
+  int rc = cnn_eval(nn->net, (long)first, (long)last, rel_err);
+  if (rc) return set_err(nn->ctx, rc, "cnn evaluation: pair range");
+  return finish(nn->ctx);
 }
 
 }  // extern "C"

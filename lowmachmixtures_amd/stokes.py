@@ -41,7 +41,7 @@ class Params(C.Structure):
         ("stag_mg_omega", C.c_double), ("stag_mg_rel_tol", C.c_double), ("gmres_rel_tol", C.c_double),
         ("gmres_abs_tol", C.c_double), ("gmres_max_outer", C.c_int), ("gmres_max_inner", C.c_int),
         ("gmres_max_iter", C.c_int), ("gmres_min_iter", C.c_int), ("gmres_verbose", C.c_int),
-        ("precision", C.c_int),
+        ("precision", C.c_int), ("stoch_stress_form", C.c_int), ("filtering_width", C.c_int),
     ]
 
 
@@ -50,6 +50,7 @@ class GmresInfo(C.Structure):
         ("total_iter", C.c_int), ("outer_iter", C.c_int), ("norm_pre_rhs", C.c_double),
         ("norm_resid", C.c_double), ("norm_b", C.c_double), ("norm_resid_true", C.c_double),
         ("status", C.c_int), ("vcycles", C.c_int), ("seconds", C.c_double),
+        ("learned_applied", C.c_int), ("learned_fallback", C.c_int),
     ]
 
     def as_dict(self):
@@ -57,12 +58,13 @@ class GmresInfo(C.Structure):
 
 
 class KernelOptions(C.Structure):
-    """lmm_kernel_options: kernel forms of a context (every choice gives the same bits)."""
+    """lmm_kernel_options: kernel forms of a context (every choice but flex_cgs gives
+    the same bits; flex_cgs selects the flexible GMRES's Gram-Schmidt form)."""
     _fields_ = [
         ("fused_sweeps", C.c_int), ("fused_min_tiles", C.c_int), ("fused_kernel", C.c_int),
         ("fused_target", C.c_int), ("fused_x3", C.c_int), ("resid_restrict", C.c_int),
         ("coarse_one_launch", C.c_int), ("am_brick", C.c_int), ("z32_direct", C.c_int),
-        ("krylov_fused", C.c_int),
+        ("krylov_fused", C.c_int), ("flex_cgs", C.c_int),
     ]
 
     def as_dict(self):
@@ -116,6 +118,7 @@ _SIGS = {
     "lmm_stag_mg_solver": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP * 3, _FP, C.c_double, _FP * 3, _FP * 3]),
     "lmm_macproject": (C.c_int, [C.c_void_p, _FP, _FP * 3, _FP * 3, _FP]),
     "lmm_stag_applyop": (C.c_int, [C.c_void_p, _FP * 3, _FP * 3, _FP * 3, _FP, _FP * 3, _FP, C.c_double, C.c_int]),
+    "lmm_diffusive_m_fluxdiv": (C.c_int, [C.c_void_p, _FP * 3, _FP * 3, _FP, _FP * 3, _FP]),
     "lmm_stag_inner_prod": (C.c_int, [C.c_void_p, _FP * 3, _FP * 3, C.POINTER(C.c_double)]),
     "lmm_cc_inner_prod": (C.c_int, [C.c_void_p, _FP, _FP, C.POINTER(C.c_double)]),
     "lmm_compute_div": (C.c_int, [C.c_void_p, _FP * 3, _FP, C.c_int]),
@@ -134,6 +137,22 @@ _SIGS = {
     "lmm_diffusive_mass_fluxdiv": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP * 3,
                                              _FP * 3, C.c_int]),
     "lmm_fill_normal_host": (C.c_int, [C.c_uint32, C.c_int64, C.POINTER(C.c_double)]),
+    "lmm_cnn_config_default": (None, [C.c_void_p]),
+    "lmm_cnn_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
+    "lmm_cnn_destroy": (C.c_int, [C.c_void_p]),
+    "lmm_cnn_num_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "lmm_cnn_get_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "lmm_cnn_set_params": (C.c_int, [C.c_void_p, C.POINTER(C.c_float)]),
+    "lmm_cnn_forward_host": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float),
+                                       C.POINTER(C.c_float)]),
+    "lmm_cnn_loss_grad_host": (C.c_int, [C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_float),
+                                         C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_float)]),
+    "lmm_cnn_generate_pairs": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP * 3, _FP, C.c_double, C.c_int, C.c_uint64]),
+    "lmm_cnn_num_pairs": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "lmm_cnn_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int64,
+                                C.POINTER(C.c_float)]),
+    "lmm_cnn_evaluate": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_double)]),
+    "lmm_ctx_set_cnn": (C.c_int, [C.c_void_p, C.c_void_p]),
 }
 
 
@@ -404,6 +423,12 @@ def stag_applyop_level(ctx, phi_fc, Lphi_fc, alpha_fc, beta_cc, beta_ed, gamma_c
                                      _h(gamma_cc), C.c_double(theta_alpha), color), "stag_applyop")
 
 
+def diffusive_m_fluxdiv(ctx, m_update, umac, eta, eta_ed, kappa):
+    """diffusive_m_fluxdiv.f90:16-73: m_update += L_eta(umac) (stag_applyop with alpha = 0, subtracted)."""
+    ctx.check(lib().lmm_diffusive_m_fluxdiv(ctx.h, _a3(m_update), _a3(umac), _h(eta), _a3(eta_ed), _h(kappa)),
+              "diffusive_m_fluxdiv")
+
+
 def stag_inner_prod(ctx, m1, m2):
     out = (C.c_double * 3)()
     ctx.check(lib().lmm_stag_inner_prod(ctx.h, _a3(m1), _a3(m2), out), "stag_inner_prod")
@@ -530,3 +555,107 @@ class DeviceProblem:
 
     def solution(self):
         return [f.download() for f in self.x_u], self.x_p.download()
+
+
+# ---------------------------------------------------------------- learned correction
+
+class CnnConfig(C.Structure):
+    """lmm_cnn_config: channels of the U-Net and the multigrid level it acts on."""
+    _fields_ = [("cin", C.c_int), ("c1", C.c_int), ("c2", C.c_int), ("c3", C.c_int), ("cout", C.c_int),
+                ("level", C.c_int)]
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Cnn:
+    """The learned coarse-grid correction network of a context (lmm_cnn)."""
+
+    def __init__(self, ctx, seed=0, **cfg):
+        c = CnnConfig()
+        lib().lmm_cnn_config_default(C.byref(c))
+        for k, v in cfg.items():
+            if not hasattr(c, k):
+                raise KeyError(k)
+            setattr(c, k, int(v))
+        self.cfg = c
+        self.ctx = ctx
+        h = C.c_void_p()
+        ctx.check(lib().lmm_cnn_create(ctx.h, C.byref(c), C.c_uint64(seed), C.byref(h)), "cnn_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().lmm_cnn_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def num_params(self):
+        n = C.c_int64()
+        self.ctx.check(lib().lmm_cnn_num_params(self.h, C.byref(n)), "cnn_num_params")
+        return n.value
+
+    def get_params(self):
+        out = np.zeros(self.num_params(), dtype=np.float32)
+        self.ctx.check(lib().lmm_cnn_get_params(self.h, _fptr(out)), "cnn_get_params")
+        return out
+
+    def set_params(self, p):
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        assert p.size == self.num_params()
+        self.ctx.check(lib().lmm_cnn_set_params(self.h, _fptr(p)), "cnn_set_params")
+
+    def forward(self, x):
+        """x: (nb, nz, ny, nx, cin) -> (nb, nz, ny, nx, cout)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        nb, nz, ny, nx, _ = x.shape
+        y = np.zeros((nb, nz, ny, nx, self.cfg.cout), dtype=np.float32)
+        n = (C.c_int * 3)(nx, ny, nz)
+        self.ctx.check(lib().lmm_cnn_forward_host(self.h, nb, n, _fptr(x), _fptr(y)), "cnn_forward")
+        return y
+
+    def loss_grad(self, x, t):
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        nb, nz, ny, nx, _ = x.shape
+        g = np.zeros(self.num_params(), dtype=np.float32)
+        loss = C.c_double()
+        n = (C.c_int * 3)(nx, ny, nz)
+        self.ctx.check(lib().lmm_cnn_loss_grad_host(self.h, nb, n, _fptr(x), _fptr(t), C.byref(loss), _fptr(g)),
+                       "cnn_loss_grad")
+        return loss.value, g
+
+    def generate_pairs(self, dp, npairs, seed):
+        """Training pairs from the plain V-cycle on a DeviceProblem's coefficients."""
+        self.ctx.check(lib().lmm_cnn_generate_pairs(self.h, _a3(dp.alpha), _h(dp.beta), _a3(dp.beta_ed), _h(dp.gamma),
+                                                    C.c_double(dp.pb.theta_alpha), int(npairs), C.c_uint64(seed)),
+                       "cnn_generate_pairs")
+
+    def num_pairs(self):
+        n = C.c_int64()
+        self.ctx.check(lib().lmm_cnn_num_pairs(self.h, C.byref(n)), "cnn_num_pairs")
+        return n.value
+
+    def train(self, steps, batch, lr, seed=0, held_out=0):
+        hist = np.zeros(steps, dtype=np.float32)
+        self.ctx.check(lib().lmm_cnn_train(self.h, int(steps), int(batch), C.c_double(lr), C.c_uint64(seed),
+                                           C.c_int64(held_out), _fptr(hist)), "cnn_train")
+        return hist
+
+    def evaluate(self, first, last):
+        r = C.c_double()
+        self.ctx.check(lib().lmm_cnn_evaluate(self.h, C.c_int64(first), C.c_int64(last), C.byref(r)), "cnn_evaluate")
+        return r.value
+
+    def attach(self):
+        """Use this network for precision LEARNED solves of its context."""
+        self.ctx.check(lib().lmm_ctx_set_cnn(self.ctx.h, self.h), "ctx_set_cnn")
+
+    def detach(self):
+        self.ctx.check(lib().lmm_ctx_set_cnn(self.ctx.h, None), "ctx_set_cnn")

@@ -129,12 +129,13 @@ int orc_average_cc_to_edge(const orc_geom *g, const double *cc, double *edge[3])
 /* ---- fluctuating fluxes (stochastic_m_fluxdiv.f90) ----
  * W_cc: dim components (cell), W_ed: 2 components per node (2D, 1 node
  * array) or per edge type (3D, 3 edge arrays); component-major, each
- * component nx*ny*nz.  stoch_m_force is incremented. */
+ * component nx*ny*nz.  stoch_m_force is incremented.  filtering_width
+ * 0, 1, 2 or 4 (multifab_filter.f90). */
 int orc_stochastic_m_fluxdiv(const orc_geom *g, const double *W_cc,
                              double *const W_ed[3], const double *eta,
                              double *const eta_ed[3], const double *temp,
                              double *const temp_ed[3], double variance,
-                             double *force[3]);
+                             int filtering_width, double *force[3]);
 
 /* ---- species diffusion (diffusive_mass_fluxdiv.f90, isothermal, no baro) ----
  * rho, molarconc: ns comps; rhotot: 1; rhoWchi, Gama: ns*ns comps (row i,

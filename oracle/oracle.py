@@ -194,6 +194,15 @@ def stag_applyop(g, visc_type, cf, theta_alpha, phi, color=0, Lphi=None):
     return out
 
 
+def diffusive_m_fluxdiv(g, visc_type, m_update, umac, eta, eta_ed, kappa):
+    """diffusive_m_fluxdiv.f90:16-73: m_update - stag_applyop_level(umac) with alpha_fc = 0
+    (multifab_sub_sub_c of :58-61)."""
+    dim = len(umac)
+    cf = coef([np.zeros_like(u) for u in umac], eta, eta_ed, kappa)
+    lphi = stag_applyop(g, visc_type, cf, 1.0, umac)
+    return [np.asarray(m, dtype=np.float64) - l for m, l in zip(m_update[:dim], lphi)]
+
+
 def stag_mg_smooth(g, p, cf, theta_alpha, phi, rhs, nsweeps):
     phi = _copy(phi)
     rhs = _copy(rhs)
@@ -362,7 +371,7 @@ def average_cc_to_edge(g, cc):
     return out
 
 
-def stochastic_m_fluxdiv(g, W_cc, W_ed, eta, eta_ed, temp, temp_ed, variance, force):
+def stochastic_m_fluxdiv(g, W_cc, W_ed, eta, eta_ed, temp, temp_ed, variance, force, filtering_width=0):
     """W_cc: (dim, *shape); W_ed: list of (2, *shape) (1 node array in 2D, 3 edges in 3D)."""
     W_cc = np.ascontiguousarray(W_cc, dtype=np.float64)
     W_ed = _copy(W_ed)
@@ -371,7 +380,8 @@ def stochastic_m_fluxdiv(g, W_cc, W_ed, eta, eta_ed, temp, temp_ed, variance, fo
     eta_ed, temp_ed = _copy(eta_ed), _copy(temp_ed)
     force = _copy(force)
     _check(lib().orc_stochastic_m_fluxdiv(C.byref(g), _p(W_cc), _arr3(W_ed), _p(eta), _arr3(eta_ed),
-                                          _p(temp), _arr3(temp_ed), C.c_double(variance), _arr3(force)),
+                                          _p(temp), _arr3(temp_ed), C.c_double(variance),
+                                          int(filtering_width), _arr3(force)),
            "stochastic_m_fluxdiv")
     return force
 
@@ -403,6 +413,41 @@ def philox4x32_10(ctr, key):
     o = (C.c_uint32 * 4)()
     lib().orc_philox4x32_10(c, k, o)
     return [int(x) for x in o]
+
+
+def fill_m_stochastic(seed, dim, shape, n_rngs, symmetric, philox_step=None):
+    """fill_m_stochastic (stochastic_m_fluxdiv.f90:556-609) on the deduplicated
+    periodic layout, per stage: W_cc (dim components), then each node/edge
+    array.  Non-symmetric: both components of a node/edge array drawn.
+    Symmetric: W_cc drawn and scaled by sqrt(2) (multifab_fill_random with
+    variance 2, multifab_fill_random.f90:58-60), component 1 of each node/edge
+    array drawn and copied to component 2 (multifab_copy_c).  Values come from
+    the host MT19937/Acklam stream (fill_normal), or the counter-based stream
+    of fill_normal_philox(seed, philox_step, ...) when philox_step is given.
+    Returns a list over stages of (W_cc (dim, *shape), [W_ed (2, *shape)])."""
+    N = int(np.prod(shape))
+    ne = 1 if dim == 2 else 3
+    edraw = 1 if symmetric else 2
+    per = (dim + edraw * ne) * N
+    if philox_step is None:
+        vals = fill_normal(seed, per * n_rngs)
+    else:
+        vals = fill_normal_philox(seed, philox_step, 0, per * n_rngs)
+    out = []
+    s2 = np.sqrt(2.0)
+    for r in range(n_rngs):
+        v = vals[r * per:(r + 1) * per]
+        wcc = v[:dim * N].reshape((dim,) + tuple(shape)).copy()
+        if symmetric:
+            wcc = s2 * wcc
+        wed = []
+        for e in range(ne):
+            base = dim * N + e * edraw * N
+            c1 = v[base:base + N].reshape(shape)
+            c2 = c1 if symmetric else v[base + N:base + 2 * N].reshape(shape)
+            wed.append(np.stack([c1, c2]).copy())
+        out.append((wcc, wed))
+    return out
 
 
 def fill_normal_philox(seed, step, v0, count):

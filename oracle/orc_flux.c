@@ -2,8 +2,9 @@
  * Fluctuating and diffusive fluxes, coefficient averaging and the Gaussian
  * generator of the CPU oracle (test infrastructure only).
  *   orc_stochastic_m_fluxdiv  <- src_common/stochastic_m_fluxdiv.f90:41-553
- *        (periodic box: mflux_bc_* and physbc are no-ops, filtering_width 0,
- *         the caller passes the already weighted noise sum of :114-123)
+ *        (periodic box: mflux_bc_* and physbc are no-ops; the caller passes
+ *         the already weighted noise sum of :114-123), with fab_filter
+ *        <- src_common/multifab_filter.f90:13-200
  *   orc_diffusive_mass_fluxdiv <- src_multiSpec/diffusive_mass_fluxdiv.f90:26-262
  *        (isothermal, barodiffusion_type 0) with matvec_mul.f90 and
  *        correction_flux.f90:71-143
@@ -89,9 +90,47 @@ int orc_average_cc_to_edge(const orc_geom *g0, const double *cc, double *edge[3]
 
 /* ---------------- stochastic momentum flux divergence ---------------- */
 
+/* multifab_filter(mfab, width): along x, then y, then z, each pass over the
+ * valid region (nodal hi+1 included) from the current ghosts into a
+ * temporary, copied back, ghosts refilled before the next pass (not after the
+ * last).  Terms in the reference's order. */
+static void fab_filter(ofab *f, int width) {
+  const int dm = f->dim;
+  const int lo0 = 0, hi0 = vhi(f, 0), hi1 = vhi(f, 1), hi2 = (dm == 3) ? vhi(f, 2) : 0;
+  const long cnt = (long)(hi0 + 1) * (hi1 + 1) * (hi2 + 1);
+  double *tmp = (double *)malloc(sizeof(double) * cnt);
+  for (int dir = 0; dir < dm; dir++) {
+    if (dir > 0) fab_fill_boundary(f);
+    const int sx = dir == 0, sy = dir == 1, sz = dir == 2;
+    long q = 0;
+    for (int k = 0; k <= hi2; k++)
+      for (int j = 0; j <= hi1; j++)
+        for (int i = lo0; i <= hi0; i++) {
+#define FP(o) F(f, i + (o) * sx, j + (o) * sy, k + (o) * sz)
+          double v;
+          if (width == 1)
+            v = FP(0) / 2 + FP(-1) / 4 + FP(1) / 4;
+          else if (width == 2)
+            v = 5 * FP(0) / 8 - FP(-2) / 16 + FP(-1) / 4 - FP(2) / 16 + FP(1) / 4;
+          else
+            v = 93 * FP(0) / 128 + 7 * FP(-1) / 32 - 7 * FP(-2) / 64 + FP(-3) / 32 - FP(-4) / 256 +
+                7 * FP(1) / 32 - 7 * FP(2) / 64 + FP(3) / 32 - FP(4) / 256;
+#undef FP
+          tmp[q++] = v;
+        }
+    q = 0;
+    for (int k = 0; k <= hi2; k++)
+      for (int j = 0; j <= hi1; j++)
+        for (int i = lo0; i <= hi0; i++) F(f, i, j, k) = tmp[q++];
+  }
+  free(tmp);
+}
+
 int orc_stochastic_m_fluxdiv(const orc_geom *g0, const double *W_cc, double *const W_ed[3],
                              const double *eta, double *const eta_ed[3], const double *temp,
-                             double *const temp_ed[3], double variance, double *force[3]) {
+                             double *const temp_ed[3], double variance, int width, double *force[3]) {
+  if (!(width == 0 || width == 1 || width == 2 || width == 4)) return 1;
+  const int ngw = width > 1 ? width : 1;
   orc_geom g = *g0;
   int dm = g.dim;
   if (dm == 2) g.n[2] = 1;
@@ -110,11 +149,11 @@ int orc_stochastic_m_fluxdiv(const orc_geom *g0, const double *W_cc, double *con
   }
   /* cell noise: variance * W * sqrt(eta*T) on valid cells, then periodic ghosts */
   for (int d = 0; d < dm; d++) {
-    cell_build(&wc[d], &g, 1);
+    cell_build(&wc[d], &g, ngw);
     fab_from_dedup(&wc[d], W_cc + d * N);
-    for (int k = (dm == 3 ? -1 : 0); k <= (dm == 3 ? g.n[2] : 0); k++)
-      for (int j = -1; j <= g.n[1]; j++)
-        for (int i = -1; i <= g.n[0]; i++) {
+    for (int k = 0; k < (dm == 3 ? g.n[2] : 1); k++)
+      for (int j = 0; j < g.n[1]; j++)
+        for (int i = 0; i < g.n[0]; i++) {
           double v = variance * F(&wc[d], i, j, k);
           F(&wc[d], i, j, k) = v * sqrt(F(&et, i, j, k) * F(&tt, i, j, k));
         }
@@ -124,7 +163,7 @@ int orc_stochastic_m_fluxdiv(const orc_geom *g0, const double *W_cc, double *con
   for (int e = 0; e < ne; e++)
     for (int q = 0; q < 2; q++) {
       ofab *w = &we[e][q];
-      edge_build(w, &g, e, 1);
+      edge_build(w, &g, e, ngw);
       fab_from_dedup(w, W_ed[e] + q * N);
       for (int k = 0; k <= vhi(w, 2); k++)
         for (int j = 0; j <= vhi(w, 1); j++)
@@ -135,6 +174,14 @@ int orc_stochastic_m_fluxdiv(const orc_geom *g0, const double *W_cc, double *con
       fab_internal_sync(w);
       fab_fill_boundary(w);
     }
+  if (width > 0) {  /* stochastic_m_fluxdiv.f90:166-174 (2D), :224-230 (3D) */
+    for (int e = 0; e < ne; e++)
+      for (int q = 0; q < 2; q++) fab_filter(&we[e][q], width);
+    for (int d = 0; d < dm; d++) {
+      fab_filter(&wc[d], width);
+      fab_fill_boundary(&wc[d]);
+    }
+  }
   double dxinv = 1.0 / g.dx;
   for (int d = 0; d < dm; d++) {
     face_build(&fo[d], &g, d, 0);

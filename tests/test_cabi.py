@@ -40,6 +40,8 @@ def test_params_default_matches_reference_probin():
             continue
         assert getattr(p, k) == v, k
     assert p.precision == 0
+    # probin_common.f90:229-230
+    assert p.stoch_stress_form == 1 and p.filtering_width == 0
 
 
 @pytest.mark.parametrize("dim,n,kw,status", [

@@ -219,6 +219,24 @@ def test_fp32_mg_fgmres_accuracy(dim, n):
     assert relerr(gu, ou) < 1e-7
 
 
+@pytest.mark.parametrize("dim,n", [(2, (16, 8)), (3, (8, 4, 8))])
+@pytest.mark.parametrize("visc", VISC)
+def test_diffusive_m_fluxdiv_bit_exact(dim, n, visc):
+    """diffusive_m_fluxdiv.f90:16-73: m_update - (0 - L_eta)umac, bit for bit."""
+    pb = P.small_problem(dim, n, visc_type=visc, seed=13)
+    g = O.geom(dim, n, pb.dx)
+    umac = _rand_faces(dim, pb.shape, 7)
+    m0 = _rand_faces(dim, pb.shape, 8)
+    kappa = 0.3 * pb.beta
+    want = O.diffusive_m_fluxdiv(g, visc, m0, umac, pb.beta, pb.beta_ed, kappa)
+    with S.Context(dim, pb.n, pb.dx, visc_type=visc) as ctx:
+        m = ctx.faces(m0)
+        S.diffusive_m_fluxdiv(ctx, m, ctx.faces(umac), ctx.field(S.CELL, 1, pb.beta), ctx.edges(pb.beta_ed),
+                              ctx.field(S.CELL, 1, kappa))
+        got = [f.download() for f in m]
+    assert bit_equal(got, want)
+
+
 @pytest.mark.parametrize("dim,n", [(2, (32, 16)), (3, (8, 16, 8))])
 def test_stochastic_flux_bit_exact(dim, n):
     pb = P.small_problem(dim, n, seed=47)
@@ -242,7 +260,7 @@ def test_stochastic_flux_bit_exact(dim, n):
 def test_stochastic_m_fluxdiv_module_noise_matches_explicit():
     dim, n = 3, (8, 8, 8)
     shape = (8, 8, 8)
-    with S.Context(dim, n, 0.5) as ctx:
+    with S.Context(dim, n, 0.5, stoch_stress_form=0) as ctx:
         S.init_m_stochastic(ctx, 2)
         S.fill_m_stochastic(ctx, 99)
         ones = np.ones(shape)
@@ -260,6 +278,47 @@ def test_stochastic_m_fluxdiv_module_noise_matches_explicit():
         S.stoch_m_force(ctx, f2, ctx.field(S.CELL, 3, Wcc), ctx.edges(Wed, ncomp=2), eta, ed, eta, ed, variance)
         want = [f.download() for f in f2]
     assert bit_equal(got, want)
+
+
+@pytest.mark.parametrize("dim,n", [(2, (16, 8)), (3, (8, 16, 8))])
+@pytest.mark.parametrize("form", [0, 1])
+@pytest.mark.parametrize("width", [0, 1, 2, 4])
+def test_stochastic_m_fluxdiv_forms_and_filtering_bit_exact(dim, n, form, width):
+    """fill_m_stochastic (host MT19937 stream) in the non-symmetric and the
+    symmetric stress form, two weighted stages, variable eta and T, every
+    filtering width: the force equals the oracle's fill and
+    stochastic_m_fluxdiv bit for bit."""
+    shape = tuple(reversed(n))
+    ne = 1 if dim == 2 else 3
+    rng = np.random.default_rng(17)
+    eta = 1.0 + rng.random(shape)
+    temp = 1.0 + rng.random(shape)
+    eta_ed = [P.avg_cc_to_node_2d(eta)] if dim == 2 else P.avg_cc_to_edge_3d(eta)
+    temp_ed = [P.avg_cc_to_node_2d(temp)] if dim == 2 else P.avg_cc_to_edge_3d(temp)
+    f0 = _rand_faces(dim, shape, 19)
+    w = [0.7, -0.4]
+    dt, dx = 0.05, 0.5
+    variance = np.sqrt(1.3 * 2.0 * 0.8 / (dx ** dim * dt))
+    stages = O.fill_m_stochastic(123, dim, shape, 2, symmetric=bool(form))
+    # saxpy of the stages from zero (stochastic_m_fluxdiv.f90:114-123)
+    Wc = 0.0 + w[0] * stages[0][0]
+    Wc = Wc + w[1] * stages[1][0]
+    We = [(0.0 + w[0] * stages[0][1][e]) + w[1] * stages[1][1][e] for e in range(ne)]
+    g = O.geom(dim, n, dx)
+    want = O.stochastic_m_fluxdiv(g, Wc, We, eta, eta_ed, temp, temp_ed, variance, f0, filtering_width=width)
+    with S.Context(dim, n, dx, stoch_stress_form=form, filtering_width=width) as ctx:
+        S.init_m_stochastic(ctx, 2)
+        S.fill_m_stochastic(ctx, 123)
+        force = ctx.faces(f0)
+        S.stochastic_m_fluxdiv(ctx, force, ctx.field(S.CELL, 1, eta), ctx.edges(eta_ed), ctx.field(S.CELL, 1, temp),
+                               ctx.edges(temp_ed), dt=dt, weights=w, variance_coef_mom=1.3, k_B=0.8)
+        got = [f.download() for f in force]
+    assert bit_equal(got, want)
+
+
+def test_filtering_width_is_checked():
+    with pytest.raises(S.LmmError):
+        S.Context(3, (8, 8, 8), 0.5, filtering_width=3)
 
 
 @pytest.mark.parametrize("dim,n,ns", [(2, (16, 8), 3), (3, (8, 8, 4), 4)])

@@ -11,7 +11,7 @@ from scipy import stats
 
 from lowmachmixtures_amd import stokes as S
 from oracle import oracle as O
-from tests.helpers import bit_equal
+from tests.helpers import bit_equal, relerr
 
 pytestmark = pytest.mark.gpu
 
@@ -48,7 +48,7 @@ def test_fill_m_stochastic_device_layout(dim, n):
     ne = 1 if dim == 2 else 3
     per = (dim + 2 * ne) * N
     ones = np.ones(shape)
-    with S.Context(dim, n, 0.5) as ctx:
+    with S.Context(dim, n, 0.5, stoch_stress_form=0) as ctx:
         S.init_m_stochastic(ctx, 2)
         S.fill_m_stochastic_device(ctx, 5, 11)
         eta = ctx.field(S.CELL, 1, ones)
@@ -68,3 +68,32 @@ def test_fill_m_stochastic_device_layout(dim, n):
             S.stoch_m_force(ctx, f2, wcc, wed, eta, ed, eta, ed, variance)
             want = [f.download() for f in f2]
             assert bit_equal(got, want)
+
+
+@pytest.mark.parametrize("dim,n", [(3, (16, 8, 8)), (2, (32, 16))])
+def test_fill_m_stochastic_device_symmetric_form(dim, n):
+    """Symmetric stress form of the device fill: the force from the context's
+    noise equals the oracle's force on the oracle's symmetric fill of the same
+    counter-based stream (sqrt(2) W_cc, one drawn node/edge component copied
+    to the second); the oracle's stream is pinned on the Random123 vectors."""
+    shape = tuple(reversed(n))
+    ne = 1 if dim == 2 else 3
+    ones = np.ones(shape)
+    stages = O.fill_m_stochastic(5, dim, shape, 2, symmetric=True, philox_step=11)
+    g = O.geom(dim, n, 0.5)
+    variance = np.sqrt(1.0 * 2.0 * 1.0 / (0.5 ** dim * 0.1))
+    with S.Context(dim, n, 0.5, stoch_stress_form=1) as ctx:
+        S.init_m_stochastic(ctx, 2)
+        S.fill_m_stochastic_device(ctx, 5, 11)
+        eta = ctx.field(S.CELL, 1, ones)
+        ed = ctx.edges([ones] * ne)
+        for r in range(2):
+            w = [0.0, 0.0]
+            w[r] = 1.0
+            f1 = ctx.faces([np.zeros(shape)] * dim)
+            S.stochastic_m_fluxdiv(ctx, f1, eta, ed, eta, ed, dt=0.1, weights=w, variance_coef_mom=1.0, k_B=1.0)
+            got = [f.download() for f in f1]
+            want = O.stochastic_m_fluxdiv(g, stages[r][0], stages[r][1], ones, [ones] * ne, ones, [ones] * ne,
+                                          variance, [np.zeros(shape)] * dim)
+            # log / cos / sin of the device may differ from the host in the last bits
+            assert relerr(got, want) < 1e-13

@@ -132,6 +132,36 @@ def test_laplacian_fourier_symbol(dim, n, dx):
     assert relerr(Lu, [lam * a for a in u]) < 1e-13
 
 
+@pytest.mark.parametrize("dim,n,dx", GEOMS)
+@pytest.mark.parametrize("visc", [-1, 1, -2])
+def test_diffusive_m_fluxdiv_adds_the_viscous_laplacian(dim, n, dx, visc):
+    """diffusive_m_fluxdiv.f90:16-73 on a Fourier mode with constant eta:
+    m_update gains +eta*Lap_h(u) = -eta*lambda_h*u (lambda_h the symbol of
+    -Lap_h), whatever alpha the caller's problem has.  For visc_type 2 the
+    mode is a gradient field (u = grad p), on which the full stress acts as
+    2*eta*Lap_h."""
+    g = O.geom(dim, n, dx)
+    c = coefs(dim, n, 4, const=True)
+    b0 = c[1].flat[0]
+    idx = np.meshgrid(*[np.arange(s) for s in shape_of(dim, n)], indexing="ij")
+    kvec = [1, 1, 1][:dim]
+    theta = [2 * np.pi * kvec[d] / n[d] for d in range(dim)]
+    lam = sum(2 - 2 * np.cos(t) for t in theta) / dx ** 2
+    if abs(visc) == 2:
+        phase = sum(theta[d] * idx[dim - 1 - d] for d in range(dim))
+        p = np.cos(phase)
+        u = mac_grad(p, dim, dx)
+        factor = 2.0
+    else:
+        phase = sum(theta[d] * idx[dim - 1 - d] for d in range(dim))
+        u = [np.cos(phase + 0.4 * d) for d in range(dim)]
+        factor = 1.0
+    m0 = rand_faces(dim, n, 9)
+    got = O.diffusive_m_fluxdiv(g, visc, m0, u, c[1], c[2], c[3])
+    want = [m - factor * b0 * lam * a for m, a in zip(m0, u)]
+    assert relerr(got, want) < 1e-12
+
+
 def mac_grad(p, dim, dx):
     return [(p - roll(p, 1, d, dim)) / dx for d in range(dim)]
 
@@ -549,3 +579,113 @@ def test_diffusive_mass_flux_conservation(dim, n, ns):
     da, _ = run(x, False)
     db, _ = run(2.0 * x, False)
     assert relerr(db, 2.0 * da) < 1e-14
+
+
+# ------------------------------------------------- stochastic stress forms
+
+def _filter_gain(width, th):
+    """Fourier gain of multifab_filter's 1-D stencil (multifab_filter.f90:67-81)."""
+    c = np.cos
+    if width == 1:
+        return 0.5 + 0.5 * c(th)
+    if width == 2:
+        return 5 / 8 + 0.5 * c(th) - 0.125 * c(2 * th)
+    return 93 / 128 + 7 / 16 * c(th) - 7 / 32 * c(2 * th) + 1 / 16 * c(3 * th) - 1 / 128 * c(4 * th)
+
+
+@pytest.mark.parametrize("dim,n", [(2, (16, 8)), (3, (8, 16, 8))])
+@pytest.mark.parametrize("width", [1, 2, 4])
+def test_filtered_stochastic_flux_is_the_filtered_divergence(dim, n, width):
+    """With eta = T = 1 the filter and the divergence are both linear and
+    translation invariant, so a noise field made of one Fourier mode comes out
+    as the unfiltered force times the product of the 1-D filter gains of its
+    wavenumbers (a wrong coefficient, sign or offset in any pass breaks it)."""
+    g = O.geom(dim, n, 0.5)
+    shp = shape_of(dim, n)
+    idx = np.meshgrid(*[np.arange(s) for s in shp], indexing="ij")
+    # theta = pi/2 along x and z, pi/4 along y: distinct gains per direction
+    kv = [n[0] // 4, n[1] // 8 if n[1] >= 8 else 1, (n[2] // 4) if dim == 3 else 0][:dim]
+    th = [2 * np.pi * kv[d] / n[d] for d in range(dim)]
+    phase = sum(th[d] * idx[dim - 1 - d] for d in range(dim))
+    Wc = np.stack([np.cos(phase + 0.3 * d) for d in range(dim)])
+    ne = 1 if dim == 2 else 3
+    We = [np.stack([np.sin(phase + 0.2 * e), np.cos(phase - 0.1 * e)]) for e in range(ne)]
+    one = np.ones(shp)
+    ones_ed = [one.copy() for _ in range(ne)]
+    zero = [np.zeros(shp) for _ in range(dim)]
+    f0 = O.stochastic_m_fluxdiv(g, Wc, We, one, ones_ed, one, ones_ed, 0.9, zero)
+    fw = O.stochastic_m_fluxdiv(g, Wc, We, one, ones_ed, one, ones_ed, 0.9, zero, filtering_width=width)
+    gain = np.prod([_filter_gain(width, t) for t in th])
+    assert relerr(fw, [gain * a for a in f0]) < 1e-12
+    assert abs(gain - 1.0) > 0.05  # the mode is really damped
+
+
+@pytest.mark.parametrize("dim,n", [(2, (4, 6)), (3, (4, 4, 4))])
+def test_symmetric_stress_noise_satisfies_fluctuation_dissipation(dim, n):
+    """Fluctuation-dissipation balance of the symmetric stress
+    (stochastic_m_fluxdiv.f90:569-604 with eta = T = 1, variance 1): the
+    covariance of the force D.W_sym, W_sym having variance-2 diagonal entries
+    and one shared draw for both off-diagonal components of a node/edge, is
+    exactly minus the full-stress operator (visc_type 2 with alpha = 0,
+    stag_applyop.f90); the non-symmetric noise gives minus the Laplacian
+    (visc_type 1).  Built by brute force over every noise degree of freedom."""
+    g = O.geom(dim, n, 0.5)
+    shp = shape_of(dim, n)
+    N = int(np.prod(shp))
+    ne = 1 if dim == 2 else 3
+    one = np.ones(shp)
+    ones_ed = [one.copy() for _ in range(ne)]
+    zero = [np.zeros(shp) for _ in range(dim)]
+
+    def force(Wc, We):
+        return flat(O.stochastic_m_fluxdiv(g, Wc, We, one, ones_ed, one, ones_ed, 1.0, zero))
+
+    def covariance(symmetric):
+        cols = []
+        for d in range(dim):
+            for q in range(N):
+                Wc = np.zeros((dim,) + shp)
+                Wc.reshape(dim, -1)[d, q] = np.sqrt(2.0) if symmetric else 1.0
+                cols.append(force(Wc, [np.zeros((2,) + shp) for _ in range(ne)]))
+        for e in range(ne):
+            for comp in ([0] if symmetric else [0, 1]):
+                for q in range(N):
+                    We = [np.zeros((2,) + shp) for _ in range(ne)]
+                    We[e].reshape(2, -1)[comp, q] = 1.0
+                    if symmetric:
+                        We[e].reshape(2, -1)[1, q] = 1.0
+                    cols.append(force(np.zeros((dim,) + shp), We))
+        Fm = np.array(cols).T
+        return Fm @ Fm.T
+
+    def operator(visc):
+        alpha = [np.zeros(shp) for _ in range(dim)]
+        cf = O.coef(alpha, one, ones_ed, np.zeros(shp))
+        return assemble(g, visc, cf, 1.0, dim, n)
+
+    C2 = covariance(True)
+    A2 = operator(2)
+    assert np.abs(C2 - A2).max() < 1e-10 * np.abs(A2).max()
+    C1 = covariance(False)
+    A1 = operator(1)
+    assert np.abs(C1 - A1).max() < 1e-10 * np.abs(A1).max()
+    # the two forms really differ
+    assert np.abs(C2 - C1).max() > 0.1 * np.abs(A1).max()
+
+
+def test_fill_m_stochastic_forms():
+    """Structure of the restated fill: the symmetric form scales W_cc by
+    sqrt(2) of the same draws and consumes one component per node/edge
+    array, copied to the second."""
+    shp = (4, 6, 8)
+    ns = O.fill_m_stochastic(5, 3, shp, 2, symmetric=False)
+    sy = O.fill_m_stochastic(5, 3, shp, 2, symmetric=True)
+    N = int(np.prod(shp))
+    raw = O.fill_normal(5, 2 * 6 * N)
+    assert np.array_equal(ns[0][0].ravel(), raw[:3 * N])
+    assert np.array_equal(sy[0][0].ravel(), np.sqrt(2.0) * raw[:3 * N])
+    for e in range(3):
+        assert np.array_equal(sy[0][1][e][0], sy[0][1][e][1])
+        assert np.array_equal(sy[0][1][e][0].ravel(), raw[(3 + e) * N:(4 + e) * N])
+    # stage 2 of the symmetric form starts after 6 N values
+    assert np.array_equal(sy[1][0].ravel(), np.sqrt(2.0) * raw[6 * N:9 * N])
