@@ -1,0 +1,343 @@
+! Fortran side of the drop-in boundary: iso_c_binding interfaces of the
+! C ABI (include/lmm.h) and wrappers that keep the reference's routine names
+! and argument order for the hot path of BoxLib-Codes/LowMachMixtures:
+!
+!   gmres                  src_gmres/gmres.f90:30-45
+!   apply_matrix           src_gmres/apply_matrix.f90:23-39
+!   apply_precon           src_gmres/apply_precon.f90:29-43
+!   stag_mg_solver         src_gmres/stag_mg_solver.f90:37-50
+!   macproject             src_gmres/macproject.f90:34-45
+!   stochastic_m_fluxdiv   src_common/stochastic_m_fluxdiv.f90:41-51
+!   diffusive_mass_fluxdiv src_multiSpec/diffusive_mass_fluxdiv.f90:26-42
+!   diffusive_m_fluxdiv    src_common/diffusive_m_fluxdiv.f90:16-17
+!
+! A multifab of the reference becomes a device-resident field handle
+! (type(c_ptr)); ml_layout, bc_tower, dx and the probin_* globals become the
+! context handle.  lmm_fab_in / lmm_fab_out move one component of one box
+! between the reference's fab (dataptr(mf, box, comp, 1): lower bound lo-ng,
+! column-major, nodal hi+1 faces and ghosts included) and a field.
+!
+! Errors: the reference aborts through bl_error; every wrapper stops the
+! program with the library's message when the C call returns a nonzero
+! status.  Non-convergence of GMRES is a printed warning, as in gmres.f90.
+module lmm_facade
+
+  use iso_c_binding
+  implicit none
+
+  private
+
+  ! status codes (enum lmm_status)
+  integer(c_int), parameter, public :: LMM_OK = 0, LMM_ERR_ARG = 1, LMM_ERR_UNSUPPORTED = 2, &
+       LMM_ERR_BREAKDOWN = 3, LMM_ERR_DEVICE = 4, LMM_ERR_NOMEM = 5, LMM_NOT_CONVERGED = 6
+  ! field kinds (enum lmm_kind)
+  integer(c_int), parameter, public :: LMM_CELL = 0, LMM_FACE_X = 1, LMM_FACE_Y = 2, LMM_FACE_Z = 3, &
+       LMM_NODE = 4, LMM_EDGE_XY = 5, LMM_EDGE_XZ = 6, LMM_EDGE_YZ = 7
+  ! precision forms (enum lmm_precision)
+  integer(c_int), parameter, public :: LMM_FP64 = 0, LMM_FP32_MG = 1, LMM_LEARNED = 2
+
+  ! lmm_params: the probin_gmres knobs (probin_gmres.f90:119-157) and the
+  ! probin_common ones the path reads
+  type, bind(c), public :: lmm_params
+     integer(c_int) :: visc_type, precon_type
+     real(c_double) :: p_norm_weight, scale_factor
+     integer(c_int) :: mg_max_vcycles, mg_minwidth
+     integer(c_int) :: mg_nsmooths_down, mg_nsmooths_up, mg_nsmooths_bottom
+     real(c_double) :: mg_rel_tol
+     integer(c_int) :: stag_mg_max_vcycles, stag_mg_minwidth
+     integer(c_int) :: stag_mg_nsmooths_down, stag_mg_nsmooths_up, stag_mg_nsmooths_bottom
+     integer(c_int) :: stag_mg_smoother
+     real(c_double) :: stag_mg_omega, stag_mg_rel_tol
+     real(c_double) :: gmres_rel_tol, gmres_abs_tol
+     integer(c_int) :: gmres_max_outer, gmres_max_inner, gmres_max_iter, gmres_min_iter
+     integer(c_int) :: gmres_verbose
+     integer(c_int) :: precision
+     integer(c_int) :: stoch_stress_form, filtering_width
+  end type lmm_params
+
+  type, bind(c), public :: lmm_gmres_info
+     integer(c_int) :: total_iter, outer_iter
+     real(c_double) :: norm_pre_rhs, norm_resid, norm_b, norm_resid_true
+     integer(c_int) :: status, vcycles
+     real(c_double) :: seconds
+     integer(c_int) :: learned_applied, learned_fallback
+  end type lmm_gmres_info
+
+  public :: lmm_params_default, lmm_ctx_create, lmm_ctx_destroy, lmm_ctx_set_params, lmm_ctx_last_error, &
+       lmm_field_create, lmm_field_destroy, lmm_field_upload_fab, lmm_field_download_fab, lmm_fill_normal_host, &
+       lmm_gmres, lmm_apply_matrix, lmm_apply_precon, lmm_stag_mg_solver, lmm_macproject, &
+       lmm_init_m_stochastic, lmm_fill_m_stochastic, lmm_stochastic_m_fluxdiv, lmm_diffusive_mass_fluxdiv, &
+       lmm_diffusive_m_fluxdiv
+  public :: lmm_check, lmm_error_message, lmm_fab_in, lmm_fab_out
+  public :: gmres, apply_matrix, apply_precon, stag_mg_solver, macproject, stochastic_m_fluxdiv, &
+       diffusive_mass_fluxdiv, diffusive_m_fluxdiv
+
+  interface
+     subroutine lmm_params_default(p) bind(c, name='lmm_params_default')
+       import :: lmm_params
+       type(lmm_params), intent(out) :: p
+     end subroutine lmm_params_default
+
+     integer(c_int) function lmm_ctx_create(dim, n_cells, dx, p, ctx) bind(c, name='lmm_ctx_create')
+       import :: c_int, c_double, c_ptr, lmm_params
+       integer(c_int), value :: dim
+       integer(c_int), intent(in) :: n_cells(3)
+       real(c_double), value :: dx
+       type(lmm_params), intent(in) :: p
+       type(c_ptr), intent(out) :: ctx
+     end function lmm_ctx_create
+
+     integer(c_int) function lmm_ctx_destroy(ctx) bind(c, name='lmm_ctx_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+     end function lmm_ctx_destroy
+
+     integer(c_int) function lmm_ctx_set_params(ctx, p) bind(c, name='lmm_ctx_set_params')
+       import :: c_int, c_ptr, lmm_params
+       type(c_ptr), value :: ctx
+       type(lmm_params), intent(in) :: p
+     end function lmm_ctx_set_params
+
+     type(c_ptr) function lmm_ctx_last_error(ctx) bind(c, name='lmm_ctx_last_error')
+       import :: c_ptr
+       type(c_ptr), value :: ctx
+     end function lmm_ctx_last_error
+
+     integer(c_int) function lmm_field_create(ctx, kind, ncomp, f) bind(c, name='lmm_field_create')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: kind, ncomp
+       type(c_ptr), intent(out) :: f
+     end function lmm_field_create
+
+     integer(c_int) function lmm_field_destroy(f) bind(c, name='lmm_field_destroy')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: f
+     end function lmm_field_destroy
+
+     integer(c_int) function lmm_field_upload_fab(f, comp, host, lo, hi, ng) bind(c, name='lmm_field_upload_fab')
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: f
+       integer(c_int), value :: comp, ng
+       real(c_double), intent(in) :: host(*)
+       integer(c_int), intent(in) :: lo(3), hi(3)
+     end function lmm_field_upload_fab
+
+     integer(c_int) function lmm_field_download_fab(f, comp, host, lo, hi, ng) bind(c, name='lmm_field_download_fab')
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: f
+       integer(c_int), value :: comp, ng
+       real(c_double), intent(inout) :: host(*)
+       integer(c_int), intent(in) :: lo(3), hi(3)
+     end function lmm_field_download_fab
+
+     integer(c_int) function lmm_fill_normal_host(seed, count, out) bind(c, name='lmm_fill_normal_host')
+       import :: c_int, c_int32_t, c_int64_t, c_double
+       integer(c_int32_t), value :: seed
+       integer(c_int64_t), value :: count
+       real(c_double), intent(out) :: out(*)
+     end function lmm_fill_normal_host
+
+     integer(c_int) function lmm_gmres(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, theta_alpha, &
+          norm_pre_rhs, info) bind(c, name='lmm_gmres')
+       import :: c_int, c_double, c_ptr, lmm_gmres_info
+       type(c_ptr), value :: ctx, b_p, x_p, beta, gamma
+       type(c_ptr), intent(in) :: b_u(3), x_u(3), alpha_fc(3), beta_ed(3)
+       real(c_double), value :: theta_alpha
+       real(c_double), intent(out) :: norm_pre_rhs
+       type(lmm_gmres_info), intent(out) :: info
+     end function lmm_gmres
+
+     integer(c_int) function lmm_apply_matrix(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, &
+          theta_alpha) bind(c, name='lmm_apply_matrix')
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx, b_p, x_p, beta, gamma
+       type(c_ptr), intent(in) :: b_u(3), x_u(3), alpha_fc(3), beta_ed(3)
+       real(c_double), value :: theta_alpha
+     end function lmm_apply_matrix
+
+     integer(c_int) function lmm_apply_precon(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, &
+          theta_alpha) bind(c, name='lmm_apply_precon')
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx, b_p, x_p, beta, gamma
+       type(c_ptr), intent(in) :: b_u(3), x_u(3), alpha_fc(3), beta_ed(3)
+       real(c_double), value :: theta_alpha
+     end function lmm_apply_precon
+
+     integer(c_int) function lmm_stag_mg_solver(ctx, alpha_fc, beta_cc, beta_ed, gamma_cc, theta_alpha, phi_fc, &
+          rhs_fc) bind(c, name='lmm_stag_mg_solver')
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx, beta_cc, gamma_cc
+       type(c_ptr), intent(in) :: alpha_fc(3), beta_ed(3), phi_fc(3), rhs_fc(3)
+       real(c_double), value :: theta_alpha
+     end function lmm_stag_mg_solver
+
+     integer(c_int) function lmm_macproject(ctx, phi, umac, alphainv_fc, mac_rhs) bind(c, name='lmm_macproject')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, phi, mac_rhs
+       type(c_ptr), intent(in) :: umac(3), alphainv_fc(3)
+     end function lmm_macproject
+
+     integer(c_int) function lmm_init_m_stochastic(ctx, n_rngs) bind(c, name='lmm_init_m_stochastic')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n_rngs
+     end function lmm_init_m_stochastic
+
+     integer(c_int) function lmm_fill_m_stochastic(ctx, seed) bind(c, name='lmm_fill_m_stochastic')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: seed
+     end function lmm_fill_m_stochastic
+
+     integer(c_int) function lmm_stochastic_m_fluxdiv(ctx, stoch_m_force, eta, eta_ed, temperature, &
+          temperature_ed, dt, weights, variance_coef_mom, k_b) bind(c, name='lmm_stochastic_m_fluxdiv')
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx, eta, temperature
+       type(c_ptr), intent(in) :: stoch_m_force(3), eta_ed(3), temperature_ed(3)
+       real(c_double), value :: dt, variance_coef_mom, k_b
+       real(c_double), intent(in) :: weights(*)
+     end function lmm_stochastic_m_fluxdiv
+
+     integer(c_int) function lmm_diffusive_mass_fluxdiv(ctx, nspecies, rho, rhotot, molarconc, rhowchi, gama, &
+          diff_fluxdiv, temp, zeta_by_temp, gradp_baro, flux_total, correct_flux) &
+          bind(c, name='lmm_diffusive_mass_fluxdiv')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, rho, rhotot, molarconc, rhowchi, gama, diff_fluxdiv, temp, zeta_by_temp
+       type(c_ptr), value :: gradp_baro
+       type(c_ptr), intent(in) :: flux_total(3)
+       integer(c_int), value :: nspecies, correct_flux
+     end function lmm_diffusive_mass_fluxdiv
+
+     integer(c_int) function lmm_diffusive_m_fluxdiv(ctx, m_update, umac, eta, eta_ed, kappa) &
+          bind(c, name='lmm_diffusive_m_fluxdiv')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, eta, kappa
+       type(c_ptr), intent(in) :: m_update(3), umac(3), eta_ed(3)
+     end function lmm_diffusive_m_fluxdiv
+
+     integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
+       import :: c_size_t, c_ptr
+       type(c_ptr), value :: s
+     end function c_strlen
+  end interface
+
+contains
+
+  ! the context's last error message as a Fortran string
+  function lmm_error_message(ctx) result(msg)
+    type(c_ptr), intent(in) :: ctx
+    character(len=:), allocatable :: msg
+    type(c_ptr) :: p
+    character(kind=c_char), pointer :: chars(:)
+    integer :: i, n
+    p = lmm_ctx_last_error(ctx)
+    if (.not. c_associated(p)) then
+       msg = ''
+       return
+    end if
+    n = int(c_strlen(p))
+    call c_f_pointer(p, chars, [n])
+    allocate(character(len=n) :: msg)
+    do i = 1, n
+       msg(i:i) = chars(i)
+    end do
+  end function lmm_error_message
+
+  ! bl_error semantics: a nonzero status stops the program with the message
+  subroutine lmm_check(ctx, rc, what)
+    type(c_ptr), intent(in) :: ctx
+    integer(c_int), intent(in) :: rc
+    character(len=*), intent(in) :: what
+    if (rc /= LMM_OK) then
+       print '(a,a,i0,a,a)', what, ': status ', rc, ': ', lmm_error_message(ctx)
+       error stop 1
+    end if
+  end subroutine lmm_check
+
+  ! one component of one box of a multifab (dataptr(mf, box, comp, 1)) into a field
+  subroutine lmm_fab_in(ctx, f, comp, fp, lo, hi, ng)
+    type(c_ptr), intent(in) :: ctx, f
+    integer, intent(in) :: comp, lo(3), hi(3), ng
+    real(c_double), intent(in) :: fp(*)
+    call lmm_check(ctx, lmm_field_upload_fab(f, int(comp - 1, c_int), fp, int(lo, c_int), int(hi, c_int), &
+         int(ng, c_int)), 'lmm_fab_in')
+  end subroutine lmm_fab_in
+
+  subroutine lmm_fab_out(ctx, f, comp, fp, lo, hi, ng)
+    type(c_ptr), intent(in) :: ctx, f
+    integer, intent(in) :: comp, lo(3), hi(3), ng
+    real(c_double), intent(inout) :: fp(*)
+    call lmm_check(ctx, lmm_field_download_fab(f, int(comp - 1, c_int), fp, int(lo, c_int), int(hi, c_int), &
+         int(ng, c_int)), 'lmm_fab_out')
+  end subroutine lmm_fab_out
+
+  ! gmres.f90:30-45 (theta_alpha and the coefficients are read only here)
+  subroutine gmres(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, theta_alpha, norm_pre_rhs)
+    type(c_ptr), intent(in) :: ctx, b_u(3), b_p, x_u(3), x_p, alpha_fc(3), beta, beta_ed(3), gamma
+    real(c_double), intent(in) :: theta_alpha
+    real(c_double), intent(out), optional :: norm_pre_rhs
+    type(lmm_gmres_info) :: info
+    real(c_double) :: npr
+    call lmm_check(ctx, lmm_gmres(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, theta_alpha, npr, info), &
+         'gmres')
+    if (info%status /= LMM_OK) print '(a,i0,a)', 'GMRES did not converge in ', info%total_iter, ' iterations'
+    if (present(norm_pre_rhs)) norm_pre_rhs = npr
+  end subroutine gmres
+
+  ! apply_matrix.f90:23-39
+  subroutine apply_matrix(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, theta_alpha)
+    type(c_ptr), intent(in) :: ctx, b_u(3), b_p, x_u(3), x_p, alpha_fc(3), beta, beta_ed(3), gamma
+    real(c_double), intent(in) :: theta_alpha
+    call lmm_check(ctx, lmm_apply_matrix(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, theta_alpha), &
+         'apply_matrix')
+  end subroutine apply_matrix
+
+  ! apply_precon.f90:29-43
+  subroutine apply_precon(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, theta_alpha)
+    type(c_ptr), intent(in) :: ctx, b_u(3), b_p, x_u(3), x_p, alpha_fc(3), beta, beta_ed(3), gamma
+    real(c_double), intent(in) :: theta_alpha
+    call lmm_check(ctx, lmm_apply_precon(ctx, b_u, b_p, x_u, x_p, alpha_fc, beta, beta_ed, gamma, theta_alpha), &
+         'apply_precon')
+  end subroutine apply_precon
+
+  ! stag_mg_solver.f90:37-50
+  subroutine stag_mg_solver(ctx, alpha_fc, beta_cc, beta_ed, gamma_cc, theta_alpha, phi_fc, rhs_fc)
+    type(c_ptr), intent(in) :: ctx, alpha_fc(3), beta_cc, beta_ed(3), gamma_cc, phi_fc(3), rhs_fc(3)
+    real(c_double), intent(in) :: theta_alpha
+    call lmm_check(ctx, lmm_stag_mg_solver(ctx, alpha_fc, beta_cc, beta_ed, gamma_cc, theta_alpha, phi_fc, rhs_fc), &
+         'stag_mg_solver')
+  end subroutine stag_mg_solver
+
+  ! macproject.f90:34-45
+  subroutine macproject(ctx, phi, umac, alphainv_fc, mac_rhs)
+    type(c_ptr), intent(in) :: ctx, phi, umac(3), alphainv_fc(3), mac_rhs
+    call lmm_check(ctx, lmm_macproject(ctx, phi, umac, alphainv_fc, mac_rhs), 'macproject')
+  end subroutine macproject
+
+  ! stochastic_m_fluxdiv.f90:41-51 (dx is the context's; k_B and
+  ! variance_coef_mom are probin_common's)
+  subroutine stochastic_m_fluxdiv(ctx, stoch_m_force, eta, eta_ed, temperature, temperature_ed, dt, weights, &
+       variance_coef_mom, k_b)
+    type(c_ptr), intent(in) :: ctx, stoch_m_force(3), eta, eta_ed(3), temperature, temperature_ed(3)
+    real(c_double), intent(in) :: dt, weights(:), variance_coef_mom, k_b
+    call lmm_check(ctx, lmm_stochastic_m_fluxdiv(ctx, stoch_m_force, eta, eta_ed, temperature, temperature_ed, dt, &
+         weights, variance_coef_mom, k_b), 'stochastic_m_fluxdiv')
+  end subroutine stochastic_m_fluxdiv
+
+  ! diffusive_mass_fluxdiv.f90:26-42 (isothermal, no barodiffusion: Temp,
+  ! zeta_by_Temp and gradp_baro absent)
+  subroutine diffusive_mass_fluxdiv(ctx, nspecies, rho, rhotot, molarconc, rhowchi, gama, diff_fluxdiv, flux_total)
+    type(c_ptr), intent(in) :: ctx, rho, rhotot, molarconc, rhowchi, gama, diff_fluxdiv, flux_total(3)
+    integer, intent(in) :: nspecies
+    call lmm_check(ctx, lmm_diffusive_mass_fluxdiv(ctx, int(nspecies, c_int), rho, rhotot, molarconc, rhowchi, gama, &
+         diff_fluxdiv, c_null_ptr, c_null_ptr, c_null_ptr, flux_total, 1_c_int), 'diffusive_mass_fluxdiv')
+  end subroutine diffusive_mass_fluxdiv
+
+  ! diffusive_m_fluxdiv.f90:16-17
+  subroutine diffusive_m_fluxdiv(ctx, m_update, umac, eta, eta_ed, kappa)
+    type(c_ptr), intent(in) :: ctx, m_update(3), umac(3), eta, eta_ed(3), kappa
+    call lmm_check(ctx, lmm_diffusive_m_fluxdiv(ctx, m_update, umac, eta, eta_ed, kappa), 'diffusive_m_fluxdiv')
+  end subroutine diffusive_m_fluxdiv
+
+end module lmm_facade

@@ -1,0 +1,54 @@
+! Calls the C ABI through the Fortran facade without a GPU: the reference
+! defaults, the argument checks that return before the device is touched,
+! the host Gaussian generator and the sizes of the bind(c) structs.  Prints
+! one "key value" line per check for the pytest driver to compare.
+program facade_check
+  use iso_c_binding
+  use lmm_facade
+  implicit none
+  type(lmm_params) :: p, bad
+  type(lmm_gmres_info) :: info
+  type(c_ptr) :: ctx, b_u(3), x_u(3), a3(3), e3(3)
+  integer(c_int) :: rc
+  real(c_double) :: z(4), npr
+
+  call lmm_params_default(p)
+  print '(a,1x,i0)', 'visc_type', p%visc_type
+  print '(a,1x,i0)', 'precon_type', p%precon_type
+  print '(a,1x,i0)', 'gmres_max_inner', p%gmres_max_inner
+  print '(a,1x,i0)', 'gmres_max_outer', p%gmres_max_outer
+  print '(a,1x,i0)', 'stag_mg_nsmooths_bottom', p%stag_mg_nsmooths_bottom
+  print '(a,1x,es23.16)', 'gmres_rel_tol', p%gmres_rel_tol
+  print '(a,1x,i0)', 'stoch_stress_form', p%stoch_stress_form
+  print '(a,1x,i0)', 'sizeof_params', c_sizeof(p)
+  print '(a,1x,i0)', 'sizeof_info', c_sizeof(info)
+
+  ! dim 4
+  rc = lmm_ctx_create(4_c_int, [8_c_int, 8_c_int, 8_c_int], 1.0_c_double, p, ctx)
+  print '(a,1x,i0)', 'create_dim4', rc
+  ! a level that coarsens to an odd extent
+  rc = lmm_ctx_create(2_c_int, [6_c_int, 8_c_int, 1_c_int], 1.0_c_double, p, ctx)
+  print '(a,1x,i0)', 'create_odd', rc
+  ! visc_type outside +-1, +-2, +-3
+  bad = p
+  bad%visc_type = 4
+  rc = lmm_ctx_create(2_c_int, [8_c_int, 8_c_int, 1_c_int], 1.0_c_double, bad, ctx)
+  print '(a,1x,i0)', 'create_visc4', rc
+  ! filtering width 3
+  bad = p
+  bad%filtering_width = 3
+  rc = lmm_ctx_create(3_c_int, [8_c_int, 8_c_int, 8_c_int], 1.0_c_double, bad, ctx)
+  print '(a,1x,i0)', 'create_filter3', rc
+  ! gmres without a context
+  b_u = c_null_ptr
+  x_u = c_null_ptr
+  a3 = c_null_ptr
+  e3 = c_null_ptr
+  rc = lmm_gmres(c_null_ptr, b_u, c_null_ptr, x_u, c_null_ptr, a3, c_null_ptr, e3, c_null_ptr, 1.0_c_double, &
+       npr, info)
+  print '(a,1x,i0)', 'gmres_null_ctx', rc
+  ! the host Gaussian stream (MT19937 + Acklam)
+  rc = lmm_fill_normal_host(99_c_int32_t, 4_c_int64_t, z)
+  print '(a,1x,i0)', 'fill_rc', rc
+  print '(a,4(1x,es25.17))', 'normals', z
+end program facade_check
