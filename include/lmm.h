@@ -115,6 +115,8 @@ typedef struct {
   int flex_cgs;            /* flexible GMRES: 1 classical Gram-Schmidt (two passes per
                               step over the basis, unnormalised stored vectors),
                               0 modified Gram-Schmidt */
+  int sweep_prefetch;      /* 1: the FP32 fused sweep loads the next plane's coefficients
+                              one step ahead */
 } lmm_kernel_options;
 
 /* ---------------- context ---------------- */
@@ -347,6 +349,7 @@ typedef struct lmm_cnn lmm_cnn;
 typedef struct {
   int cin, c1, c2, c3, cout;  /* 8, 16, 32, 64, 3; c1..c3 multiples of 16, cin of 4 */
   int level;                  /* staggered multigrid level of the correction (default 3) */
+  int post_sweeps;            /* finest-level sweeps after the correction is added (default 1) */
 } lmm_cnn_config;
 void lmm_cnn_config_default(lmm_cnn_config *cfg);
 /* He-initialised network (seeded) on the context's stream; cfg may be NULL. */
@@ -371,12 +374,22 @@ int lmm_cnn_generate_pairs(lmm_cnn *net, lmm_field *const alpha_fc[3], lmm_field
                            uint64_t seed);
 int lmm_cnn_num_pairs(const lmm_cnn *net, int64_t *count);
 /* steps of Adam (lr) on random batches of `batch` pairs, the last held_out
- * pairs left out; loss_history (steps values) may be NULL. */
+ * pairs left out; loss_history (steps values, the loss of the targets divided
+ * by their RMS over the training pairs) may be NULL.  Training runs on those
+ * unit-scale targets; the RMS is folded into the linear output layer after. */
 int lmm_cnn_train(lmm_cnn *net, int steps, int batch, double lr, uint64_t seed, int64_t held_out,
                   float *loss_history);
 /* Mean over pairs [first, last) of |t - y| / |t| (1 = no better than a zero
  * prediction). */
 int lmm_cnn_evaluate(lmm_cnn *net, int64_t first, int64_t last, double *rel_err);
+/* The ablation of the learned stage on fresh samples of the plain V-cycle:
+ * for npairs samples made as lmm_cnn_generate_pairs makes them (phi* = one
+ * V-cycle of white noise, b = A phi*), the mean over samples of
+ * |phi* - phi1| / |phi*| with phi1 the plain V-cycle's answer (err_plain)
+ * and with phi1 corrected by the learned stage (err_learned). */
+int lmm_cnn_error_reduction(lmm_cnn *net, lmm_field *const alpha_fc[3], lmm_field *beta,
+                            lmm_field *const beta_ed[3], lmm_field *gamma, double theta_alpha, int npairs,
+                            uint64_t seed, double *err_plain, double *err_learned);
 /* Network used by precision LMM_LEARNED on this context (NULL detaches). */
 int lmm_ctx_set_cnn(lmm_ctx *ctx, lmm_cnn *net);
 

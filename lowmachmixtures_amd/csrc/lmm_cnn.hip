@@ -357,7 +357,7 @@ __device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
 
 // batch item b of a training step: pair hash(seed, step, b) mod ntrain
 __global__ void k_gather(long nv1, int cin, int nb, const uint16_t *__restrict__ dsx, const float *__restrict__ dst,
-                         long ntrain, uint64_t seed, uint64_t step, uint16_t *x, float *t) {
+                         long ntrain, uint64_t seed, uint64_t step, float tinv, uint16_t *x, float *t) {
   const long q = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (q >= nv1 * nb) return;
   const int b = (int)(q / nv1);
@@ -365,7 +365,26 @@ __global__ void k_gather(long nv1, int cin, int nb, const uint16_t *__restrict__
   const long pair = (long)(splitmix64(seed ^ splitmix64(step * 1024 + b)) % (uint64_t)ntrain);
   for (int c = 0; c < cin; c++) x[q * cin + c] = dsx[(pair * nv1 + v) * cin + c];
   for (int c = 0; c < CNN_OUT_STRIDE; c++)
-    t[q * CNN_OUT_STRIDE + c] = dst[(pair * nv1 + v) * CNN_OUT_STRIDE + c];
+    t[q * CNN_OUT_STRIDE + c] = dst[(pair * nv1 + v) * CNN_OUT_STRIDE + c] * tinv;
+}
+
+// sum of squares of n floats into partial[blk]
+__global__ void __launch_bounds__(256) k_sumsq_f(long n, const float *__restrict__ a, double *partial) {
+  __shared__ double sh[4];
+  double acc = 0.0;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += stride) acc += (double)a[q] * a[q];
+  const double r = block_sum_d(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+// p[q] *= a, m[q] *= a, v[q] *= a*a for q < n (one layer's weights and Adam moments)
+__global__ void k_rescale_layer(long n, float *p, float *m, float *v, float a) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  p[t] *= a;
+  m[t] *= a;
+  v[t] *= a * a;
 }
 
 // |t - y|^2 and |t|^2 of one pair into out[0], out[1] (atomics of a few doubles)
@@ -421,6 +440,7 @@ struct CnnNet {
   float *part = nullptr;
   long part_cap = 0;
   double *lpart = nullptr, *dscal = nullptr;
+  bool trained = false;  // the output layer predicts targets in their own units
   // dataset
   long ds_cap = 0, ds_count = 0;
   int ds_n[3] = {0, 0, 0};
@@ -831,20 +851,48 @@ int cnn_train(CnnNet *n, int steps, int batch, double lr, uint64_t seed, long he
   const long nv1 = (long)g[0] * g[1] * g[2];
   uint16_t *xb = cmalloc<uint16_t>((long)batch * nv1 * n->cfg.cin);
   float *tb = cmalloc<float>((long)batch * nv1 * CNN_OUT_STRIDE);
-  double *lh = cmalloc<double>(steps);
+  double *lh = cmalloc<double>(std::max(steps, 256));  // loss history, first the partial sums
   if (!xb || !tb || !lh) {
     cfree(xb);
     cfree(tb);
     cfree(lh);
     return LMM_ERR_NOMEM;
   }
+  // The network trains on targets divided by their RMS over the training
+  // pairs (unit scale, whatever the units of the error); the output layer is
+  // linear and bias-free, so multiplying its weights by that RMS afterwards
+  // makes it predict the targets themselves.  A network trained before is
+  // brought back to unit scale first.
+  const long nt_val = ntrain * nv1 * CNN_OUT_STRIDE;
+  CLAUNCH(k_sumsq_f, dim3(256), dim3(256), 0, n->s, nt_val, n->dst, lh);
+  CLAUNCH(k_sum_partials, dim3(1), dim3(256), 0, n->s, lh, 256L, 1.0, n->dscal);
+  double ssq = 0.0;
+  hipMemcpyAsync(&ssq, n->dscal, sizeof(double), hipMemcpyDeviceToHost, n->s);
+  if (hipStreamSynchronize(n->s) != hipSuccess) {
+    cfree(xb);
+    cfree(tb);
+    cfree(lh);
+    return LMM_ERR_DEVICE;
+  }
+  const double rms = std::sqrt(ssq / ((double)ntrain * nv1 * n->cfg.cout));
+  const float tscale = (rms > 0.0 && std::isfinite(rms)) ? (float)rms : 1.f;
+  const CnnLayer &lo = n->L[10];
+  const long nlo = 27L * lo.ci * lo.co;
+  if (n->trained)
+    CLAUNCH(k_rescale_layer, dim3(cblk(nlo)), dim3(CTPB), 0, n->s, nlo, n->p + lo.off, n->m + lo.off, n->v + lo.off,
+            1.f / tscale);
+  prep_weights(n);
   int rc = 0;
   for (int st = 0; st < steps && !rc; st++) {
     CLAUNCH(k_gather, dim3(cblk(nv1 * batch)), dim3(CTPB), 0, n->s, nv1, n->cfg.cin, batch, n->dsx, n->dst, ntrain,
-            seed, (uint64_t)st, xb, tb);
+            seed, (uint64_t)st, 1.f / tscale, xb, tb);
     rc = cnn_loss_grad(n, batch, g, xb, tb, lh + st);
     if (!rc) rc = cnn_adam_step(n, lr, st + 1);
   }
+  CLAUNCH(k_rescale_layer, dim3(cblk(nlo)), dim3(CTPB), 0, n->s, nlo, n->p + lo.off, n->m + lo.off, n->v + lo.off,
+          tscale);
+  prep_weights(n);
+  n->trained = true;
   std::vector<double> hl(steps);
   hipMemcpyAsync(hl.data(), lh, sizeof(double) * steps, hipMemcpyDeviceToHost, n->s);
   if (hipStreamSynchronize(n->s) != hipSuccess && !rc) rc = LMM_ERR_DEVICE;

@@ -199,7 +199,12 @@ __global__ void __launch_bounds__(256) k_rb_fused(Grid g, int tz, int ntx, int n
 // read once, at one time; the red points of the 1-point halo ring (and of
 // the two halo planes) use scalar loads of lines that the neighbouring
 // tiles read in the same step.
-template <typename T, typename P, int TX>
+//
+// PF (prefetch): the coefficients of the next interior plane's red step
+// (the thread's pair and its halo-ring point) are loaded one step ahead, so
+// a step's global loads are consumed by the following step instead of
+// stalling the one that issued them.  Same arithmetic, same bits.
+template <typename T, typename P, int TX, bool PF>
 __device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int nty, int tile,
                                             const T *__restrict__ pin, T *__restrict__ pout,
                                             const typename P::Args &args, T (*sp)[512 / TX + 4][TX + 4],
@@ -241,41 +246,64 @@ __device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int 
       }
     }
   };
-  // one red point of plane kk at red-region position (row, rr), scalar loads
-  auto red_point = [&](int kk, int row, int rr) {
+  // red point of plane kk at red-region position (row, rr) with coefficients c
+  auto red_with = [&](int kk, int row, int rr, const Coef &c) {
     const int s = kk & 3, sm = (kk - 1) & 3, spl = (kk + 1) & 3;
     const int hr = row + 1, hc = rr + 1;
-    const Coef c = P::load(g, wrapc(i0 + rr - 1, nx), wrapc(j0 + row - 1, ny), wrapc(kk, nz), args);
     sr[s][row][rr] = P::update(c, sp[s][hr][hc], sp[s][hr][hc + 1], sp[s][hr][hc - 1], sp[s][hr + 1][hc],
                                sp[s][hr - 1][hc], sp[spl][hr][hc], sp[sm][hr][hc], args);
   };
-  // red on plane kk; for interior planes returns the black half of the pair
-  auto red_plane = [&](int kk, Coef &cb) {
-    const int s = kk & 3, sm = (kk - 1) & 3, spl = (kk + 1) & 3;
-    if (kk >= k0 && kk < k0 + tz) {
-      Coef c0, c1;
-      P::load_pair(g, i0 + 2 * pm, pj, kk, args, c0, c1);
+  auto point_coef = [&](int kk, int row, int rr) -> Coef {
+    return P::load(g, wrapc(i0 + rr - 1, nx), wrapc(j0 + row - 1, ny), wrapc(kk, nz), args);
+  };
+  // one red point of plane kk at red-region position (row, rr), scalar loads
+  auto red_point = [&](int kk, int row, int rr) { red_with(kk, row, rr, point_coef(kk, row, rr)); };
+  // halo ring of an interior plane: rows -1 and FY (RX / 2 red points each),
+  // then one end point of each interior row; NHALO <= 256, one per thread
+  constexpr int NHALO = 2 * (RX / 2) + FY;
+  static_assert(NHALO <= 256, "one halo-ring point per thread");
+  auto halo_pos = [&](int kk, int h, int &row2, int &rr2) {
+    if (h < 2 * (RX / 2)) {
+      row2 = (h < RX / 2) ? 0 : RY - 1;
+      const int jj = j0 + row2 - 1;
+      rr2 = 2 * (h % (RX / 2)) + ((jj + kk + 1) & 1);
+    } else {
+      row2 = 1 + (h - 2 * (RX / 2));
+      const int jj = j0 + row2 - 1;
+      rr2 = ((jj + kk + 1) & 1) ? RX - 1 : 0;  // red: i + j + k even, i = i0 + rr - 1
+    }
+  };
+  auto interior = [&](int kk) { return kk >= k0 && kk < k0 + tz; };
+  // the coefficients an interior plane's red step needs: the thread's pair
+  // and, for tid < NHALO, its halo-ring point
+  auto load_red = [&](int kk, Coef &c0, Coef &c1, Coef &ch) {
+    P::load_pair(g, i0 + 2 * pm, pj, kk, args, c0, c1);
+    if (tid < NHALO) {
+      int row2, rr2;
+      halo_pos(kk, tid, row2, rr2);
+      ch = point_coef(kk, row2, rr2);
+    }
+  };
+  // red on plane kk; for interior planes returns the black half of the pair.
+  // PF: c0, c1, ch hold the plane's coefficients, loaded a step earlier.
+  auto red_plane = [&](int kk, Coef &cb, const Coef &c0, const Coef &c1, const Coef &ch) {
+    if (interior(kk)) {
+      Coef x0, x1, xh;
+      if constexpr (PF) {
+        x0 = c0;
+        x1 = c1;
+        xh = ch;
+      } else {
+        load_red(kk, x0, x1, xh);
+      }
       const bool first_red = ((pj + kk) & 1) == 0;  // column 2m is red when j + k is even
-      const Coef cr = first_red ? c0 : c1;
-      cb = first_red ? c1 : c0;
-      const int rr = 2 * pm + (first_red ? 0 : 1) + 1, row = pr + 1;
-      const int hr = row + 1, hc = rr + 1;
-      sr[s][row][rr] = P::update(cr, sp[s][hr][hc], sp[s][hr][hc + 1], sp[s][hr][hc - 1], sp[s][hr + 1][hc],
-                                 sp[s][hr - 1][hc], sp[spl][hr][hc], sp[sm][hr][hc], args);
-      // halo ring: rows -1 and FY (RX / 2 red points each), then one end
-      // point of each interior row
-      for (int h = tid; h < 2 * (RX / 2) + FY; h += 256) {
+      const Coef cr = first_red ? x0 : x1;
+      cb = first_red ? x1 : x0;
+      red_with(kk, pr + 1, 2 * pm + (first_red ? 0 : 1) + 1, cr);
+      if (tid < NHALO) {
         int row2, rr2;
-        if (h < 2 * (RX / 2)) {
-          row2 = (h < RX / 2) ? 0 : RY - 1;
-          const int jj = j0 + row2 - 1;
-          rr2 = 2 * (h % (RX / 2)) + ((jj + kk + 1) & 1);
-        } else {
-          row2 = 1 + (h - 2 * (RX / 2));
-          const int jj = j0 + row2 - 1;
-          rr2 = ((jj + kk + 1) & 1) ? RX - 1 : 0;  // red: i + j + k even, i = i0 + rr - 1
-        }
-        red_point(kk, row2, rr2);
+        halo_pos(kk, tid, row2, rr2);
+        red_with(kk, row2, rr2, xh);
       }
     } else {
       for (int q = tid; q < NRED; q += 256) {
@@ -298,11 +326,18 @@ __device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int 
 
   T pva[NLD] = {}, pvb[NLD] = {};
   Coef cba = {}, cbb = {};
-  auto step = [&](int kk, const T *pv, T *pvn, const Coef &cb_prev, Coef &cb_cur) {
+  // red coefficients of the current plane (a) and the next one (b)
+  Coef ra0 = {}, ra1 = {}, rah = {}, rb0 = {}, rb1 = {}, rbh = {};
+  auto step = [&](int kk, const T *pv, T *pvn, const Coef &cb_prev, Coef &cb_cur, Coef &c0, Coef &c1, Coef &ch,
+                  Coef &n0, Coef &n1, Coef &nh) {
     store_plane(kk + 1, pv);
     __syncthreads();
     if (kk + 2 <= k0 + tz + 1) fetch_plane(kk + 2, pvn);
-    red_plane(kk, cb_cur);
+    red_plane(kk, cb_cur, c0, c1, ch);
+    // the next plane's red coefficients, issued once this plane's are used
+    if constexpr (PF) {
+      if (interior(kk + 1)) load_red(kk + 1, n0, n1, nh);
+    }
     __syncthreads();
     if (kk - 1 >= k0) black_plane(kk - 1, cb_prev);
   };
@@ -313,12 +348,12 @@ __device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int 
   fetch_plane(k0, pva);
   // tz + 2 steps, an even count
   for (int kk = k0 - 1; kk <= k0 + tz; kk += 2) {
-    step(kk, pva, pvb, cba, cbb);
-    step(kk + 1, pvb, pva, cbb, cba);
+    step(kk, pva, pvb, cba, cbb, ra0, ra1, rah, rb0, rb1, rbh);
+    step(kk + 1, pvb, pva, cbb, cba, rb0, rb1, rbh, ra0, ra1, rah);
   }
 }
 
-template <typename T, typename P>
+template <typename T, typename P, bool PF>
 __global__ void __launch_bounds__(256) k_rb_march2(Grid g, int tz, int ntx, int nty, int ntiles,
                                                    const T *__restrict__ pin, T *__restrict__ pout,
                                                    typename P::Args args0, const T *__restrict__ q0,
@@ -329,7 +364,7 @@ __global__ void __launch_bounds__(256) k_rb_march2(Grid g, int tz, int ntx, int 
   __shared__ T sr[4][RY][RX];
   int w = blockIdx.x, tile = w;
   if (ntiles % 8 == 0) tile = (w % 8) * (ntiles / 8) + w / 8;
-  march2_tile<T, P, FX>(g, tz, ntx, nty, tile, pin, pout, args, sp, sr);
+  march2_tile<T, P, FX, PF>(g, tz, ntx, nty, tile, pin, pout, args, sp, sr);
 }
 
 // The three face components of one sweep in one launch (visc_type -1 / 1:
@@ -340,7 +375,7 @@ __global__ void __launch_bounds__(256) k_rb_march2(Grid g, int tz, int ntx, int 
 // separate launches stream those arrays from memory once per component.
 // Each workgroup runs the single-component tile code, so every point is
 // computed exactly as in k_rb_march2.
-template <typename T, typename P0, typename P1, typename P2, int TX>
+template <typename T, typename P0, typename P1, typename P2, int TX, bool PF>
 __global__ void __launch_bounds__(256)
     k_rb_march2x3(Grid g, int tz, int ntx, int nty, int ntiles, const T *__restrict__ pin0, T *__restrict__ pout0,
                   const T *__restrict__ pin1, T *__restrict__ pout1, const T *__restrict__ pin2,
@@ -357,11 +392,11 @@ __global__ void __launch_bounds__(256)
   if (nslots % 8 == 0) slot = (w % 8) * (nslots / 8) + w / 8;
   const int tile = slot / 3, comp = slot - 3 * tile;
   if (comp == 0) {
-    march2_tile<T, P0, TX>(g, tz, ntx, nty, tile, pin0, pout0, P0::bind(a0, q00, q01, q02, q03, q04), sp, sr);
+    march2_tile<T, P0, TX, PF>(g, tz, ntx, nty, tile, pin0, pout0, P0::bind(a0, q00, q01, q02, q03, q04), sp, sr);
   } else if (comp == 1) {
-    march2_tile<T, P1, TX>(g, tz, ntx, nty, tile, pin1, pout1, P1::bind(a1, q10, q11, q12, q13, q14), sp, sr);
+    march2_tile<T, P1, TX, PF>(g, tz, ntx, nty, tile, pin1, pout1, P1::bind(a1, q10, q11, q12, q13, q14), sp, sr);
   } else {
-    march2_tile<T, P2, TX>(g, tz, ntx, nty, tile, pin2, pout2, P2::bind(a2, q20, q21, q22, q23, q24), sp, sr);
+    march2_tile<T, P2, TX, PF>(g, tz, ntx, nty, tile, pin2, pout2, P2::bind(a2, q20, q21, q22, q23, q24), sp, sr);
   }
 }
 
@@ -484,13 +519,16 @@ void launch_rb(hipStream_t s, const Grid &g, int tz, const T *pin, T *pout, cons
   const int ntiles = ntx * nty * ntz;
   if (tuning().rb_variant == 2) {
     LaunchEvents *ev = launch_events();
+    // FP32 prefetches the next plane's coefficients (FP64 would lose occupancy)
+    const bool pf = sizeof(T) == 4 && tuning().rb_prefetch;
+    auto kern = pf ? k_rb_march2<T, P, true> : k_rb_march2<T, P, false>;
     if (ev->start) {
-      hipExtLaunchKernelGGL((k_rb_march2<T, P>), dim3(ntiles), dim3(256), 0, s, ev->start, ev->stop, 0, g, tz, ntx,
-                            nty, ntiles, pin, pout, args, q[0], q[1], q[2], q[3], q[4]);
+      hipExtLaunchKernelGGL(kern, dim3(ntiles), dim3(256), 0, s, ev->start, ev->stop, 0, g, tz, ntx, nty, ntiles, pin,
+                            pout, args, q[0], q[1], q[2], q[3], q[4]);
       ev->start = ev->stop = nullptr;
     } else {
-      hipLaunchKernelGGL((k_rb_march2<T, P>), dim3(ntiles), dim3(256), 0, s, g, tz, ntx, nty, ntiles, pin, pout,
-                         args, q[0], q[1], q[2], q[3], q[4]);
+      hipLaunchKernelGGL(kern, dim3(ntiles), dim3(256), 0, s, g, tz, ntx, nty, ntiles, pin, pout, args, q[0], q[1],
+                         q[2], q[3], q[4]);
     }
     post_launch(s, kMarchName);
     return;
@@ -661,20 +699,22 @@ void launch_stag_three(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz,
   P1::ptrs(a1, q1);
   P2::ptrs(a2, q2);
   LaunchEvents *ev = launch_events();
+  // FP32 prefetches the next plane's coefficients (FP64 would lose occupancy)
+  const bool pf = sizeof(T) == 4 && tuning().rb_prefetch;
   if (tx == FX) {
     const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
     const int ntiles = ntx * nty * ntz;
-    hipExtLaunchKernelGGL((k_rb_march2x3<T, P0, P1, P2, FX>), dim3(3 * ntiles), dim3(256), 0, s, ev->start, ev->stop,
-                          0, g, tz, ntx, nty, ntiles, pin[0], pout[0], pin[1], pout[1], pin[2], pout[2], a0, a1, a2,
-                          q0[0], q0[1], q0[2], q0[3], q0[4], q1[0], q1[1], q1[2], q1[3], q1[4], q2[0], q2[1], q2[2],
-                          q2[3], q2[4]);
+    auto kern = pf ? k_rb_march2x3<T, P0, P1, P2, FX, true> : k_rb_march2x3<T, P0, P1, P2, FX, false>;
+    hipExtLaunchKernelGGL(kern, dim3(3 * ntiles), dim3(256), 0, s, ev->start, ev->stop, 0, g, tz, ntx, nty, ntiles,
+                          pin[0], pout[0], pin[1], pout[1], pin[2], pout[2], a0, a1, a2, q0[0], q0[1], q0[2], q0[3],
+                          q0[4], q1[0], q1[1], q1[2], q1[3], q1[4], q2[0], q2[1], q2[2], q2[3], q2[4]);
   } else {
     const int ntx = g.n[0] / FX2, nty = g.n[1] / FY2, ntz = g.n[2] / tz;
     const int ntiles = ntx * nty * ntz;
-    hipExtLaunchKernelGGL((k_rb_march2x3<T, P0, P1, P2, FX2>), dim3(3 * ntiles), dim3(256), 0, s, ev->start,
-                          ev->stop, 0, g, tz, ntx, nty, ntiles, pin[0], pout[0], pin[1], pout[1], pin[2], pout[2], a0,
-                          a1, a2, q0[0], q0[1], q0[2], q0[3], q0[4], q1[0], q1[1], q1[2], q1[3], q1[4], q2[0], q2[1],
-                          q2[2], q2[3], q2[4]);
+    auto kern = pf ? k_rb_march2x3<T, P0, P1, P2, FX2, true> : k_rb_march2x3<T, P0, P1, P2, FX2, false>;
+    hipExtLaunchKernelGGL(kern, dim3(3 * ntiles), dim3(256), 0, s, ev->start, ev->stop, 0, g, tz, ntx, nty, ntiles,
+                          pin[0], pout[0], pin[1], pout[1], pin[2], pout[2], a0, a1, a2, q0[0], q0[1], q0[2], q0[3],
+                          q0[4], q1[0], q1[1], q1[2], q1[3], q1[4], q2[0], q2[1], q2[2], q2[3], q2[4]);
   }
   ev->start = ev->stop = nullptr;
   post_launch(s, kMarch3Name);

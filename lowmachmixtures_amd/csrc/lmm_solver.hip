@@ -886,6 +886,8 @@ static void learned_stage(lmm_ctx *c, std::vector<SLevel<float>> &S, float *cons
     }
     launch_stag_prolong<float>(s, S[0].g, xu, S[1].phi);
   }
+  // smooth the interpolation error of the correction on the finest level
+  if (nn->cfg.post_sweeps > 0) smooth<float>(c, S[0], xu, bu, nn->cfg.post_sweeps);
   c->learned_applied++;
 }
 
@@ -1272,6 +1274,8 @@ static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
     }
     if (total_iter >= p.gmres_max_iter || iter >= p.gmres_max_outer) break;
     iter++;
+    // b32_fresh: b32 holds V(i-1) in FP32 (written by the pass that made V(i-1))
+    bool b32_fresh = c->fused_krylov;
     if (c->fused_krylov) launch_div_by_f32(s, Vk(0), c->b32, c->tmp, L, c->scal + 3008);
     else launch_div_by(s, Vk(0), c->tmp, L, c->scal + 3008);
     std::fill(hs.s.begin(), hs.s.end(), 0.0);
@@ -1283,12 +1287,12 @@ static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
       total_iter++;
       used = i;
       for (int attempt = 0; attempt < 2; attempt++) {
-        // b32 already holds V(i-1) in FP32 when the step that made V(i-1)
-        // wrote it too (not on a redone step: that step overwrote it)
-        precon32(c, Vk(i - 1), Zk(i - 1), theta_alpha, attempt == 0 && (c->fused_krylov || c->flex_cgs));
+        // a redone step converts V(i-1) again: its first try overwrote b32
+        precon32(c, Vk(i - 1), Zk(i - 1), theta_alpha, attempt == 0 && b32_fresh);
         apply_matrix_z32(c, Zk(i - 1), w);
         if (c->flex_cgs) {
-          cgs_step(c, Vk, i, sig, hs);
+          cgs_step(c, Vk, i, sig, hs);  // writes V(i) and its FP32 copy
+          b32_fresh = true;
         } else {
           double *col = dH + (long)(m + 1) * (i - 1);
           double *sl = c->scal + 3100;
@@ -1296,6 +1300,7 @@ static int gmres_flex(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
           hipMemcpyAsync(col + i, sl, sizeof(double), hipMemcpyDeviceToDevice, s);
           if (c->fused_krylov) launch_div_by_f32(s, Vk(i), c->b32, c->w, L, sl);
           else launch_div_by(s, Vk(i), c->w, L, sl);
+          b32_fresh = c->fused_krylov;
           fetch(c, dH + (long)(m + 1) * (i - 1), i + 1, &hs.h(1, i));
         }
         bool finite = true;
@@ -1384,6 +1389,8 @@ static lmm_kernel_options env_options() {
   if (co && co[0] == '0') o.coarse_one_launch = 0;
   const char *gs = getenv("LMM_FLEX_GS");
   if (gs && strcmp(gs, "mgs") == 0) o.flex_cgs = 0;
+  const char *pf = getenv("LMM_SWEEP_PREFETCH");
+  if (pf && pf[0] == '0') o.sweep_prefetch = 0;
   return o;
 }
 
@@ -1399,6 +1406,7 @@ static void apply_options(lmm_ctx *c, const lmm_kernel_options &o) {
   c->tune.rb_variant = o.fused_kernel;
   c->tune.rb_target = o.fused_target;
   c->tune.am_dz = o.am_brick;
+  c->tune.rb_prefetch = o.sweep_prefetch;
   c->kopt = o;
 }
 
@@ -1589,6 +1597,7 @@ void lmm_kernel_options_default(lmm_kernel_options *o) {
   o->z32_direct = 1;
   o->krylov_fused = 1;
   o->flex_cgs = 1;
+  o->sweep_prefetch = 1;
 }
 
 int lmm_ctx_get_kernel_options(const lmm_ctx *c, lmm_kernel_options *o) {
@@ -2358,6 +2367,7 @@ void lmm_cnn_config_default(lmm_cnn_config *cfg) {
   cfg->c3 = 64;
   cfg->cout = 3;
   cfg->level = 3;
+  cfg->post_sweeps = 1;
 }
 
 int lmm_cnn_create(lmm_ctx *c, const lmm_cnn_config *cfg, uint64_t seed, lmm_cnn **out) {
@@ -2432,6 +2442,7 @@ static int cnn_level_ok(lmm_ctx *c, lmm_cnn *nn) {
   if (c->g.dim != 3) return set_err(c, LMM_ERR_UNSUPPORTED, "the learned correction is 3D only");
   if (f.cin != 8 || f.cout != 3) return set_err(c, LMM_ERR_ARG, "the Stokes correction needs cin 8 and cout 3");
   if (f.level < 1 || f.level >= c->nl_s) return set_err(c, LMM_ERR_ARG, "cnn level %d outside 1..%d", f.level, c->nl_s - 1);
+  if (f.post_sweeps < 0 || f.post_sweeps > 8) return set_err(c, LMM_ERR_ARG, "cnn post_sweeps %d outside 0..8", f.post_sweeps);
   Grid gc = c->g;
   for (int n = 0; n < f.level; n++) gc = coarsen(gc);
   for (int d = 0; d < 3; d++)
@@ -2513,6 +2524,61 @@ int lmm_cnn_generate_pairs(lmm_cnn *nn, lmm_field *const alpha_fc[3], lmm_field 
     launch_cnn_pack_target(s, S[lc].g, S[lc].phi, nn->ss, cnn_dataset_t(nn->net, slot));
     cnn_dataset_push(nn->net);
   }
+  return finish(c);
+}
+
+int lmm_cnn_error_reduction(lmm_cnn *nn, lmm_field *const alpha_fc[3], lmm_field *beta, lmm_field *const beta_ed[3],
+                            lmm_field *gamma, double theta_alpha, int npairs, uint64_t seed, double *err_plain,
+                            double *err_learned) {
+  if (!nn || npairs < 1 || !err_plain || !err_learned) return LMM_ERR_ARG;
+  lmm_ctx *c = nn->ctx;
+  TuningScope ts(&c->tune);
+  int rc;
+  if ((rc = check_coefs(c, alpha_fc, beta, beta_ed, gamma))) return rc;
+  if ((rc = cnn_level_ok(c, nn))) return rc;
+  if ((rc = ensure_lvl32(c))) return rc;
+  const Grid &g = c->g;
+  const long N = g.N;
+  if (!c->gen32) {
+    c->gen32 = dalloc<float>(c->stream, 6 * N);
+    if (!c->gen32) return set_err(c, LMM_ERR_NOMEM, "sample work arrays");
+  }
+  prepare_coefs(c, coef_in(c, alpha_fc, beta, beta_ed, gamma), theta_alpha, 1.0, false, true);
+  hipStream_t s = c->stream;
+  const int visc = c->prm.visc_type;
+  auto &S = c->s32;
+  float *ps[3], *x[3], *b[3];
+  for (int d = 0; d < 3; d++) {
+    ps[d] = c->gen32 + d * N;
+    x[d] = c->gen32 + (3 + d) * N;
+    b[d] = c->b32 + d * N;
+  }
+  lmm_cnn *saved = c->cnn;
+  c->cnn = nn;
+  double sp = 0.0, sl = 0.0;
+  double *slot = c->scal + 4300;
+  for (int q = 0; q < npairs; q++) {
+    launch_fill_normal_philox(s, c->r, 3 * N, 0, seed, (uint64_t)q);
+    launch_convert<float, double>(s, c->b32, c->r, 3 * N);
+    stag_vcycles<float>(c, S, ps, b, true);
+    launch_stag_applyop<float>(s, g, visc, 0, ps, b, S[0].K());
+    stag_vcycles<float>(c, S, x, b, true);
+    // |phi*|^2, |phi* - phi1|^2, then the learned stage on phi1 and |phi* - phi2|^2
+    launch_sumsq3(s, g, ps, c->partial, slot);
+    for (int d = 0; d < 3; d++) launch_sub_f32(s, S[0].phi[d], ps[d], x[d], N);
+    launch_sumsq3(s, g, S[0].phi, c->partial, slot + 1);
+    learned_stage(c, S, b, x);
+    for (int d = 0; d < 3; d++) launch_sub_f32(s, S[0].phi[d], ps[d], x[d], N);
+    launch_sumsq3(s, g, S[0].phi, c->partial, slot + 2);
+    double h[3];
+    fetch(c, slot, 3, h);
+    sp += std::sqrt(h[1] / h[0]);
+    sl += std::sqrt(h[2] / h[0]);
+  }
+  c->cnn = saved;
+  c->learned_applied = 0;
+  *err_plain = sp / npairs;
+  *err_learned = sl / npairs;
   return finish(c);
 }
 

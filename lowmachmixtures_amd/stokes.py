@@ -64,7 +64,7 @@ class KernelOptions(C.Structure):
         ("fused_sweeps", C.c_int), ("fused_min_tiles", C.c_int), ("fused_kernel", C.c_int),
         ("fused_target", C.c_int), ("fused_x3", C.c_int), ("resid_restrict", C.c_int),
         ("coarse_one_launch", C.c_int), ("am_brick", C.c_int), ("z32_direct", C.c_int),
-        ("krylov_fused", C.c_int), ("flex_cgs", C.c_int),
+        ("krylov_fused", C.c_int), ("flex_cgs", C.c_int), ("sweep_prefetch", C.c_int),
     ]
 
     def as_dict(self):
@@ -149,6 +149,8 @@ _SIGS = {
                                          C.POINTER(C.c_float), C.POINTER(C.c_double), C.POINTER(C.c_float)]),
     "lmm_cnn_generate_pairs": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP * 3, _FP, C.c_double, C.c_int, C.c_uint64]),
     "lmm_cnn_num_pairs": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64)]),
+    "lmm_cnn_error_reduction": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP * 3, _FP, C.c_double, C.c_int, C.c_uint64,
+                                          C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "lmm_cnn_train": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_uint64, C.c_int64,
                                 C.POINTER(C.c_float)]),
     "lmm_cnn_evaluate": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_double)]),
@@ -562,7 +564,7 @@ class DeviceProblem:
 class CnnConfig(C.Structure):
     """lmm_cnn_config: channels of the U-Net and the multigrid level it acts on."""
     _fields_ = [("cin", C.c_int), ("c1", C.c_int), ("c2", C.c_int), ("c3", C.c_int), ("cout", C.c_int),
-                ("level", C.c_int)]
+                ("level", C.c_int), ("post_sweeps", C.c_int)]
 
 
 def _fptr(a):
@@ -636,6 +638,15 @@ class Cnn:
         self.ctx.check(lib().lmm_cnn_generate_pairs(self.h, _a3(dp.alpha), _h(dp.beta), _a3(dp.beta_ed), _h(dp.gamma),
                                                     C.c_double(dp.pb.theta_alpha), int(npairs), C.c_uint64(seed)),
                        "cnn_generate_pairs")
+
+    def error_reduction(self, dp, npairs, seed):
+        """Mean |phi* - phi1| / |phi*| after the plain V-cycle and after V-cycle + learned stage."""
+        a, b = C.c_double(), C.c_double()
+        self.ctx.check(lib().lmm_cnn_error_reduction(self.h, _a3(dp.alpha), _h(dp.beta), _a3(dp.beta_ed),
+                                                     _h(dp.gamma), C.c_double(dp.pb.theta_alpha), int(npairs),
+                                                     C.c_uint64(seed), C.byref(a), C.byref(b)),
+                       "cnn_error_reduction")
+        return a.value, b.value
 
     def num_pairs(self):
         n = C.c_int64()

@@ -48,7 +48,9 @@ def test_forward_matches_oracle(nb, n):
 
 
 def test_gradient_matches_oracle():
-    nb, n = 2, (8, 8, 8)
+    # 16^3 keeps at least 128 voxels on the coarsest level, so that a ReLU
+    # mask flipped by a BF16 rounding difference moves no layer's sum by much
+    nb, n = 2, (16, 16, 16)
     x, t = _inputs(nb, n, 2)
     with S.Context(3, (16, 16, 16), 1.0) as ctx, S.Cnn(ctx, seed=4) as net:
         p = net.get_params().astype(np.float64)
@@ -58,12 +60,13 @@ def test_gradient_matches_oracle():
     assert abs(loss - wl) <= 1e-2 * wl
     assert _rel(g, wg) < 2e-2
     # every layer, including the first (input) and last (linear) ones
-    q = 0
-    for li, (ci, co) in enumerate(SHAPES):
+    q, errs = 0, []
+    for ci, co in SHAPES:
         m = 27 * ci * co
-        e = _rel(g[q:q + m], wg[q:q + m])
-        assert e < 5e-2, (li, e)
+        errs.append(_rel(g[q:q + m], wg[q:q + m]))
         q += m
+    print("per-layer grad rel err " + " ".join(f"{e:.1e}" for e in errs), flush=True)
+    assert max(errs) < 5e-2
 
 
 def test_set_params_round_trip():
@@ -90,10 +93,12 @@ def test_training_lowers_loss_and_predicts_held_out_errors():
         total = net.num_pairs()
         assert total == 192
         before = net.evaluate(total - held, total)
-        hist = net.train(steps=300, batch=16, lr=2e-3, seed=7, held_out=held)
+        hist = net.train(steps=300, batch=16, lr=1e-3, seed=7, held_out=held)
         after = net.evaluate(total - held, total)
+        seen = net.evaluate(0, held)
     first, last = float(np.mean(hist[:20])), float(np.mean(hist[-20:]))
-    print(f"loss {first:.4e} -> {last:.4e}; held-out rel err {before:.3f} -> {after:.3f}", flush=True)
+    print(f"loss {first:.4e} -> {last:.4e}; held-out rel err {before:.3f} -> {after:.3f}; "
+          f"training pairs {seen:.3f}", flush=True)
     assert np.all(np.isfinite(hist))
     assert last < 0.7 * first
     assert after < min(before, 0.95)
@@ -117,7 +122,7 @@ def test_learned_fgmres_reaches_the_same_tolerance():
     with S.Context(3, n, pb.dx, visc_type=-1, precision=S.FP32_MG) as ctx, S.Cnn(ctx, seed=8, level=2) as net:
         dp = S.DeviceProblem(ctx, pb, rhs=b_u)
         net.generate_pairs(dp, 128, seed=200)
-        net.train(steps=300, batch=16, lr=2e-3, seed=9)
+        net.train(steps=300, batch=16, lr=1e-3, seed=9)
         info_mg, res_mg, (u_mg, p_mg) = _solve(ctx, pb, b_u)
         net.attach()
         ctx.set_params(precision=S.LEARNED)
