@@ -1390,7 +1390,7 @@ static lmm_kernel_options env_options() {
   const char *gs = getenv("LMM_FLEX_GS");
   if (gs && strcmp(gs, "mgs") == 0) o.flex_cgs = 0;
   const char *pf = getenv("LMM_SWEEP_PREFETCH");
-  if (pf && pf[0] == '0') o.sweep_prefetch = 0;
+  if (pf && pf[0] == '1') o.sweep_prefetch = 1;
   return o;
 }
 
@@ -1597,7 +1597,7 @@ void lmm_kernel_options_default(lmm_kernel_options *o) {
   o->z32_direct = 1;
   o->krylov_fused = 1;
   o->flex_cgs = 1;
-  o->sweep_prefetch = 1;
+  o->sweep_prefetch = 0;
 }
 
 int lmm_ctx_get_kernel_options(const lmm_ctx *c, lmm_kernel_options *o) {

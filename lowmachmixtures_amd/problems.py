@@ -205,6 +205,33 @@ def config3(n: int = 256, seed: int = 1, contrast: float = 100.0) -> StokesProbl
     )
 
 
+def config4_member(n: int, seed: int, contrast: float = 10.0) -> StokesProblem:
+    """One member of the batched 2D ensemble: a periodic n x n box, dx = 1.
+
+    eta = exp(sigma Z), sigma = ln(contrast)/4 (contrast 10), one periodic
+    3x3 box filter, eta on nodes by 4-point averaging; theta*rho = 1;
+    gamma = 0; RHS = divergence of the member's own white-noise stress;
+    reference-default solver settings (one V-cycle of each multigrid),
+    tol 1e-9, FP32 V-cycles inside FP64 flexible GMRES.
+    """
+    shape = (n, n)
+    z = normals(seed, n * n).reshape(shape)
+    eta = box_filter(np.exp(z * (math.log(contrast) / 4.0)), 1)
+    W_cc, W_ed = _noise(seed + 1, 2, shape)
+    return StokesProblem(
+        dim=2, n=(n, n, 1), dx=1.0, visc_type=-1, theta_alpha=1.0,
+        alpha=[np.ones(shape), np.ones(shape)], beta=eta, beta_ed=[avg_cc_to_node_2d(eta)],
+        gamma=np.zeros(shape), W_cc=W_cc, W_ed=W_ed, b_p=np.zeros(shape),
+        solver=dict(visc_type=-1, gmres_rel_tol=1e-9, precision=1),
+        name=f"2d_periodic_{n}x{n}_contrast{contrast:g}_seed{seed}",
+    )
+
+
+def config4(per_size: int = 32, sizes=(64, 128, 256), seed: int = 1):
+    """The ragged ensemble: per_size members of each size, each with its own seeds."""
+    return [config4_member(n, seed + 1000 * k + 2 * m) for k, n in enumerate(sizes) for m in range(per_size)]
+
+
 def rhs_from_noise_host(p: StokesProblem):
     """b_u = div(W) on faces with eta = T = 1, variance 1 (stoch_m_force_*), numpy.
 

@@ -162,6 +162,9 @@ void orc_fill_normal(uint32_t seed, int64_t count, double *out);
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 void orc_fill_normal_philox(uint64_t seed, uint64_t step, int64_t v0, int64_t count, double *out);
 
+/* OpenMP threads of the oracle's parallel loops */
+int orc_max_threads(void);
+
 #ifdef __cplusplus
 }
 #endif

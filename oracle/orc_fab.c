@@ -180,3 +180,7 @@ void ocoef_free(ocoef *c, const orc_geom *g) {
   int ne = (g->dim == 2) ? 1 : 3;
   for (int e = 0; e < ne; e++) fab_free(&c->bed[e]);
 }
+
+/* OpenMP threads the oracle's parallel loops use (the CPU baseline reports it) */
+#include <omp.h>
+int orc_max_threads(void) { return omp_get_max_threads(); }
