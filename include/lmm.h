@@ -393,6 +393,20 @@ int lmm_cnn_error_reduction(lmm_cnn *net, lmm_field *const alpha_fc[3], lmm_fiel
 /* Network used by precision LMM_LEARNED on this context (NULL detaches). */
 int lmm_ctx_set_cnn(lmm_ctx *ctx, lmm_cnn *net);
 
+/* No counterpart in the reference.  lmm_diffusive_mass_fluxdiv for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc): same arguments and
+ * results, bit for bit, but the face fluxes go to a file-scope device array
+ * of the library instead of a per-call allocation, so the call makes no
+ * hipMalloc / hipFree.  Limit: dim * nspecies * cells <= 4194304 (32 MiB of
+ * FP64), else LMM_ERR_ARG.  Every call on one GPU shares that GPU's array:
+ * calls that overlap on one GPU (two contexts' streams, two threads) race. */
+int lmm_diffusive_mass_fluxdiv_noalloc(lmm_ctx *ctx, int nspecies, lmm_field *rho, lmm_field *rhotot,
+                                       lmm_field *molarconc, lmm_field *rhoWchi, lmm_field *Gama,
+                                       lmm_field *diff_fluxdiv, lmm_field *Temp, lmm_field *zeta_by_Temp,
+                                       lmm_field *const gradp_baro[3], lmm_field *flux_total[3],
+                                       int correct_flux);
+
 /* Host Gaussian generator of the noise fields (RNGs.c:247 genrandn stream:
  * MT19937 init_genrand(seed) + Acklam inverse normal). */
 int lmm_fill_normal_host(uint32_t seed, int64_t count, double *out);

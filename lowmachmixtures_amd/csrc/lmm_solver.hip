@@ -2357,6 +2357,71 @@ int lmm_diffusive_mass_fluxdiv(lmm_ctx *c, int ns, lmm_field *rho, lmm_field *rh
   return rc;
 }
 
+// ---------------- allocation-free species diffusion ----------------
+
+}  // extern "C"
+
+// Device scratch of lmm_diffusive_mass_fluxdiv_noalloc: the face fluxes of
+// every species (dim*nspecies*N doubles), at most NOALLOC_MAX doubles
+// (32 MiB).  Each GPU holds its own copy until the process ends; host code
+// uses only the address hipGetSymbolAddress returns on that device.
+static constexpr long NOALLOC_MAX = 1L << 22;
+static __device__ __align__(256) double scratch[NOALLOC_MAX];
+static std::mutex g_scratch_mu;
+static std::map<int, double *> g_scratch_ptr;  // device -> address of its scratch
+
+// The scratch address on the current device, looked up once per device.
+static int scratch_address(lmm_ctx *c, double **out) {
+  int dev = 0;
+  HIPCHK(c, hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) {
+    *out = it->second;
+    return LMM_OK;
+  }
+  double *p = nullptr;
+  HIPCHK(c, hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+  g_scratch_ptr[dev] = p;
+  *out = p;
+  return LMM_OK;
+}
+
+extern "C" {
+
+int lmm_diffusive_mass_fluxdiv_noalloc(lmm_ctx *c, int ns, lmm_field *rho, lmm_field *rhotot, lmm_field *molarconc,
+                                       lmm_field *rhoWchi, lmm_field *Gama, lmm_field *diff_fluxdiv, lmm_field *Temp,
+                                       lmm_field *zeta_by_Temp, lmm_field *const gradp_baro[3],
+                                       lmm_field *flux_total[3], int correct_flux) {
+  if (!c) return LMM_ERR_ARG;
+  if (ns < 1 || ns > 8) return set_err(c, LMM_ERR_UNSUPPORTED, "nspecies %d outside 1..8", ns);
+  if (Temp || zeta_by_Temp || (gradp_baro && gradp_baro[0]))
+    return set_err(c, LMM_ERR_UNSUPPORTED, "non-isothermal and barodiffusion terms are not implemented");
+  (void)rho;
+  (void)rhotot;
+  if (!field_ok(c, molarconc, LMM_CELL, ns) || !field_ok(c, rhoWchi, LMM_CELL, ns * ns) ||
+      !field_ok(c, Gama, LMM_CELL, ns * ns) || !field_ok(c, diff_fluxdiv, LMM_CELL, ns))
+    return set_err(c, LMM_ERR_ARG, "diffusive_mass_fluxdiv: cell fields have wrong components");
+  const int dim = c->g.dim;
+  const long need = (long)dim * ns * c->g.N;
+  if (need > NOALLOC_MAX)
+    return set_err(c, LMM_ERR_ARG,
+                   "diffusive_mass_fluxdiv_noalloc: dim*nspecies*cells = %ld exceeds the %ld-value device scratch",
+                   need, NOALLOC_MAX);
+  double *ft[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < dim; d++) {
+    if (!field_ok(c, flux_total[d], LMM_FACE_X + d, ns)) return set_err(c, LMM_ERR_ARG, "flux_total[%d] wrong", d);
+    ft[d] = flux_total[d]->d;
+  }
+  double *flux = nullptr;
+  int rc = scratch_address(c, &flux);
+  if (rc) return rc;
+  // every flux value is written before the divergence reads it: nothing to clear
+  launch_diffusive_mass_flux(c->stream, c->g, ns, molarconc->d, rhoWchi->d, Gama->d, flux, ft, correct_flux);
+  launch_div_species(c->stream, c->g, ns, flux, diff_fluxdiv->d);
+  return finish(c);
+}
+
 // ---------------- learned coarse-grid correction ----------------
 
 void lmm_cnn_config_default(lmm_cnn_config *cfg) {

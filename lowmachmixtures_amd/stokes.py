@@ -136,6 +136,8 @@ _SIGS = {
     "lmm_stoch_m_force": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, _FP, _FP * 3, C.c_double]),
     "lmm_diffusive_mass_fluxdiv": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP * 3,
                                              _FP * 3, C.c_int]),
+    "lmm_diffusive_mass_fluxdiv_noalloc": (C.c_int, [C.c_void_p, C.c_int, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP,
+                                                     _FP * 3, _FP * 3, C.c_int]),
     "lmm_fill_normal_host": (C.c_int, [C.c_uint32, C.c_int64, C.POINTER(C.c_double)]),
     "lmm_cnn_config_default": (None, [C.c_void_p]),
     "lmm_cnn_create": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint64, C.POINTER(C.c_void_p)]),
