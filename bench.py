@@ -8,11 +8,12 @@ stress, reference-default solver: projection preconditioner, one V-cycle of
 each multigrid, 2/2/8 sweeps, 6-colour Gauss-Seidel, restart 5, tol 1e-9).
 
 One step = one complete GMRES solve from a zero initial guess with every
-input already resident in HBM.  Two forms are timed on the same device
-problem: plain FP64 (the reference algorithm) and FP32 V-cycles inside an
-FP64 flexible GMRES.  The JSON line's ``value`` is the FP32-MG solve time;
-the plain FP64 time, iteration counts, the true relative residual of each
-and the FP32-MG error against the FP64 solution sit beside it.
+input already resident in HBM.  Three forms are timed on the same device
+problem: plain FP64 (the reference algorithm), FP32 V-cycles inside an FP64
+flexible GMRES, and the same with the learned correction stage.  The JSON
+line's ``value`` is the faster mixed-precision form's solve time; every
+form's time, iteration count, true relative residual and error against the
+FP64 solution sit beside it.
 
 ``roofline`` is for the dominant kernel, the finest-level fused red-black
 sweep (k_rb_march2x3: the three face components in one launch) in the
@@ -28,11 +29,28 @@ workload (FETCH_SIZE scaled by 2, the factor that makes it read the true byte
 count of a plain conversion kernel); it is read from the committed counter
 summary and is null when that file is absent.
 
+The learned form (FP32-MG plus the learned coarse-grid correction stage) is
+trained before the timed region: training pairs from the plain FP32
+preconditioner (samples made of its slowest error modes) on 128^3 problems
+of the same family with contrasts 10, 100 and 1000 (the network acts on
+multigrid level 3, 16^3 there), Adam on the
+device, then the weights move to a network on the 256^3 context's level 3
+(32^3; the same spacing, 8 cells).  Its pair count, steps, losses, held-out
+error and the ablation (the preconditioner's error on fresh 256^3 samples
+with and without the stage) are reported beside its time.  ``value`` is the
+faster of the FP32-MG and learned forms.
+
+``ensemble`` times the ragged batched 2D workload: 32 members each of 64^2,
+128^2 and 256^2 (contrast 10, FP32 V-cycles in FP64 flexible GMRES), one
+lmm_gmres_batched call, in solves per second.
+
 ``cpu_baseline`` is the CPU oracle (the FP64 restatement of the reference
-algorithm, OpenMP) solving the same problem family at a bounded size (128^3
-by default, all host threads it is given) to the same tolerance; its time is
-scaled by the cell ratio to a per-256^3-solve figure.  The FP64 device path
-is checked against that oracle solve at the same size.
+algorithm, OpenMP over z-planes) solving the same problem family at 128^3
+(all host threads it is given; the median of three runs) to the same
+tolerance; its time is scaled by the cell ratio to a per-256^3-solve figure,
+because three 256^3 oracle solves (about 35 s each) would exceed the bounded
+CPU sample.  The FP64 device path is checked against that oracle solve at
+the same size.
 """
 from __future__ import annotations
 
@@ -60,7 +78,11 @@ def parse():
     ap.add_argument("--n", type=int, default=256, help="cells per direction")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--cpu-n", type=int, default=128, help="size of the CPU oracle sample (0: skip)")
-    ap.add_argument("--forms", default="fp64,fp32_mg")
+    ap.add_argument("--forms", default="fp64,fp32_mg,learned")
+    ap.add_argument("--pairs", type=int, default=600, help="training pairs per contrast for the learned form")
+    ap.add_argument("--train-steps", type=int, default=1500)
+    ap.add_argument("--ensemble", type=int, default=32, help="members per size of the batched 2D ensemble (0: skip)")
+    ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -111,8 +133,43 @@ def measured_traffic(form, n):
     return None
 
 
+FAMILIES = ("stag_smooth_l0", "stag_smooth", "stag_restrict_l0", "stag_restrict", "stag_prolong", "stag_coarse",
+            "cc_smooth_l0", "cc_smooth", "cc_restrict_l0", "cc_coarse", "apply_matrix", "krylov", "learned")
+
+
+def train_learned(P, S, a, quiet):
+    """Train the correction network on 128^3 problems of the family; returns (params, stats)."""
+    stats = {"train_grid": "128^3", "level": 3, "contrasts": [10, 100, 1000], "pairs_per_contrast": a.pairs,
+             "steps": a.train_steps, "batch": 8, "lr": 5e-4, "sample_steps": 3}
+    held = 32
+    t0 = time.perf_counter()
+    with S.Context(3, (128, 128, 128), 1.0, visc_type=-1, precision=S.FP32_MG) as tctx, \
+            S.Cnn(tctx, seed=a.seed, level=3) as net:
+        for k, contrast in enumerate((10.0, 100.0, 1000.0)):
+            pb = P.config3(128, seed=a.seed + 10 * (k + 1), contrast=contrast)
+            dp = S.DeviceProblem(tctx, pb, rhs=[np.zeros(pb.shape)] * 3)
+            net.generate_pairs(dp, a.pairs, seed=a.seed + 100 + k)
+        tctx.synchronize()
+        stats["generate_s"] = round(time.perf_counter() - t0, 2)
+        total = net.num_pairs()
+        stats["pairs"] = total
+        stats["held_out"] = held
+        t1 = time.perf_counter()
+        hist = net.train(steps=a.train_steps, batch=8, lr=5e-4, seed=a.seed + 7, held_out=held)
+        tctx.synchronize()
+        stats["train_s"] = round(time.perf_counter() - t1, 2)
+        stats["loss_first"] = float(np.mean(hist[:20]))
+        stats["loss_last"] = float(np.mean(hist[-20:]))
+        stats["held_out_rel_err"] = round(net.evaluate(total - held, total), 4)
+        params = net.get_params()
+    log(f"learned: {stats['pairs']} pairs in {stats['generate_s']} s, {a.train_steps} steps in {stats['train_s']} s, "
+        f"loss {stats['loss_first']:.3e} -> {stats['loss_last']:.3e}, held-out rel err {stats['held_out_rel_err']}",
+        quiet)
+    return params, stats
+
+
 def time_form(S, ctx, dp, form, steps, warmup, quiet):
-    prec = {"fp64": S.FP64, "fp32_mg": S.FP32_MG}[form]
+    prec = {"fp64": S.FP64, "fp32_mg": S.FP32_MG, "learned": S.LEARNED}[form]
     ctx.set_params(precision=prec)
     info = None
     for w in range(warmup):
@@ -143,25 +200,69 @@ def time_form(S, ctx, dp, form, steps, warmup, quiet):
     dp.solve()
     ctx.synchronize()
     ctx.profiling(False)
-    fams = {k: ctx.kernel_stats(k) for k in ("stag_smooth_l0", "stag_smooth", "stag_restrict_l0", "stag_restrict",
-                                              "stag_prolong",
-                                              "stag_coarse", "cc_smooth_l0", "cc_smooth", "cc_restrict_l0", "cc_coarse",
-                                              "apply_matrix", "krylov")}
+    fams = {k: ctx.kernel_stats(k) for k in FAMILIES}
     ms = (t1 - t0) * 1e3 / steps
     log(f"{form}: {ms:.2f} ms/solve, iters {iters}, status {info['status']}", quiet)
     return ms, info, iters, sm, fams
 
 
-def cpu_baseline(P, O, helpers, n, seed, quiet):
-    """Oracle FP64 solve of the same problem family at n^3; returns (seconds, info, problem, solution)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(P, O, helpers, n, seed, runs, quiet):
+    """Oracle FP64 solves of the same problem family at n^3; returns (median seconds, all, info, problem, solution)."""
     pb = P.config3(n, seed)
     b_u = P.rhs_from_noise_host(pb)
     g, p, cf = helpers.oracle_args(pb)
-    t0 = time.perf_counter()
-    xu, xp, info = O.gmres(g, p, cf, pb.theta_alpha, b_u, pb.b_p)
-    sec = time.perf_counter() - t0
-    log(f"cpu oracle {n}^3: {sec:.2f} s, iters {info['total_iter']}", quiet)
-    return sec, info, pb, b_u, (xu, xp)
+    secs = []
+    for r in range(runs):
+        t0 = time.perf_counter()
+        xu, xp, info = O.gmres(g, p, cf, pb.theta_alpha, b_u, pb.b_p)
+        secs.append(time.perf_counter() - t0)
+        log(f"cpu oracle {n}^3 run {r}: {secs[-1]:.2f} s, iters {info['total_iter']}", quiet)
+    return float(np.median(secs)), secs, info, pb, b_u, (xu, xp)
+
+
+def ensemble(P, S, per_size, steps, quiet):
+    """Batched ragged 2D ensemble: solves per second over `steps` batches after one warm-up batch."""
+    pbs = P.config4(per_size=per_size)
+    members = []
+    try:
+        for pb in pbs:
+            ctx = S.Context(2, pb.n, pb.dx, **pb.solver)
+            members.append((ctx, S.DeviceProblem(ctx, pb, rhs=P.rhs_from_noise_host(pb))))
+        dps = [dp for _, dp in members]
+        S.gmres_batched(dps)
+        times, iters = [], []
+        for _ in range(steps):
+            for dp in dps:
+                dp.zero_solution()
+            for ctx, _ in members:
+                ctx.synchronize()
+            t0 = time.perf_counter()
+            res = S.gmres_batched(dps)
+            times.append(time.perf_counter() - t0)
+            iters = [info["total_iter"] for _, _, info in res]
+            assert all(st == 0 and info["status"] == 0 for st, _, info in res)
+        sec = float(np.median(times))
+        out = {"members": len(pbs), "sizes": "32x64^2 + 32x128^2 + 32x256^2" if per_size == 32 else
+               f"{per_size}x64^2 + {per_size}x128^2 + {per_size}x256^2", "batches": steps,
+               "s_per_batch": round(sec, 4), "solves_per_s": round(len(pbs) / sec, 1),
+               "gmres_iters_mean": round(float(np.mean(iters)), 2), "form": "fp32_mg",
+               "mechanism": "one context and stream per member, host thread pool (lmm_gmres_batched)"}
+        log(f"ensemble: {out['solves_per_s']} solves/s ({sec * 1e3:.1f} ms per batch of {len(pbs)})", quiet)
+        return out
+    finally:
+        for ctx, _ in members:
+            ctx.close()
 
 
 def main():
@@ -182,12 +283,25 @@ def main():
     pb = P.config3(n, a.seed)
     log(f"host inputs {pb.name}: {time.perf_counter() - t:.1f} s", a.quiet)
 
+    learned_params = learned_stats = None
+    if "learned" in forms:
+        learned_params, learned_stats = train_learned(P, S, a, a.quiet)
+
     results = {}
     with S.Context(3, pb.n, pb.dx, **pb.solver) as ctx:
         dp = S.DeviceProblem(ctx, pb)
         tmp_u = ctx.faces()
         tmp_p = ctx.field(S.CELL)
         sols = {}
+        net = None
+        if learned_params is not None:
+            net = S.Cnn(ctx, seed=a.seed, level=3)
+            net.set_params(learned_params)
+            net.attach()
+            e0, e1 = net.error_reduction(dp, 4, seed=a.seed + 999)
+            learned_stats["ablation_err_plain"] = round(e0, 5)
+            learned_stats["ablation_err_learned"] = round(e1, 5)
+            log(f"learned ablation at {n}^3: preconditioner error {e0:.4f} plain, {e1:.4f} with the stage", a.quiet)
         for form in forms:
             ms, info, iters, sm, fams = time_form(S, ctx, dp, form, a.steps, a.warmup, a.quiet)
             res = rel_true_residual(S, ctx, dp, tmp_u, tmp_p)
@@ -210,6 +324,15 @@ def main():
                 },
                 "time_by_family_ms": {k: round(v["ms"], 3) for k, v in fams.items()},
             }
+            # whole-solve bytes of the families that account them (all but the
+            # coarse single-workgroup V-cycles, prolongation and the learned stage)
+            sb = sum(v["bytes"] for v in fams.values())
+            results[form]["solve_algorithmic_GB"] = round(sb / 1e9, 3)
+            results[form]["solve_frac_hbm"] = round(sb / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)
+            if form == "learned":
+                results[form]["learned_applied"] = info["learned_applied"]
+                results[form]["learned_fallback"] = info["learned_fallback"]
+                results[form]["training"] = learned_stats
         if "fp64" in sols:
             for form in forms:
                 if form != "fp64":
@@ -218,8 +341,11 @@ def main():
                     results[form]["err_vs_fp64_p"] = ep
         for f in tmp_u + [tmp_p]:
             f.free()
+        if net is not None:
+            net.close()
 
-    head = "fp32_mg" if "fp32_mg" in results else forms[0]
+    fast = [f for f in ("fp32_mg", "learned") if f in results]
+    head = min(fast, key=lambda f: results[f]["ms_per_solve"]) if fast else forms[0]
     hr = results[head]
     sm = hr["smoother_l0"]
     out = {
@@ -233,7 +359,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if head == "fp32_mg" else "f64",
+        "dtype": "f64" if head == "fp64" else ("f32+f64+bf16" if head == "learned" else "f32+f64"),
         "data": "synthetic",
         "config": {
             "workload": f"3D periodic {n}^3 variable-viscosity staggered Stokes solve to rel-res 1e-9 "
@@ -241,7 +367,8 @@ def main():
             "form": head,
             "cells": N,
             "seed": a.seed,
-            "solver": "GMRES restart 5, projection preconditioner, 1 V-cycle each (2/2/8 sweeps, 6-colour GS)",
+            "solver": "GMRES restart 5, projection preconditioner, 1 V-cycle each (2/2/8 sweeps, 6-colour GS)"
+                      + ("; learned coarse-grid correction stage (BF16 U-Net on level 3)" if head == "learned" else ""),
         },
         "roofline": {
             "bound": "hbm",
@@ -258,20 +385,28 @@ def main():
     if "fp64" in results and head != "fp64":
         out["speedup_vs_fp64"] = round(results["fp64"]["ms_per_solve"] / hr["ms_per_solve"], 3)
 
+    if a.ensemble > 0:
+        out["ensemble"] = ensemble(P, S, a.ensemble, 3, a.quiet)
+
     if a.cpu_n > 0:
         from oracle import oracle as O
         from tests import helpers
 
-        sec, oinfo, pbc, b_c, (ou, op) = cpu_baseline(P, O, helpers, a.cpu_n, a.seed, a.quiet)
+        sec, secs, oinfo, pbc, b_c, (ou, op) = cpu_baseline(P, O, helpers, a.cpu_n, a.seed, a.cpu_runs, a.quiet)
         scale = (n / a.cpu_n) ** 3
-        cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        cores = int(O.lib().orc_max_threads())
         out["cpu_baseline"] = {
             "value": round(sec * scale, 3),
             "unit": "s",
             "cores": cores,
             "kind": "port",
-            "sample": f"oracle FP64 GMRES on the {a.cpu_n}^3 instance of the same problem family "
-                      f"({oinfo['total_iter']} iterations, {sec:.2f} s), scaled by the cell ratio {scale:g}",
+            "cpu": cpu_model(),
+            "sample": f"oracle FP64 GMRES (OpenMP, {cores} threads) on the {a.cpu_n}^3 instance of the same problem "
+                      f"family, median of {len(secs)} runs ({', '.join(f'{x:.2f}' for x in secs)} s, "
+                      f"{oinfo['total_iter']} iterations), scaled by the cell ratio {scale:g}",
+            "measured_s": round(sec, 3),
+            "measured_cells_per_s": round(a.cpu_n ** 3 / sec, 1),
+            "measured_solves_per_s": round(1.0 / sec, 4),
         }
         # the FP64 device path against the oracle on that instance
         with S.Context(3, pbc.n, pbc.dx, **pbc.solver) as ctx:

@@ -326,30 +326,35 @@ int lmm_diffusive_mass_fluxdiv(lmm_ctx *ctx, int nspecies, lmm_field *rho, lmm_f
  * No counterpart in the reference.  A small bias-free 3-level U-Net
  * (3x3x3 periodic convolutions, channels c1/c2/c3 = 16/32/64 by default,
  * BF16 weights and activations, FP32 accumulation on the gfx950 matrix
- * cores) predicts the error left by the staggered V-cycle from its
- * residual.  Precision LMM_LEARNED inserts it into the flexible GMRES's FP32
- * preconditioner: after the staggered V-cycle of apply_precon.f90:103-195
- * (the stage gmres.f90:335 reaches through apply_precon), the residual of
- * x_u is restricted (12-point weights) to multigrid level `level`; the
- * network maps [r/rms(r), log beta, log theta*alpha] there to the error in
- * units of rms(r)*dx_level^2; the prediction is prolongated back to the
- * finest level and added to x_u.  The plain V-cycle is the fallback: a step
- * whose Hessenberg column is not finite is redone without the stage, and an
- * inner step that cuts the residual estimate by less than 10 % ends the
- * stage for the rest of the solve (lmm_gmres_info.learned_fallback).  The
- * FGMRES stopping test on the true residual is unchanged, so the solution
- * meets the same tolerance.
+ * cores) predicts the error that the projection preconditioner leaves.
+ * Precision LMM_LEARNED inserts it into the flexible GMRES as an extra
+ * stage after the FP32 preconditioner (apply_precon.f90:103-195, the stage
+ * gmres.f90:335 reaches): the Stokes residual v - A z of the preconditioned
+ * vector z is restricted to multigrid level `level` (velocity with the
+ * 12-point weights, pressure with the 8-cell mean); the network maps
+ * [r_u dx^2/s, r_p dx/s, log beta, log theta*alpha] there (s the residual's
+ * scale in velocity units) to the error [e_u/s, e_p dx/s]; the prediction,
+ * its pressure made mean-free, is prolongated to the finest level and added
+ * to z.  The plain preconditioner is the fallback: a step whose Hessenberg
+ * column is not finite is redone without the stage, and an inner step that
+ * cuts the residual estimate by less than 10 % ends the stage for the rest
+ * of the solve (lmm_gmres_info.learned_fallback).  The FGMRES stopping test
+ * on the true residual is unchanged, so the solution meets the same
+ * tolerance.
  *
- * Training pairs come from the plain FP32 V-cycle of the context on given
- * coefficients: phi* = one V-cycle of counter-based white noise (seed, pair
- * index), b = A phi*, phi1 = one V-cycle of b; the input is the residual
- * b - A phi1 restricted to `level`, the target the error phi* - phi1
- * restricted with face averages.  Pairs stay on the device. */
+ * Training pairs come from the plain FP32 preconditioner of the context on
+ * given coefficients: phi* = M(counter-based white noise) (seed, pair
+ * index), then sample_steps times phi* <- phi* - M(A phi*), which leaves the
+ * modes the preconditioner handles worst (those the flexible GMRES iterates
+ * on); b = A phi*, z = M(b); the input is the residual b - A z, the target
+ * the error phi* - z, both restricted to `level` (the error with face and
+ * cell means).  Pairs stay on the device. */
 typedef struct lmm_cnn lmm_cnn;
 typedef struct {
-  int cin, c1, c2, c3, cout;  /* 8, 16, 32, 64, 3; c1..c3 multiples of 16, cin of 4 */
-  int level;                  /* staggered multigrid level of the correction (default 3) */
-  int post_sweeps;            /* finest-level sweeps after the correction is added (default 1) */
+  int cin, c1, c2, c3, cout;  /* 8, 16, 32, 64, 4; c1..c3 multiples of 16, cin of 4, cout <= 4 */
+  int level;                  /* multigrid level of the correction (default 3) */
+  int sample_steps;           /* training samples: applications of the preconditioned error
+                                 propagation I - M A to the noise before a sample (default 3) */
 } lmm_cnn_config;
 void lmm_cnn_config_default(lmm_cnn_config *cfg);
 /* He-initialised network (seeded) on the context's stream; cfg may be NULL. */
@@ -382,11 +387,11 @@ int lmm_cnn_train(lmm_cnn *net, int steps, int batch, double lr, uint64_t seed, 
 /* Mean over pairs [first, last) of |t - y| / |t| (1 = no better than a zero
  * prediction). */
 int lmm_cnn_evaluate(lmm_cnn *net, int64_t first, int64_t last, double *rel_err);
-/* The ablation of the learned stage on fresh samples of the plain V-cycle:
- * for npairs samples made as lmm_cnn_generate_pairs makes them (phi* = one
- * V-cycle of white noise, b = A phi*), the mean over samples of
- * |phi* - phi1| / |phi*| with phi1 the plain V-cycle's answer (err_plain)
- * and with phi1 corrected by the learned stage (err_learned). */
+/* The ablation of the learned stage on fresh samples of the plain
+ * preconditioner: for npairs samples made as lmm_cnn_generate_pairs makes
+ * them (phi* = M(white noise), b = A phi*), the mean over samples of
+ * |phi* - z| / |phi*| (velocity and pressure) with z = M(b) (err_plain) and
+ * with z corrected by the learned stage (err_learned). */
 int lmm_cnn_error_reduction(lmm_cnn *net, lmm_field *const alpha_fc[3], lmm_field *beta,
                             lmm_field *const beta_ed[3], lmm_field *gamma, double theta_alpha, int npairs,
                             uint64_t seed, double *err_plain, double *err_learned);

@@ -63,18 +63,24 @@ int cnn_loss_grad_host(CnnNet *net, int nb, const int n[3], const float *x, cons
 hipStream_t cnn_stream(CnnNet *net);
 void cnn_set_stream(CnnNet *net, hipStream_t s);
 
-// ---- features of the Stokes coarse-grid correction (3D, FP32 faces) ----
-// sum of squares of the three face arrays of grid g into *out (device double)
-void launch_sumsq3(hipStream_t s, const Grid &g, const float *const r[3], double *partial, double *out);
-// x[v][0..2] = r_d / sqrt(ss/(3N)), x[v][3] = log(beta), x[v][4..6] =
-// log(alpha_d), x[v][7] = 0 (cin = 8), in BF16
-void launch_cnn_pack(hipStream_t s, const Grid &g, const float *const r[3], const float *beta,
+// ---- features of the Stokes coarse-grid correction (3D, FP32) ----
+// The residual r = (r_u, r_p) of a preconditioned Stokes vector on one
+// multigrid level, its scale in velocity units s = sqrt(mean(r_u^2) dx^4 +
+// mean(r_p^2) dx^2), the network input [r_u dx^2/s, r_p dx/s, log beta,
+// log theta*alpha_d] (cin = 8, BF16) and target [e_u/s, e_p dx/s] (FP32, 4
+// channels); the prediction back in the level's units with the mean of its
+// pressure removed.
+// sums of squares of r_u and r_p into out[0], out[1] (device doubles)
+void launch_stokes_ss(hipStream_t s, const Grid &g, const float *const ru[3], const float *rp, double *partial,
+                      double *out);
+void launch_cnn_pack(hipStream_t s, const Grid &g, const float *const ru[3], const float *rp, const float *beta,
                      const float *const alpha[3], const double *ss, uint16_t *x);
-// target t[v][d] = e_d / (rms(r) * dx^2), t[v][3] = 0
-void launch_cnn_pack_target(hipStream_t s, const Grid &g, const float *const e[3], const double *ss, float *t);
+void launch_cnn_pack_target(hipStream_t s, const Grid &g, const float *const eu[3], const float *ep, const double *ss,
+                            float *t);
 // out = a - b (n values)
 void launch_sub_f32(hipStream_t s, float *out, const float *a, const float *b, long n);
-// phi_d = y[v][d] * rms(r) * dx^2 (the prediction, back in the level's units)
-void launch_cnn_unpack(hipStream_t s, const Grid &g, const float *y, const double *ss, float *const phi[3]);
+// partial: RED_BLOCKS doubles of scratch; psum: one device double
+void launch_cnn_unpack(hipStream_t s, const Grid &g, const float *y, const double *ss, double *partial, double *psum,
+                       float *const phu[3], float *php);
 
 }  // namespace lmm

@@ -985,51 +985,79 @@ int cnn_loss_grad_host(CnnNet *n, int nb, const int g[3], const float *x, const 
 
 namespace {
 
-__global__ void __launch_bounds__(256) k_sumsq3(long N, const float *__restrict__ r0, const float *__restrict__ r1,
-                                                const float *__restrict__ r2, double *partial) {
+// sums of squares of the velocity residual (3 face arrays) and of the
+// pressure residual (1 cell array): partial[2*blk], partial[2*blk+1]
+__global__ void __launch_bounds__(256) k_stokes_ss(long N, const float *__restrict__ r0, const float *__restrict__ r1,
+                                                   const float *__restrict__ r2, const float *__restrict__ rp,
+                                                   double *partial) {
   __shared__ double sh[4];
-  double acc = 0.0;
+  double au = 0.0, ap = 0.0;
   const long stride = (long)gridDim.x * blockDim.x;
   for (long q = (long)blockIdx.x * blockDim.x + threadIdx.x; q < N; q += stride) {
-    const double a = r0[q], b = r1[q], c = r2[q];
-    acc += a * a + b * b + c * c;
+    const double a = r0[q], b = r1[q], c = r2[q], p = rp[q];
+    au += a * a + b * b + c * c;
+    ap += p * p;
   }
-  const double r = block_sum_d(acc, sh);
-  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+  const double su = block_sum_d(au, sh);
+  const double sp = block_sum_d(ap, sh);
+  if (threadIdx.x == 0) {
+    partial[2 * blockIdx.x] = su;
+    partial[2 * blockIdx.x + 1] = sp;
+  }
 }
 
-__device__ __forceinline__ float inv_rms(const double *ss, long N) {
-  const double ms = ss[0] / (3.0 * (double)N);
-  return ms > 0.0 ? (float)(1.0 / sqrt(ms)) : 0.f;
+__global__ void k_sum_pairs(const double *__restrict__ p, long n, double *out) {
+  __shared__ double sh[4];
+  double a = 0.0, b = 0.0;
+  for (long q = threadIdx.x; q < n; q += blockDim.x) {
+    a += p[2 * q];
+    b += p[2 * q + 1];
+  }
+  const double ra = block_sum_d(a, sh);
+  const double rb = block_sum_d(b, sh);
+  if (threadIdx.x == 0) {
+    out[0] = ra;
+    out[1] = rb;
+  }
 }
 
-__global__ void k_cnn_pack(long N, const float *__restrict__ r0, const float *__restrict__ r1,
-                           const float *__restrict__ r2, const float *__restrict__ beta,
-                           const float *__restrict__ a0, const float *__restrict__ a1, const float *__restrict__ a2,
-                           const double *__restrict__ ss, uint16_t *x) {
+// The residual's scale in velocity units: s = sqrt(mean(r_u^2) dx^4 + mean(r_p^2) dx^2)
+__device__ __forceinline__ double stokes_scale(const double *ss, long N, double dx) {
+  const double dx2 = dx * dx;
+  return sqrt(ss[0] / (3.0 * (double)N) * dx2 * dx2 + ss[1] / (double)N * dx2);
+}
+
+__global__ void k_cnn_pack(long N, double dx, const float *__restrict__ r0, const float *__restrict__ r1,
+                           const float *__restrict__ r2, const float *__restrict__ rp,
+                           const float *__restrict__ beta, const float *__restrict__ a0,
+                           const float *__restrict__ a1, const float *__restrict__ a2, const double *__restrict__ ss,
+                           uint16_t *x) {
   const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= N) return;
-  const float s = inv_rms(ss, N);
+  const double sc = stokes_scale(ss, N, dx);
+  const float iu = sc > 0.0 ? (float)(dx * dx / sc) : 0.f, ip = sc > 0.0 ? (float)(dx / sc) : 0.f;
   uint16_t *o = x + v * 8;
-  o[0] = f2bf(r0[v] * s);
-  o[1] = f2bf(r1[v] * s);
-  o[2] = f2bf(r2[v] * s);
-  o[3] = f2bf(logf(beta[v]));
-  o[4] = f2bf(logf(a0[v]));
-  o[5] = f2bf(logf(a1[v]));
-  o[6] = f2bf(logf(a2[v]));
-  o[7] = 0;
+  o[0] = f2bf(r0[v] * iu);
+  o[1] = f2bf(r1[v] * iu);
+  o[2] = f2bf(r2[v] * iu);
+  o[3] = f2bf(rp[v] * ip);
+  o[4] = f2bf(logf(beta[v]));
+  o[5] = f2bf(logf(a0[v]));
+  o[6] = f2bf(logf(a1[v]));
+  o[7] = f2bf(logf(a2[v]));
 }
 
-__global__ void k_cnn_pack_target(long N, float dx2inv, const float *__restrict__ e0, const float *__restrict__ e1,
-                                  const float *__restrict__ e2, const double *__restrict__ ss, float *t) {
+__global__ void k_cnn_pack_target(long N, double dx, const float *__restrict__ e0, const float *__restrict__ e1,
+                                  const float *__restrict__ e2, const float *__restrict__ ep,
+                                  const double *__restrict__ ss, float *t) {
   const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= N) return;
-  const float s = inv_rms(ss, N) * dx2inv;
-  t[v * 4 + 0] = e0[v] * s;
-  t[v * 4 + 1] = e1[v] * s;
-  t[v * 4 + 2] = e2[v] * s;
-  t[v * 4 + 3] = 0.f;
+  const double sc = stokes_scale(ss, N, dx);
+  const float iu = sc > 0.0 ? (float)(1.0 / sc) : 0.f, ip = sc > 0.0 ? (float)(dx / sc) : 0.f;
+  t[v * 4 + 0] = e0[v] * iu;
+  t[v * 4 + 1] = e1[v] * iu;
+  t[v * 4 + 2] = e2[v] * iu;
+  t[v * 4 + 3] = ep[v] * ip;
 }
 
 __global__ void k_sub_f32(long n, float *out, const float *__restrict__ a, const float *__restrict__ b) {
@@ -1037,41 +1065,58 @@ __global__ void k_sub_f32(long n, float *out, const float *__restrict__ a, const
   if (t < n) out[t] = a[t] - b[t];
 }
 
-__global__ void k_cnn_unpack(long N, float dx2, const float *__restrict__ y, const double *__restrict__ ss, float *p0,
-                             float *p1, float *p2) {
+// sum of the pressure channel of y (for its mean)
+__global__ void __launch_bounds__(256) k_sum_p(long N, const float *__restrict__ y, double *partial) {
+  __shared__ double sh[4];
+  double a = 0.0;
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < N; v += stride) a += y[v * 4 + 3];
+  const double r = block_sum_d(a, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+// phi_u = y_u * s, phi_p = (y_p - mean y_p) * s / dx (the prediction in the
+// level's units; a constant pressure is in the operator's null space)
+__global__ void k_cnn_unpack(long N, double dx, const float *__restrict__ y, const double *__restrict__ ss,
+                             const double *__restrict__ psum, float *p0, float *p1, float *p2, float *pp) {
   const long v = (long)blockIdx.x * blockDim.x + threadIdx.x;
   if (v >= N) return;
-  const double ms = ss[0] / (3.0 * (double)N);
-  const float s = (float)sqrt(ms) * dx2;
-  p0[v] = y[v * 4 + 0] * s;
-  p1[v] = y[v * 4 + 1] * s;
-  p2[v] = y[v * 4 + 2] * s;
+  const double sc = stokes_scale(ss, N, dx);
+  const float su = (float)sc, sp = (float)(sc / dx), mean = (float)(psum[0] / (double)N);
+  p0[v] = y[v * 4 + 0] * su;
+  p1[v] = y[v * 4 + 1] * su;
+  p2[v] = y[v * 4 + 2] * su;
+  pp[v] = (y[v * 4 + 3] - mean) * sp;
 }
 
 }  // namespace
 
-void launch_sumsq3(hipStream_t s, const Grid &g, const float *const r[3], double *partial, double *out) {
-  CLAUNCH(k_sumsq3, dim3(RED_BLOCKS), dim3(256), 0, s, g.N, r[0], r[1], r[2], partial);
-  CLAUNCH(k_sum_partials, dim3(1), dim3(256), 0, s, partial, (long)RED_BLOCKS, 1.0, out);
+void launch_stokes_ss(hipStream_t s, const Grid &g, const float *const ru[3], const float *rp, double *partial,
+                      double *out) {
+  CLAUNCH(k_stokes_ss, dim3(RED_BLOCKS), dim3(256), 0, s, g.N, ru[0], ru[1], ru[2], rp, partial);
+  CLAUNCH(k_sum_pairs, dim3(1), dim3(256), 0, s, partial, (long)RED_BLOCKS, out);
 }
 
-void launch_cnn_pack(hipStream_t s, const Grid &g, const float *const r[3], const float *beta,
+void launch_cnn_pack(hipStream_t s, const Grid &g, const float *const ru[3], const float *rp, const float *beta,
                      const float *const alpha[3], const double *ss, uint16_t *x) {
-  CLAUNCH(k_cnn_pack, dim3(cblk(g.N)), dim3(CTPB), 0, s, g.N, r[0], r[1], r[2], beta, alpha[0], alpha[1], alpha[2],
-          ss, x);
+  CLAUNCH(k_cnn_pack, dim3(cblk(g.N)), dim3(CTPB), 0, s, g.N, g.dx, ru[0], ru[1], ru[2], rp, beta, alpha[0], alpha[1],
+          alpha[2], ss, x);
 }
 
-void launch_cnn_pack_target(hipStream_t s, const Grid &g, const float *const e[3], const double *ss, float *t) {
-  CLAUNCH(k_cnn_pack_target, dim3(cblk(g.N)), dim3(CTPB), 0, s, g.N, (float)(1.0 / (g.dx * g.dx)), e[0], e[1], e[2],
-          ss, t);
+void launch_cnn_pack_target(hipStream_t s, const Grid &g, const float *const eu[3], const float *ep, const double *ss,
+                            float *t) {
+  CLAUNCH(k_cnn_pack_target, dim3(cblk(g.N)), dim3(CTPB), 0, s, g.N, g.dx, eu[0], eu[1], eu[2], ep, ss, t);
 }
 
 void launch_sub_f32(hipStream_t s, float *out, const float *a, const float *b, long n) {
   CLAUNCH(k_sub_f32, dim3(cblk(n)), dim3(CTPB), 0, s, n, out, a, b);
 }
 
-void launch_cnn_unpack(hipStream_t s, const Grid &g, const float *y, const double *ss, float *const phi[3]) {
-  CLAUNCH(k_cnn_unpack, dim3(cblk(g.N)), dim3(CTPB), 0, s, g.N, (float)(g.dx * g.dx), y, ss, phi[0], phi[1], phi[2]);
+void launch_cnn_unpack(hipStream_t s, const Grid &g, const float *y, const double *ss, double *partial, double *psum,
+                       float *const phu[3], float *php) {
+  CLAUNCH(k_sum_p, dim3(RED_BLOCKS), dim3(256), 0, s, g.N, y, partial);
+  CLAUNCH(k_sum_partials, dim3(1), dim3(256), 0, s, partial, (long)RED_BLOCKS, 1.0, psum);
+  CLAUNCH(k_cnn_unpack, dim3(cblk(g.N)), dim3(CTPB), 0, s, g.N, g.dx, y, ss, psum, phu[0], phu[1], phu[2], php);
 }
 
 }  // namespace lmm

@@ -860,34 +860,45 @@ static void apply_matrix_z32(lmm_ctx *c, float *z, const SV &b) {
   apply_matrix_dev(c, sv_of(c, c->r), b);
 }
 
-// The learned coarse-grid correction (no counterpart in the reference): the
-// residual of x_u after the staggered V-cycle, restricted to level lc with
-// the 12-point weights, goes through the network with the level's
-// coefficients; its prediction of the error, prolongated back to level 0,
-// is added to x_u.
-static void learned_stage(lmm_ctx *c, std::vector<SLevel<float>> &S, float *const bu[3], float *const xu[3]) {
+// The learned coarse-grid correction (no counterpart in the reference) of a
+// preconditioned Stokes vector z = M(v): the residual v - A z (FP32),
+// restricted to level lc (velocity with the 12-point weights, pressure with
+// the 8-cell mean), goes through the network with the level's
+// coefficients; its prediction of the error, prolongated back to the finest
+// level (staggered and cell-centred prolongation), is added to z.
+static void learned_stage(lmm_ctx *c, float *const vu[3], const float *vp, float *const zu[3], float *zp) {
   hipStream_t s = c->stream;
+  auto &S = c->s32;
+  auto &C = c->c32;
   lmm_cnn *nn = c->cnn;
   const int lc = nn->cfg.level, visc = c->prm.visc_type;
-  {
-    ProfScope ps(c, "learned", 0.0);
-    launch_stag_residual<float>(s, S[0].g, visc, xu, bu, S[0].res, S[0].K());
-    launch_stag_restrict<float>(s, S[1].g, S[0].res, S[1].rhs, 0);
-    for (int n = 1; n < lc; n++) launch_stag_restrict<float>(s, S[n + 1].g, S[n].rhs, S[n + 1].rhs, 0);
-    const SLevel<float> &C = S[lc];
-    launch_sumsq3(s, C.g, C.rhs, c->partial, nn->ss);
-    launch_cnn_pack(s, C.g, C.rhs, C.beta, C.alpha, nn->ss, nn->xin);
-    const int gn[3] = {C.g.n[0], C.g.n[1], C.g.n[2]};
-    cnn_forward(nn->net, 1, gn, nn->xin, nn->yout);
-    launch_cnn_unpack(s, C.g, nn->yout, nn->ss, S[lc].phi);
-    for (int n = lc - 1; n >= 1; n--) {
-      for (int d = 0; d < 3; d++) launch_fill<float>(s, S[n].phi[d], S[n].g.N, 0.f);
-      launch_stag_prolong<float>(s, S[n].g, S[n].phi, S[n + 1].phi);
-    }
-    launch_stag_prolong<float>(s, S[0].g, xu, S[1].phi);
+  const long N = c->g.N;
+  ProfScope ps(c, "learned", 0.0);
+  float *ru[3] = {S[0].res[0], S[0].res[1], S[0].res[2]};
+  float *rp = C[0].res;
+  launch_apply_matrix<float>(s, S[0].g, visc, zu, zp, ru, rp, S[0].K());
+  for (int d = 0; d < 3; d++) launch_sub_f32(s, ru[d], vu[d], ru[d], N);
+  launch_sub_f32(s, rp, vp, rp, N);
+  launch_stag_restrict<float>(s, S[1].g, ru, S[1].rhs, 0);
+  launch_cc_restrict<float>(s, C[1].g, rp, C[1].rhs);
+  for (int n = 1; n < lc; n++) {
+    launch_stag_restrict<float>(s, S[n + 1].g, S[n].rhs, S[n + 1].rhs, 0);
+    launch_cc_restrict<float>(s, C[n + 1].g, C[n].rhs, C[n + 1].rhs);
   }
-  // smooth the interpolation error of the correction on the finest level
-  if (nn->cfg.post_sweeps > 0) smooth<float>(c, S[0], xu, bu, nn->cfg.post_sweeps);
+  const Grid &gc = S[lc].g;
+  launch_stokes_ss(s, gc, S[lc].rhs, C[lc].rhs, c->partial, nn->ss);
+  launch_cnn_pack(s, gc, S[lc].rhs, C[lc].rhs, S[lc].beta, S[lc].alpha, nn->ss, nn->xin);
+  const int gn[3] = {gc.n[0], gc.n[1], gc.n[2]};
+  cnn_forward(nn->net, 1, gn, nn->xin, nn->yout);
+  launch_cnn_unpack(s, gc, nn->yout, nn->ss, c->partial, nn->ss + 2, S[lc].phi, C[lc].phi);
+  for (int n = lc - 1; n >= 1; n--) {
+    for (int d = 0; d < 3; d++) launch_fill<float>(s, S[n].phi[d], S[n].g.N, 0.f);
+    launch_fill<float>(s, C[n].phi, C[n].g.N, 0.f);
+    launch_stag_prolong<float>(s, S[n].g, S[n].phi, S[n + 1].phi);
+    launch_cc_prolong_add<float>(s, C[n].g, C[n].phi, C[n + 1].phi);
+  }
+  launch_stag_prolong<float>(s, S[0].g, zu, S[1].phi);
+  launch_cc_prolong_add<float>(s, C[0].g, zp, C[1].phi);
   c->learned_applied++;
 }
 
@@ -900,9 +911,6 @@ static void apply_precon_t(lmm_ctx *c, std::vector<SLevel<T>> &S, std::vector<CL
   const Grid &g = c->g;
   int dim = g.dim;
   stag_vcycles<T>(c, S, xu, bu, true);  // x_u from a zero guess
-  if constexpr (std::is_same<T, float>::value) {
-    if (c->learned_on && c->cnn) learned_stage(c, S, bu, xu);
-  }
   launch_div_plus<T>(s, g, xu, bp, mac_rhs, 1);
   cc_vcycles<T>(c, C, phi, mac_rhs);
   T *ainv[3] = {C[0].a[0], C[0].a[1], C[0].a[2]};
@@ -1186,6 +1194,7 @@ static void precon32(lmm_ctx *c, const double *v64, float *z32, double theta_alp
   if (!have32) launch_convert<float, double>(s, c->b32, v64, L);
   SVT<float> b = svt_of<float>(c, c->b32), z = svt_of<float>(c, z32);
   apply_precon_t<float>(c, c->s32, c->c32, b.u, b.p, z.u, z.p, c->mac_rhs32, c->c32[0].phi, theta_alpha);
+  if (c->learned_on && c->cnn) learned_stage(c, b.u, b.p, z.u, z.p);
 }
 
 // One classical Gram-Schmidt step of the flexible GMRES.  The stored basis
@@ -2430,9 +2439,9 @@ void lmm_cnn_config_default(lmm_cnn_config *cfg) {
   cfg->c1 = 16;
   cfg->c2 = 32;
   cfg->c3 = 64;
-  cfg->cout = 3;
+  cfg->cout = 4;
   cfg->level = 3;
-  cfg->post_sweeps = 1;
+  cfg->sample_steps = 3;
 }
 
 int lmm_cnn_create(lmm_ctx *c, const lmm_cnn_config *cfg, uint64_t seed, lmm_cnn **out) {
@@ -2500,14 +2509,16 @@ int lmm_cnn_loss_grad_host(lmm_cnn *nn, int nb, const int n[3], const float *x, 
   return finish(nn->ctx);
 }
 
-// the network's level must exist and its grid suit the network (3D, 8 input
-// and 3 output channels: residual, log beta, log theta*alpha)
+// the network's level must exist in both multigrids and its grid suit the
+// network (3D; 8 input channels: residual (u, p), log beta, log theta*alpha;
+// 4 output channels: error (u, p))
 static int cnn_level_ok(lmm_ctx *c, lmm_cnn *nn) {
   const lmm_cnn_config &f = nn->cfg;
   if (c->g.dim != 3) return set_err(c, LMM_ERR_UNSUPPORTED, "the learned correction is 3D only");
-  if (f.cin != 8 || f.cout != 3) return set_err(c, LMM_ERR_ARG, "the Stokes correction needs cin 8 and cout 3");
-  if (f.level < 1 || f.level >= c->nl_s) return set_err(c, LMM_ERR_ARG, "cnn level %d outside 1..%d", f.level, c->nl_s - 1);
-  if (f.post_sweeps < 0 || f.post_sweeps > 8) return set_err(c, LMM_ERR_ARG, "cnn post_sweeps %d outside 0..8", f.post_sweeps);
+  if (f.cin != 8 || f.cout != 4) return set_err(c, LMM_ERR_ARG, "the Stokes correction needs cin 8 and cout 4");
+  if (f.sample_steps < 0 || f.sample_steps > 16) return set_err(c, LMM_ERR_ARG, "cnn sample_steps outside 0..16");
+  const int nl = std::min(c->nl_s, c->nl_c);
+  if (f.level < 1 || f.level >= nl) return set_err(c, LMM_ERR_ARG, "cnn level %d outside 1..%d", f.level, nl - 1);
   Grid gc = c->g;
   for (int n = 0; n < f.level; n++) gc = coarsen(gc);
   for (int d = 0; d < 3; d++)
@@ -2517,7 +2528,7 @@ static int cnn_level_ok(lmm_ctx *c, lmm_cnn *nn) {
   if (!nn->xin) {
     nn->xin = dalloc<uint16_t>(c->stream, gc.N * 8);
     nn->yout = dalloc<float>(c->stream, gc.N * CNN_OUT_STRIDE);
-    nn->ss = dalloc<double>(c->stream, 4);
+    nn->ss = dalloc<double>(c->stream, 4);  // sums of squares (u, p) and the prediction's pressure sum
     if (!nn->xin || !nn->yout || !nn->ss) return set_err(c, LMM_ERR_NOMEM, "cnn application buffers");
   }
   return LMM_OK;
@@ -2536,57 +2547,92 @@ int lmm_ctx_set_cnn(lmm_ctx *c, lmm_cnn *nn) {
   return LMM_OK;
 }
 
+// One sample of the preconditioner's error on the coefficients the levels
+// hold: phi* = M(white noise) (a Stokes vector of the solutions' kind),
+// then `steps` times phi* <- phi* - M(A phi*) (the preconditioned error
+// propagation, which leaves the modes M handles worst: the ones the flexible
+// GMRES iterates on); b = A phi*, z = M(b) (FP32 projection preconditioner,
+// no learned stage).  Leaves phi* in ps, b in c->b32 and z in z.
+static void precon_sample(lmm_ctx *c, double theta_alpha, uint64_t seed, uint64_t q, int steps, const SVT<float> &ps,
+                          const SVT<float> &z) {
+  hipStream_t s = c->stream;
+  const long N = c->g.N;
+  auto &S = c->s32;
+  auto &C = c->c32;
+  const int visc = c->prm.visc_type;
+  SVT<float> b = svt_of<float>(c, c->b32);
+  launch_fill_normal_philox(s, c->r, 4 * N, 0, seed, q);
+  launch_convert<float, double>(s, c->b32, c->r, 4 * N);
+  apply_precon_t<float>(c, S, C, b.u, b.p, ps.u, ps.p, c->mac_rhs32, C[0].phi, theta_alpha);
+  for (int k = 0; k < steps; k++) {
+    launch_apply_matrix<float>(s, c->g, visc, ps.u, ps.p, b.u, b.p, S[0].K());
+    apply_precon_t<float>(c, S, C, b.u, b.p, z.u, z.p, c->mac_rhs32, C[0].phi, theta_alpha);
+    for (int d = 0; d < 3; d++) launch_sub_f32(s, ps.u[d], ps.u[d], z.u[d], N);
+    launch_sub_f32(s, ps.p, ps.p, z.p, N);
+  }
+  launch_apply_matrix<float>(s, c->g, visc, ps.u, ps.p, b.u, b.p, S[0].K());
+  apply_precon_t<float>(c, S, C, b.u, b.p, z.u, z.p, c->mac_rhs32, C[0].phi, theta_alpha);
+}
+
+// common set-up of pair generation and the ablation: coefficients staged
+// (FP64 and FP32 hierarchies, both multigrids), work arrays, checks
+static int learned_setup(lmm_ctx *c, lmm_cnn *nn, lmm_field *const alpha_fc[3], lmm_field *beta,
+                         lmm_field *const beta_ed[3], lmm_field *gamma, double theta_alpha) {
+  int rc;
+  if ((rc = check_coefs(c, alpha_fc, beta, beta_ed, gamma))) return rc;
+  if ((rc = cnn_level_ok(c, nn))) return rc;
+  if ((rc = ensure_lvl32(c))) return rc;
+  if (!c->gen32) {
+    c->gen32 = dalloc<float>(c->stream, 8 * c->g.N);
+    if (!c->gen32) return set_err(c, LMM_ERR_NOMEM, "sample work arrays");
+  }
+  prepare_coefs(c, coef_in(c, alpha_fc, beta, beta_ed, gamma), theta_alpha, 1.0, true, true);
+  c->learned_on = false;
+  return LMM_OK;
+}
+
 int lmm_cnn_generate_pairs(lmm_cnn *nn, lmm_field *const alpha_fc[3], lmm_field *beta, lmm_field *const beta_ed[3],
                            lmm_field *gamma, double theta_alpha, int npairs, uint64_t seed) {
   if (!nn || npairs < 1) return LMM_ERR_ARG;
   lmm_ctx *c = nn->ctx;
   TuningScope ts(&c->tune);
   int rc;
-  if ((rc = check_coefs(c, alpha_fc, beta, beta_ed, gamma))) return rc;
-  if ((rc = cnn_level_ok(c, nn))) return rc;
-  if ((rc = ensure_lvl32(c))) return rc;
-  const Grid &g = c->g;
-  const long N = g.N;
-  if (!c->gen32) {
-    c->gen32 = dalloc<float>(c->stream, 6 * N);
-    if (!c->gen32) return set_err(c, LMM_ERR_NOMEM, "training-pair work arrays");
-  }
+  if ((rc = learned_setup(c, nn, alpha_fc, beta, beta_ed, gamma, theta_alpha))) return rc;
   const int lc = nn->cfg.level;
   auto &S = c->s32;
+  auto &C = c->c32;
   const int gn[3] = {S[lc].g.n[0], S[lc].g.n[1], S[lc].g.n[2]};
   const long have = cnn_dataset_count(nn->net);
   if ((rc = cnn_dataset_reserve(nn->net, have + npairs, gn)))
     return set_err(c, rc, "training pairs: out of memory or a grid other than the stored pairs'");
-  prepare_coefs(c, coef_in(c, alpha_fc, beta, beta_ed, gamma), theta_alpha, 1.0, false, true);
   hipStream_t s = c->stream;
-  const int visc = c->prm.visc_type;
-  float *ps[3], *e[3], *b[3];
-  for (int d = 0; d < 3; d++) {
-    ps[d] = c->gen32 + d * N;
-    e[d] = c->gen32 + (3 + d) * N;
-    b[d] = c->b32 + d * N;
-  }
+  const long N = c->g.N;
+  const SVT<float> ps = svt_of<float>(c, c->gen32), z = svt_of<float>(c, c->gen32 + 4 * N);
+  const SVT<float> b = svt_of<float>(c, c->b32);
+  float *ru[3] = {S[0].res[0], S[0].res[1], S[0].res[2]};
   for (int q = 0; q < npairs; q++) {
-    // phi* = one V-cycle of white noise (a sample of the solutions), b = A phi*
-    launch_fill_normal_philox(s, c->r, 3 * N, 0, seed, (uint64_t)q);
-    launch_convert<float, double>(s, c->b32, c->r, 3 * N);
-    stag_vcycles<float>(c, S, ps, b, true);
-    launch_stag_applyop<float>(s, g, visc, 0, ps, b, S[0].K());
-    // the plain V-cycle's answer phi1 (in e), its residual r = b - A phi1 and error e = phi* - phi1
-    stag_vcycles<float>(c, S, e, b, true);
-    launch_stag_residual<float>(s, g, visc, e, b, S[0].res, S[0].K());
-    for (int d = 0; d < 3; d++) launch_sub_f32(s, e[d], ps[d], e[d], N);
-    // the residual with the 12-point weights, the error with face averages
-    launch_stag_restrict<float>(s, S[1].g, S[0].res, S[1].rhs, 0);
-    launch_stag_restrict<float>(s, S[1].g, e, S[1].phi, 1);
+    precon_sample(c, theta_alpha, seed, (uint64_t)q, nn->cfg.sample_steps, ps, z);
+    // residual r = b - A z, error e = phi* - z (into z)
+    launch_apply_matrix<float>(s, c->g, c->prm.visc_type, z.u, z.p, ru, C[0].res, S[0].K());
+    for (int d = 0; d < 3; d++) launch_sub_f32(s, ru[d], b.u[d], ru[d], N);
+    launch_sub_f32(s, C[0].res, b.p, C[0].res, N);
+    for (int d = 0; d < 3; d++) launch_sub_f32(s, z.u[d], ps.u[d], z.u[d], N);
+    launch_sub_f32(s, z.p, ps.p, z.p, N);
+    // the residual with the multigrid restrictions, the error with face / cell means
+    launch_stag_restrict<float>(s, S[1].g, ru, S[1].rhs, 0);
+    launch_cc_restrict<float>(s, C[1].g, C[0].res, C[1].rhs);
+    launch_stag_restrict<float>(s, S[1].g, z.u, S[1].phi, 1);
+    launch_cc_restrict<float>(s, C[1].g, z.p, C[1].phi);
     for (int n = 1; n < lc; n++) {
       launch_stag_restrict<float>(s, S[n + 1].g, S[n].rhs, S[n + 1].rhs, 0);
+      launch_cc_restrict<float>(s, C[n + 1].g, C[n].rhs, C[n + 1].rhs);
       launch_stag_restrict<float>(s, S[n + 1].g, S[n].phi, S[n + 1].phi, 1);
+      launch_cc_restrict<float>(s, C[n + 1].g, C[n].phi, C[n + 1].phi);
     }
     const long slot = have + q;
-    launch_sumsq3(s, S[lc].g, S[lc].rhs, c->partial, nn->ss);
-    launch_cnn_pack(s, S[lc].g, S[lc].rhs, S[lc].beta, S[lc].alpha, nn->ss, cnn_dataset_x(nn->net, slot));
-    launch_cnn_pack_target(s, S[lc].g, S[lc].phi, nn->ss, cnn_dataset_t(nn->net, slot));
+    launch_stokes_ss(s, S[lc].g, S[lc].rhs, C[lc].rhs, c->partial, nn->ss);
+    launch_cnn_pack(s, S[lc].g, S[lc].rhs, C[lc].rhs, S[lc].beta, S[lc].alpha, nn->ss, cnn_dataset_x(nn->net, slot));
+    launch_cnn_pack_target(s, S[lc].g, S[lc].phi, C[lc].phi, nn->ss, cnn_dataset_t(nn->net, slot));
     cnn_dataset_push(nn->net);
   }
   return finish(c);
@@ -2599,46 +2645,36 @@ int lmm_cnn_error_reduction(lmm_cnn *nn, lmm_field *const alpha_fc[3], lmm_field
   lmm_ctx *c = nn->ctx;
   TuningScope ts(&c->tune);
   int rc;
-  if ((rc = check_coefs(c, alpha_fc, beta, beta_ed, gamma))) return rc;
-  if ((rc = cnn_level_ok(c, nn))) return rc;
-  if ((rc = ensure_lvl32(c))) return rc;
-  const Grid &g = c->g;
-  const long N = g.N;
-  if (!c->gen32) {
-    c->gen32 = dalloc<float>(c->stream, 6 * N);
-    if (!c->gen32) return set_err(c, LMM_ERR_NOMEM, "sample work arrays");
-  }
-  prepare_coefs(c, coef_in(c, alpha_fc, beta, beta_ed, gamma), theta_alpha, 1.0, false, true);
+  if ((rc = learned_setup(c, nn, alpha_fc, beta, beta_ed, gamma, theta_alpha))) return rc;
   hipStream_t s = c->stream;
-  const int visc = c->prm.visc_type;
+  const long N = c->g.N;
+  const SVT<float> ps = svt_of<float>(c, c->gen32), z = svt_of<float>(c, c->gen32 + 4 * N);
+  const SVT<float> b = svt_of<float>(c, c->b32);
   auto &S = c->s32;
-  float *ps[3], *x[3], *b[3];
-  for (int d = 0; d < 3; d++) {
-    ps[d] = c->gen32 + d * N;
-    x[d] = c->gen32 + (3 + d) * N;
-    b[d] = c->b32 + d * N;
-  }
+  auto &C = c->c32;
+  float *du[3] = {S[0].phi[0], S[0].phi[1], S[0].phi[2]};
+  float *dp = C[0].rhs;
   lmm_cnn *saved = c->cnn;
   c->cnn = nn;
   double sp = 0.0, sl = 0.0;
   double *slot = c->scal + 4300;
+  // |phi* - z| / |phi*| over velocity and pressure (both mean-free in pressure)
+  auto err2 = [&](double *out) {
+    for (int d = 0; d < 3; d++) launch_sub_f32(s, du[d], ps.u[d], z.u[d], N);
+    launch_sub_f32(s, dp, ps.p, z.p, N);
+    launch_stokes_ss(s, c->g, du, dp, c->partial, out);
+  };
   for (int q = 0; q < npairs; q++) {
-    launch_fill_normal_philox(s, c->r, 3 * N, 0, seed, (uint64_t)q);
-    launch_convert<float, double>(s, c->b32, c->r, 3 * N);
-    stag_vcycles<float>(c, S, ps, b, true);
-    launch_stag_applyop<float>(s, g, visc, 0, ps, b, S[0].K());
-    stag_vcycles<float>(c, S, x, b, true);
-    // |phi*|^2, |phi* - phi1|^2, then the learned stage on phi1 and |phi* - phi2|^2
-    launch_sumsq3(s, g, ps, c->partial, slot);
-    for (int d = 0; d < 3; d++) launch_sub_f32(s, S[0].phi[d], ps[d], x[d], N);
-    launch_sumsq3(s, g, S[0].phi, c->partial, slot + 1);
-    learned_stage(c, S, b, x);
-    for (int d = 0; d < 3; d++) launch_sub_f32(s, S[0].phi[d], ps[d], x[d], N);
-    launch_sumsq3(s, g, S[0].phi, c->partial, slot + 2);
-    double h[3];
-    fetch(c, slot, 3, h);
-    sp += std::sqrt(h[1] / h[0]);
-    sl += std::sqrt(h[2] / h[0]);
+    precon_sample(c, theta_alpha, seed, (uint64_t)q, nn->cfg.sample_steps, ps, z);
+    launch_stokes_ss(s, c->g, ps.u, ps.p, c->partial, slot);
+    err2(slot + 2);
+    learned_stage(c, b.u, b.p, z.u, z.p);
+    err2(slot + 4);
+    double h[6];
+    fetch(c, slot, 6, h);
+    const double n0 = h[0] + h[1];
+    sp += std::sqrt((h[2] + h[3]) / n0);
+    sl += std::sqrt((h[4] + h[5]) / n0);
   }
   c->cnn = saved;
   c->learned_applied = 0;

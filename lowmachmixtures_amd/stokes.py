@@ -566,7 +566,7 @@ class DeviceProblem:
 class CnnConfig(C.Structure):
     """lmm_cnn_config: channels of the U-Net and the multigrid level it acts on."""
     _fields_ = [("cin", C.c_int), ("c1", C.c_int), ("c2", C.c_int), ("c3", C.c_int), ("cout", C.c_int),
-                ("level", C.c_int), ("post_sweeps", C.c_int)]
+                ("level", C.c_int), ("sample_steps", C.c_int)]
 
 
 def _fptr(a):
@@ -642,7 +642,7 @@ class Cnn:
                        "cnn_generate_pairs")
 
     def error_reduction(self, dp, npairs, seed):
-        """Mean |phi* - phi1| / |phi*| after the plain V-cycle and after V-cycle + learned stage."""
+        """Mean |phi* - z| / |phi*| after the plain preconditioner and after it plus the learned stage."""
         a, b = C.c_double(), C.c_double()
         self.ctx.check(lib().lmm_cnn_error_reduction(self.h, _a3(dp.alpha), _h(dp.beta), _a3(dp.beta_ed),
                                                      _h(dp.gamma), C.c_double(dp.pb.theta_alpha), int(npairs),
