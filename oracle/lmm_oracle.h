@@ -147,6 +147,68 @@ int orc_diffusive_mass_fluxdiv(const orc_geom *g, int ns, const double *rho,
                                double *diff_fluxdiv, double *flux_total[3],
                                int correct_flux);
 
+/* ---- species mass-flux chain (orc_species.c) ----
+ * Knobs of probin_multispecies.f90:11-90 and probin_common.f90 (molmass,
+ * rhobar, k_B, variance_coef_mass, barodiffusion_type).  Dbar and H_offdiag
+ * list the strict lower triangle by rows: (2,1), (3,1), (3,2), (4,1), ...
+ * (compute_mixture_properties.f90:176-186). */
+#define ORC_MAX_SPECIES 8
+typedef struct {
+  int nspecies;
+  double molmass[8], rhobar[8];
+  double Dbar[28], Dtherm[8], H_offdiag[28], H_diag[8];
+  double k_B, variance_coef_mass, fraction_tolerance;
+  int is_ideal_mixture, is_nonisothermal, use_lapack, inverse_type, chi_iterations;
+  int correct_flux, barodiffusion_type;
+} orc_species;
+
+/* per-cell algebra; matrices column-major (ns, ns) */
+void orc_sp_correct_rho(const orc_species *sp, double *rho, double *drho);
+void orc_sp_cons_to_prim(const orc_species *sp, const double *rho, double *rhotot,
+                         double *molarconc, double *molmtot);
+void orc_sp_mixture(const orc_species *sp, double *D_bar, double *D_therm, double *H);
+void orc_sp_gama(const orc_species *sp, const double *molarconc, const double *H, double *Gama);
+void orc_sp_chi(const orc_species *sp, const double *rho, double rhotot, const double *molarconc,
+                const double *D_bar, double Temp, const double *D_therm, double *chi,
+                double *zeta_by_Temp);
+void orc_sp_rhowchi(const orc_species *sp, const double *rho, const double *chi, double *rhoWchi);
+void orc_sp_lonsager(const orc_species *sp, const double *rho, double rhotot, double molmtot,
+                     const double *chi, double *L);
+void orc_sp_baro_coef(const orc_species *sp, const double *rho, double Temp, double *baro);
+int orc_sp_invert(int ns, double *A);
+void orc_sp_pinv(int ns, const double *A, double tol, double *out);
+
+/* whole fields (component-major, deduplicated) */
+int orc_correct_rho_with_drho(const orc_geom *g, const orc_species *sp, double *rho, double *drho);
+int orc_convert_cons_to_prim(const orc_geom *g, const orc_species *sp, const double *rho,
+                             double *rhotot, double *molarconc, double *molmtot);
+int orc_compute_mixture_properties(const orc_geom *g, const orc_species *sp, double *D_bar,
+                                   double *D_therm, double *Hessian);
+int orc_compute_Gama(const orc_geom *g, const orc_species *sp, const double *molarconc,
+                     const double *Hessian, double *Gama);
+int orc_compute_chi(const orc_geom *g, const orc_species *sp, const double *rho, const double *rhotot,
+                    const double *molarconc, double *chi, const double *D_bar, const double *D_therm,
+                    const double *Temp, double *zeta_by_Temp);
+int orc_compute_rhoWchi(const orc_geom *g, const orc_species *sp, const double *rho,
+                        const double *chi, double *rhoWchi);
+int orc_compute_Lonsager(const orc_geom *g, const orc_species *sp, const double *rho,
+                         const double *rhotot, const double *molmtot, const double *chi,
+                         double *Lonsager);
+int orc_diffusive_mass_flux_full(const orc_geom *g, const orc_species *sp, const double *rho,
+                                 const double *rhotot, const double *molarconc,
+                                 const double *rhoWchi, const double *Gama, double *diff_fluxdiv,
+                                 const double *Temp, const double *zeta_by_Temp,
+                                 double *const gradp_baro[3], double *flux_total[3]);
+int orc_stochastic_mass_fluxdiv(const orc_geom *g, const orc_species *sp, const double *rho,
+                                const double *rhotot, const double *molarconc,
+                                const double *molmtot, const double *chi, const double *Gama,
+                                double *stoch_fluxdiv, double *flux_total[3], double dt,
+                                const double *W, int n_rngs, const double *weights, int increment);
+int orc_compute_mass_fluxdiv(const orc_geom *g, const orc_species *sp, double *rho,
+                             double *const gradp_baro[3], double *diff_fluxdiv,
+                             double *stoch_fluxdiv, const double *Temp, double *flux_total[3],
+                             double dt, const double *W, int n_rngs, const double *weights);
+
 /* ---- random numbers (src_random/RNGs.c restated) ---- */
 typedef struct {
   uint32_t mt[624];

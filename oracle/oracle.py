@@ -464,3 +464,200 @@ def inv_norm_dist(p):
 
 def compute_nlevs_mg(g, minwidth=2):
     return lib().orc_compute_nlevs_mg(C.byref(g), minwidth)
+
+
+# ---------------- species mass-flux chain (orc_species.c) ----------------
+
+MAX_SPECIES = 8
+
+
+class Species(C.Structure):
+    _fields_ = [
+        ("nspecies", C.c_int),
+        ("molmass", C.c_double * 8), ("rhobar", C.c_double * 8),
+        ("Dbar", C.c_double * 28), ("Dtherm", C.c_double * 8),
+        ("H_offdiag", C.c_double * 28), ("H_diag", C.c_double * 8),
+        ("k_B", C.c_double), ("variance_coef_mass", C.c_double), ("fraction_tolerance", C.c_double),
+        ("is_ideal_mixture", C.c_int), ("is_nonisothermal", C.c_int), ("use_lapack", C.c_int),
+        ("inverse_type", C.c_int), ("chi_iterations", C.c_int),
+        ("correct_flux", C.c_int), ("barodiffusion_type", C.c_int),
+    ]
+
+
+# probin_multispecies.f90:70-90 and probin_common.f90:210-220 defaults
+SPECIES_DEFAULTS = dict(
+    molmass=1.0, rhobar=1.0, Dbar=1.0, Dtherm=0.0, H_offdiag=0.0, H_diag=0.0,
+    k_B=1.0, variance_coef_mass=1.0, fraction_tolerance=1e-13,
+    is_ideal_mixture=1, is_nonisothermal=1, use_lapack=0, inverse_type=1, chi_iterations=10,
+    correct_flux=1, barodiffusion_type=0,
+)
+
+
+def species(nspecies, **kw) -> Species:
+    """Species knobs; array knobs take a scalar (every entry) or a sequence."""
+    d = dict(SPECIES_DEFAULTS)
+    d.update(kw)
+    s = Species()
+    s.nspecies = int(nspecies)
+    for k, v in d.items():
+        field = getattr(s, k)
+        if isinstance(field, C.Array):
+            vals = np.broadcast_to(np.asarray(v, dtype=np.float64), (len(field),)) if np.ndim(v) == 0 else v
+            for i, x in enumerate(vals):
+                field[i] = float(x)
+        else:
+            setattr(s, k, v)
+    return s
+
+
+def _c(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def sp_cell_chi(sp, rho, molarconc, D_bar, Temp=1.0, D_therm=None):
+    """compute_chi_local on one cell: returns (chi (ns, ns), zeta_by_Temp)."""
+    ns = sp.nspecies
+    rho, x = _c(rho), _c(molarconc)
+    Db = _c(np.asarray(D_bar).T)  # column-major
+    Dt = _c(D_therm if D_therm is not None else np.zeros(ns))
+    chi = np.zeros(ns * ns)
+    z = np.zeros(ns)
+    lib().orc_sp_chi(C.byref(sp), _p(rho), C.c_double(float(rho.sum())), _p(x), _p(Db), C.c_double(Temp),
+                     _p(Dt), _p(chi), _p(z))
+    return chi.reshape(ns, ns).T.copy(), z
+
+
+def sp_cell_lonsager(sp, rho, rhotot, molmtot, chi):
+    ns = sp.nspecies
+    L = np.zeros(ns * ns)
+    lib().orc_sp_lonsager(C.byref(sp), _p(_c(rho)), C.c_double(rhotot), C.c_double(molmtot),
+                          _p(_c(np.asarray(chi).T)), _p(L))
+    return L.reshape(ns, ns).T.copy()
+
+
+def sp_cell_gama(sp, molarconc, H):
+    ns = sp.nspecies
+    G = np.zeros(ns * ns)
+    lib().orc_sp_gama(C.byref(sp), _p(_c(molarconc)), _p(_c(np.asarray(H).T)), _p(G))
+    return G.reshape(ns, ns).T.copy()
+
+
+def sp_invert(A):
+    ns = A.shape[0]
+    a = _c(np.asarray(A).T).copy()
+    _check(lib().orc_sp_invert(ns, _p(a)), "sp_invert")
+    return a.reshape(ns, ns).T.copy()
+
+
+def sp_pinv(A, tol):
+    ns = A.shape[0]
+    out = np.zeros(ns * ns)
+    lib().orc_sp_pinv(ns, _p(_c(np.asarray(A).T)), C.c_double(tol), _p(out))
+    return out.reshape(ns, ns).T.copy()
+
+
+def correct_rho_with_drho(g, sp, rho):
+    rho = _c(rho).copy()
+    drho = np.zeros_like(rho)
+    _check(lib().orc_correct_rho_with_drho(C.byref(g), C.byref(sp), _p(rho), _p(drho)), "correct_rho_with_drho")
+    return rho, drho
+
+
+def convert_cons_to_prim(g, sp, rho):
+    rho = _c(rho)
+    rhotot = np.zeros(rho.shape[1:])
+    molarconc = np.zeros_like(rho)
+    molmtot = np.zeros(rho.shape[1:])
+    _check(lib().orc_convert_cons_to_prim(C.byref(g), C.byref(sp), _p(rho), _p(rhotot), _p(molarconc),
+                                          _p(molmtot)), "convert_cons_to_prim")
+    return rhotot, molarconc, molmtot
+
+
+def compute_mixture_properties(g, sp, shape):
+    ns = sp.nspecies
+    D_bar = np.zeros((ns * ns,) + tuple(shape))
+    D_therm = np.zeros((ns,) + tuple(shape))
+    H = np.zeros((ns * ns,) + tuple(shape))
+    _check(lib().orc_compute_mixture_properties(C.byref(g), C.byref(sp), _p(D_bar), _p(D_therm), _p(H)),
+           "compute_mixture_properties")
+    return D_bar, D_therm, H
+
+
+def compute_Gama(g, sp, molarconc, Hessian):
+    molarconc, Hessian = _c(molarconc), _c(Hessian)
+    Gama = np.zeros_like(Hessian)
+    _check(lib().orc_compute_Gama(C.byref(g), C.byref(sp), _p(molarconc), _p(Hessian), _p(Gama)), "compute_Gama")
+    return Gama
+
+
+def compute_chi(g, sp, rho, rhotot, molarconc, D_bar, D_therm, Temp):
+    rho, rhotot, molarconc, D_bar, D_therm, Temp = map(_c, (rho, rhotot, molarconc, D_bar, D_therm, Temp))
+    chi = np.zeros_like(D_bar)
+    zeta = np.zeros_like(rho)
+    _check(lib().orc_compute_chi(C.byref(g), C.byref(sp), _p(rho), _p(rhotot), _p(molarconc), _p(chi), _p(D_bar),
+                                 _p(D_therm), _p(Temp), _p(zeta)), "compute_chi")
+    return chi, zeta
+
+
+def compute_rhoWchi(g, sp, rho, chi):
+    rho, chi = _c(rho), _c(chi)
+    out = np.zeros_like(chi)
+    _check(lib().orc_compute_rhoWchi(C.byref(g), C.byref(sp), _p(rho), _p(chi), _p(out)), "compute_rhoWchi")
+    return out
+
+
+def compute_Lonsager(g, sp, rho, rhotot, molmtot, chi):
+    rho, rhotot, molmtot, chi = map(_c, (rho, rhotot, molmtot, chi))
+    out = np.zeros_like(chi)
+    _check(lib().orc_compute_Lonsager(C.byref(g), C.byref(sp), _p(rho), _p(rhotot), _p(molmtot), _p(chi), _p(out)),
+           "compute_Lonsager")
+    return out
+
+
+def diffusive_mass_flux_full(g, sp, rho, rhotot, molarconc, rhoWchi, Gama, flux_total, Temp=None,
+                             zeta_by_Temp=None, gradp_baro=None):
+    rho, rhotot, molarconc, rhoWchi, Gama = map(_c, (rho, rhotot, molarconc, rhoWchi, Gama))
+    flux_total = _copy(flux_total)
+    div = np.zeros_like(molarconc)
+    T = _c(Temp) if Temp is not None else None
+    z = _c(zeta_by_Temp) if zeta_by_Temp is not None else None
+    gb = _copy(gradp_baro) if gradp_baro is not None else None
+    _check(lib().orc_diffusive_mass_flux_full(C.byref(g), C.byref(sp), _p(rho), _p(rhotot), _p(molarconc),
+                                              _p(rhoWchi), _p(Gama), _p(div), _p(T) if T is not None else None,
+                                              _p(z) if z is not None else None,
+                                              _arr3(gb) if gb is not None else None, _arr3(flux_total)),
+           "diffusive_mass_flux_full")
+    return div, flux_total
+
+
+def stochastic_mass_fluxdiv(g, sp, rho, rhotot, molarconc, molmtot, chi, Gama, flux_total, dt, W, weights,
+                            stoch_fluxdiv=None, increment=False):
+    """W: array (n_rngs, dim, ns, *shape)."""
+    rho, rhotot, molarconc, molmtot, chi, Gama, W = map(_c, (rho, rhotot, molarconc, molmtot, chi, Gama, W))
+    w = _c(weights)
+    flux_total = _copy(flux_total)
+    div = _c(stoch_fluxdiv).copy() if stoch_fluxdiv is not None else np.zeros_like(rho)
+    _check(lib().orc_stochastic_mass_fluxdiv(C.byref(g), C.byref(sp), _p(rho), _p(rhotot), _p(molarconc),
+                                             _p(molmtot), _p(chi), _p(Gama), _p(div), _arr3(flux_total),
+                                             C.c_double(dt), _p(W), int(W.shape[0]), _p(w), int(increment)),
+           "stochastic_mass_fluxdiv")
+    return div, flux_total
+
+
+def compute_mass_fluxdiv(g, sp, rho, Temp, dt, W, weights, gradp_baro=None):
+    """compute_mass_fluxdiv.f90:24-147; returns (rho after the call, diff_fluxdiv,
+    stoch_fluxdiv, flux_total)."""
+    rho = _c(rho).copy()
+    T = _c(Temp)
+    W = _c(W)
+    w = _c(weights)
+    shape = rho.shape[1:]
+    ns = sp.nspecies
+    flux_total = [np.zeros((ns,) + tuple(shape)) for _ in range(g.dim)]
+    diff = np.zeros_like(rho)
+    stoch = np.zeros_like(rho)
+    gb = _copy(gradp_baro) if gradp_baro is not None else None
+    _check(lib().orc_compute_mass_fluxdiv(C.byref(g), C.byref(sp), _p(rho), _arr3(gb) if gb is not None else None,
+                                          _p(diff), _p(stoch), _p(T), _arr3(flux_total), C.c_double(dt), _p(W),
+                                          int(W.shape[0]), _p(w)), "compute_mass_fluxdiv")
+    return rho, diff, stoch, flux_total
