@@ -311,16 +311,124 @@ int lmm_stoch_m_force(lmm_ctx *ctx, lmm_field *stoch_m_force[3], lmm_field *W_cc
                       lmm_field *const W_ed[3], lmm_field *eta, lmm_field *const eta_ed[3],
                       lmm_field *temperature, lmm_field *const temperature_ed[3], double variance);
 
+/* ---------------- species mass fluxes ----------------
+ * The reference keeps these knobs in module globals (probin_multispecies.f90:
+ * 11-90, probin_common.f90:31-32,214-220); a context carries one set.  Dbar
+ * and H_offdiag list the strict lower triangle by rows, (2,1), (3,1), (3,2),
+ * (4,1), ... (compute_mixture_properties.f90:209-230).  Species matrices are
+ * fields of nspecies^2 components in Fortran (ns,ns) order (row r, column c
+ * in component r + ns*c).  Defaults are the reference's. */
+typedef struct {
+  int nspecies;               /* 2..8 */
+  double molmass[8];          /* 1 */
+  double rhobar[8];           /* 1 */
+  double Dbar[28];            /* 1 */
+  double Dtherm[8];           /* 0 */
+  double H_offdiag[28];       /* 0 */
+  double H_diag[8];           /* 0 */
+  double k_B;                 /* 1 */
+  double variance_coef_mass;  /* 1 */
+  double fraction_tolerance;  /* 1e-13 */
+  int is_ideal_mixture;       /* 1 */
+  int is_nonisothermal;       /* 1 */
+  int use_lapack;             /* 0: Dbar2chi_iterative and choldc */
+  int inverse_type;           /* 1 inverse, 2 pseudo-inverse (use_lapack only) */
+  int chi_iterations;         /* 10 */
+  int correct_flux;           /* 1 */
+  int barodiffusion_type;     /* 0 */
+} lmm_species_params;
+
+void lmm_species_params_default(lmm_species_params *p, int nspecies);
+int lmm_ctx_set_species(lmm_ctx *ctx, const lmm_species_params *p);
+int lmm_ctx_get_species(const lmm_ctx *ctx, lmm_species_params *p);
+
+/* mass_flux_utilities.f90:26-150 correct_rho_with_drho(mla,rho,drho):
+ * rho = max(rho, fraction_tolerance*sum(rho)) in place, drho = the change. */
+int lmm_correct_rho_with_drho(lmm_ctx *ctx, lmm_field *rho, lmm_field *drho);
+/* :152-313 convert_cons_to_prim(mla,rho,rhotot,molarconc,molmtot) */
+int lmm_convert_cons_to_prim(lmm_ctx *ctx, lmm_field *rho, lmm_field *rhotot, lmm_field *molarconc,
+                             lmm_field *molmtot);
+/* compute_mixture_properties.f90:24-233 compute_mixture_properties(mla,rho,rhotot,
+ * D_bar,D_therm,Hessian,Temp): the constant-coefficient model (prob_type 9's
+ * water-glycerol fit is not provided); rho, rhotot, Temp may be NULL. */
+int lmm_compute_mixture_properties(lmm_ctx *ctx, lmm_field *rho, lmm_field *rhotot, lmm_field *D_bar,
+                                   lmm_field *D_therm, lmm_field *Hessian, lmm_field *Temp);
+/* mass_flux_utilities.f90:428-599 compute_Gama(mla,rho,rhotot,molarconc,molmtot,Hessian,Gama);
+ * rho, rhotot, molmtot may be NULL (unused by the reference). */
+int lmm_compute_Gama(lmm_ctx *ctx, lmm_field *rho, lmm_field *rhotot, lmm_field *molarconc,
+                     lmm_field *molmtot, lmm_field *Hessian, lmm_field *Gama);
+/* :601-927 compute_chi(mla,rho,rhotot,molarconc,chi,D_bar,D_therm,Temp,zeta_by_Temp);
+ * zeta_by_Temp is written when is_nonisothermal (then D_therm and Temp are
+ * required).  use_lapack computes the inverse (inverse_type 1) or the
+ * pseudo-inverse (2) that LAPACK dgetrf/dgetri or gesvd would, by Gauss-Jordan
+ * elimination and a one-sided Jacobi SVD: equal to LAPACK to rounding. */
+int lmm_compute_chi(lmm_ctx *ctx, lmm_field *rho, lmm_field *rhotot, lmm_field *molarconc, lmm_field *chi,
+                    lmm_field *D_bar, lmm_field *D_therm, lmm_field *Temp, lmm_field *zeta_by_Temp);
+/* :1120-1273 compute_rhoWchi(mla,rho,rhotot,chi,rhoWchi): rhoWchi = -rho_r chi_rc;
+ * rhotot may be NULL. */
+int lmm_compute_rhoWchi(lmm_ctx *ctx, lmm_field *rho, lmm_field *rhotot, lmm_field *chi, lmm_field *rhoWchi);
+/* :929-1118 compute_Lonsager(mla,rho,rhotot,molarconc,molmtot,chi,Gama,Lonsager):
+ * Cholesky factor (lower triangle, upper zeroed) of molmtot rho W chi W / k_B,
+ * by choldc (matrix_utilities.f90:160-239) or, with use_lapack, the dpotrf
+ * factor with its (ns,ns) entry zeroed.  molarconc and Gama may be NULL. */
+int lmm_compute_Lonsager(lmm_ctx *ctx, lmm_field *rho, lmm_field *rhotot, lmm_field *molarconc,
+                         lmm_field *molmtot, lmm_field *chi, lmm_field *Gama, lmm_field *Lonsager);
+
 /* diffusive_mass_fluxdiv.f90:26-42  diffusive_mass_fluxdiv(mla,rho,rhotot,molarconc,rhoWchi,
  *        Gama,diff_fluxdiv,Temp,zeta_by_Temp,gradp_baro,flux_total,dx,the_bc_tower)
- * Isothermal, no barodiffusion: Temp, zeta_by_Temp, gradp_baro must be NULL.
- * rhoWchi and Gama hold nspecies^2 components (Fortran (ns,ns) order);
- * flux_total[d] is incremented; diff_fluxdiv is overwritten. */
+ * F = rhoWchi_f (Gama_f grad x [+ zeta_f grad T] [+ baro_f gradp_baro]) per
+ * face (:92-262), corrected so the species fluxes sum to zero when
+ * correct_flux is nonzero (correction_flux.f90:71-251); flux_total[d] is
+ * incremented, diff_fluxdiv overwritten.  The temperature piece enters when
+ * the context's species knobs say is_nonisothermal and Temp and
+ * zeta_by_Temp are given (both or neither); the barodiffusion piece when
+ * barodiffusion_type > 0 (2D only, as compute_baro_coef,
+ * mass_flux_utilities.f90:1291-1340; rho,
+ * Temp, gradp_baro required).  Without lmm_ctx_set_species the context uses
+ * the default knobs for nspecies species.  rhoWchi and Gama hold
+ * nspecies^2 components. */
 int lmm_diffusive_mass_fluxdiv(lmm_ctx *ctx, int nspecies, lmm_field *rho, lmm_field *rhotot,
                                lmm_field *molarconc, lmm_field *rhoWchi, lmm_field *Gama,
                                lmm_field *diff_fluxdiv, lmm_field *Temp, lmm_field *zeta_by_Temp,
                                lmm_field *const gradp_baro[3], lmm_field *flux_total[3],
                                int correct_flux);
+
+/* stochastic_mass_fluxdiv.f90:158-188 init_mass_stochastic(mla,n_rngs): n_rngs
+ * stages of face noise, nspecies components per face direction. */
+int lmm_init_mass_stochastic(lmm_ctx *ctx, int n_rngs);
+/* :219-246 fill_mass_stochastic(mla,the_bc_level): every stage filled with
+ * N(0,1) numbers of the host generator (as lmm_fill_m_stochastic), in the
+ * order stage, face direction, species, x fastest (periodic: no wall
+ * boundary conditions to apply). */
+int lmm_fill_mass_stochastic(lmm_ctx *ctx, uint32_t seed);
+/* No counterpart in the reference: the same fill on the device from the
+ * counter-based stream of lmm_fill_m_stochastic_device, same value order. */
+int lmm_fill_mass_stochastic_device(lmm_ctx *ctx, uint64_t seed, uint64_t step);
+/* stochastic_mass_fluxdiv.f90:32-154  stochastic_mass_fluxdiv(mla,rho,rhotot,molarconc,
+ *        molmtot,chi,Gama,stoch_fluxdiv,flux_total,dx,dt,weights,the_bc_level,increment)
+ * F = sqrt(2 k_B variance_coef_mass/(dV dt)) * L_f * (sum_r weights[r] W_r)
+ * with L_f the face average of the cell Cholesky factors of L-Onsager,
+ * corrected when correct_flux; flux_total[d] incremented; stoch_fluxdiv
+ * overwritten, or incremented when increment is nonzero. */
+int lmm_stochastic_mass_fluxdiv(lmm_ctx *ctx, lmm_field *rho, lmm_field *rhotot, lmm_field *molarconc,
+                                lmm_field *molmtot, lmm_field *chi, lmm_field *Gama,
+                                lmm_field *stoch_fluxdiv, lmm_field *flux_total[3], double dt,
+                                const double *weights, int increment);
+/* compute_mass_fluxdiv.f90:24-147  compute_mass_fluxdiv(mla,rho,gradp_baro,diff_fluxdiv,
+ *        stoch_fluxdiv,Temp,flux_total,dt,stage_time,dx,weights,the_bc_tower)
+ * The whole chain in two device passes per flux: rho corrected
+ * (correct_rho_with_drho), primitive variables, mixture properties, Gama,
+ * chi, rhoWchi, the diffusive flux, the stochastic flux from the context's
+ * mass noise (when variance_coef_mass != 0; else stoch_fluxdiv = 0), rho
+ * restored as rho_c - drho.  flux_total is overwritten with the sum of both
+ * face fluxes.  Temp is required when is_nonisothermal or
+ * barodiffusion_type > 0.  stage_time only feeds the reference's
+ * manufactured-solution sources (external_source, prob_types 4/5), which
+ * this library does not provide. */
+int lmm_compute_mass_fluxdiv(lmm_ctx *ctx, lmm_field *rho, lmm_field *const gradp_baro[3],
+                             lmm_field *diff_fluxdiv, lmm_field *stoch_fluxdiv, lmm_field *Temp,
+                             lmm_field *flux_total[3], double dt, double stage_time,
+                             const double *weights);
 
 /* ---------------- learned coarse-grid correction ----------------
  * No counterpart in the reference.  A small bias-free 3-level U-Net

@@ -247,9 +247,5 @@ void launch_stoch_m_force(hipStream_t s, const Grid &g, double *const force[3], 
                           long stage_stride_ed, const double *eta, double *const eta_ed[3],
                           const double *temp, double *const temp_ed[3], double variance, int width = 0,
                           double *work = nullptr);
-void launch_diffusive_mass_flux(hipStream_t s, const Grid &g, int ns, const double *molarconc,
-                                const double *rhoWchi, const double *Gama, double *flux,
-                                double *const flux_total[3], int correct_flux);
-void launch_div_species(hipStream_t s, const Grid &g, int ns, const double *flux, double *div);
 
 }  // namespace lmm

@@ -1452,85 +1452,6 @@ void launch_stoch_m_force(hipStream_t s, const Grid &g, double *const force[3], 
                      p3(temp_ed), variance);
 }
 
-// ------------------------------------------------------------------------
-// species diffusion flux (diffusive_mass_fluxdiv.f90:92-262, isothermal)
-// flux layout: [dir][species][N]
-// ------------------------------------------------------------------------
-
-constexpr int MAXNS = 8;
-
-__global__ void __launch_bounds__(256) k_diff_mass_flux(Grid g, int ns, const double *__restrict__ x,
-                                                        const double *__restrict__ rw,
-                                                        const double *__restrict__ gm, double *flux,
-                                                        Ptr3<double> ft, int correct) {
-  long t = xcd_block() * blockDim.x + threadIdx.x;
-  if (t >= g.N) return;
-  int i, j, k;
-  decode(g, t, i, j, k);
-  Pt P = make_pt(g, i, j, k);
-  const long N = g.N;
-  double dxinv = 1.0 / g.dx;
-  for (int d = 0; d < g.dim; d++) {
-    long m = at_sh(g, P, d, -1, -1, 0);
-    double gx[MAXNS], y[MAXNS];
-    for (int q = 0; q < ns; q++) gx[q] = (x[q * N + t] - x[q * N + m]) * dxinv;
-    for (int r = 0; r < ns; r++) {
-      double acc = 0.0;
-      for (int c = 0; c < ns; c++) {
-        long q = r + ns * c;
-        acc = acc + 0.5 * (gm[q * N + t] + gm[q * N + m]) * gx[c];
-      }
-      y[r] = acc;
-    }
-    for (int r = 0; r < ns; r++) {
-      double acc = 0.0;
-      for (int c = 0; c < ns; c++) {
-        long q = r + ns * c;
-        acc = acc + 0.5 * (rw[q * N + t] + rw[q * N + m]) * y[c];
-      }
-      gx[r] = acc;
-    }
-    if (correct && ns > 1) {
-      double sum = 0.0;
-      for (int q = 0; q < ns - 1; q++) sum = sum + gx[q];
-      gx[ns - 1] = -sum;
-    }
-    for (int q = 0; q < ns; q++) {
-      flux[((long)d * ns + q) * N + t] = gx[q];
-      ft.c[d][q * N + t] = ft.c[d][q * N + t] + gx[q];
-    }
-  }
-}
-
-void launch_diffusive_mass_flux(hipStream_t s, const Grid &g, int ns, const double *molarconc,
-                                const double *rhoWchi, const double *Gama, double *flux,
-                                double *const flux_total[3], int correct_flux) {
-  LAUNCH(k_diff_mass_flux, dim3(nblk(g.N)), dim3(TPB), 0, s, g, ns, molarconc, rhoWchi, Gama, flux,
-                     p3(flux_total), correct_flux);
-}
-
-__global__ void k_div_species(Grid g, int ns, const double *__restrict__ flux, double *div) {
-  long c = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= g.N) return;
-  const int q = (int)blockIdx.y;
-  const long t = q * g.N + c;
-  int i, j, k;
-  decode(g, c, i, j, k);
-  Pt P = make_pt(g, i, j, k);
-  const long N = g.N;
-  double dxinv = 1.0 / g.dx;
-  const double *fx = flux + ((long)0 * ns + q) * N, *fy = flux + ((long)1 * ns + q) * N;
-  double v = (fx[at3(g, P, 1, 0, 0)] - fx[c]) * dxinv;
-  v = v + (fy[at3(g, P, 0, 1, 0)] - fy[c]) * dxinv;
-  if (g.dim == 3) {
-    const double *fz = flux + ((long)2 * ns + q) * N;
-    v = v + (fz[at3(g, P, 0, 0, 1)] - fz[c]) * dxinv;
-  }
-  div[t] = v;
-}
-void launch_div_species(hipStream_t s, const Grid &g, int ns, const double *flux, double *div) {
-  LAUNCH(k_div_species, dim3(nblk(g.N), ns), dim3(TPB), 0, s, g, ns, flux, div);
-}
 
 // ------------------------------------------------------------------------
 // explicit instantiations

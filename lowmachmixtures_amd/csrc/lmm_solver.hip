@@ -20,6 +20,7 @@
 
 #include "../../include/lmm.h"
 #include "lmm_cnn.h"
+#include "lmm_species.h"
 #include "lmm_kernels.h"
 
 using namespace lmm;
@@ -152,6 +153,15 @@ struct lmm_ctx {
   bool learned_on = false;    // the FP32 preconditioner applies the learned stage
   int learned_applied = 0;
   float *gen32 = nullptr;     // training-pair work: phi* and e (6 face arrays)
+  // species knobs (lmm_ctx_set_species; reference defaults until set) and
+  // the mass noise of init_mass_stochastic ([n_rngs][dim][ns][N])
+  bool species_set = false;
+  lmm_species_params species{};
+  int n_rngs_mass = 0;
+  double *Wmass = nullptr;
+  double *dweights_mass = nullptr;
+  double *sp_work = nullptr;  // species scratch, grown on demand
+  long sp_work_n = 0;
 };
 
 // A network bound to a context: the correction acts on staggered level
@@ -1556,7 +1566,8 @@ int lmm_ctx_destroy(lmm_ctx *c) {
   for (void *p : {(void *)c->partial, (void *)c->partial_cgs, (void *)c->partial2, (void *)c->scal, (void *)c->scaled_beta,
                   (void *)c->scaled_gamma, (void *)c->V, (void *)c->r, (void *)c->w, (void *)c->tmp, (void *)c->b,
                   (void *)c->x, (void *)c->Z32, (void *)c->b32, (void *)c->mac_rhs, (void *)c->phi_cc,
-                  (void *)c->mac_rhs32, (void *)c->Wcc, (void *)c->Wed, (void *)c->dweights, (void *)c->stoch_work})
+                  (void *)c->mac_rhs32, (void *)c->Wcc, (void *)c->Wed, (void *)c->dweights, (void *)c->stoch_work,
+                  (void *)c->Wmass, (void *)c->dweights_mass, (void *)c->sp_work})
     dfree(p);
   for (int d = 0; d < 3; d++) {
     dfree(c->ainv[d]);
@@ -2338,32 +2349,439 @@ int lmm_stoch_m_force(lmm_ctx *c, lmm_field *stoch_m_force[3], lmm_field *W_cc, 
                       temperature_ed, variance);
 }
 
+// ---------------- species mass fluxes ----------------
+
+void lmm_species_params_default(lmm_species_params *p, int nspecies) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  p->nspecies = nspecies;
+  for (int q = 0; q < 8; q++) {
+    p->molmass[q] = 1.0;
+    p->rhobar[q] = 1.0;
+  }
+  for (int q = 0; q < 28; q++) p->Dbar[q] = 1.0;
+  p->k_B = 1.0;
+  p->variance_coef_mass = 1.0;
+  p->fraction_tolerance = 1e-13;
+  p->is_ideal_mixture = 1;
+  p->is_nonisothermal = 1;
+  p->use_lapack = 0;
+  p->inverse_type = 1;
+  p->chi_iterations = 10;
+  p->correct_flux = 1;
+  p->barodiffusion_type = 0;
+}
+
+int lmm_ctx_set_species(lmm_ctx *c, const lmm_species_params *p) {
+  if (!c || !p) return LMM_ERR_ARG;
+  if (p->nspecies < 2 || p->nspecies > SP_MAX)
+    return set_err(c, LMM_ERR_UNSUPPORTED, "nspecies %d outside 2..%d", p->nspecies, SP_MAX);
+  if (p->use_lapack && p->inverse_type != 1 && p->inverse_type != 2)
+    return set_err(c, LMM_ERR_ARG, "inverse_type must be 1 or 2");
+  if (p->chi_iterations < 0 || !(p->k_B > 0) || !(p->fraction_tolerance >= 0))
+    return set_err(c, LMM_ERR_ARG, "chi_iterations, k_B or fraction_tolerance out of range");
+  for (int q = 0; q < p->nspecies; q++)
+    if (!(p->molmass[q] > 0) || !(p->rhobar[q] > 0)) return set_err(c, LMM_ERR_ARG, "molmass / rhobar must be positive");
+  if (c->n_rngs_mass > 0 && c->species_set && p->nspecies != c->species.nspecies) {
+    dfree(c->Wmass);
+    c->Wmass = nullptr;
+    c->n_rngs_mass = 0;  // the noise of another species count no longer fits
+  }
+  c->species = *p;
+  c->species_set = true;
+  return LMM_OK;
+}
+
+int lmm_ctx_get_species(const lmm_ctx *c, lmm_species_params *p) {
+  if (!c || !p) return LMM_ERR_ARG;
+  if (c->species_set)
+    *p = c->species;
+  else
+    lmm_species_params_default(p, 2);
+  return LMM_OK;
+}
+
+// Kernel-side knobs: the context's set, or the defaults for ns species.
+static SpParams species_knobs(const lmm_ctx *c, int ns) {
+  lmm_species_params p;
+  if (c->species_set)
+    p = c->species;
+  else
+    lmm_species_params_default(&p, ns);
+  SpParams k;
+  memset(&k, 0, sizeof(k));
+  k.ns = p.nspecies;
+  for (int q = 0; q < 8; q++) {
+    k.molmass[q] = p.molmass[q];
+    k.rhobar[q] = p.rhobar[q];
+    k.Dtherm[q] = p.Dtherm[q];
+    k.Hdiag[q] = p.H_diag[q];
+  }
+  for (int q = 0; q < 28; q++) {
+    k.Dbar[q] = p.Dbar[q];
+    k.Hoff[q] = p.H_offdiag[q];
+  }
+  k.k_B = p.k_B;
+  k.vcm = p.variance_coef_mass;
+  k.ftol = p.fraction_tolerance;
+  k.ideal = p.is_ideal_mixture;
+  k.noniso = p.is_nonisothermal;
+  k.lapack = p.use_lapack;
+  k.inverse_type = p.inverse_type;
+  k.chi_iter = p.chi_iterations;
+  k.correct = p.correct_flux;
+  k.baro = p.barodiffusion_type;
+  return k;
+}
+
+// The species count of the context's knobs (set, else 0 = not set).
+static int species_ns(lmm_ctx *c) { return c->species_set ? c->species.nspecies : 0; }
+
+// Species scratch of at least n doubles, kept by the context.
+static double *sp_scratch(lmm_ctx *c, long n) {
+  if (n > c->sp_work_n) {
+    dfree(c->sp_work);
+    c->sp_work = dalloc<double>(c->stream, n);
+    c->sp_work_n = c->sp_work ? n : 0;
+  }
+  return c->sp_work;
+}
+
+static int sp_status(lmm_ctx *c, int rc, const char *what) {
+  if (rc) return set_err(c, rc, "%s: nspecies outside the supported range", what);
+  return finish(c);
+}
+
+#define SP_FIELD(f, kind, nc, what)                                                              \
+  if (!field_ok(c, f, kind, nc)) return set_err(c, LMM_ERR_ARG, "%s: %s missing or with fewer than %d components", \
+                                                what, #f, (int)(nc))
+
+static int need_species(lmm_ctx *c, const char *what) {
+  if (!c) return LMM_ERR_ARG;
+  if (!c->species_set) return set_err(c, LMM_ERR_ARG, "%s: set the species knobs first (lmm_ctx_set_species)", what);
+  return LMM_OK;
+}
+
+int lmm_correct_rho_with_drho(lmm_ctx *c, lmm_field *rho, lmm_field *drho) {
+  if (int rc = need_species(c, "correct_rho_with_drho")) return rc;
+  const int ns = species_ns(c);
+  SP_FIELD(rho, LMM_CELL, ns, "correct_rho_with_drho");
+  SP_FIELD(drho, LMM_CELL, ns, "correct_rho_with_drho");
+  return sp_status(c, sp_correct_rho(c->stream, c->g, species_knobs(c, ns), rho->d, drho->d), "correct_rho_with_drho");
+}
+
+int lmm_convert_cons_to_prim(lmm_ctx *c, lmm_field *rho, lmm_field *rhotot, lmm_field *molarconc, lmm_field *molmtot) {
+  if (int rc = need_species(c, "convert_cons_to_prim")) return rc;
+  const int ns = species_ns(c);
+  SP_FIELD(rho, LMM_CELL, ns, "convert_cons_to_prim");
+  SP_FIELD(rhotot, LMM_CELL, 1, "convert_cons_to_prim");
+  SP_FIELD(molarconc, LMM_CELL, ns, "convert_cons_to_prim");
+  SP_FIELD(molmtot, LMM_CELL, 1, "convert_cons_to_prim");
+  return sp_status(c, sp_cons_to_prim(c->stream, c->g, species_knobs(c, ns), rho->d, rhotot->d, molarconc->d, molmtot->d),
+                   "convert_cons_to_prim");
+}
+
+int lmm_compute_mixture_properties(lmm_ctx *c, lmm_field *rho, lmm_field *rhotot, lmm_field *D_bar, lmm_field *D_therm,
+                                   lmm_field *Hessian, lmm_field *Temp) {
+  if (int rc = need_species(c, "compute_mixture_properties")) return rc;
+  (void)rho;
+  (void)rhotot;
+  (void)Temp;
+  const int ns = species_ns(c);
+  SP_FIELD(D_bar, LMM_CELL, ns * ns, "compute_mixture_properties");
+  SP_FIELD(D_therm, LMM_CELL, ns, "compute_mixture_properties");
+  SP_FIELD(Hessian, LMM_CELL, ns * ns, "compute_mixture_properties");
+  return sp_status(c, sp_mixture(c->stream, c->g, species_knobs(c, ns), D_bar->d, D_therm->d, Hessian->d),
+                   "compute_mixture_properties");
+}
+
+int lmm_compute_Gama(lmm_ctx *c, lmm_field *rho, lmm_field *rhotot, lmm_field *molarconc, lmm_field *molmtot,
+                     lmm_field *Hessian, lmm_field *Gama) {
+  if (int rc = need_species(c, "compute_Gama")) return rc;
+  (void)rho;
+  (void)rhotot;
+  (void)molmtot;
+  const int ns = species_ns(c);
+  SP_FIELD(molarconc, LMM_CELL, ns, "compute_Gama");
+  SP_FIELD(Hessian, LMM_CELL, ns * ns, "compute_Gama");
+  SP_FIELD(Gama, LMM_CELL, ns * ns, "compute_Gama");
+  return sp_status(c, sp_gama(c->stream, c->g, species_knobs(c, ns), molarconc->d, Hessian->d, Gama->d), "compute_Gama");
+}
+
+int lmm_compute_chi(lmm_ctx *c, lmm_field *rho, lmm_field *rhotot, lmm_field *molarconc, lmm_field *chi,
+                    lmm_field *D_bar, lmm_field *D_therm, lmm_field *Temp, lmm_field *zeta_by_Temp) {
+  if (int rc = need_species(c, "compute_chi")) return rc;
+  const int ns = species_ns(c);
+  const SpParams sp = species_knobs(c, ns);
+  SP_FIELD(rho, LMM_CELL, ns, "compute_chi");
+  SP_FIELD(rhotot, LMM_CELL, 1, "compute_chi");
+  SP_FIELD(molarconc, LMM_CELL, ns, "compute_chi");
+  SP_FIELD(chi, LMM_CELL, ns * ns, "compute_chi");
+  SP_FIELD(D_bar, LMM_CELL, ns * ns, "compute_chi");
+  if (sp.noniso) {
+    SP_FIELD(D_therm, LMM_CELL, ns, "compute_chi");
+    SP_FIELD(Temp, LMM_CELL, 1, "compute_chi");
+    SP_FIELD(zeta_by_Temp, LMM_CELL, ns, "compute_chi");
+  }
+  return sp_status(c, sp_chi(c->stream, c->g, sp, rho->d, rhotot->d, molarconc->d, D_bar->d,
+                             sp.noniso ? D_therm->d : nullptr, sp.noniso ? Temp->d : nullptr, chi->d,
+                             sp.noniso ? zeta_by_Temp->d : nullptr),
+                   "compute_chi");
+}
+
+int lmm_compute_rhoWchi(lmm_ctx *c, lmm_field *rho, lmm_field *rhotot, lmm_field *chi, lmm_field *rhoWchi) {
+  if (int rc = need_species(c, "compute_rhoWchi")) return rc;
+  (void)rhotot;
+  const int ns = species_ns(c);
+  SP_FIELD(rho, LMM_CELL, ns, "compute_rhoWchi");
+  SP_FIELD(chi, LMM_CELL, ns * ns, "compute_rhoWchi");
+  SP_FIELD(rhoWchi, LMM_CELL, ns * ns, "compute_rhoWchi");
+  return sp_status(c, sp_rhowchi(c->stream, c->g, species_knobs(c, ns), rho->d, chi->d, rhoWchi->d), "compute_rhoWchi");
+}
+
+int lmm_compute_Lonsager(lmm_ctx *c, lmm_field *rho, lmm_field *rhotot, lmm_field *molarconc, lmm_field *molmtot,
+                         lmm_field *chi, lmm_field *Gama, lmm_field *Lonsager) {
+  if (int rc = need_species(c, "compute_Lonsager")) return rc;
+  (void)molarconc;
+  (void)Gama;
+  const int ns = species_ns(c);
+  SP_FIELD(rho, LMM_CELL, ns, "compute_Lonsager");
+  SP_FIELD(rhotot, LMM_CELL, 1, "compute_Lonsager");
+  SP_FIELD(molmtot, LMM_CELL, 1, "compute_Lonsager");
+  SP_FIELD(chi, LMM_CELL, ns * ns, "compute_Lonsager");
+  SP_FIELD(Lonsager, LMM_CELL, ns * ns, "compute_Lonsager");
+  return sp_status(c, sp_lonsager(c->stream, c->g, species_knobs(c, ns), rho->d, rhotot->d, molmtot->d, chi->d,
+                                  Lonsager->d),
+                   "compute_Lonsager");
+}
+
+// Arguments of diffusive_mass_fluxdiv common to the allocating and the
+// allocation-free entry: checks the fields and resolves the optional pieces.
+struct DiffArgs {
+  SpParams sp;
+  const double *temp = nullptr, *zeta = nullptr;
+  const double *gpb[3] = {nullptr, nullptr, nullptr};
+  bool baro = false;
+  double *ft[3] = {nullptr, nullptr, nullptr};
+};
+
+static int diff_args(lmm_ctx *c, int ns, lmm_field *rho, lmm_field *molarconc, lmm_field *rhoWchi, lmm_field *Gama,
+                     lmm_field *diff_fluxdiv, lmm_field *Temp, lmm_field *zeta_by_Temp, lmm_field *const gradp_baro[3],
+                     lmm_field *flux_total[3], int correct_flux, DiffArgs *a) {
+  if (ns < 1 || ns > SP_MAX) return set_err(c, LMM_ERR_UNSUPPORTED, "nspecies %d outside 1..%d", ns, SP_MAX);
+  if (c->species_set && ns != c->species.nspecies)
+    return set_err(c, LMM_ERR_ARG, "nspecies %d differs from the context's species knobs (%d)", ns,
+                   c->species.nspecies);
+  a->sp = species_knobs(c, ns);
+  a->sp.ns = ns;
+  a->sp.correct = correct_flux;
+  if (!field_ok(c, molarconc, LMM_CELL, ns) || !field_ok(c, rhoWchi, LMM_CELL, ns * ns) ||
+      !field_ok(c, Gama, LMM_CELL, ns * ns) || !field_ok(c, diff_fluxdiv, LMM_CELL, ns))
+    return set_err(c, LMM_ERR_ARG, "diffusive_mass_fluxdiv: cell fields have wrong components");
+  const int dim = c->g.dim;
+  for (int d = 0; d < dim; d++) {
+    if (!field_ok(c, flux_total[d], LMM_FACE_X + d, ns)) return set_err(c, LMM_ERR_ARG, "flux_total[%d] wrong", d);
+    a->ft[d] = flux_total[d]->d;
+  }
+  if ((Temp == nullptr) != (zeta_by_Temp == nullptr))
+    return set_err(c, LMM_ERR_ARG, "diffusive_mass_fluxdiv: Temp and zeta_by_Temp go together");
+  if (a->sp.noniso && Temp) {
+    if (!field_ok(c, Temp, LMM_CELL) || !field_ok(c, zeta_by_Temp, LMM_CELL, ns))
+      return set_err(c, LMM_ERR_ARG, "diffusive_mass_fluxdiv: Temp / zeta_by_Temp wrong");
+    a->temp = Temp->d;
+    a->zeta = zeta_by_Temp->d;
+  }
+  if (a->sp.baro > 0) {
+    if (dim != 2)
+      return set_err(c, LMM_ERR_UNSUPPORTED, "barodiffusion is 2D only (compute_baro_coef_3d is not written)");
+    if (!field_ok(c, rho, LMM_CELL, ns) || !field_ok(c, Temp, LMM_CELL) || !gradp_baro)
+      return set_err(c, LMM_ERR_ARG, "barodiffusion needs rho, Temp and gradp_baro");
+    for (int d = 0; d < dim; d++) {
+      if (!field_ok(c, gradp_baro[d], LMM_FACE_X + d)) return set_err(c, LMM_ERR_ARG, "gradp_baro[%d] wrong", d);
+      a->gpb[d] = gradp_baro[d]->d;
+    }
+    a->temp = Temp->d;
+    a->baro = true;
+  }
+  return LMM_OK;
+}
+
 int lmm_diffusive_mass_fluxdiv(lmm_ctx *c, int ns, lmm_field *rho, lmm_field *rhotot, lmm_field *molarconc,
                                lmm_field *rhoWchi, lmm_field *Gama, lmm_field *diff_fluxdiv, lmm_field *Temp,
                                lmm_field *zeta_by_Temp, lmm_field *const gradp_baro[3], lmm_field *flux_total[3],
                                int correct_flux) {
   if (!c) return LMM_ERR_ARG;
-  if (ns < 1 || ns > 8) return set_err(c, LMM_ERR_UNSUPPORTED, "nspecies %d outside 1..8", ns);
-  if (Temp || zeta_by_Temp || (gradp_baro && gradp_baro[0]))
-    return set_err(c, LMM_ERR_UNSUPPORTED, "non-isothermal and barodiffusion terms are not implemented");
-  (void)rho;
   (void)rhotot;
-  if (!field_ok(c, molarconc, LMM_CELL, ns) || !field_ok(c, rhoWchi, LMM_CELL, ns * ns) ||
-      !field_ok(c, Gama, LMM_CELL, ns * ns) || !field_ok(c, diff_fluxdiv, LMM_CELL, ns))
-    return set_err(c, LMM_ERR_ARG, "diffusive_mass_fluxdiv: cell fields have wrong components");
-  int dim = c->g.dim;
+  DiffArgs a;
+  if (int rc = diff_args(c, ns, rho, molarconc, rhoWchi, Gama, diff_fluxdiv, Temp, zeta_by_Temp, gradp_baro,
+                         flux_total, correct_flux, &a))
+    return rc;
+  const long N = c->g.N;
+  double *work = sp_scratch(c, (long)(c->g.dim + (a.baro ? 1 : 0)) * ns * N);
+  if (!work) return set_err(c, LMM_ERR_NOMEM, "species scratch");
+  double *baro = a.baro ? work + (long)c->g.dim * ns * N : nullptr;
+  return sp_status(c, sp_diffusive(c->stream, c->g, a.sp, a.baro ? rho->d : nullptr, molarconc->d, rhoWchi->d, Gama->d,
+                                   a.temp, a.zeta, a.baro ? a.gpb : nullptr, baro, work, a.ft, diff_fluxdiv->d),
+                   "diffusive_mass_fluxdiv");
+}
+
+int lmm_init_mass_stochastic(lmm_ctx *c, int n_rngs) {
+  if (int rc = need_species(c, "init_mass_stochastic")) return rc;
+  if (n_rngs < 1 || n_rngs > 8) return set_err(c, LMM_ERR_ARG, "n_rngs %d outside 1..8", n_rngs);
+  dfree(c->Wmass);
+  dfree(c->dweights_mass);
+  const long cnt = (long)n_rngs * c->g.dim * c->species.nspecies * c->g.N;
+  c->Wmass = dalloc<double>(c->stream, cnt);
+  c->dweights_mass = dalloc<double>(c->stream, 8);
+  c->n_rngs_mass = (c->Wmass && c->dweights_mass) ? n_rngs : 0;
+  if (!c->n_rngs_mass) return set_err(c, LMM_ERR_NOMEM, "mass noise allocation failed");
+  return finish(c);
+}
+
+static long mass_noise_count(const lmm_ctx *c) {
+  return (long)c->n_rngs_mass * c->g.dim * c->species.nspecies * c->g.N;
+}
+
+int lmm_fill_mass_stochastic(lmm_ctx *c, uint32_t seed) {
+  if (!c || c->n_rngs_mass < 1) return c ? set_err(c, LMM_ERR_ARG, "fill_mass_stochastic before init") : LMM_ERR_ARG;
+  std::vector<double> h((size_t)mass_noise_count(c));
+  lmm_fill_normal_host(seed, (int64_t)h.size(), h.data());
+  HIPCHK(c, hipMemcpyAsync(c->Wmass, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return finish(c);
+}
+
+int lmm_fill_mass_stochastic_device(lmm_ctx *c, uint64_t seed, uint64_t step) {
+  if (!c || c->n_rngs_mass < 1) return c ? set_err(c, LMM_ERR_ARG, "fill_mass_stochastic before init") : LMM_ERR_ARG;
+  launch_fill_normal_philox(c->stream, c->Wmass, mass_noise_count(c), 0, seed, step);
+  return finish(c);
+}
+
+// stochastic_mass_fluxdiv.f90:69: sqrt(2 k_B variance_coef_mass / (product(dx) dt))
+static double mass_variance(const lmm_ctx *c, double dt) {
+  double dv = 1.0;
+  for (int d = 0; d < c->g.dim; d++) dv = dv * c->g.dx;
+  return std::sqrt(2.0 * c->species.k_B * c->species.variance_coef_mass / (dv * dt));
+}
+
+static int upload_mass_weights(lmm_ctx *c, const double *weights) {
+  memcpy(c->hscal, weights, sizeof(double) * c->n_rngs_mass);
+  HIPCHK(c, hipMemcpyAsync(c->dweights_mass, c->hscal, sizeof(double) * c->n_rngs_mass, hipMemcpyHostToDevice,
+                           c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return LMM_OK;
+}
+
+int lmm_stochastic_mass_fluxdiv(lmm_ctx *c, lmm_field *rho, lmm_field *rhotot, lmm_field *molarconc, lmm_field *molmtot,
+                                lmm_field *chi, lmm_field *Gama, lmm_field *stoch_fluxdiv, lmm_field *flux_total[3],
+                                double dt, const double *weights, int increment) {
+  if (int rc = need_species(c, "stochastic_mass_fluxdiv")) return rc;
+  (void)molarconc;
+  (void)Gama;
+  const int ns = species_ns(c), dim = c->g.dim;
+  if (c->n_rngs_mass < 1 || !weights || !(dt > 0))
+    return set_err(c, LMM_ERR_ARG, "stochastic_mass_fluxdiv: noise not initialised, no weights or dt <= 0");
+  SP_FIELD(rho, LMM_CELL, ns, "stochastic_mass_fluxdiv");
+  SP_FIELD(rhotot, LMM_CELL, 1, "stochastic_mass_fluxdiv");
+  SP_FIELD(molmtot, LMM_CELL, 1, "stochastic_mass_fluxdiv");
+  SP_FIELD(chi, LMM_CELL, ns * ns, "stochastic_mass_fluxdiv");
+  SP_FIELD(stoch_fluxdiv, LMM_CELL, ns, "stochastic_mass_fluxdiv");
   double *ft[3] = {nullptr, nullptr, nullptr};
   for (int d = 0; d < dim; d++) {
     if (!field_ok(c, flux_total[d], LMM_FACE_X + d, ns)) return set_err(c, LMM_ERR_ARG, "flux_total[%d] wrong", d);
     ft[d] = flux_total[d]->d;
   }
-  double *flux = dalloc<double>(c->stream, (long)dim * ns * c->g.N);
-  if (!flux) return set_err(c, LMM_ERR_NOMEM, "flux scratch");
-  launch_diffusive_mass_flux(c->stream, c->g, ns, molarconc->d, rhoWchi->d, Gama->d, flux, ft, correct_flux);
-  launch_div_species(c->stream, c->g, ns, flux, diff_fluxdiv->d);
-  int rc = finish(c);
-  dfree(flux);
-  return rc;
+  const long N = c->g.N;
+  double *work = sp_scratch(c, (long)(ns * ns + dim * ns) * N);
+  if (!work) return set_err(c, LMM_ERR_NOMEM, "species scratch");
+  if (int rc = upload_mass_weights(c, weights)) return rc;
+  const SpParams sp = species_knobs(c, ns);
+  double *L = work, *flux = work + (long)ns * ns * N;
+  int rc = sp_lonsager(c->stream, c->g, sp, rho->d, rhotot->d, molmtot->d, chi->d, L);
+  if (!rc)
+    rc = sp_stochastic(c->stream, c->g, sp, L, c->Wmass, c->n_rngs_mass, c->dweights_mass, mass_variance(c, dt), flux,
+                       ft, stoch_fluxdiv->d, increment);
+  return sp_status(c, rc, "stochastic_mass_fluxdiv");
+}
+
+int lmm_compute_mass_fluxdiv(lmm_ctx *c, lmm_field *rho, lmm_field *const gradp_baro[3], lmm_field *diff_fluxdiv,
+                             lmm_field *stoch_fluxdiv, lmm_field *Temp, lmm_field *flux_total[3], double dt,
+                             double stage_time, const double *weights) {
+  if (int rc = need_species(c, "compute_mass_fluxdiv")) return rc;
+  (void)stage_time;
+  const int ns = species_ns(c), dim = c->g.dim;
+  const SpParams sp = species_knobs(c, ns);
+  SP_FIELD(rho, LMM_CELL, ns, "compute_mass_fluxdiv");
+  SP_FIELD(diff_fluxdiv, LMM_CELL, ns, "compute_mass_fluxdiv");
+  SP_FIELD(stoch_fluxdiv, LMM_CELL, ns, "compute_mass_fluxdiv");
+  double *ft[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < dim; d++) {
+    if (!field_ok(c, flux_total[d], LMM_FACE_X + d, ns)) return set_err(c, LMM_ERR_ARG, "flux_total[%d] wrong", d);
+    ft[d] = flux_total[d]->d;
+  }
+  if ((sp.noniso || sp.baro > 0) && !field_ok(c, Temp, LMM_CELL))
+    return set_err(c, LMM_ERR_ARG, "compute_mass_fluxdiv: Temp is required (non-isothermal or barodiffusion)");
+  const bool baro = sp.baro > 0;
+  SpFaceIn in{};
+  if (baro) {
+    if (dim != 2)
+      return set_err(c, LMM_ERR_UNSUPPORTED, "barodiffusion is 2D only (compute_baro_coef_3d is not written)");
+    for (int d = 0; d < dim; d++) {
+      if (!gradp_baro || !field_ok(c, gradp_baro[d], LMM_FACE_X + d))
+        return set_err(c, LMM_ERR_ARG, "gradp_baro[%d] missing or wrong", d);
+      in.gpb[d] = gradp_baro[d]->d;
+    }
+  }
+  const bool stoch = sp.vcm != 0.0;
+  if (stoch && (c->n_rngs_mass < 1 || !weights || !(dt > 0)))
+    return set_err(c, LMM_ERR_ARG, "compute_mass_fluxdiv: mass noise not initialised, no weights or dt <= 0");
+  // scratch: molarconc, rhoWchi, [Gama], [zeta], [L factor], [baro], face flux
+  const long N = c->g.N;
+  const long nlp = (long)ns * (ns + 1) / 2;
+  const long need = (long)(ns + ns * ns + (sp.ideal ? 0 : ns * ns) + (sp.noniso ? ns : 0) + (stoch ? nlp : 0) +
+                           (baro ? ns : 0) + dim * ns) * N;
+  double *w = sp_scratch(c, need);
+  if (!w) return set_err(c, LMM_ERR_NOMEM, "species scratch");
+  if (stoch)
+    if (int rc = upload_mass_weights(c, weights)) return rc;
+  SpCellOut o{};
+  o.rho = rho->d;
+  o.xm = w;
+  w += (long)ns * N;
+  o.rw = w;
+  w += (long)ns * ns * N;
+  if (!sp.ideal) {
+    o.gama = w;
+    w += (long)ns * ns * N;
+  }
+  if (sp.noniso) {
+    o.zeta = w;
+    w += (long)ns * N;
+  }
+  if (stoch) {
+    o.lpk = w;
+    w += nlp * N;
+  }
+  if (baro) {
+    o.baro = w;
+    w += (long)ns * N;
+  }
+  double *flux = w;
+  in.xm = o.xm;
+  in.rw = o.rw;
+  in.gama = o.gama;
+  in.zeta = o.zeta;
+  in.temp = Temp ? Temp->d : nullptr;
+  in.baro = o.baro;
+  in.lpk = o.lpk;
+  in.W = stoch ? c->Wmass : nullptr;
+  in.weights = c->dweights_mass;
+  in.n_rngs = c->n_rngs_mass;
+  in.variance = stoch ? mass_variance(c, dt) : 0.0;
+  int rc = sp_chain_cell(c->stream, c->g, sp, Temp ? Temp->d : nullptr, o);
+  if (!rc) rc = sp_chain_face(c->stream, c->g, sp, in, flux, diff_fluxdiv->d, stoch_fluxdiv->d, ft);
+  if (!rc && !stoch) launch_fill<double>(c->stream, stoch_fluxdiv->d, (long)ns * N, 0.0);
+  return sp_status(c, rc, "compute_mass_fluxdiv");
 }
 
 // ---------------- allocation-free species diffusion ----------------
@@ -2403,32 +2821,26 @@ int lmm_diffusive_mass_fluxdiv_noalloc(lmm_ctx *c, int ns, lmm_field *rho, lmm_f
                                        lmm_field *zeta_by_Temp, lmm_field *const gradp_baro[3],
                                        lmm_field *flux_total[3], int correct_flux) {
   if (!c) return LMM_ERR_ARG;
-  if (ns < 1 || ns > 8) return set_err(c, LMM_ERR_UNSUPPORTED, "nspecies %d outside 1..8", ns);
   if (Temp || zeta_by_Temp || (gradp_baro && gradp_baro[0]))
-    return set_err(c, LMM_ERR_UNSUPPORTED, "non-isothermal and barodiffusion terms are not implemented");
-  (void)rho;
+    return set_err(c, LMM_ERR_UNSUPPORTED, "diffusive_mass_fluxdiv_noalloc: isothermal, no barodiffusion");
   (void)rhotot;
-  if (!field_ok(c, molarconc, LMM_CELL, ns) || !field_ok(c, rhoWchi, LMM_CELL, ns * ns) ||
-      !field_ok(c, Gama, LMM_CELL, ns * ns) || !field_ok(c, diff_fluxdiv, LMM_CELL, ns))
-    return set_err(c, LMM_ERR_ARG, "diffusive_mass_fluxdiv: cell fields have wrong components");
-  const int dim = c->g.dim;
-  const long need = (long)dim * ns * c->g.N;
+  DiffArgs a;
+  if (int rc = diff_args(c, ns, rho, molarconc, rhoWchi, Gama, diff_fluxdiv, nullptr, nullptr, nullptr, flux_total,
+                         correct_flux, &a))
+    return rc;
+  if (a.baro) return set_err(c, LMM_ERR_UNSUPPORTED, "diffusive_mass_fluxdiv_noalloc: isothermal, no barodiffusion");
+  const long need = (long)c->g.dim * ns * c->g.N;
   if (need > NOALLOC_MAX)
     return set_err(c, LMM_ERR_ARG,
                    "diffusive_mass_fluxdiv_noalloc: dim*nspecies*cells = %ld exceeds the %ld-value device scratch",
                    need, NOALLOC_MAX);
-  double *ft[3] = {nullptr, nullptr, nullptr};
-  for (int d = 0; d < dim; d++) {
-    if (!field_ok(c, flux_total[d], LMM_FACE_X + d, ns)) return set_err(c, LMM_ERR_ARG, "flux_total[%d] wrong", d);
-    ft[d] = flux_total[d]->d;
-  }
   double *flux = nullptr;
   int rc = scratch_address(c, &flux);
   if (rc) return rc;
   // every flux value is written before the divergence reads it: nothing to clear
-  launch_diffusive_mass_flux(c->stream, c->g, ns, molarconc->d, rhoWchi->d, Gama->d, flux, ft, correct_flux);
-  launch_div_species(c->stream, c->g, ns, flux, diff_fluxdiv->d);
-  return finish(c);
+  return sp_status(c, sp_diffusive(c->stream, c->g, a.sp, nullptr, molarconc->d, rhoWchi->d, Gama->d, nullptr, nullptr,
+                                   nullptr, nullptr, flux, a.ft, diff_fluxdiv->d),
+                   "diffusive_mass_fluxdiv_noalloc");
 }
 
 // ---------------- learned coarse-grid correction ----------------

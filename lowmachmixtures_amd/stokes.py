@@ -71,6 +71,20 @@ class KernelOptions(C.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class SpeciesParams(C.Structure):
+    """lmm_species_params: probin_multispecies / probin_common species knobs."""
+    _fields_ = [
+        ("nspecies", C.c_int),
+        ("molmass", C.c_double * 8), ("rhobar", C.c_double * 8),
+        ("Dbar", C.c_double * 28), ("Dtherm", C.c_double * 8),
+        ("H_offdiag", C.c_double * 28), ("H_diag", C.c_double * 8),
+        ("k_B", C.c_double), ("variance_coef_mass", C.c_double), ("fraction_tolerance", C.c_double),
+        ("is_ideal_mixture", C.c_int), ("is_nonisothermal", C.c_int), ("use_lapack", C.c_int),
+        ("inverse_type", C.c_int), ("chi_iterations", C.c_int),
+        ("correct_flux", C.c_int), ("barodiffusion_type", C.c_int),
+    ]
+
+
 class GmresMember(C.Structure):
     """lmm_gmres_member: one solve of a batch."""
     _fields_ = [
@@ -157,6 +171,23 @@ _SIGS = {
                                 C.POINTER(C.c_float)]),
     "lmm_cnn_evaluate": (C.c_int, [C.c_void_p, C.c_int64, C.c_int64, C.POINTER(C.c_double)]),
     "lmm_ctx_set_cnn": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lmm_species_params_default": (None, [C.POINTER(SpeciesParams), C.c_int]),
+    "lmm_ctx_set_species": (C.c_int, [C.c_void_p, C.POINTER(SpeciesParams)]),
+    "lmm_ctx_get_species": (C.c_int, [C.c_void_p, C.POINTER(SpeciesParams)]),
+    "lmm_correct_rho_with_drho": (C.c_int, [C.c_void_p, _FP, _FP]),
+    "lmm_convert_cons_to_prim": (C.c_int, [C.c_void_p, _FP, _FP, _FP, _FP]),
+    "lmm_compute_mixture_properties": (C.c_int, [C.c_void_p, _FP, _FP, _FP, _FP, _FP, _FP]),
+    "lmm_compute_Gama": (C.c_int, [C.c_void_p, _FP, _FP, _FP, _FP, _FP, _FP]),
+    "lmm_compute_chi": (C.c_int, [C.c_void_p, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP]),
+    "lmm_compute_rhoWchi": (C.c_int, [C.c_void_p, _FP, _FP, _FP, _FP]),
+    "lmm_compute_Lonsager": (C.c_int, [C.c_void_p, _FP, _FP, _FP, _FP, _FP, _FP, _FP]),
+    "lmm_init_mass_stochastic": (C.c_int, [C.c_void_p, C.c_int]),
+    "lmm_fill_mass_stochastic": (C.c_int, [C.c_void_p, C.c_uint32]),
+    "lmm_fill_mass_stochastic_device": (C.c_int, [C.c_void_p, C.c_uint64, C.c_uint64]),
+    "lmm_stochastic_mass_fluxdiv": (C.c_int, [C.c_void_p, _FP, _FP, _FP, _FP, _FP, _FP, _FP, _FP * 3, C.c_double,
+                                              C.POINTER(C.c_double), C.c_int]),
+    "lmm_compute_mass_fluxdiv": (C.c_int, [C.c_void_p, _FP, _FP * 3, _FP, _FP, _FP, _FP * 3, C.c_double, C.c_double,
+                                           C.POINTER(C.c_double)]),
 }
 
 
@@ -494,9 +525,103 @@ def stoch_m_force(ctx, force, W_cc, W_ed, eta, eta_ed, temperature, temperature_
                                       _a3(temperature_ed), C.c_double(variance)), "stoch_m_force")
 
 
+def species_params(nspecies, **kw) -> SpeciesParams:
+    """Reference defaults for nspecies species; array knobs take a scalar or a sequence."""
+    p = SpeciesParams()
+    lib().lmm_species_params_default(C.byref(p), int(nspecies))
+    for k, v in kw.items():
+        field = getattr(p, k)
+        if isinstance(field, C.Array):
+            vals = [v] * len(field) if np.ndim(v) == 0 else list(v)
+            for i, x in enumerate(vals):
+                field[i] = float(x)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def set_species(ctx, nspecies, **kw):
+    """Sets the context's species knobs (lmm_ctx_set_species); returns them."""
+    p = species_params(nspecies, **kw)
+    ctx.check(lib().lmm_ctx_set_species(ctx.h, C.byref(p)), "ctx_set_species")
+    return p
+
+
+def correct_rho_with_drho(ctx, rho, drho):
+    """mass_flux_utilities.f90:26-150."""
+    ctx.check(lib().lmm_correct_rho_with_drho(ctx.h, _h(rho), _h(drho)), "correct_rho_with_drho")
+
+
+def convert_cons_to_prim(ctx, rho, rhotot, molarconc, molmtot):
+    """mass_flux_utilities.f90:152-313."""
+    ctx.check(lib().lmm_convert_cons_to_prim(ctx.h, _h(rho), _h(rhotot), _h(molarconc), _h(molmtot)),
+              "convert_cons_to_prim")
+
+
+def compute_mixture_properties(ctx, rho, rhotot, D_bar, D_therm, Hessian, Temp):
+    """compute_mixture_properties.f90:24-233 (constant coefficients)."""
+    ctx.check(lib().lmm_compute_mixture_properties(ctx.h, _h(rho), _h(rhotot), _h(D_bar), _h(D_therm), _h(Hessian),
+                                                   _h(Temp)), "compute_mixture_properties")
+
+
+def compute_Gama(ctx, rho, rhotot, molarconc, molmtot, Hessian, Gama):
+    """mass_flux_utilities.f90:428-599."""
+    ctx.check(lib().lmm_compute_Gama(ctx.h, _h(rho), _h(rhotot), _h(molarconc), _h(molmtot), _h(Hessian), _h(Gama)),
+              "compute_Gama")
+
+
+def compute_chi(ctx, rho, rhotot, molarconc, chi, D_bar, D_therm, Temp, zeta_by_Temp):
+    """mass_flux_utilities.f90:601-927."""
+    ctx.check(lib().lmm_compute_chi(ctx.h, _h(rho), _h(rhotot), _h(molarconc), _h(chi), _h(D_bar), _h(D_therm),
+                                    _h(Temp), _h(zeta_by_Temp)), "compute_chi")
+
+
+def compute_rhoWchi(ctx, rho, rhotot, chi, rhoWchi):
+    """mass_flux_utilities.f90:1120-1273."""
+    ctx.check(lib().lmm_compute_rhoWchi(ctx.h, _h(rho), _h(rhotot), _h(chi), _h(rhoWchi)), "compute_rhoWchi")
+
+
+def compute_Lonsager(ctx, rho, rhotot, molarconc, molmtot, chi, Gama, Lonsager):
+    """mass_flux_utilities.f90:929-1118."""
+    ctx.check(lib().lmm_compute_Lonsager(ctx.h, _h(rho), _h(rhotot), _h(molarconc), _h(molmtot), _h(chi), _h(Gama),
+                                         _h(Lonsager)), "compute_Lonsager")
+
+
+def init_mass_stochastic(ctx, n_rngs):
+    ctx.check(lib().lmm_init_mass_stochastic(ctx.h, int(n_rngs)), "init_mass_stochastic")
+
+
+def fill_mass_stochastic(ctx, seed):
+    ctx.check(lib().lmm_fill_mass_stochastic(ctx.h, C.c_uint32(seed)), "fill_mass_stochastic")
+
+
+def fill_mass_stochastic_device(ctx, seed, step=0):
+    ctx.check(lib().lmm_fill_mass_stochastic_device(ctx.h, C.c_uint64(seed), C.c_uint64(step)),
+              "fill_mass_stochastic_device")
+
+
+def stochastic_mass_fluxdiv(ctx, rho, rhotot, molarconc, molmtot, chi, Gama, stoch_fluxdiv, flux_total, dt, weights,
+                            increment=False):
+    """stochastic_mass_fluxdiv.f90:32-154."""
+    w = (C.c_double * 8)(*[float(x) for x in weights])
+    ctx.check(lib().lmm_stochastic_mass_fluxdiv(ctx.h, _h(rho), _h(rhotot), _h(molarconc), _h(molmtot), _h(chi),
+                                                _h(Gama), _h(stoch_fluxdiv), _a3(flux_total), C.c_double(dt), w,
+                                                int(increment)), "stochastic_mass_fluxdiv")
+
+
+def compute_mass_fluxdiv(ctx, rho, gradp_baro, diff_fluxdiv, stoch_fluxdiv, Temp, flux_total, dt, stage_time=0.0,
+                         weights=(1.0,)):
+    """compute_mass_fluxdiv.f90:24-147."""
+    w = (C.c_double * 8)(*[float(x) for x in weights])
+    ctx.check(lib().lmm_compute_mass_fluxdiv(ctx.h, _h(rho), _a3(gradp_baro), _h(diff_fluxdiv), _h(stoch_fluxdiv),
+                                             _h(Temp), _a3(flux_total), C.c_double(dt), C.c_double(stage_time), w),
+              "compute_mass_fluxdiv")
+
+
 def diffusive_mass_fluxdiv(ctx, nspecies, rho, rhotot, molarconc, rhoWchi, Gama, diff_fluxdiv, Temp,
                            zeta_by_Temp, gradp_baro, flux_total, correct_flux=True):
-    """diffusive_mass_fluxdiv.f90:26-42 (isothermal, no barodiffusion)."""
+    """diffusive_mass_fluxdiv.f90:26-42 (temperature piece with Temp and
+    zeta_by_Temp, barodiffusion per the context's species knobs)."""
     ctx.check(lib().lmm_diffusive_mass_fluxdiv(ctx.h, nspecies, _h(rho), _h(rhotot), _h(molarconc), _h(rhoWchi),
                                                _h(Gama), _h(diff_fluxdiv), _h(Temp), _h(zeta_by_Temp),
                                                _a3(gradp_baro), _a3(flux_total), int(correct_flux)),

@@ -151,7 +151,7 @@ int orc_diffusive_mass_fluxdiv(const orc_geom *g, int ns, const double *rho,
  * Knobs of probin_multispecies.f90:11-90 and probin_common.f90 (molmass,
  * rhobar, k_B, variance_coef_mass, barodiffusion_type).  Dbar and H_offdiag
  * list the strict lower triangle by rows: (2,1), (3,1), (3,2), (4,1), ...
- * (compute_mixture_properties.f90:176-186). */
+ * (compute_mixture_properties.f90:209-230). */
 #define ORC_MAX_SPECIES 8
 typedef struct {
   int nspecies;

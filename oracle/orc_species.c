@@ -5,20 +5,20 @@
  * fluxes of diffusive_mass_fluxdiv.f90 / stochastic_mass_fluxdiv.f90 as
  * composed by compute_mass_fluxdiv.f90:24-147.
  *
- *   orc_sp_correct_rho      <- mass_flux_utilities.f90:129-149 correct_rho_with_drho_local
- *   orc_sp_cons_to_prim     <- :265-296 compute_molconc_rhotot_local
- *   orc_sp_mixture          <- compute_mixture_properties.f90:146-200
+ *   orc_sp_correct_rho      <- mass_flux_utilities.f90:127-150 correct_rho_with_drho_local
+ *   orc_sp_cons_to_prim     <- :279-313 compute_molconc_rhotot_local
+ *   orc_sp_mixture          <- compute_mixture_properties.f90:165-233
  *                               mixture_properties_mass_local (constant D_bar,
  *                               D_therm, H; prob_type 9 is not restated)
- *   orc_sp_gama             <- mass_flux_utilities.f90:564-599 compute_Gama_local
- *   orc_sp_chi              <- :767-845 compute_chi_local, with
- *                               matrix_utilities.f90:21-154 Dbar2chi_iterative
- *                               (use_lapack = F) and :847-927 compute_chi_lapack
+ *   orc_sp_gama             <- mass_flux_utilities.f90:562-599 compute_Gama_local
+ *   orc_sp_chi              <- :767-831 compute_chi_local, with
+ *                               matrix_utilities.f90:21-156 Dbar2chi_iterative
+ *                               (use_lapack = F) and :833-927 compute_chi_lapack
  *                               (use_lapack = T)
- *   orc_sp_rhowchi          <- :1198-1218 compute_rhoWchi_local
- *   orc_sp_lonsager         <- :1067-1117 compute_Lonsager_local with
+ *   orc_sp_rhowchi          <- :1250-1273 compute_rhoWchi_local
+ *   orc_sp_lonsager         <- :1068-1118 compute_Lonsager_local with
  *                               matrix_utilities.f90:160-239 choldc (use_lapack = F)
- *   orc_sp_baro_coef        <- :1297-1368 compute_baro_coef_2d
+ *   orc_sp_baro_coef        <- :1342-1369 compute_baro_coef_2d
  *   orc_diffusive_mass_flux_full <- diffusive_mass_fluxdiv.f90:26-262 with the
  *                               temperature and barodiffusion pieces
  *   orc_stochastic_mass_fluxdiv <- stochastic_mass_fluxdiv.f90:32-154
@@ -113,7 +113,7 @@ void orc_sp_gama(const orc_species *sp, const double *molarconc, const double *H
     }
 }
 
-/* matrix_utilities.f90:21-154 */
+/* matrix_utilities.f90:21-156 */
 static void dbar2chi_iterative(const orc_species *sp, const double *D_bar, const double *Xk, double *chi) {
   int ns = sp->nspecies;
   double MW = 0.0, Y[MS], Di[MS], Mm[MS], Mi[MS];
@@ -264,7 +264,7 @@ void orc_sp_pinv(int ns, const double *A0, double tol, double *out) {
   }
 }
 
-/* mass_flux_utilities.f90:847-927 compute_chi_lapack */
+/* mass_flux_utilities.f90:833-927 compute_chi_lapack */
 static void chi_lapack(const orc_species *sp, const double *Lambda, const double *W, double *chi) {
   int ns = sp->nspecies;
   double alpha = 0.0, A[MS * MS];
@@ -527,7 +527,7 @@ static long nb(const orc_geom *g, long t, int d, int s) {
 
 /* diffusive_mass_fluxdiv.f90:92-262 on one face: grad x (compute_grad),
  * Gama_face * grad x, + zeta_face * grad T, + baro_face * gradp_baro,
- * rhoWchi_face * (...), then correction_flux.f90:71-143.  Fields are
+ * rhoWchi_face * (...), then correction_flux.f90:71-251.  Fields are
  * component-major with N cells; (t, m) are the cells above and below the face. */
 static void diff_face(const orc_species *sp, long N, double dxinv, long t, long m, const double *molarconc,
                       const double *rhoWchi, const double *Gama, const double *Temp, const double *zeta,
@@ -591,7 +591,7 @@ int orc_diffusive_mass_flux_full(const orc_geom *g, const orc_species *sp, const
   const double *zeta = (sp->is_nonisothermal && Temp) ? zeta_by_Temp : NULL;
   double *baro = NULL;
   if (sp->barodiffusion_type > 0) {
-    if (dm != 2 || !gradp_baro || !Temp) return 2; /* compute_baro_coef is 2D only (:1340-1342) */
+    if (dm != 2 || !gradp_baro || !Temp) return 2; /* compute_baro_coef is 2D only (mass_flux_utilities.f90:1291-1340) */
     baro = malloc(sizeof(double) * ns * N);
     for (long t = 0; t < N; t++) {
       double r[MS], b[MS];

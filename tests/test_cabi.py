@@ -19,7 +19,7 @@ def declared():
     src = open(HDR).read()
     src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
     src = re.sub(r"//[^\n]*", "", src)
-    return sorted(set(re.findall(r"\b(lmm_[a-z0-9_]+)\s*\(", src)))
+    return sorted(set(re.findall(r"\b(lmm_[A-Za-z0-9_]+)\s*\(", src)))
 
 
 def test_library_exports_every_declared_entry_point():

@@ -2,14 +2,14 @@
 the mathematics fixes, independent of the oracle's own loops:
 
 * convert_cons_to_prim: mole fractions sum to one; equal molar masses give
-  mole fraction = mass fraction (mass_flux_utilities.f90:265-296);
+  mole fraction = mass fraction (mass_flux_utilities.f90:279-313);
 * compute_mixture_properties: the Dbar / H_offdiag input order is the
-  strict lower triangle by rows (compute_mixture_properties.f90:176-186);
+  strict lower triangle by rows (compute_mixture_properties.f90:209-230);
 * compute_Gama: ideal mixtures give the identity exactly; otherwise
   I + (diag(x) - x x^T) H (mass_flux_utilities.f90:564-599);
-* compute_chi: the Dbar2chi_iterative series (matrix_utilities.f90:21-154)
+* compute_chi: the Dbar2chi_iterative series (matrix_utilities.f90:21-156)
   converges to the closed form (Lambda + alpha W W^T)^-1 - 1/alpha of
-  compute_chi_lapack (:847-927); chi W = 0; a binary mixture gives Fick's
+  compute_chi_lapack (:833-927); chi W = 0; a binary mixture gives Fick's
   law w_1 (chi_11 - chi_12) = D_12; the 10-term series equals the same
   series written as numpy matrix products;
 * compute_Lonsager: the Cholesky factor reproduces
