@@ -83,6 +83,7 @@ def parse():
     ap.add_argument("--train-steps", type=int, default=1500)
     ap.add_argument("--ensemble", type=int, default=32, help="members per size of the batched 2D ensemble (0: skip)")
     ap.add_argument("--cpu-runs", type=int, default=3)
+    ap.add_argument("--species-n", type=int, default=128, help="box of the species mass-flux line (0: skip)")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -265,6 +266,73 @@ def ensemble(P, S, per_size, steps, quiet):
             ctx.close()
 
 
+def species_chain(S, n, ns, steps, cpu, quiet):
+    """compute_mass_fluxdiv (the species mass-flux chain) on a 3D n^3 periodic box
+    with ns species: time per call, algorithmic bytes per call, the oracle's time
+    on the same inputs and the relative difference of the results."""
+    shape = (n, n, n)
+    rng = np.random.default_rng(5)
+    knobs = dict(molmass=[1.0, 2.0, 3.0, 4.0, 1, 1, 1, 1], Dbar=[1.0, 0.8, 1.2, 0.9, 1.1, 0.7] + [1.0] * 22,
+                 Dtherm=[0.1, -0.1, 0.2, 0.0, 0, 0, 0, 0], k_B=1e-3)
+    grid = np.meshgrid(*[np.arange(k) for k in shape], indexing="ij")
+    rho = np.stack([1.0 + 0.5 * np.sin(2 * np.pi * (s + 1) * grid[2] / n + s) * np.cos(2 * np.pi * grid[0] / n)
+                    for s in range(ns)]) * np.array([0.4, 0.3, 0.2, 0.1][:ns])[:, None, None, None]
+    rho = rho + 0.01 * rng.random(rho.shape)
+    T = 1.0 + 0.05 * np.cos(2 * np.pi * grid[1] / n)
+    n_rngs, dt, weights = 2, 1e-4, [1.0, 0.5]
+    out = {"workload": f"compute_mass_fluxdiv, 3D periodic {n}^3, {ns} species, non-isothermal (Dtherm), "
+                       f"ideal mixture, chi by 10 series terms, stochastic flux from {n_rngs} noise stages",
+           "cells": n ** 3}
+    with S.Context(3, shape, 1.0 / n) as ctx:
+        S.set_species(ctx, ns, **knobs)
+        S.init_mass_stochastic(ctx, n_rngs)
+        S.fill_mass_stochastic_device(ctx, 7, 0)
+        f_rho = ctx.field(S.CELL, ns, rho)
+        f_T = ctx.field(S.CELL, 1, T)
+        diff, stoch = ctx.field(S.CELL, ns), ctx.field(S.CELL, ns)
+        ft = [ctx.field(S.FACE_X + d, ns) for d in range(3)]
+        for _ in range(2):
+            S.compute_mass_fluxdiv(ctx, f_rho, None, diff, stoch, f_T, ft, dt, 0.0, weights)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            S.compute_mass_fluxdiv(ctx, f_rho, None, diff, stoch, f_T, ft, dt, 0.0, weights)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) / steps * 1e3
+        ctx.reset_stats()
+        ctx.profiling(1)
+        f_rho.upload(rho)
+        S.compute_mass_fluxdiv(ctx, f_rho, None, diff, stoch, f_T, ft, dt, 0.0, weights)
+        ctx.synchronize()
+        ctx.profiling(False)
+        st = {k: ctx.kernel_stats(k) for k in ("species_cell", "species_face")}
+        gd, gs = diff.download(), stoch.download()
+    alg = sum(v["bytes"] for v in st.values())
+    out.update({
+        "ms_per_call": round(ms, 4), "cells_per_s": round(n ** 3 / (ms * 1e-3), 1),
+        "algorithmic_bytes_per_call": alg, "achieved_GBs": round(alg / (ms * 1e-3) / 1e9, 1),
+        "frac_hbm": round(alg / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+        "passes_ms": {k: round(v["ms"], 4) for k, v in st.items()},
+    })
+    log(f"species chain {n}^3 x {ns}: {ms:.3f} ms per call, {out['achieved_GBs']} GB/s algorithmic", quiet)
+    if cpu:
+        from oracle import oracle as O
+
+        sp = O.species(ns, **knobs)
+        g = O.geom(3, shape, 1.0 / n)
+        W = O.fill_normal_philox(7, 0, 0, n_rngs * 3 * ns * n ** 3).reshape((n_rngs, 3, ns) + shape)
+        t0 = time.perf_counter()
+        _, od, osd, _ = O.compute_mass_fluxdiv(g, sp, rho, T, dt, W, weights)
+        sec = time.perf_counter() - t0
+        out["cpu_oracle"] = {"s_per_call": round(sec, 3), "cores": 1, "kind": "port",
+                             "sample": f"oracle compute_mass_fluxdiv on the same {n}^3 inputs, one call, one thread"}
+        # the device's Box-Muller log/cos/sin may differ from the host's in the last bits
+        rd = lambda a_, b_: float(np.linalg.norm(a_ - b_) / max(np.linalg.norm(b_), 1e-300))
+        out["rel_diff_vs_oracle"] = {"diff_fluxdiv": rd(gd, od), "stoch_fluxdiv": rd(gs, osd)}
+        log(f"species chain oracle: {sec:.2f} s per call; rel diff {out['rel_diff_vs_oracle']}", quiet)
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -387,6 +455,9 @@ def main():
 
     if a.ensemble > 0:
         out["ensemble"] = ensemble(P, S, a.ensemble, 3, a.quiet)
+
+    if a.species_n > 0:
+        out["species_mass_flux"] = species_chain(S, a.species_n, 4, 20, a.cpu_n > 0, a.quiet)
 
     if a.cpu_n > 0:
         from oracle import oracle as O

@@ -263,10 +263,25 @@ int lmm_cc_inner_prod(lmm_ctx *ctx, lmm_field *m1, lmm_field *m2, double *prod_v
  * div_and_grad.f90:16-19 compute_grad (periodic box). */
 int lmm_compute_div(lmm_ctx *ctx, lmm_field *const phi_fc[3], lmm_field *div, int increment);
 int lmm_compute_grad(lmm_ctx *ctx, lmm_field *phi, lmm_field *gradp[3]);
+/* The same with the reference's component arguments (0-based here):
+ * div_and_grad.f90:304 compute_div(mla,phi_fc,div,dx,start_incomp,start_outcomp,num_comp,
+ * increment_in) and :16-18 compute_grad(mla,phi,gradp,dx,start_incomp,start_bccomp,
+ * start_outcomp,num_comp,the_bc_level) -- on the periodic box the boundary
+ * component is unused; the wall-face 2/dx stencils (:139-155) need walls,
+ * which this data model does not carry. */
+int lmm_compute_div_comps(lmm_ctx *ctx, lmm_field *const phi_fc[3], lmm_field *div, int start_incomp,
+                          int start_outcomp, int num_comp, int increment);
+int lmm_compute_grad_comps(lmm_ctx *ctx, lmm_field *phi, lmm_field *gradp[3], int start_incomp,
+                           int start_outcomp, int num_comp);
 
 /* convert_stag.f90:132 average_cc_to_face, :395 average_cc_to_node (2D), :612
  * average_cc_to_edge (3D) for component 0 of a cell field. */
 int lmm_average_cc_to_face(lmm_ctx *ctx, lmm_field *cc, lmm_field *face[3]);
+/* convert_stag.f90:132 average_cc_to_face(nlevs,cc,face,start_scomp,start_bccomp,num_comp,...):
+ * components [start_comp, start_comp + num_comp) of cc to the same components
+ * of the faces (0-based). */
+int lmm_average_cc_to_face_comps(lmm_ctx *ctx, lmm_field *cc, lmm_field *face[3], int start_comp,
+                                 int num_comp);
 int lmm_average_cc_to_node(lmm_ctx *ctx, lmm_field *cc, lmm_field *node);
 int lmm_average_cc_to_edge(lmm_ctx *ctx, lmm_field *cc, lmm_field *edge[3]);
 

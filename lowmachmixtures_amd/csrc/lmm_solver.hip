@@ -162,6 +162,8 @@ struct lmm_ctx {
   double *dweights_mass = nullptr;
   double *sp_work = nullptr;  // species scratch, grown on demand
   long sp_work_n = 0;
+  double hweights_mass[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // weights last copied to dweights_mass
+  bool weights_mass_valid = false;
 };
 
 // A network bound to a context: the correction acts on staggered level
@@ -2110,6 +2112,62 @@ int lmm_compute_grad(lmm_ctx *c, lmm_field *phi, lmm_field *gradp[3]) {
   return finish(c);
 }
 
+// Multi-component forms: component q of the input to component q of the
+// output, q over [start, start + num_comp) (0-based).
+static bool comps_ok(const lmm_field *f, int start, int num) {
+  return start >= 0 && num >= 1 && start + num <= f->ncomp;
+}
+
+int lmm_compute_div_comps(lmm_ctx *c, lmm_field *const phi_fc[3], lmm_field *div, int start_incomp,
+                          int start_outcomp, int num_comp, int increment) {
+  if (!c) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, phi_fc, div, "compute_div"))) return rc;
+  for (int d = 0; d < c->g.dim; d++)
+    if (!comps_ok(phi_fc[d], start_incomp, num_comp)) return set_err(c, LMM_ERR_ARG, "compute_div: input components");
+  if (!comps_ok(div, start_outcomp, num_comp)) return set_err(c, LMM_ERR_ARG, "compute_div: output components");
+  const long N = c->g.N;
+  for (int q = 0; q < num_comp; q++) {
+    double *u[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < c->g.dim; d++) u[d] = phi_fc[d]->d + (long)(start_incomp + q) * N;
+    launch_div_plus<double>(c->stream, c->g, u, nullptr, div->d + (long)(start_outcomp + q) * N, increment ? 3 : 0);
+  }
+  return finish(c);
+}
+
+int lmm_compute_grad_comps(lmm_ctx *c, lmm_field *phi, lmm_field *gradp[3], int start_incomp, int start_outcomp,
+                           int num_comp) {
+  if (!c) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, gradp, phi, "compute_grad"))) return rc;
+  if (!comps_ok(phi, start_incomp, num_comp)) return set_err(c, LMM_ERR_ARG, "compute_grad: input components");
+  for (int d = 0; d < c->g.dim; d++)
+    if (!comps_ok(gradp[d], start_outcomp, num_comp)) return set_err(c, LMM_ERR_ARG, "compute_grad: output components");
+  const long N = c->g.N;
+  for (int q = 0; q < num_comp; q++) {
+    double *gp[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < c->g.dim; d++) gp[d] = gradp[d]->d + (long)(start_outcomp + q) * N;
+    launch_grad(c->stream, c->g, phi->d + (long)(start_incomp + q) * N, gp);
+  }
+  return finish(c);
+}
+
+int lmm_average_cc_to_face_comps(lmm_ctx *c, lmm_field *cc, lmm_field *face[3], int start_comp, int num_comp) {
+  if (!c) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, face, cc, "average_cc_to_face"))) return rc;
+  if (!comps_ok(cc, start_comp, num_comp)) return set_err(c, LMM_ERR_ARG, "average_cc_to_face: components");
+  for (int d = 0; d < c->g.dim; d++)
+    if (!comps_ok(face[d], start_comp, num_comp)) return set_err(c, LMM_ERR_ARG, "average_cc_to_face: components");
+  const long N = c->g.N;
+  for (int q = 0; q < num_comp; q++) {
+    double *f[3] = {nullptr, nullptr, nullptr};
+    for (int d = 0; d < c->g.dim; d++) f[d] = face[d]->d + (long)(start_comp + q) * N;
+    launch_avg_cc_to_face(c->stream, c->g, cc->d + (long)(start_comp + q) * N, f);
+  }
+  return finish(c);
+}
+
 int lmm_average_cc_to_face(lmm_ctx *c, lmm_field *cc, lmm_field *face[3]) {
   if (!c) return LMM_ERR_ARG;
   int rc;
@@ -2634,6 +2692,7 @@ int lmm_init_mass_stochastic(lmm_ctx *c, int n_rngs) {
   c->Wmass = dalloc<double>(c->stream, cnt);
   c->dweights_mass = dalloc<double>(c->stream, 8);
   c->n_rngs_mass = (c->Wmass && c->dweights_mass) ? n_rngs : 0;
+  c->weights_mass_valid = false;
   if (!c->n_rngs_mass) return set_err(c, LMM_ERR_NOMEM, "mass noise allocation failed");
   return finish(c);
 }
@@ -2665,6 +2724,9 @@ static double mass_variance(const lmm_ctx *c, double dt) {
 }
 
 static int upload_mass_weights(lmm_ctx *c, const double *weights) {
+  if (c->weights_mass_valid && !memcmp(c->hweights_mass, weights, sizeof(double) * c->n_rngs_mass)) return LMM_OK;
+  memcpy(c->hweights_mass, weights, sizeof(double) * c->n_rngs_mass);
+  c->weights_mass_valid = true;
   memcpy(c->hscal, weights, sizeof(double) * c->n_rngs_mass);
   HIPCHK(c, hipMemcpyAsync(c->dweights_mass, c->hscal, sizeof(double) * c->n_rngs_mass, hipMemcpyHostToDevice,
                            c->stream));
@@ -2778,8 +2840,20 @@ int lmm_compute_mass_fluxdiv(lmm_ctx *c, lmm_field *rho, lmm_field *const gradp_
   in.weights = c->dweights_mass;
   in.n_rngs = c->n_rngs_mass;
   in.variance = stoch ? mass_variance(c, dt) : 0.0;
-  int rc = sp_chain_cell(c->stream, c->g, sp, Temp ? Temp->d : nullptr, o);
-  if (!rc) rc = sp_chain_face(c->stream, c->g, sp, in, flux, diff_fluxdiv->d, stoch_fluxdiv->d, ft);
+  // algorithmic bytes (each input read once, each output written once):
+  // the cell pass reads rho and Temp and writes rho back; the face passes
+  // read the noise and write both divergences and flux_total
+  const double bN = 8.0 * (double)N;
+  int rc;
+  {
+    ProfScope ps(c, "species_cell", bN * (2 * ns + ((sp.noniso || baro) ? 1 : 0)));
+    rc = sp_chain_cell(c->stream, c->g, sp, Temp ? Temp->d : nullptr, o);
+  }
+  if (!rc) {
+    ProfScope ps(c, "species_face", bN * ((stoch ? c->n_rngs_mass * dim * ns : 0) + 2 * ns + dim * ns +
+                                          (baro ? dim : 0)));
+    rc = sp_chain_face(c->stream, c->g, sp, in, flux, diff_fluxdiv->d, stoch_fluxdiv->d, ft);
+  }
   if (!rc && !stoch) launch_fill<double>(c->stream, stoch_fluxdiv->d, (long)ns * N, 0.0);
   return sp_status(c, rc, "compute_mass_fluxdiv");
 }

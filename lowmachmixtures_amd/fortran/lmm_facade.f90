@@ -63,14 +63,28 @@ module lmm_facade
      integer(c_int) :: learned_applied, learned_fallback
   end type lmm_gmres_info
 
+  ! lmm_species_params: the probin_multispecies knobs (probin_multispecies.f90:
+  ! 11-90) and probin_common's molmass, rhobar, k_B, variance_coef_mass,
+  ! barodiffusion_type
+  type, bind(c), public :: lmm_species_params
+     integer(c_int) :: nspecies
+     real(c_double) :: molmass(8), rhobar(8)
+     real(c_double) :: dbar(28), dtherm(8), h_offdiag(28), h_diag(8)
+     real(c_double) :: k_b, variance_coef_mass, fraction_tolerance
+     integer(c_int) :: is_ideal_mixture, is_nonisothermal, use_lapack, inverse_type, chi_iterations
+     integer(c_int) :: correct_flux, barodiffusion_type
+  end type lmm_species_params
+
   public :: lmm_params_default, lmm_ctx_create, lmm_ctx_destroy, lmm_ctx_set_params, lmm_ctx_last_error, &
        lmm_field_create, lmm_field_destroy, lmm_field_upload_fab, lmm_field_download_fab, lmm_fill_normal_host, &
        lmm_gmres, lmm_apply_matrix, lmm_apply_precon, lmm_stag_mg_solver, lmm_macproject, &
        lmm_init_m_stochastic, lmm_fill_m_stochastic, lmm_stochastic_m_fluxdiv, lmm_diffusive_mass_fluxdiv, &
-       lmm_diffusive_m_fluxdiv
+       lmm_diffusive_m_fluxdiv, lmm_species_params_default, lmm_ctx_set_species, lmm_init_mass_stochastic, &
+       lmm_fill_mass_stochastic, lmm_stochastic_mass_fluxdiv, lmm_compute_mass_fluxdiv, lmm_compute_div_comps
   public :: lmm_check, lmm_error_message, lmm_fab_in, lmm_fab_out
   public :: gmres, apply_matrix, apply_precon, stag_mg_solver, macproject, stochastic_m_fluxdiv, &
-       diffusive_mass_fluxdiv, diffusive_m_fluxdiv
+       diffusive_mass_fluxdiv, diffusive_m_fluxdiv, compute_mass_fluxdiv, stochastic_mass_fluxdiv, &
+       init_mass_stochastic, fill_mass_stochastic, compute_div
 
   interface
      subroutine lmm_params_default(p) bind(c, name='lmm_params_default')
@@ -216,6 +230,58 @@ module lmm_facade
        type(c_ptr), intent(in) :: m_update(3), umac(3), eta_ed(3)
      end function lmm_diffusive_m_fluxdiv
 
+     subroutine lmm_species_params_default(p, nspecies) bind(c, name='lmm_species_params_default')
+       import :: lmm_species_params, c_int
+       type(lmm_species_params), intent(out) :: p
+       integer(c_int), value :: nspecies
+     end subroutine lmm_species_params_default
+
+     integer(c_int) function lmm_ctx_set_species(ctx, p) bind(c, name='lmm_ctx_set_species')
+       import :: c_int, c_ptr, lmm_species_params
+       type(c_ptr), value :: ctx
+       type(lmm_species_params), intent(in) :: p
+     end function lmm_ctx_set_species
+
+     integer(c_int) function lmm_init_mass_stochastic(ctx, n_rngs) bind(c, name='lmm_init_mass_stochastic')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int), value :: n_rngs
+     end function lmm_init_mass_stochastic
+
+     integer(c_int) function lmm_fill_mass_stochastic(ctx, seed) bind(c, name='lmm_fill_mass_stochastic')
+       import :: c_int, c_int32_t, c_ptr
+       type(c_ptr), value :: ctx
+       integer(c_int32_t), value :: seed
+     end function lmm_fill_mass_stochastic
+
+     integer(c_int) function lmm_stochastic_mass_fluxdiv(ctx, rho, rhotot, molarconc, molmtot, chi, gama, &
+          stoch_fluxdiv, flux_total, dt, weights, increment) bind(c, name='lmm_stochastic_mass_fluxdiv')
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx, rho, rhotot, molarconc, molmtot, chi, gama, stoch_fluxdiv
+       type(c_ptr), intent(in) :: flux_total(3)
+       real(c_double), value :: dt
+       real(c_double), intent(in) :: weights(*)
+       integer(c_int), value :: increment
+     end function lmm_stochastic_mass_fluxdiv
+
+     integer(c_int) function lmm_compute_mass_fluxdiv(ctx, rho, gradp_baro, diff_fluxdiv, stoch_fluxdiv, temp, &
+          flux_total, dt, stage_time, weights) bind(c, name='lmm_compute_mass_fluxdiv')
+       import :: c_int, c_double, c_ptr
+       type(c_ptr), value :: ctx, rho, diff_fluxdiv, stoch_fluxdiv, temp
+       type(c_ptr), value :: gradp_baro
+       type(c_ptr), intent(in) :: flux_total(3)
+       real(c_double), value :: dt, stage_time
+       real(c_double), intent(in) :: weights(*)
+     end function lmm_compute_mass_fluxdiv
+
+     integer(c_int) function lmm_compute_div_comps(ctx, phi_fc, div, start_incomp, start_outcomp, num_comp, &
+          increment) bind(c, name='lmm_compute_div_comps')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, div
+       type(c_ptr), intent(in) :: phi_fc(3)
+       integer(c_int), value :: start_incomp, start_outcomp, num_comp, increment
+     end function lmm_compute_div_comps
+
      integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
        import :: c_size_t, c_ptr
        type(c_ptr), value :: s
@@ -325,14 +391,76 @@ contains
          weights, variance_coef_mom, k_b), 'stochastic_m_fluxdiv')
   end subroutine stochastic_m_fluxdiv
 
-  ! diffusive_mass_fluxdiv.f90:26-42 (isothermal, no barodiffusion: Temp,
-  ! zeta_by_Temp and gradp_baro absent)
-  subroutine diffusive_mass_fluxdiv(ctx, nspecies, rho, rhotot, molarconc, rhowchi, gama, diff_fluxdiv, flux_total)
+  ! diffusive_mass_fluxdiv.f90:26-42; temp and zeta_by_temp give the
+  ! temperature piece (non-isothermal knobs), gradp_baro (a c_loc of three
+  ! face handles) the barodiffusion piece; correct_flux is the context's
+  subroutine diffusive_mass_fluxdiv(ctx, nspecies, rho, rhotot, molarconc, rhowchi, gama, diff_fluxdiv, flux_total, &
+       temp, zeta_by_temp, gradp_baro)
     type(c_ptr), intent(in) :: ctx, rho, rhotot, molarconc, rhowchi, gama, diff_fluxdiv, flux_total(3)
     integer, intent(in) :: nspecies
+    type(c_ptr), intent(in), optional :: temp, zeta_by_temp, gradp_baro
+    type(c_ptr) :: t, z, g
+    t = c_null_ptr
+    z = c_null_ptr
+    g = c_null_ptr
+    if (present(temp)) t = temp
+    if (present(zeta_by_temp)) z = zeta_by_temp
+    if (present(gradp_baro)) g = gradp_baro
     call lmm_check(ctx, lmm_diffusive_mass_fluxdiv(ctx, int(nspecies, c_int), rho, rhotot, molarconc, rhowchi, gama, &
-         diff_fluxdiv, c_null_ptr, c_null_ptr, c_null_ptr, flux_total, 1_c_int), 'diffusive_mass_fluxdiv')
+         diff_fluxdiv, t, z, g, flux_total, 1_c_int), 'diffusive_mass_fluxdiv')
   end subroutine diffusive_mass_fluxdiv
+
+  ! stochastic_mass_fluxdiv.f90:158-188 / :219-246 (periodic: no wall conditions)
+  subroutine init_mass_stochastic(ctx, n_rngs)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: n_rngs
+    call lmm_check(ctx, lmm_init_mass_stochastic(ctx, int(n_rngs, c_int)), 'init_mass_stochastic')
+  end subroutine init_mass_stochastic
+
+  subroutine fill_mass_stochastic(ctx, seed)
+    type(c_ptr), intent(in) :: ctx
+    integer, intent(in) :: seed
+    call lmm_check(ctx, lmm_fill_mass_stochastic(ctx, int(seed, c_int32_t)), 'fill_mass_stochastic')
+  end subroutine fill_mass_stochastic
+
+  ! stochastic_mass_fluxdiv.f90:32-154 (dx is the context's)
+  subroutine stochastic_mass_fluxdiv(ctx, rho, rhotot, molarconc, molmtot, chi, gama, stoch_fluxdiv, flux_total, dt, &
+       weights, increment_in)
+    type(c_ptr), intent(in) :: ctx, rho, rhotot, molarconc, molmtot, chi, gama, stoch_fluxdiv, flux_total(3)
+    real(c_double), intent(in) :: dt, weights(:)
+    logical, intent(in), optional :: increment_in
+    integer(c_int) :: inc
+    inc = 0
+    if (present(increment_in)) then
+       if (increment_in) inc = 1
+    end if
+    call lmm_check(ctx, lmm_stochastic_mass_fluxdiv(ctx, rho, rhotot, molarconc, molmtot, chi, gama, stoch_fluxdiv, &
+         flux_total, dt, weights, inc), 'stochastic_mass_fluxdiv')
+  end subroutine stochastic_mass_fluxdiv
+
+  ! compute_mass_fluxdiv.f90:24-38 (dx and the bc tower are the context's)
+  subroutine compute_mass_fluxdiv(ctx, rho, gradp_baro, diff_fluxdiv, stoch_fluxdiv, temp, flux_total, dt, &
+       stage_time, weights)
+    type(c_ptr), intent(in) :: ctx, rho, gradp_baro, diff_fluxdiv, stoch_fluxdiv, temp, flux_total(3)
+    real(c_double), intent(in) :: dt, stage_time, weights(:)
+    call lmm_check(ctx, lmm_compute_mass_fluxdiv(ctx, rho, gradp_baro, diff_fluxdiv, stoch_fluxdiv, temp, flux_total, &
+         dt, stage_time, weights), 'compute_mass_fluxdiv')
+  end subroutine compute_mass_fluxdiv
+
+  ! div_and_grad.f90:304 compute_div(mla,phi_fc,div,dx,start_incomp,start_outcomp,num_comp,increment_in),
+  ! components 1-based as in the reference
+  subroutine compute_div(ctx, phi_fc, div, start_incomp, start_outcomp, num_comp, increment_in)
+    type(c_ptr), intent(in) :: ctx, phi_fc(3), div
+    integer, intent(in) :: start_incomp, start_outcomp, num_comp
+    logical, intent(in), optional :: increment_in
+    integer(c_int) :: inc
+    inc = 0
+    if (present(increment_in)) then
+       if (increment_in) inc = 1
+    end if
+    call lmm_check(ctx, lmm_compute_div_comps(ctx, phi_fc, div, int(start_incomp - 1, c_int), &
+         int(start_outcomp - 1, c_int), int(num_comp, c_int), inc), 'compute_div')
+  end subroutine compute_div
 
   ! diffusive_m_fluxdiv.f90:16-17
   subroutine diffusive_m_fluxdiv(ctx, m_update, umac, eta, eta_ed, kappa)

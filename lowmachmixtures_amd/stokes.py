@@ -137,6 +137,9 @@ _SIGS = {
     "lmm_cc_inner_prod": (C.c_int, [C.c_void_p, _FP, _FP, C.POINTER(C.c_double)]),
     "lmm_compute_div": (C.c_int, [C.c_void_p, _FP * 3, _FP, C.c_int]),
     "lmm_compute_grad": (C.c_int, [C.c_void_p, _FP, _FP * 3]),
+    "lmm_compute_div_comps": (C.c_int, [C.c_void_p, _FP * 3, _FP, C.c_int, C.c_int, C.c_int, C.c_int]),
+    "lmm_compute_grad_comps": (C.c_int, [C.c_void_p, _FP, _FP * 3, C.c_int, C.c_int, C.c_int]),
+    "lmm_average_cc_to_face_comps": (C.c_int, [C.c_void_p, _FP, _FP * 3, C.c_int, C.c_int]),
     "lmm_average_cc_to_face": (C.c_int, [C.c_void_p, _FP, _FP * 3]),
     "lmm_average_cc_to_node": (C.c_int, [C.c_void_p, _FP, _FP]),
     "lmm_average_cc_to_edge": (C.c_int, [C.c_void_p, _FP, _FP * 3]),
@@ -482,6 +485,24 @@ def compute_div(ctx, phi_fc, div, increment=False):
 
 def compute_grad(ctx, phi, gradp):
     ctx.check(lib().lmm_compute_grad(ctx.h, _h(phi), _a3(gradp)), "compute_grad")
+
+
+def compute_div_comps(ctx, phi_fc, div, start_incomp, start_outcomp, num_comp, increment=False):
+    """div_and_grad.f90:304 with component arguments (0-based)."""
+    ctx.check(lib().lmm_compute_div_comps(ctx.h, _a3(phi_fc), _h(div), start_incomp, start_outcomp, num_comp,
+                                          int(increment)), "compute_div_comps")
+
+
+def compute_grad_comps(ctx, phi, gradp, start_incomp, start_outcomp, num_comp):
+    """div_and_grad.f90:16 with component arguments (0-based)."""
+    ctx.check(lib().lmm_compute_grad_comps(ctx.h, _h(phi), _a3(gradp), start_incomp, start_outcomp, num_comp),
+              "compute_grad_comps")
+
+
+def average_cc_to_face_comps(ctx, cc, face, start_comp, num_comp):
+    """convert_stag.f90:132 with component arguments (0-based)."""
+    ctx.check(lib().lmm_average_cc_to_face_comps(ctx.h, _h(cc), _a3(face), start_comp, num_comp),
+              "average_cc_to_face_comps")
 
 
 def average_cc_to_face(ctx, cc, face):

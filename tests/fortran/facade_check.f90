@@ -7,6 +7,7 @@ program facade_check
   use lmm_facade
   implicit none
   type(lmm_params) :: p, bad
+  type(lmm_species_params) :: sp
   type(lmm_gmres_info) :: info
   type(c_ptr) :: ctx, b_u(3), x_u(3), a3(3), e3(3)
   integer(c_int) :: rc
@@ -47,6 +48,18 @@ program facade_check
   rc = lmm_gmres(c_null_ptr, b_u, c_null_ptr, x_u, c_null_ptr, a3, c_null_ptr, e3, c_null_ptr, 1.0_c_double, &
        npr, info)
   print '(a,1x,i0)', 'gmres_null_ctx', rc
+  ! species knobs: the reference defaults and the null-context rejections
+  call lmm_species_params_default(sp, 4_c_int)
+  print '(a,1x,i0)', 'species_n', sp%nspecies
+  print '(a,1x,i0)', 'chi_iterations', sp%chi_iterations
+  print '(a,1x,es23.16)', 'fraction_tolerance', sp%fraction_tolerance
+  print '(a,1x,i0,1x,i0,1x,i0)', 'species_flags', sp%is_ideal_mixture, sp%is_nonisothermal, sp%use_lapack
+  print '(a,1x,i0)', 'sizeof_species', c_sizeof(sp)
+  rc = lmm_ctx_set_species(c_null_ptr, sp)
+  print '(a,1x,i0)', 'set_species_null_ctx', rc
+  rc = lmm_compute_mass_fluxdiv(c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, a3, &
+       1.0_c_double, 0.0_c_double, z)
+  print '(a,1x,i0)', 'mass_fluxdiv_null_ctx', rc
   ! the host Gaussian stream (MT19937 + Acklam)
   rc = lmm_fill_normal_host(99_c_int32_t, 4_c_int64_t, z)
   print '(a,1x,i0)', 'fill_rc', rc

@@ -59,6 +59,15 @@ def test_defaults_through_fortran(run_check):
 def test_struct_layouts_match_the_c_abi(run_check):
     assert int(run_check["sizeof_params"][0]) == C.sizeof(S.Params)
     assert int(run_check["sizeof_info"][0]) == C.sizeof(S.GmresInfo)
+    assert int(run_check["sizeof_species"][0]) == C.sizeof(S.SpeciesParams)
+
+
+def test_species_knobs_through_fortran(run_check):
+    v = run_check
+    assert v["species_n"] == ["4"] and v["chi_iterations"] == ["10"]
+    assert float(v["fraction_tolerance"][0]) == 1e-13
+    assert v["species_flags"] == ["1", "1", "0"]   # ideal, non-isothermal, iterative chi
+    assert v["set_species_null_ctx"] == ["1"] and v["mass_fluxdiv_null_ctx"] == ["1"]
 
 
 def test_argument_rejections_through_fortran(run_check):

@@ -27,3 +27,45 @@ def test_compute_div_bit_exact(dim, n, increment):
         S.compute_div(ctx, uf, div, increment=increment)
         got = div.download()
     assert bit_equal(got, want), relerr(got, want)
+
+
+@pytest.mark.parametrize("dim,n", [(2, (16, 8)), (3, (8, 4, 8))])
+def test_multi_component_div_grad_average(dim, n):
+    """compute_div / compute_grad / average_cc_to_face with component ranges
+    (div_and_grad.f90:16-18, :304; convert_stag.f90:132): each component q of
+    the range maps to the same offset of the output, the other output
+    components are untouched."""
+    shape = tuple(reversed(n))
+    rng = np.random.default_rng(17)
+    g = O.geom(dim, n, 0.5)
+    nin, nout = 4, 5
+    u = [rng.standard_normal((nin,) + shape) for _ in range(dim)]
+    cc = rng.standard_normal((nin,) + shape)
+    d0 = rng.standard_normal((nout,) + shape)
+    g0 = [rng.standard_normal((nout,) + shape) for _ in range(dim)]
+    with S.Context(dim, n, 0.5) as ctx:
+        uf = [ctx.field(S.FACE_X + d, nin, u[d]) for d in range(dim)]
+        div = ctx.field(S.CELL, nout, d0)
+        S.compute_div_comps(ctx, uf, div, 1, 2, 3, increment=True)
+        got = div.download()
+        for q in range(3):
+            want = O.compute_div(g, [u[d][1 + q] for d in range(dim)], div=d0[2 + q], increment=True)
+            assert bit_equal(got[2 + q], want)
+        assert bit_equal(got[:2], d0[:2])
+        cf = ctx.field(S.CELL, nin, cc)
+        gp = [ctx.field(S.FACE_X + d, nout, g0[d]) for d in range(dim)]
+        S.compute_grad_comps(ctx, cf, gp, 0, 1, 2)
+        gg = [f.download() for f in gp]
+        for q in range(2):
+            want = O.compute_grad(g, np.ascontiguousarray(cc[q]))
+            assert bit_equal([gg[d][1 + q] for d in range(dim)], want)
+        assert bit_equal([gg[d][[0, 3, 4]] for d in range(dim)], [g0[d][[0, 3, 4]] for d in range(dim)])
+        fa = [ctx.field(S.FACE_X + d, nin) for d in range(dim)]
+        S.average_cc_to_face_comps(ctx, cf, fa, 1, 3)
+        fv = [f.download() for f in fa]
+        for q in range(1, 4):
+            want = O.average_cc_to_face(g, np.ascontiguousarray(cc[q]))
+            assert bit_equal([fv[d][q] for d in range(dim)], want)
+        assert not any(fv[d][0].any() for d in range(dim))
+        with pytest.raises(S.LmmError):
+            S.compute_div_comps(ctx, uf, div, 2, 0, 3)  # input range beyond 4 components
