@@ -39,6 +39,21 @@ inline Grid coarsen(const Grid &f) {
   return make_grid(f.dim, n, f.dx * 2.0);
 }
 
+// The division of a Gauss-Seidel / Jacobi update, x / diag.  FP64 keeps the
+// correctly rounded quotient (the reference's rounding, bit-exact parity).
+// FP32 is only the multigrid preconditioner inside the FP64 flexible GMRES,
+// which has no reference rounding to follow: there the hardware reciprocal
+// (v_rcp_f32, 1 ulp) and a multiply replace the ten-instruction IEEE
+// division sequence.
+template <typename T>
+__device__ __forceinline__ T gs_div(T x, T diag) {
+  return x / diag;
+}
+template <>
+__device__ __forceinline__ float gs_div<float>(float x, float diag) {
+  return x * __builtin_amdgcn_rcpf(diag);
+}
+
 __device__ __forceinline__ int wp(int i, int n) { return (i == n - 1) ? 0 : i + 1; }
 __device__ __forceinline__ int wm(int i, int n) { return (i == 0) ? n - 1 : i - 1; }
 

@@ -117,7 +117,7 @@ __global__ void __launch_bounds__(256) k_stag_jacobi(Grid g, int visc, T omega, 
   T L, fac;
   stag_point<T>(visc, g, a, P, phr, K, dxsqinv, L, fac);
   long p = i + g.s[1] * j + g.s[2] * k;
-  tmp.c[a][p] = ph.c[a][p] + omega * (rh.c[a][p] - L) / fac;
+  tmp.c[a][p] = ph.c[a][p] + gs_div(omega * (rh.c[a][p] - L), fac);
 }
 
 template <typename T>

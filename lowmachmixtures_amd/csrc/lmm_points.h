@@ -67,7 +67,7 @@ __device__ __forceinline__ void gs_colour_at(const Grid &g, int visc, int a, int
   T L, fac;
   stag_point<T>(visc, g, a, P, phr, K, dxsqinv, L, fac);
   long p = i + g.s[1] * j + g.s[2] * k;
-  ph.c[a][p] = ph.c[a][p] + omega * (rhs[p] - L) / fac;
+  ph.c[a][p] = ph.c[a][p] + gs_div(omega * (rhs[p] - L), fac);
 }
 
 // res = rhs - L phi at point t of component a (stag_mg_solver.f90:404-416: -(Lphi - rhs))
@@ -222,7 +222,7 @@ __device__ __forceinline__ void cc_rb_at(const Grid &g, int colour, T inv, T *ph
   T diag;
   T lp = cc_apply<T>(g, i, j, k, phi, A, inv, diag);
   long p = i + g.s[1] * j + g.s[2] * k;
-  phi[p] = phi[p] + (rhs[p] - lp) / diag;
+  phi[p] = phi[p] + gs_div(rhs[p] - lp, diag);
 }
 
 template <typename T>
@@ -412,7 +412,7 @@ struct StagRB {
       T s = xp + xm + yp + ym + zp + zm;
       T L = F * (c.av + (T)6.0 * b * dxsqinv) - s * b * dxsqinv;
       T fac = c.av + (T)6.0 * b * dxsqinv;
-      return F + omega * (c.rhs - L) / fac;
+      return F + gs_div(omega * (c.rhs - L), fac);
     } else {
       // along / transverse neighbours of component A
       const T Fa = (A == 0) ? xp : (A == 1 ? yp : zp);
@@ -425,7 +425,7 @@ struct StagRB {
       T off = Fa * c.B0 + Fma * c.B1 + F1p * c.E11 + F1m * c.E10 + F2p * c.E21 + F2m * c.E20;
       T fac = c.av + dg * dxsqinv;
       T L = F * fac - off * dxsqinv;
-      return F + omega * (c.rhs - L) / fac;
+      return F + gs_div(omega * (c.rhs - L), fac);
     }
   }
 };
@@ -490,7 +490,7 @@ struct CCRB {
     off = off + c.azl * zm + c.azh * zp;
     const T diag = -s * a.inv;
     const T lp = (-s * F + off) * a.inv;
-    return F + (c.rhs - lp) / diag;
+    return F + gs_div(c.rhs - lp, diag);
   }
 };
 
