@@ -232,13 +232,14 @@ def cpu_baseline(P, O, helpers, n, seed, runs, quiet):
     return float(np.median(secs)), secs, info, pb, b_u, (xu, xp)
 
 
-def ensemble(P, S, per_size, steps, quiet):
+def ensemble(P, S, per_size, steps, quiet, graph=1):
     """Batched ragged 2D ensemble: solves per second over `steps` batches after one warm-up batch."""
     pbs = P.config4(per_size=per_size)
     members = []
     try:
         for pb in pbs:
             ctx = S.Context(2, pb.n, pb.dx, **pb.solver)
+            ctx.kernel_options(precon_graph=graph)
             members.append((ctx, S.DeviceProblem(ctx, pb, rhs=P.rhs_from_noise_host(pb))))
         dps = [dp for _, dp in members]
         S.gmres_batched(dps)
@@ -258,8 +259,10 @@ def ensemble(P, S, per_size, steps, quiet):
                f"{per_size}x64^2 + {per_size}x128^2 + {per_size}x256^2", "batches": steps,
                "s_per_batch": round(sec, 4), "solves_per_s": round(len(pbs) / sec, 1),
                "gmres_iters_mean": round(float(np.mean(iters)), 2), "form": "fp32_mg",
-               "mechanism": "one context and stream per member, host thread pool (lmm_gmres_batched)"}
-        log(f"ensemble: {out['solves_per_s']} solves/s ({sec * 1e3:.1f} ms per batch of {len(pbs)})", quiet)
+               "mechanism": "one context and stream per member, host thread pool (lmm_gmres_batched)"
+                            + (", each member's FP32 preconditioner replayed from HIP graphs" if graph else "")}
+        log(f"ensemble (graphs {graph}): {out['solves_per_s']} solves/s ({sec * 1e3:.1f} ms per batch of {len(pbs)})",
+            quiet)
         return out
     finally:
         for ctx, _ in members:
@@ -455,6 +458,8 @@ def main():
 
     if a.ensemble > 0:
         out["ensemble"] = ensemble(P, S, a.ensemble, 3, a.quiet)
+        eager = ensemble(P, S, a.ensemble, 3, a.quiet, graph=0)
+        out["ensemble"]["solves_per_s_without_graphs"] = eager["solves_per_s"]
 
     if a.species_n > 0:
         out["species_mass_flux"] = species_chain(S, a.species_n, 4, 20, a.cpu_n > 0, a.quiet)

@@ -117,6 +117,9 @@ typedef struct {
                               0 modified Gram-Schmidt */
   int sweep_prefetch;      /* 1: the FP32 fused sweep loads the next plane's coefficients
                               one step ahead */
+  int precon_graph;        /* 1: each FP32 preconditioner application of the flexible
+                              GMRES is captured once per output slot into a HIP graph and
+                              replayed (same kernels, same bits; fewer host launches) */
 } lmm_kernel_options;
 
 /* ---------------- context ---------------- */

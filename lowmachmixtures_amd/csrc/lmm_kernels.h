@@ -17,6 +17,7 @@ constexpr int RED_BLOCKS = 1024;
 // hipGetLastError() and, when debug_sync is set (environment LMM_DEBUG_SYNC=1),
 // synchronises the stream so a failing kernel is named in first_error().
 void set_debug_sync(int on);
+int debug_sync_on();
 
 // Kernel-choice settings of one context.  Every setting gives the same bits;
 // they exist for measurement and for the tests that compare kernel forms.

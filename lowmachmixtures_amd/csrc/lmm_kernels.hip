@@ -18,6 +18,7 @@ static int g_debug_sync = 0;
 // per host thread: each thread drives its own contexts (lmm_gmres_batched)
 static thread_local char g_first_error[256] = {0};
 void set_debug_sync(int on) { g_debug_sync = on; }
+int debug_sync_on() { return g_debug_sync; }
 // the context settings in force on this thread (null: the defaults)
 static thread_local const Tuning *t_tuning = nullptr;
 const Tuning &tuning() {

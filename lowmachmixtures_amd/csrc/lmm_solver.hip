@@ -164,6 +164,17 @@ struct lmm_ctx {
   long sp_work_n = 0;
   double hweights_mass[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // weights last copied to dweights_mass
   bool weights_mass_valid = false;
+  // HIP graphs of the FP32 preconditioner application, one per output slot
+  // (kernel option precon_graph); dropped whenever what they captured may
+  // have changed (parameters, kernel options, storage)
+  bool precon_graph = true;
+  struct PGraph {
+    hipGraphExec_t exec = nullptr;
+    double theta = 0.0;
+    int vcycles = 0;    // V-cycles one application counts (added on each replay)
+    bool warm = false;  // one eager application done: storage allocated
+  };
+  std::map<const float *, PGraph> pgraphs;
 };
 
 // A network bound to a context: the correction acts on staggered level
@@ -377,8 +388,15 @@ static void free_levels(std::vector<SLevel<T>> &S, std::vector<CLevel<T>> &CC, b
   CC.clear();
 }
 
+static void drop_graphs(lmm_ctx *c) {
+  for (auto &kv : c->pgraphs)
+    if (kv.second.exec) hipGraphExecDestroy(kv.second.exec);
+  c->pgraphs.clear();
+}
+
 static int ensure_krylov(lmm_ctx *c, int m) {
   if (c->kry_m >= m) return LMM_OK;
+  drop_graphs(c);  // the slots move
   long L = (long)(c->g.dim + 1) * c->g.N;
   dfree(c->V);
   dfree(c->Z32);
@@ -1200,11 +1218,55 @@ static int gmres_ref(lmm_ctx *c, double theta_alpha, lmm_gmres_info *info) {
 // Flexible GMRES with FP32 V-cycles (right preconditioned, MGS, FP64 outer)
 // ---------------------------------------------------------------------------
 
+// The FP32 preconditioner application z32 = M(b32) is a fixed sequence of
+// device work for a given output slot, theta_alpha and configuration (the FP32
+// V-cycles have no host-side stopping tests).  With precon_graph it is
+// captured into a HIP graph at its second use per slot (the first runs
+// eagerly, so any lazy allocation happens outside capture) and replayed
+// afterwards: the same kernels with the same arguments, hence the same bits.
+// The V-cycles clear memory with fill kernels, never hipMemsetAsync.
+static bool graphs_usable(const lmm_ctx *c) {
+  return c->precon_graph && !c->prof && !(c->learned_on && c->cnn) && !debug_sync_on();
+}
+
 static void precon32(lmm_ctx *c, const double *v64, float *z32, double theta_alpha, bool have32) {
   hipStream_t s = c->stream;
   long L = (long)(c->g.dim + 1) * c->g.N;
   if (!have32) launch_convert<float, double>(s, c->b32, v64, L);
   SVT<float> b = svt_of<float>(c, c->b32), z = svt_of<float>(c, z32);
+  if (graphs_usable(c)) {
+    lmm_ctx::PGraph &e = c->pgraphs[z32];
+    if (e.exec && e.theta != theta_alpha) {
+      hipGraphExecDestroy(e.exec);
+      e.exec = nullptr;
+    }
+    if (!e.exec && e.warm) {
+      hipGraph_t gr = nullptr;
+      if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+        const int vc0 = c->vcycles;
+        apply_precon_t<float>(c, c->s32, c->c32, b.u, b.p, z.u, z.p, c->mac_rhs32, c->c32[0].phi, theta_alpha);
+        e.vcycles = c->vcycles - vc0;
+        c->vcycles = vc0;  // counted when the graph runs
+        if (hipStreamEndCapture(s, &gr) == hipSuccess && gr) {
+          if (hipGraphInstantiate(&e.exec, gr, nullptr, nullptr, 0) != hipSuccess) e.exec = nullptr;
+          hipGraphDestroy(gr);
+        }
+        if (e.exec) e.theta = theta_alpha;
+      }
+      if (!e.exec) {  // capture unavailable: stay eager for this slot
+        hipGetLastError();
+        c->precon_graph = false;
+        apply_precon_t<float>(c, c->s32, c->c32, b.u, b.p, z.u, z.p, c->mac_rhs32, c->c32[0].phi, theta_alpha);
+        return;
+      }
+    }
+    if (e.exec) {
+      if (hipGraphLaunch(e.exec, s) != hipSuccess) report_error("hipGraphLaunch of the FP32 preconditioner failed");
+      c->vcycles += e.vcycles;
+      return;
+    }
+    e.warm = true;
+  }
   apply_precon_t<float>(c, c->s32, c->c32, b.u, b.p, z.u, z.p, c->mac_rhs32, c->c32[0].phi, theta_alpha);
   if (c->learned_on && c->cnn) learned_stage(c, b.u, b.p, z.u, z.p);
 }
@@ -1412,6 +1474,8 @@ static lmm_kernel_options env_options() {
   if (gs && strcmp(gs, "mgs") == 0) o.flex_cgs = 0;
   const char *pf = getenv("LMM_SWEEP_PREFETCH");
   if (pf && pf[0] == '1') o.sweep_prefetch = 1;
+  const char *pg = getenv("LMM_PRECON_GRAPH");
+  if (pg && pg[0] == '0') o.precon_graph = 0;
   return o;
 }
 
@@ -1428,7 +1492,9 @@ static void apply_options(lmm_ctx *c, const lmm_kernel_options &o) {
   c->tune.rb_target = o.fused_target;
   c->tune.am_dz = o.am_brick;
   c->tune.rb_prefetch = o.sweep_prefetch;
+  c->precon_graph = o.precon_graph != 0;
   c->kopt = o;
+  drop_graphs(c);
 }
 
 extern "C" {
@@ -1560,6 +1626,7 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
 int lmm_ctx_destroy(lmm_ctx *c) {
   if (!c) return LMM_ERR_ARG;
   if (c->stream) hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   prof_flush(c);
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   free_levels<double>(c->s64, c->c64, false);
@@ -1587,6 +1654,8 @@ int lmm_ctx_set_params(lmm_ctx *c, const lmm_params *p) {
   if (rc) return rc;
   if (p->stag_mg_minwidth != c->prm.stag_mg_minwidth || p->mg_minwidth != c->prm.mg_minwidth)
     return set_err(c, LMM_ERR_UNSUPPORTED, "minwidth is fixed at context creation");
+  hipStreamSynchronize(c->stream);
+  drop_graphs(c);
   if (p->gmres_max_inner != c->prm.gmres_max_inner) c->kry_m = std::min(c->kry_m, p->gmres_max_inner - 1);
   if (p->precision != c->prm.precision) {
     dfree(c->V);
@@ -1620,6 +1689,7 @@ void lmm_kernel_options_default(lmm_kernel_options *o) {
   o->krylov_fused = 1;
   o->flex_cgs = 1;
   o->sweep_prefetch = 0;
+  o->precon_graph = 1;
 }
 
 int lmm_ctx_get_kernel_options(const lmm_ctx *c, lmm_kernel_options *o) {

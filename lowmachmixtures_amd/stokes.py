@@ -65,6 +65,7 @@ class KernelOptions(C.Structure):
         ("fused_target", C.c_int), ("fused_x3", C.c_int), ("resid_restrict", C.c_int),
         ("coarse_one_launch", C.c_int), ("am_brick", C.c_int), ("z32_direct", C.c_int),
         ("krylov_fused", C.c_int), ("flex_cgs", C.c_int), ("sweep_prefetch", C.c_int),
+        ("precon_graph", C.c_int),
     ]
 
     def as_dict(self):
