@@ -2871,7 +2871,7 @@ int lmm_compute_mass_fluxdiv(lmm_ctx *c, lmm_field *rho, lmm_field *const gradp_
   const long N = c->g.N;
   const long nlp = (long)ns * (ns + 1) / 2;
   const long need = (long)(ns + ns * ns + (sp.ideal ? 0 : ns * ns) + (sp.noniso ? ns : 0) + (stoch ? nlp : 0) +
-                           (baro ? ns : 0) + dim * ns) * N;
+                           (baro ? ns : 0) + (stoch ? 2 : 1) * dim * ns) * N;
   double *w = sp_scratch(c, need);
   if (!w) return set_err(c, LMM_ERR_NOMEM, "species scratch");
   if (stoch)

@@ -69,9 +69,10 @@ int sp_diffusive(hipStream_t s, const Grid &g, const SpParams &sp, const double 
 int sp_stochastic(hipStream_t s, const Grid &g, const SpParams &sp, const double *L, const double *W, int n_rngs,
                   const double *weights, double variance, double *flux, double *const flux_total[3], double *div,
                   int increment);
-// compute_mass_fluxdiv: one cell pass, then the diffusive and (when W is
-// given) stochastic face passes with their divergences; flux is dim*ns*N
-// scratch; flux_total is overwritten (setval 0, then both fluxes added)
+// compute_mass_fluxdiv: one cell pass, then one face pass for both fluxes
+// (the diffusive one alone when W is null) and one divergence pass; flux is
+// scratch of 2*dim*ns*N values (dim*ns*N without W); flux_total is
+// overwritten with (0 + F_diff) + F_stoch, the reference's setval then adds
 int sp_chain_cell(hipStream_t s, const Grid &g, const SpParams &sp, const double *Temp, const SpCellOut &o);
 int sp_chain_face(hipStream_t s, const Grid &g, const SpParams &sp, const SpFaceIn &in, double *flux,
                   double *diff_div, double *stoch_div, double *const flux_total[3]);

@@ -29,8 +29,10 @@
 // compute_mass_fluxdiv.f90 runs as one cell pass (sp_chain_cell), which
 // writes only what the faces need (mole fractions, rho W chi, Gama unless
 // the mixture is ideal, zeta/T, the packed lower triangle of the Cholesky
-// factor of L-Onsager) and restores rho, then the face passes of the two
-// fluxes, each followed by its divergence (sp_chain_face).  Giving each cell
+// factor of L-Onsager) and restores rho, then (up to 4 species) one face
+// pass computing both fluxes of every face (flux_total written once) and one
+// pass for both divergences, or (more species) a pass and a divergence per
+// flux (sp_chain_face).  Giving each cell
 // its upper faces too (recomputed, so the face arrays need not be stored)
 // was tried: the two matrix-vector chains per face kept so many loads live
 // that the kernel ran at one wave per SIMD with spills from 5 species up.
@@ -757,6 +759,65 @@ __global__ void __launch_bounds__(STPB) k_sp_div(Grid g, const double *flux, dou
   }
 }
 
+// compute_mass_fluxdiv's face pass: both fluxes of each lower face in one
+// thread (the diffusive then the stochastic, fenced so their loads do not
+// pile up), flux_total = (0 + F_diff) + F_stoch written once.
+template <int NS>
+__global__ void __launch_bounds__(STPB) k_sp_chain_flux(Grid g, SpParams sp, SpFaceIn in, double *fd, double *fs,
+                                                         Ptr3<double> ft) {
+  const long t = xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  const Pt P = make_pt(g, i, j, k);
+  const long N = g.N;
+  const double dxinv = 1.0 / g.dx;
+  for (int d = 0; d < g.dim; d++) {
+    const long m = at_sh(g, P, d, -1, -1, 0);
+    double od[NS], os[NS];
+    diff_face<NS>(sp, N, dxinv, t, m, in.xm, in.rw, in.gama, in.temp, in.zeta, in.baro,
+                  in.baro ? in.gpb[d][t] : 0.0, od);
+    __builtin_amdgcn_sched_barrier(0);
+    stoch_face<NS, true>(sp, N, t, m, d, g.dim, in.lpk, in.W, in.n_rngs, in.weights, in.variance, os);
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const long q = ((long)d * NS + s) * N + t;
+      fd[q] = od[s];
+      fs[q] = os[s];
+      ft.c[d][(long)s * N + t] = (0.0 + od[s]) + os[s];
+    }
+  }
+}
+
+// compute_div of two species face-flux sets in one pass
+template <int NS>
+__global__ void __launch_bounds__(STPB) k_sp_div2(Grid g, const double *fa, double *da, const double *fb,
+                                                   double *db) {
+  const long t = xcd_block() * blockDim.x + threadIdx.x;
+  if (t >= g.N) return;
+  int i, j, k;
+  decode(g, t, i, j, k);
+  const Pt P = make_pt(g, i, j, k);
+  const long N = g.N;
+  const double dxinv = 1.0 / g.dx;
+  const long ip = at3(g, P, 1, 0, 0), jp = at3(g, P, 0, 1, 0), kp = at3(g, P, 0, 0, 1);
+  for (int w = 0; w < 2; w++) {
+    const double *flux = w ? fb : fa;
+    double *div = w ? db : da;
+#pragma unroll
+    for (int s = 0; s < NS; s++) {
+      const double *fx = flux + (long)s * N, *fy = flux + (long)(NS + s) * N;
+      double v = (fx[ip] - fx[t]) * dxinv;
+      v = v + (fy[jp] - fy[t]) * dxinv;
+      if (g.dim == 3) {
+        const double *fz = flux + (long)(2 * NS + s) * N;
+        v = v + (fz[kp] - fz[t]) * dxinv;
+      }
+      div[(long)s * N + t] = v;
+    }
+  }
+}
+
 // ---------------- compute_mass_fluxdiv, fused ----------------
 
 template <int NS, bool LP>
@@ -951,6 +1012,14 @@ struct RunChainFace {
   static int run(hipStream_t s, const Grid &g, const SpParams &sp, const SpFaceIn &in, double *flux, double *dd,
                  double *sd, double *const ft[3]) {
     SP_NEEDS2;
+    // up to 4 species both fluxes share one face pass (at 5 and more its
+    // registers drop it to 1-3 waves per SIMD: separate passes there)
+    if (in.W && NS <= 4) {  // flux holds 2 * dim * ns * N: the diffusive then the stochastic face fluxes
+      double *fs = flux + (long)g.dim * NS * g.N;
+      SLAUNCH(k_sp_chain_flux<NS>, dim3(sblk(g.N)), dim3(STPB), 0, s, g, sp, in, flux, fs, p3(ft));
+      SLAUNCH(k_sp_div2<NS>, dim3(sblk(g.N)), dim3(STPB), 0, s, g, flux, dd, fs, sd);
+      return 0;
+    }
     SLAUNCH(k_sp_diff_flux<NS>, dim3(sblk(g.N)), dim3(STPB), 0, s, g, sp, in.xm, in.rw, in.gama, in.temp, in.zeta,
             in.baro, cp3(in.baro ? in.gpb : nullptr), flux, p3(ft), 1);
     SLAUNCH(k_sp_div<NS>, dim3(sblk(g.N)), dim3(STPB), 0, s, g, flux, dd, 0);
