@@ -74,6 +74,13 @@ bool fused_rb3_ok(const Grid &g, int visc, int min_tiles);
 template <typename T>
 void launch_stag_rb_fused3(hipStream_t s, const Grid &g, int visc, T omega, T *const pin[3], T *const pout[3],
                            T *const rhs[3], const StagCoef<T> &K, int min_tiles, bool zero_in);
+// Both components of one fused red-black sweep in one launch (2D, visc -1 /
+// 1, x a multiple of 64, y of 8, at least min_tiles workgroups); reads pin[a]
+// (zero_in: a field of zeros, pin not read), writes pout[a].
+bool fused_rb2d_ok(const Grid &g, int visc, int min_tiles);
+template <typename T>
+void launch_stag_rb2d(hipStream_t s, const Grid &g, int visc, T omega, T *const pin[3], T *const pout[3],
+                      T *const rhs[3], const StagCoef<T> &K, bool zero_in);
 // Residual of every face component and its full-weighting restriction into
 // the coarse right-hand side crhs in one pass per component (3D, visc_type
 // -1 / 1, x a multiple of 64, y of 8); false (nothing launched) elsewhere.

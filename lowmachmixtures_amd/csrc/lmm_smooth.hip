@@ -720,6 +720,169 @@ void launch_stag_three(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz,
   post_launch(s, kMarch3Name);
 }
 
+// ---------------------------------------------------------------------------
+// 2D fused red-black sweep (visc_type -1 / 1): both face components in one
+// launch, a workgroup per (tile, component).  In 2D a component's colours
+// read only that component, red (i + j even) reads only black (old)
+// neighbours, so a workgroup stages the old field of its 64 x 8 tile with a
+// 2-point halo in LDS, computes the red values of the tile and its 1-point
+// ring, then the black values of the tile from them, and writes the tile:
+// the reference's colours 2a+1, 2a+2 (stag_mg_solver.f90:373-382, the 2D
+// stag_applyop / stag_mg_update arithmetic term by term), in one pass over
+// memory instead of four colour passes.
+template <typename T, int A, int VISC>
+struct Stag2RB {
+  static constexpr int A_ = A;
+  struct Args {
+    T omega, dxsqinv;
+    const T *rhs, *alpha, *beta, *bnd;  // bnd: node beta (2D edges)
+  };
+  struct Coef {
+    T av, rhs, B0, B1, E10, E11;
+  };
+  static __device__ __forceinline__ Coef load(const Grid &g, int i, int j, const Args &a) {
+    Coef c;
+    const long sy = g.s[1];
+    const long p = i + sy * j;
+    c.av = a.alpha[p];
+    c.rhs = a.rhs[p];
+    if (VISC == -1) {
+      const long pma = (A == 0) ? wrapc(i - 1, g.n[0]) + sy * j : i + sy * wrapc(j - 1, g.n[1]);
+      const long p1p = (A == 0) ? i + sy * wrapc(j + 1, g.n[1]) : wrapc(i + 1, g.n[0]) + sy * j;
+      c.B0 = a.beta[p];
+      c.B1 = a.beta[pma];
+      c.E10 = a.bnd[p];
+      c.E11 = a.bnd[p1p];
+    } else {
+      c.B0 = a.beta[0];
+      c.B1 = c.E10 = c.E11 = (T)0;
+    }
+    return c;
+  }
+  // F: the point; (fpa, fpma) along A; (fp1p, fp1m) along the other direction
+  static __device__ __forceinline__ T update(const Coef &c, T F, T fpa, T fpma, T fp1p, T fp1m, const Args &a) {
+    const T dxsqinv = a.dxsqinv;
+    T L, fac;
+    if (VISC == 1) {
+      const T b = c.B0;
+      T s = fpa + fpma + fp1p + fp1m;
+      L = F * (c.av + (T)4.0 * b * dxsqinv) - s * b * dxsqinv;
+      fac = c.av + (T)4.0 * b * dxsqinv;
+    } else {
+      T dg = c.B0 + c.B1 + c.E11 + c.E10;
+      T off = fpa * c.B0 + fpma * c.B1 + fp1p * c.E11 + fp1m * c.E10;
+      L = F * (c.av + dg * dxsqinv) - off * dxsqinv;
+      fac = c.av + (c.B0 + c.B1 + c.E10 + c.E11) * dxsqinv;
+    }
+    return F + gs_div(a.omega * (c.rhs - L), fac);
+  }
+};
+
+constexpr int TX2D = 64, TY2D = 8;
+
+template <typename T, typename P>
+__device__ __forceinline__ void tile2d(const Grid &g, int ntx, int tile, const T *__restrict__ pin,
+                                       T *__restrict__ pout, const typename P::Args &args,
+                                       T (*sp)[TX2D + 4], T (*sr)[TX2D + 2]) {
+  constexpr int HX2 = TX2D + 4, HY2 = TY2D + 4, RX2 = TX2D + 2, RY2 = TY2D + 2;
+  const int bx = tile % ntx, by = tile / ntx;
+  const int i0 = bx * TX2D, j0 = by * TY2D;
+  const int nx = g.n[0], ny = g.n[1];
+  const long sy = g.s[1];
+  const int tid = threadIdx.x;
+  // the old field of the tile and a 2-point halo (a null input: zeros)
+  for (int q = tid; q < HY2 * HX2; q += 256) {
+    const int r = q / HX2, c = q - r * HX2;
+    sp[r][c] = pin ? pin[wrapc(i0 - 2 + c, nx) + sy * wrapc(j0 - 2 + r, ny)] : (T)0;
+  }
+  __syncthreads();
+  // red points of the tile and its 1-point ring: region (row, rr) is point
+  // (i0 + rr - 1, j0 + row - 1); red where i + j is even (i0 is even)
+  constexpr int NRED2 = RY2 * (RX2 / 2);
+  for (int q = tid; q < NRED2; q += 256) {
+    const int row = q / (RX2 / 2), cc = q - row * (RX2 / 2);
+    const int jj = j0 + row - 1;
+    const int rr = 2 * cc + ((jj + 1) & 1);
+    const int hr = row + 1, hc = rr + 1;
+    const typename P::Coef c = P::load(g, wrapc(i0 + rr - 1, nx), wrapc(jj, ny), args);
+    const T xp = sp[hr][hc + 1], xm = sp[hr][hc - 1], yp = sp[hr + 1][hc], ym = sp[hr - 1][hc];
+    sr[row][rr] = (P::A_ == 0) ? P::update(c, sp[hr][hc], xp, xm, yp, ym, args)
+                               : P::update(c, sp[hr][hc], yp, ym, xp, xm, args);
+  }
+  __syncthreads();
+  // black: thread t owns the pair (2m, 2m+1) of tile row r
+  const int pr = tid / (TX2D / 2), pm = tid - pr * (TX2D / 2);
+  const int j = j0 + pr;
+  const bool first_red = (j & 1) == 0;  // column 2m is red when j is even
+  const int bc = 2 * pm + (first_red ? 1 : 0), rc = 2 * pm + (first_red ? 0 : 1);
+  const typename P::Coef cb = P::load(g, i0 + bc, j, args);
+  const int rrow = pr + 1, rcol = bc + 1;
+  const T xp = sr[rrow][rcol + 1], xm = sr[rrow][rcol - 1], yp = sr[rrow + 1][rcol], ym = sr[rrow - 1][rcol];
+  const T F = sp[pr + 2][bc + 2];
+  const T v = (P::A_ == 0) ? P::update(cb, F, xp, xm, yp, ym, args) : P::update(cb, F, yp, ym, xp, xm, args);
+  const T r = sr[rrow][rc + 1];
+  st2(pout, i0 + 2 * pm + sy * j, first_red ? r : v, first_red ? v : r);
+}
+
+template <typename T, typename P0, typename P1>
+__global__ void __launch_bounds__(256) k_rb_tile2d(Grid g, int ntx, int ntiles, const T *__restrict__ pin0,
+                                                   T *__restrict__ pout0, const T *__restrict__ pin1,
+                                                   T *__restrict__ pout1, typename P0::Args a0,
+                                                   typename P1::Args a1) {
+  __shared__ T sp[TY2D + 4][TX2D + 4];
+  __shared__ T sr[TY2D + 2][TX2D + 2];
+  const int nslots = 2 * ntiles, w = blockIdx.x;
+  int slot = w;
+  if (nslots % 8 == 0) slot = (w % 8) * (nslots / 8) + w / 8;
+  const int tile = slot / 2, comp = slot - 2 * tile;
+  if (comp == 0)
+    tile2d<T, P0>(g, ntx, tile, pin0, pout0, a0, sp, sr);
+  else
+    tile2d<T, P1>(g, ntx, tile, pin1, pout1, a1, sp, sr);
+}
+
+}  // namespace
+
+bool fused_rb2d_ok(const Grid &g, int visc, int min_tiles) {
+  return g.dim == 2 && (visc == -1 || visc == 1) && g.n[0] % TX2D == 0 && g.n[1] % TY2D == 0 &&
+         2 * (g.n[0] / TX2D) * (g.n[1] / TY2D) >= min_tiles;
+}
+
+template <typename T, int VISC>
+static void launch_rb2d(hipStream_t s, const Grid &g, T omega, T dxsqinv, T *const pin[3], T *const pout[3],
+                        T *const rhs[3], const StagCoef<T> &K) {
+  using P0 = Stag2RB<T, 0, VISC>;
+  using P1 = Stag2RB<T, 1, VISC>;
+  typename P0::Args a0 = {omega, dxsqinv, rhs[0], K.alpha[0], K.beta, K.bed[0]};
+  typename P1::Args a1 = {omega, dxsqinv, rhs[1], K.alpha[1], K.beta, K.bed[0]};
+  const int ntx = g.n[0] / TX2D, ntiles = ntx * (g.n[1] / TY2D);
+  LaunchEvents *ev = launch_events();
+  hipExtLaunchKernelGGL((k_rb_tile2d<T, P0, P1>), dim3(2 * ntiles), dim3(256), 0, s, ev->start, ev->stop, 0, g, ntx,
+                        ntiles, pin[0], pout[0], pin[1], pout[1], a0, a1);
+  ev->start = ev->stop = nullptr;
+  post_launch(s, "k_rb_tile2d");
+}
+
+template <typename T>
+void launch_stag_rb2d(hipStream_t s, const Grid &g, int visc, T omega, T *const pin_[3], T *const pout[3],
+                      T *const rhs[3], const StagCoef<T> &K, bool zero_in) {
+  if (!fused_rb2d_ok(g, visc, 1)) {
+    report_error("launch_stag_rb2d: level not eligible");
+    return;
+  }
+  T *const zeros[3] = {nullptr, nullptr, nullptr};
+  T *const *pin = zero_in ? zeros : pin_;
+  const T dxsqinv = (T)(1.0 / (g.dx * g.dx));
+  if (visc == -1) launch_rb2d<T, -1>(s, g, omega, dxsqinv, pin, pout, rhs, K);
+  else launch_rb2d<T, 1>(s, g, omega, dxsqinv, pin, pout, rhs, K);
+}
+template void launch_stag_rb2d<double>(hipStream_t, const Grid &, int, double, double *const[3], double *const[3],
+                                       double *const[3], const StagCoef<double> &, bool);
+template void launch_stag_rb2d<float>(hipStream_t, const Grid &, int, float, float *const[3], float *const[3],
+                                      float *const[3], const StagCoef<float> &, bool);
+
+namespace {
+
 template <typename T, int A, int VISC>
 void launch_stag_one(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, const T *pin, T *pout,
                      const T *rhs, const StagCoef<T> &K) {

@@ -508,12 +508,21 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
       continue;
     }
     const bool x3 = c->fused && c->fused3 && fused_rb3_ok(L.g, p.visc_type, c->fused_min_tiles);
-    if (x3 || (c->fused && fused_rb_ok(L.g, p.visc_type, c->fused_min_tiles))) {
+    const bool x2 = c->fused && fused_rb2d_ok(L.g, p.visc_type, c->fused_min_tiles);
+    if (x2 || x3 || (c->fused && fused_rb_ok(L.g, p.visc_type, c->fused_min_tiles))) {
       // sweep m reads one buffer and writes the other (phi <-> L.res); an
       // odd sweep count ends with a copy back
       T *const *src = (m % 2 == 0) ? phi : L.res;
       T *const *dst = (m % 2 == 0) ? L.res : phi;
-      if (x3) {
+      if (x2) {
+        // both components in one launch: phi, rhs, alpha of each component,
+        // beta_cc and the nodes (visc_type -1); phi written per component
+        const bool zin = zero_in && m == 0;
+        const double bytes = (double)(4 * dim + 1 + (p.visc_type < 0 ? 1 : 0) - (zin ? dim : 0)) * L.g.N * sizeof(T);
+        ProfScope pf(c, "stag_rb_fused", bytes);
+        ProfScope ps(c, fam, bytes, true);
+        launch_stag_rb2d<T>(c->stream, L.g, p.visc_type, (T)p.stag_mg_omega, src, dst, rhs, K, zin);
+      } else if (x3) {
         // all components in one launch: each array is compulsory once per
         // sweep (beta_cc and the edges shared by the components)
         // (a sweep from a zero guess does not read phi)
@@ -568,13 +577,15 @@ static void rhs_inf(lmm_ctx *c, SLevel<double> &L, double *const rhs[3], double 
 // phi0/rhs0 are the level-0 arrays; phi_zero tells that phi0 is zero on
 // entry, in which case the initial-residual exit of :276-306 cannot trigger
 // unless rhs is zero, and a V-cycle of zero data returns zero exactly.
-// A level whose sweeps run as three-component sweeps can start its first
+// A level whose sweeps run as all-component fused sweeps can start its first
 // smoothing of a cycle from a zero field without reading it: then its phi
 // need not be zeroed first (every point of the sweep's output is written).
 template <typename T>
 static bool sweep_takes_zeros(lmm_ctx *c, const SLevel<T> &L) {
   const lmm_params &p = c->prm;
-  return c->fused && c->fused3 && p.stag_mg_smoother == 1 && fused_rb3_ok(L.g, p.visc_type, c->fused_min_tiles);
+  return c->fused && p.stag_mg_smoother == 1 &&
+         ((c->fused3 && fused_rb3_ok(L.g, p.visc_type, c->fused_min_tiles)) ||
+          fused_rb2d_ok(L.g, p.visc_type, c->fused_min_tiles));
 }
 
 template <typename T>

@@ -1,5 +1,5 @@
-"""The fused red-black smoother (3D, visc_type -1 and 1) on levels whose
-extents allow it (x a multiple of 64, y of 8), against the oracle and against
+"""The fused red-black smoother (visc_type -1 and 1; 3D and 2D) on levels
+whose extents allow it (x a multiple of 64, y of 8), against the oracle and against
 the colour-by-colour kernels.  Both must agree bit for bit: every point is
 computed with the reference's operation order.  Two forms of the fused sweep
 exist (LMM_FUSED_KERNEL=march2, the default, block and march).
@@ -21,13 +21,13 @@ pytestmark = pytest.mark.gpu
 N3 = (64, 16, 32)
 
 
-def _rand_faces(shape, seed):
+def _rand_faces(shape, seed, ncomp=3):
     rng = np.random.default_rng(seed)
-    return [rng.standard_normal(shape) for _ in range(3)]
+    return [rng.standard_normal(shape) for _ in range(ncomp)]
 
 
 def _mg_solve(pb, kw, rhs, guess):
-    with S.Context(3, pb.n, pb.dx, **kw) as ctx:
+    with S.Context(pb.dim, pb.n, pb.dx, **kw) as ctx:
         dp = S.DeviceProblem(ctx, pb, rhs=rhs)
         phi = ctx.faces(guess)
         ctx.profiling(True)
@@ -179,5 +179,61 @@ def test_first_sweep_from_zeros_equals_filled_guess(n, coarse, prec, monkeypatch
             dp = S.DeviceProblem(ctx, pb, rhs=b_u)
             _, info = dp.solve()
             out[mode] = (info["total_iter"], dp.solution())
+    assert out["fused"][0] == out["colour"][0]
+    assert bit_equal(out["fused"][1][0], out["colour"][1][0]) and bit_equal(out["fused"][1][1], out["colour"][1][1])
+
+
+# 2D: both face components in one launch on 64 x 8 tiles
+
+
+@pytest.mark.parametrize("visc", [-1, 1])
+@pytest.mark.parametrize("n,vcycles,sweeps", [((128, 64), 1, (2, 2, 8)), ((256, 128), 2, (2, 2, 8)),
+                                              ((128, 128), 1, (1, 3, 3))])
+def test_fused_2d_mg_solver_bit_exact(n, visc, vcycles, sweeps, monkeypatch):
+    monkeypatch.setenv("LMM_SMOOTHER", "fused")
+    pb = P.small_problem(2, n, visc_type=visc, seed=113)
+    kw = dict(visc_type=visc, stag_mg_max_vcycles=vcycles, stag_mg_nsmooths_down=sweeps[0],
+              stag_mg_nsmooths_up=sweeps[1], stag_mg_nsmooths_bottom=sweeps[2])
+    g, p, cf = oracle_args(pb, **kw)
+    rhs = _rand_faces(pb.shape, 12, 2)
+    guess = _rand_faces(pb.shape, 13, 2)
+    want = O.stag_mg_solver(g, p, cf, 0.9, rhs, phi=guess)
+    got, fused = _mg_solve(pb, kw, rhs, guess)
+    assert fused > 0, "the fused smoother did not run"
+    assert bit_equal(got, want), relerr(got, want)
+
+
+@pytest.mark.parametrize("visc", [-1, 1])
+def test_fused_2d_equals_colour_kernels(visc, monkeypatch):
+    """At the default launch threshold (64 workgroups: 128^2 and up)."""
+    pb = P.small_problem(2, (512, 256), visc_type=visc, seed=127)
+    kw = dict(visc_type=visc, stag_mg_max_vcycles=2)
+    rhs, guess = _rand_faces(pb.shape, 14, 2), _rand_faces(pb.shape, 15, 2)
+    monkeypatch.setenv("LMM_SMOOTHER", "colour")
+    ref, fused0 = _mg_solve(pb, kw, rhs, guess)
+    monkeypatch.delenv("LMM_SMOOTHER")
+    got, fused1 = _mg_solve(pb, kw, rhs, guess)
+    assert fused0 == 0 and fused1 > 0
+    assert bit_equal(got, ref)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_mg"])
+@pytest.mark.parametrize("visc", [-1, 1])
+def test_fused_2d_gmres_equals_colour(prec, visc, monkeypatch):
+    """Whole solves: the V-cycles' first sweeps start from zeros that are not
+    written (levels 1 and down, and the zero guess of each preconditioner
+    application), in both multigrids."""
+    pb = P.small_problem(2, (256, 128), visc_type=visc, seed=131, contrast=30.0)
+    b_u = P.rhs_from_noise_host(pb)
+    precision = {"fp64": S.FP64, "fp32_mg": S.FP32_MG}[prec]
+    out = {}
+    for mode in ("fused", "colour"):
+        monkeypatch.setenv("LMM_SMOOTHER", mode)
+        with S.Context(2, pb.n, pb.dx, visc_type=visc, precision=precision, gmres_max_iter=12) as ctx:
+            dp = S.DeviceProblem(ctx, pb, rhs=b_u)
+            ctx.profiling(True)
+            _, info = dp.solve()
+            out[mode] = (info["total_iter"], dp.solution(), ctx.kernel_stats("stag_rb_fused")["launches"])
+    assert out["fused"][2] > 0 and out["colour"][2] == 0
     assert out["fused"][0] == out["colour"][0]
     assert bit_equal(out["fused"][1][0], out["colour"][1][0]) and bit_equal(out["fused"][1][1], out["colour"][1][1])
