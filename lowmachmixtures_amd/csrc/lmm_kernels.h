@@ -81,6 +81,12 @@ bool fused_rb2d_ok(const Grid &g, int visc, int min_tiles);
 template <typename T>
 void launch_stag_rb2d(hipStream_t s, const Grid &g, int visc, T omega, T *const pin[3], T *const pout[3],
                       T *const rhs[3], const StagCoef<T> &K, bool zero_in);
+// The cell-centred red-black sweep of one 2D level in one launch (x a
+// multiple of 64, y of 8, at least min_tiles workgroups); a null pin is a
+// field of zeros.
+bool cc_rb2d_ok(const Grid &g, int min_tiles);
+template <typename T>
+void launch_cc_rb2d(hipStream_t s, const Grid &g, const T *pin, T *pout, const T *rhs, T *const a[3]);
 // Residual of every face component and its full-weighting restriction into
 // the coarse right-hand side crhs in one pass per component (3D, visc_type
 // -1 / 1, x a multiple of 64, y of 8); false (nothing launched) elsewhere.

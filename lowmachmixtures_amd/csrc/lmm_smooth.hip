@@ -732,7 +732,6 @@ void launch_stag_three(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz,
 // memory instead of four colour passes.
 template <typename T, int A, int VISC>
 struct Stag2RB {
-  static constexpr int A_ = A;
   struct Args {
     T omega, dxsqinv;
     const T *rhs, *alpha, *beta, *bnd;  // bnd: node beta (2D edges)
@@ -759,8 +758,11 @@ struct Stag2RB {
     }
     return c;
   }
-  // F: the point; (fpa, fpma) along A; (fp1p, fp1m) along the other direction
-  static __device__ __forceinline__ T update(const Coef &c, T F, T fpa, T fpma, T fp1p, T fp1m, const Args &a) {
+  // F: the point and its x / y neighbours
+  static __device__ __forceinline__ T update(const Coef &c, T F, T xp, T xm, T yp, T ym, const Args &a) {
+    // (fpa, fpma) along A; (fp1p, fp1m) along the other direction
+    const T fpa = (A == 0) ? xp : yp, fpma = (A == 0) ? xm : ym;
+    const T fp1p = (A == 0) ? yp : xp, fp1m = (A == 0) ? ym : xm;
     const T dxsqinv = a.dxsqinv;
     T L, fac;
     if (VISC == 1) {
@@ -775,6 +777,37 @@ struct Stag2RB {
       fac = c.av + (c.B0 + c.B1 + c.E10 + c.E11) * dxsqinv;
     }
     return F + gs_div(a.omega * (c.rhs - L), fac);
+  }
+};
+
+// The 2D cell-centred red-black update of the MAC-projection multigrid:
+// cc_apply's arithmetic for dim 2 (x terms, then y).
+template <typename T>
+struct CC2RB {
+  struct Args {
+    T inv;
+    const T *rhs, *ax, *ay;
+  };
+  struct Coef {
+    T rhs, axl, axh, ayl, ayh;
+  };
+  static __device__ __forceinline__ Coef load(const Grid &g, int i, int j, const Args &a) {
+    Coef c;
+    const long sy = g.s[1];
+    const long p = i + sy * j;
+    c.rhs = a.rhs[p];
+    c.axl = a.ax[p];
+    c.axh = a.ax[wrapc(i + 1, g.n[0]) + sy * j];
+    c.ayl = a.ay[p];
+    c.ayh = a.ay[i + sy * wrapc(j + 1, g.n[1])];
+    return c;
+  }
+  static __device__ __forceinline__ T update(const Coef &c, T F, T xp, T xm, T yp, T ym, const Args &a) {
+    const T s = c.axl + c.axh + c.ayl + c.ayh;
+    const T off = c.axl * xm + c.axh * xp + c.ayl * ym + c.ayh * yp;
+    const T diag = -s * a.inv;
+    const T lp = (-s * F + off) * a.inv;
+    return F + gs_div(c.rhs - lp, diag);
   }
 };
 
@@ -806,8 +839,7 @@ __device__ __forceinline__ void tile2d(const Grid &g, int ntx, int tile, const T
     const int hr = row + 1, hc = rr + 1;
     const typename P::Coef c = P::load(g, wrapc(i0 + rr - 1, nx), wrapc(jj, ny), args);
     const T xp = sp[hr][hc + 1], xm = sp[hr][hc - 1], yp = sp[hr + 1][hc], ym = sp[hr - 1][hc];
-    sr[row][rr] = (P::A_ == 0) ? P::update(c, sp[hr][hc], xp, xm, yp, ym, args)
-                               : P::update(c, sp[hr][hc], yp, ym, xp, xm, args);
+    sr[row][rr] = P::update(c, sp[hr][hc], xp, xm, yp, ym, args);
   }
   __syncthreads();
   // black: thread t owns the pair (2m, 2m+1) of tile row r
@@ -819,7 +851,7 @@ __device__ __forceinline__ void tile2d(const Grid &g, int ntx, int tile, const T
   const int rrow = pr + 1, rcol = bc + 1;
   const T xp = sr[rrow][rcol + 1], xm = sr[rrow][rcol - 1], yp = sr[rrow + 1][rcol], ym = sr[rrow - 1][rcol];
   const T F = sp[pr + 2][bc + 2];
-  const T v = (P::A_ == 0) ? P::update(cb, F, xp, xm, yp, ym, args) : P::update(cb, F, yp, ym, xp, xm, args);
+  const T v = P::update(cb, F, xp, xm, yp, ym, args);
   const T r = sr[rrow][rc + 1];
   st2(pout, i0 + 2 * pm + sy * j, first_red ? r : v, first_red ? v : r);
 }
@@ -839,6 +871,16 @@ __global__ void __launch_bounds__(256) k_rb_tile2d(Grid g, int ntx, int ntiles, 
     tile2d<T, P0>(g, ntx, tile, pin0, pout0, a0, sp, sr);
   else
     tile2d<T, P1>(g, ntx, tile, pin1, pout1, a1, sp, sr);
+}
+
+template <typename T, typename P>
+__global__ void __launch_bounds__(256) k_rb_tile2d1(Grid g, int ntx, int ntiles, const T *__restrict__ pin,
+                                                    T *__restrict__ pout, typename P::Args a) {
+  __shared__ T sp[TY2D + 4][TX2D + 4];
+  __shared__ T sr[TY2D + 2][TX2D + 2];
+  const int w = blockIdx.x;
+  const int tile = (ntiles % 8 == 0) ? (w % 8) * (ntiles / 8) + w / 8 : w;
+  tile2d<T, P>(g, ntx, tile, pin, pout, a, sp, sr);
 }
 
 }  // namespace
@@ -880,6 +922,33 @@ template void launch_stag_rb2d<double>(hipStream_t, const Grid &, int, double, d
                                        double *const[3], const StagCoef<double> &, bool);
 template void launch_stag_rb2d<float>(hipStream_t, const Grid &, int, float, float *const[3], float *const[3],
                                       float *const[3], const StagCoef<float> &, bool);
+
+bool cc_rb2d_ok(const Grid &g, int min_tiles) {
+  return g.dim == 2 && g.n[0] % TX2D == 0 && g.n[1] % TY2D == 0 && (g.n[0] / TX2D) * (g.n[1] / TY2D) >= min_tiles;
+}
+
+template <typename T>
+void launch_cc_rb2d(hipStream_t s, const Grid &g, const T *pin, T *pout, const T *rhs, T *const a[3]) {
+  if (!cc_rb2d_ok(g, 1)) {
+    report_error("launch_cc_rb2d: level not eligible");
+    return;
+  }
+  typename CC2RB<T>::Args args = {};  // zeroed padding: kernel arguments are copied bytewise
+  args.inv = (T)(1.0 / (g.dx * g.dx));
+  args.rhs = rhs;
+  args.ax = a[0];
+  args.ay = a[1];
+  const int ntx = g.n[0] / TX2D, ntiles = ntx * (g.n[1] / TY2D);
+  LaunchEvents *ev = launch_events();
+  hipExtLaunchKernelGGL((k_rb_tile2d1<T, CC2RB<T>>), dim3(ntiles), dim3(256), 0, s, ev->start, ev->stop, 0, g, ntx,
+                        ntiles, pin, pout, args);
+  ev->start = ev->stop = nullptr;
+  post_launch(s, "k_rb_tile2d1");
+}
+template void launch_cc_rb2d<double>(hipStream_t, const Grid &, const double *, double *, const double *,
+                                     double *const[3]);
+template void launch_cc_rb2d<float>(hipStream_t, const Grid &, const float *, float *, const float *,
+                                    float *const[3]);
 
 namespace {
 

@@ -733,13 +733,16 @@ template <typename T>
 static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *rhs0) {
   const lmm_params &p = c->prm;
   int nl = c->nl_c;
-  auto fused_at = [&](int n) { return c->fused && fused_rb_tz(C[n].g, c->fused_min_tiles, 4, 1) > 0; };
+  auto fused2d_at = [&](int n) { return c->fused && cc_rb2d_ok(C[n].g, c->fused_min_tiles); };
+  auto fused_at = [&](int n) {
+    return fused2d_at(n) || (c->fused && fused_rb_tz(C[n].g, c->fused_min_tiles, 4, 1) > 0);
+  };
   // a level's first down-leg sweep from its zero guess can take the zeros
   // without reading them (every point of its output is written): then phi is
   // not zeroed first
   auto takes_zeros = [&](int n) {
     const int first = (n < nl - 1) ? p.mg_nsmooths_down : p.mg_nsmooths_bottom;
-    return first >= 1 && fused_at(n) && fused_rb_takes_zeros();
+    return first >= 1 && fused_at(n) && (fused2d_at(n) || fused_rb_takes_zeros());
   };
   // The stopping test of the cell-centred multigrid: max |rhs - L phi| <=
   // mg_rel_tol * max |rhs| or <= 1e-16, the eps and abs_eps that
@@ -771,7 +774,8 @@ static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *
                      (double)(3 + C[n].g.dim - (zin ? 1 : 0)) * C[n].g.N * sizeof(T), true);
         const T *src = zin ? nullptr : (m % 2 == 0) ? ph : C[n].res;
         T *dst = (m % 2 == 0) ? C[n].res : ph;
-        launch_cc_rb_fused<T>(c->stream, C[n].g, src, dst, rh, C[n].a, c->fused_min_tiles);
+        if (fused2d_at(n)) launch_cc_rb2d<T>(c->stream, C[n].g, src, dst, rh, C[n].a);
+        else launch_cc_rb_fused<T>(c->stream, C[n].g, src, dst, rh, C[n].a, c->fused_min_tiles);
       }
       if (sweeps % 2 == 1) hipMemcpyAsync(ph, C[n].res, sizeof(T) * C[n].g.N, hipMemcpyDeviceToDevice, c->stream);
       return;

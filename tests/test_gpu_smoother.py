@@ -237,3 +237,34 @@ def test_fused_2d_gmres_equals_colour(prec, visc, monkeypatch):
     assert out["fused"][2] > 0 and out["colour"][2] == 0
     assert out["fused"][0] == out["colour"][0]
     assert bit_equal(out["fused"][1][0], out["colour"][1][0]) and bit_equal(out["fused"][1][1], out["colour"][1][1])
+
+
+@pytest.mark.parametrize("n,vcycles,sweeps,mode", [((128, 64), 1, (2, 2), "fused"), ((256, 128), 2, (3, 1), "fused"),
+                                                   ((512, 256), 2, (2, 2), None)])
+def test_fused_2d_cc_multigrid_matches_oracle_and_colour(n, vcycles, sweeps, mode, monkeypatch):
+    """The 2D cell-centred red-black sweep in one launch per sweep (64 x 8
+    tiles): with LMM_SMOOTHER=fused on every eligible level, and at the
+    default threshold (64 workgroups: 32768 cells and up, e.g. 256 x 128)."""
+    rng = np.random.default_rng(93)
+    shape = tuple(reversed(n))
+    ainv = [0.5 + rng.random(shape) for _ in range(2)]
+    rhs = rng.standard_normal(shape)
+    rhs -= rhs.mean()
+    kw = dict(mg_max_vcycles=vcycles, mg_nsmooths_down=sweeps[0], mg_nsmooths_up=sweeps[1])
+    want = O.macproject(O.geom(2, n, 0.5), O.params(**kw), ainv, rhs)
+    out = {}
+    for m in ("colour", mode):
+        if m is None:
+            monkeypatch.delenv("LMM_SMOOTHER", raising=False)
+        else:
+            monkeypatch.setenv("LMM_SMOOTHER", m)
+        with S.Context(2, n, 0.5, **kw) as ctx:
+            a = ctx.faces(ainv)
+            r = ctx.field(S.CELL, 1, rhs)
+            phi = ctx.field(S.CELL)
+            ctx.profiling(True)
+            S.macproject(ctx, phi, None, a, r)
+            out[m] = (phi.download(), ctx.kernel_stats("cc_smooth_l0")["launches"])
+    assert out[mode][1] > 0 and out["colour"][1] == 0, "level 0 of the colour run is not fused; of the other run is"
+    assert bit_equal(out[mode][0], out["colour"][0])
+    assert bit_equal(out[mode][0], want)
