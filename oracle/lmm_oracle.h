@@ -209,6 +209,18 @@ int orc_compute_mass_fluxdiv(const orc_geom *g, const orc_species *sp, double *r
                              double *stoch_fluxdiv, const double *Temp, double *flux_total[3],
                              double dt, const double *W, int n_rngs, const double *weights);
 
+/* ---- advection and gravity of the inertial step (orc_inertial.c) ---- */
+/* mk_advective_s_fluxdiv.f90:14-166: s_update (ncomp cell components) -=
+ * div(umac s_fc) component by component; s_fc[d] holds ncomp components. */
+int orc_mk_advective_s_fluxdiv(const orc_geom *g, int ncomp, double *const umac[3],
+                               double *const s_fc[3], double *s_update);
+/* mk_advective_m_fluxdiv.f90:16-331: m_update[d] += -div(m_d u) (centred). */
+int orc_mk_advective_m_fluxdiv(const orc_geom *g, double *const umac[3], double *const m[3],
+                               double *m_update[3]);
+/* mk_grav_force.f90:17-161: m_force[d] += 0.5 grav[d] (s_old[d] + s_new[d]). */
+int orc_mk_grav_force(const orc_geom *g, const double grav[3], double *const s_old[3],
+                      double *const s_new[3], double *m_force[3]);
+
 /* ---- random numbers (src_random/RNGs.c restated) ---- */
 typedef struct {
   uint32_t mt[624];

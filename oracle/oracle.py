@@ -12,6 +12,7 @@ of shape ``(nz, ny, nx)`` (``(ny, nx)`` in 2D), x fastest.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 import subprocess
 
@@ -661,3 +662,227 @@ def compute_mass_fluxdiv(g, sp, rho, Temp, dt, W, weights, gradp_baro=None):
                                           _p(diff), _p(stoch), _p(T), _arr3(flux_total), C.c_double(dt), _p(W),
                                           int(W.shape[0]), _p(w)), "compute_mass_fluxdiv")
     return rho, diff, stoch, flux_total
+
+
+# ---------------------------------------------------------------------------
+# Inertial time step (advance_timestep_inertial.f90:58-990) composed from the
+# oracle's routines.  Periodic box, centred advection (advection_type 0),
+# barodiffusion_type 0 or 2; the wall, reservoir and ghost-cell calls of the
+# reference do nothing on a periodic box.
+
+
+def mk_advective_s_fluxdiv(g, umac, s_fc, s_update):
+    """mk_advective_s_fluxdiv.f90:14-166: s_update - div(umac s_fc); s_fc[d] is
+    (ncomp, *shape), s_update (ncomp, *shape)."""
+    s_update = _c(s_update).copy()
+    ncomp = s_update.shape[0]
+    u = _copy(umac)
+    s = _copy(s_fc)
+    _check(lib().orc_mk_advective_s_fluxdiv(C.byref(g), ncomp, _arr3(u), _arr3(s), _p(s_update)),
+           "mk_advective_s_fluxdiv")
+    return s_update
+
+
+def mk_advective_m_fluxdiv(g, umac, m, m_update):
+    """mk_advective_m_fluxdiv.f90:16-331: m_update + (-div(m u)) per face component."""
+    u, mm, out = _copy(umac), _copy(m), _copy(m_update)
+    _check(lib().orc_mk_advective_m_fluxdiv(C.byref(g), _arr3(u), _arr3(mm), _arr3(out)),
+           "mk_advective_m_fluxdiv")
+    return out
+
+
+def mk_grav_force(g, grav, s_old, s_new, m_force):
+    """mk_grav_force.f90:17-161: m_force + 0.5 grav_d (s_old + s_new)."""
+    gv = (C.c_double * 3)(*[float(x) for x in (list(grav) + [0.0, 0.0, 0.0])[:3]])
+    so, sn, out = _copy(s_old), _copy(s_new), _copy(m_force)
+    _check(lib().orc_mk_grav_force(C.byref(g), gv, _arr3(so), _arr3(sn), _arr3(out)), "mk_grav_force")
+    return out
+
+
+def _faces_comps(g, cc):
+    """average_cc_to_face of every component of cc (ncomp, *shape) -> list over d of (ncomp, *shape)."""
+    per = [average_cc_to_face(g, c) for c in cc]
+    return [np.stack([per[c][d] for c in range(len(cc))]) for d in range(g.dim)]
+
+
+def _edges(g, cc):
+    return [average_cc_to_node(g, cc)] if g.dim == 2 else average_cc_to_edge(g, cc)
+
+
+def compute_rhotot(rho):
+    """mass_flux_utilities.f90:376-420: rhotot = sum over species, from 0."""
+    t = np.zeros_like(rho[0])
+    for c in range(rho.shape[0]):
+        t = t + rho[c]
+    return t
+
+
+def compute_eta_kappa(g, rho, rhotot, visc_coef, eta_model):
+    """compute_mixture_properties.f90:235-298 with compute_eta_2d/3d: eta =
+    visc_coef (the default prob_type branch) or visc_coef*(-5 + 6 rhotot)
+    (prob_type -12); kappa = 1; eta on nodes (2D) / edges (3D)."""
+    if eta_model == 0:
+        eta = np.full_like(rhotot, float(visc_coef))
+    else:
+        eta = visc_coef * (-5.0 + 6.0 * rhotot)
+    return eta, _edges(g, eta), np.ones_like(rhotot)
+
+
+def advance_timestep_inertial(g, p, sp, st, dt, time, grav=(0.0, 0.0, 0.0), variance_coef_mom=1.0,
+                              visc_coef=1.0, eta_model=0, W_m=None, W_m_new=None, W_mass=None,
+                              W_mass_new=None, filtering_width=0):
+    """One inertial step (advance_timestep_inertial.f90:58-990), steps 2-7 in
+    the reference's order of operations.  st holds the state: umac (dim
+    faces), rho_old (ns, *shape), rhotot_old, pi, eta, eta_ed, kappa, Temp,
+    Temp_ed, diff_mass_fluxdiv, stoch_mass_fluxdiv, gradp_baro (dim faces).
+    W_m / W_mass: the momentum noise (W_cc, [W_ed]) and mass noise (1, dim, ns,
+    *shape) the step starts with; W_m_new / W_mass_new: what fill_m_stochastic
+    and fill_mass_stochastic draw in step 7.  Returns the new state and the
+    two solves' info dicts."""
+    dim = g.dim
+    ns = sp.nspecies
+    var_mass = sp.variance_coef_mass
+    baro = sp.barodiffusion_type
+    umac = _copy(st["umac"])
+    rho_old = _c(st["rho_old"])
+    rhotot_old = _c(st["rhotot_old"])
+    pi = _c(st["pi"]).copy()
+    eta, eta_ed, kappa = _c(st["eta"]), _copy(st["eta_ed"]), _c(st["kappa"])
+    Temp, Temp_ed = _c(st["Temp"]), _copy(st["Temp_ed"])
+    diff, stoch = _c(st["diff_mass_fluxdiv"]).copy(), _c(st["stoch_mass_fluxdiv"]).copy()
+    gradp_baro = _copy(st["gradp_baro"]) if st.get("gradp_baro") is not None else [np.zeros_like(pi)
+                                                                                     for _ in range(dim)]
+    shape = pi.shape
+    zf = [np.zeros(shape) for _ in range(dim)]
+    theta_alpha = 1.0 / dt
+    dv = 1.0
+    for _ in range(dim):
+        dv *= g.dx
+    variance = math.sqrt(variance_coef_mom * 2.0 * sp.k_B / (dv * dt))
+
+    def stoch_m(force, e, e_ed, W):
+        wcc = 0.0 + 1.0 * W[0]
+        wed = [0.0 + 1.0 * w for w in W[1]]
+        return stochastic_m_fluxdiv(g, wcc, wed, e, e_ed, Temp, Temp_ed, variance, force,
+                                    filtering_width=filtering_width)
+
+    def mass_flux(rho):
+        W = W_mass if W_mass is not None else np.zeros((1, dim, ns) + shape)
+        return compute_mass_fluxdiv(g, sp, rho, Temp, dt, W, [1.0], gradp_baro if baro != 0 else None)
+
+    # ---- step 2: predictor Euler step for rho (:169-257)
+    rho_fc = _faces_comps(g, rho_old)
+    rhotot_fc_old = average_cc_to_face(g, rhotot_old)
+    ru = 0.0 + diff
+    if var_mass != 0.0:
+        ru = ru + stoch
+    ru = mk_advective_s_fluxdiv(g, umac, rho_fc, ru)
+    ru = ru * dt
+    rho_new = rho_old + ru
+    rhotot_new = compute_rhotot(rho_new)
+    rho_new = (rho_new / rhotot_new) * rhotot_new
+    rho_fc = _faces_comps(g, rho_new)
+    rhotot_fc = average_cc_to_face(g, rhotot_new)
+
+    # ---- step 4: predictor Crank-Nicolson step (:264-496)
+    mold = [u * r for u, r in zip(umac, rhotot_fc_old)]
+    v = [m * (1.0 / dt) for m in mold]
+    gradpi = compute_grad(g, pi)
+    if baro == 2:
+        gradp_baro = _copy(gradpi)
+    v = [a - b for a, b in zip(v, gradpi)]
+    m_a = mk_advective_m_fluxdiv(g, umac, mold, zf)
+    v = [a + b for a, b in zip(v, m_a)]
+    m_d = diffusive_m_fluxdiv(g, p.visc_type, zf, umac, eta, eta_ed, kappa)
+    m_d = [a * 0.5 for a in m_d]
+    v = [a + b for a, b in zip(v, m_d)]
+    m_s = _copy(zf)
+    if variance_coef_mom != 0.0:
+        m_s = stoch_m(m_s, eta, eta_ed, W_m)
+    v = [a + b for a, b in zip(v, m_s)]
+    if any(x != 0.0 for x in grav[:dim]):
+        v = mk_grav_force(g, grav, rhotot_fc_old, rhotot_fc_old, v)
+    eta, eta_ed, kappa = compute_eta_kappa(g, rho_new, rhotot_new, visc_coef, eta_model)
+    rho_new, diff, stoch, _ = mass_flux(rho_new)
+    diff = diff * -1.0
+    if var_mass != 0.0:
+        stoch = stoch * -1.0
+    bp = np.zeros(shape)
+    for c in range(ns):
+        a = -1.0 / sp.rhobar[c]
+        bp = bp + a * diff[c]
+        if var_mass != 0.0:
+            bp = bp + a * stoch[c]
+    mt = [u * r for u, r in zip(umac, rhotot_fc)]
+    mt = [m * (1.0 / dt) for m in mt]
+    v = [a - b for a, b in zip(v, mt)]
+    mt = diffusive_m_fluxdiv(g, p.visc_type, zf, umac, eta, eta_ed, kappa)
+    mt = [a * 0.5 for a in mt]
+    v = [a + b for a, b in zip(v, mt)]
+    ru = 0.0 + diff
+    if var_mass != 0.0:
+        ru = ru + stoch
+    divu = compute_div(g, umac)
+    bp = bp + divu
+    cf = coef(rhotot_fc, eta * 0.5, [e * 0.5 for e in eta_ed], kappa * 0.5)
+    du, dpi, info1 = gmres(g, p, cf, theta_alpha, v, bp)
+    p2 = Params.from_buffer_copy(p)
+    p2.gmres_abs_tol = max(p.gmres_abs_tol, info1["norm_pre_rhs"] * p.gmres_rel_tol)
+    umac = [u + d for u, d in zip(umac, du)]
+    pi = pi + dpi
+    mtemp = [u * r for u, r in zip(umac, rhotot_fc)]
+
+    # ---- step 5: trapezoidal scalar corrector (:585-606)
+    ru = mk_advective_s_fluxdiv(g, umac, rho_fc, ru)
+    rho_new = rho_new + rho_old
+    rho_new = rho_new * 0.5
+    ru = ru * (dt / 2.0)
+    rho_new = rho_new + ru
+    rhotot_new = compute_rhotot(rho_new)
+    rho_new = (rho_new / rhotot_new) * rhotot_new
+    rho_fc = _faces_comps(g, rho_new)
+    rhotot_fc = average_cc_to_face(g, rhotot_new)
+    eta, eta_ed, kappa = compute_eta_kappa(g, rho_new, rhotot_new, visc_coef, eta_model)
+
+    # ---- step 7: corrector Crank-Nicolson step (:641-957)
+    v = [m * (1.0 / dt) for m in mold]
+    gradpi = compute_grad(g, pi)
+    if baro == 2:
+        gradp_baro = _copy(gradpi)
+    v = [a - b for a, b in zip(v, gradpi)]
+    m_a = mk_advective_m_fluxdiv(g, umac, mtemp, m_a)
+    m_a = [a * 0.5 for a in m_a]
+    v = [a + b for a, b in zip(v, m_a)]
+    v = [a + b for a, b in zip(v, m_d)]
+    if variance_coef_mom != 0.0:
+        m_s = stoch_m(m_s, eta, eta_ed, W_m)
+        m_s = [a * 0.5 for a in m_s]
+    v = [a + b for a, b in zip(v, m_s)]
+    if any(x != 0.0 for x in grav[:dim]):
+        v = mk_grav_force(g, grav, rhotot_fc_old, rhotot_fc, v)
+    W_m, W_mass = W_m_new, W_mass_new
+    rho_new, diff, stoch, _ = mass_flux(rho_new)
+    diff = diff * -1.0
+    if var_mass != 0.0:
+        stoch = stoch * -1.0
+    bp = np.zeros(shape)
+    for c in range(ns):
+        a = -1.0 / sp.rhobar[c]
+        bp = bp + a * diff[c]
+        if var_mass != 0.0:
+            bp = bp + a * stoch[c]
+    mt = [u * r for u, r in zip(umac, rhotot_fc)]
+    mt = [m * (1.0 / dt) for m in mt]
+    v = [a - b for a, b in zip(v, mt)]
+    m_d = diffusive_m_fluxdiv(g, p.visc_type, zf, umac, eta, eta_ed, kappa)
+    m_d = [a * 0.5 for a in m_d]
+    v = [a + b for a, b in zip(v, m_d)]
+    divu = compute_div(g, umac)
+    bp = bp + divu
+    cf = coef(rhotot_fc, eta * 0.5, [e * 0.5 for e in eta_ed], kappa * 0.5)
+    du, dpi, info2 = gmres(g, p2, cf, theta_alpha, v, bp)
+    umac = [u + d for u, d in zip(umac, du)]
+    pi = pi + dpi
+    out = dict(umac=umac, rho_new=rho_new, rhotot_new=rhotot_new, pi=pi, eta=eta, eta_ed=eta_ed, kappa=kappa,
+               diff_mass_fluxdiv=diff, stoch_mass_fluxdiv=stoch, gradp_baro=gradp_baro, W_m=W_m, W_mass=W_mass)
+    return out, info1, info2
