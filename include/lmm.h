@@ -448,6 +448,72 @@ int lmm_compute_mass_fluxdiv(lmm_ctx *ctx, lmm_field *rho, lmm_field *const grad
                              lmm_field *flux_total[3], double dt, double stage_time,
                              const double *weights);
 
+/* ---------------- inertial time step ----------------
+ * advance_timestep_inertial.f90:58-990 (algorithm_type 0) on the periodic
+ * box: the predictor Euler step for rho, the Crank-Nicolson predictor solve,
+ * the trapezoidal scalar corrector and the corrector solve, each gmres on
+ * theta_alpha = 1/dt, alpha_fc = rhotot on faces, beta = eta/2, gamma =
+ * kappa/2 with the corrector stopping at max(gmres_abs_tol,
+ * norm_pre_rhs * gmres_rel_tol) of the predictor (:480-486).  Centred
+ * advection only (advection_type 0: mk_advective_s_fluxdiv and
+ * mk_advective_m_fluxdiv; BDS is not built); barodiffusion_type 0 or 2 (3
+ * needs compute_HSE_pres, which needs walls).  The knobs the reference reads
+ * from probin_common (grav, variance_coef_mom, visc_coef, advection_type)
+ * travel here; the species knobs come from lmm_ctx_set_species. */
+typedef struct {
+  double grav[3];           /* probin_common.f90:209 */
+  double variance_coef_mom; /* :213 */
+  double visc_coef;         /* :225 */
+  int eta_model;            /* compute_eta_3d (compute_mixture_properties.f90:352-406):
+                               0 = visc_coef (the default prob_type branch),
+                               1 = visc_coef*(-5 + 6 rhotot) (prob_type -12) */
+  int advection_type;       /* :227; only 0 */
+  int noise;                /* the fills of step 7: 0 = device counter-based stream
+                               (seed, 2*istep) for the momentum noise and (seed,
+                               2*istep+1) for the mass noise; 1 = host MT19937 seeded
+                               with seed + 2*istep and seed + 2*istep + 1 */
+  int reserved;
+  uint64_t noise_seed;
+} lmm_inertial_params;
+
+/* probin_common defaults: no gravity, variance_coef_mom 1, visc_coef 1,
+ * centred advection, device noise with seed 1. */
+void lmm_inertial_params_default(lmm_inertial_params *p);
+
+/* advance_timestep_inertial.f90:58-79 advance_timestep_inertial(mla,umac,
+ * rho_old,rho_new,rhotot_old,rhotot_new,gradp_baro,pi,eta,eta_ed,kappa,Temp,
+ * Temp_ed,diff_mass_fluxdiv,stoch_mass_fluxdiv,dx,dt,time,the_bc_tower,istep).
+ * rho fields have nspecies components.  On entry diff/stoch_mass_fluxdiv
+ * hold the step's starting mass fluxes F^n (as the reference's step 0 or
+ * previous step leaves them) and the context holds the noise the step starts
+ * with (lmm_init_m_stochastic(ctx, 1) / lmm_init_mass_stochastic(ctx, 1) and
+ * a first fill, when the variance coefficients are nonzero).  On return
+ * umac, pi, rho_new, rhotot_new, eta, eta_ed, kappa, the mass fluxes and the
+ * noise are those of t + dt; gradp_baro is written when barodiffusion_type
+ * is 2 (null otherwise).  info (may be null) receives the predictor's and
+ * the corrector's gmres_info. */
+int lmm_advance_timestep_inertial(lmm_ctx *ctx, lmm_field *umac[3], lmm_field *rho_old,
+                                  lmm_field *rho_new, lmm_field *rhotot_old, lmm_field *rhotot_new,
+                                  lmm_field *gradp_baro[3], lmm_field *pi, lmm_field *eta,
+                                  lmm_field *eta_ed[3], lmm_field *kappa, lmm_field *Temp,
+                                  lmm_field *Temp_ed[3], lmm_field *diff_mass_fluxdiv,
+                                  lmm_field *stoch_mass_fluxdiv, double dt, double time,
+                                  int64_t istep, const lmm_inertial_params *params,
+                                  lmm_gmres_info info[2]);
+/* mk_advective_s_fluxdiv.f90:14-21 mk_advective_s_fluxdiv(mla,umac,s_fc,s_update,dx,
+ * start_comp,num_comp): s_update -= div(umac s_fc) for components
+ * start_comp .. start_comp+num_comp-1 (0-based). */
+int lmm_mk_advective_s_fluxdiv(lmm_ctx *ctx, lmm_field *const umac[3], lmm_field *const s_fc[3],
+                               lmm_field *s_update, int start_comp, int num_comp);
+/* mk_advective_m_fluxdiv.f90:16-22 mk_advective_m_fluxdiv(mla,umac,m,m_update,dx,
+ * the_bc_level): m_update += -div(m u), centred. */
+int lmm_mk_advective_m_fluxdiv(lmm_ctx *ctx, lmm_field *const umac[3], lmm_field *const m[3],
+                               lmm_field *m_update[3]);
+/* mk_grav_force.f90:17-22 mk_grav_force(mla,m_force,s_fc_old,s_fc_new,the_bc_tower):
+ * m_force += 0.5 grav (s_fc_old + s_fc_new); grav is probin_common's. */
+int lmm_mk_grav_force(lmm_ctx *ctx, lmm_field *m_force[3], lmm_field *const s_fc_old[3],
+                      lmm_field *const s_fc_new[3], const double grav[3]);
+
 /* ---------------- learned coarse-grid correction ----------------
  * No counterpart in the reference.  A small bias-free 3-level U-Net
  * (3x3x3 periodic convolutions, channels c1/c2/c3 = 16/32/64 by default,

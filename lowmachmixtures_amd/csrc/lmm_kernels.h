@@ -253,6 +253,46 @@ void launch_sub_scalar(hipStream_t s, T *x, long n, const double *scal);
 template <typename T>
 void launch_sub_scalar3(hipStream_t s, const Grid &g, T *const u[3], const double *scal);
 
+// ---- inertial time step (lmm_inertial.hip) ----
+struct AdvArgs {
+  const double *u[3];  // umac
+  const double *s[3];  // face values of the advected components
+  double dxinv;
+};
+struct RhoArgs {
+  double *upd;          // mode 0: the updated cell components
+  const double *diff;   // modes 1, 2: diffusive mass fluxdiv
+  const double *stoch;  // modes 1, 2: stochastic mass fluxdiv, or null
+  const double *rho_old;
+  double *rho_new;
+  double dt;            // mode 1: dt; mode 2: dt/2
+};
+struct RhsVArgs {
+  double *v[3], *mold[3], *gradpi[3], *m_a[3], *m_d[3], *m_s[3], *rfo[3], *rfn[3];
+  double g[3];
+  double inv_dt;
+  int half_ma, half_md, half_ms, grav;
+};
+struct RhobarInv {
+  double v[8];  // -1/rhobar per species
+};
+// s_update (mode 0) or the rho update of the predictor (1) / corrector (2)
+void launch_adv_s(hipStream_t s, const Grid &g, int ncomp, const AdvArgs &a, const RhoArgs &r, int mode);
+void launch_rhotot_conc(hipStream_t s, const Grid &g, int ns, double *rho, double *rhotot);
+void launch_adv_m(hipStream_t s, const Grid &g, double *const u[3], double *const m[3], double *const upd[3]);
+void launch_grav(hipStream_t s, const Grid &g, const double grav[3], double *const f[3], double *const so[3],
+                 double *const sn[3]);
+void launch_mul_faces(hipStream_t s, const Grid &g, double *const out[3], double *const a[3], double *const b[3]);
+void launch_rhs_v(hipStream_t s, const Grid &g, const RhsVArgs &r);
+void launch_v_sub_mt(hipStream_t s, const Grid &g, double *const v[3], double *const u[3], double *const rf[3],
+                     double inv_dt);
+void launch_v_add_half(hipStream_t s, const Grid &g, double *const v[3], double *const w[3]);
+void launch_add(hipStream_t s, double *a, const double *b, long n);
+void launch_rhs_p(hipStream_t s, const Grid &g, int ns, double *bp, double *diff, double *stoch, const double *divu,
+                  const RhobarInv &a);
+void launch_eta(hipStream_t s, const Grid &g, double *eta, double *kappa, const double *rhotot, double visc_coef,
+                int model);
+
 // ---- fluctuating and diffusive fluxes ----
 // width > 0: the scaled fluxes are written to work (2*(dim + 2*ne)*N doubles,
 // ne = 1 in 2D, 3 in 3D) and filtered (multifab_filter) before the divergence

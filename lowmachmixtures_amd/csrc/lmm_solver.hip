@@ -162,6 +162,8 @@ struct lmm_ctx {
   double *dweights_mass = nullptr;
   double *sp_work = nullptr;  // species scratch, grown on demand
   long sp_work_n = 0;
+  double *iw = nullptr;       // inertial-step temporaries, grown on demand
+  long iw_n = 0;
   double hweights_mass[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // weights last copied to dweights_mass
   bool weights_mass_valid = false;
   // HIP graphs of the FP32 preconditioner application, one per output slot
@@ -1651,7 +1653,7 @@ int lmm_ctx_destroy(lmm_ctx *c) {
                   (void *)c->scaled_gamma, (void *)c->V, (void *)c->r, (void *)c->w, (void *)c->tmp, (void *)c->b,
                   (void *)c->x, (void *)c->Z32, (void *)c->b32, (void *)c->mac_rhs, (void *)c->phi_cc,
                   (void *)c->mac_rhs32, (void *)c->Wcc, (void *)c->Wed, (void *)c->dweights, (void *)c->stoch_work,
-                  (void *)c->Wmass, (void *)c->dweights_mass, (void *)c->sp_work})
+                  (void *)c->Wmass, (void *)c->dweights_mass, (void *)c->sp_work, (void *)c->iw})
     dfree(p);
   for (int d = 0; d < 3; d++) {
     dfree(c->ainv[d]);
@@ -2941,6 +2943,327 @@ int lmm_compute_mass_fluxdiv(lmm_ctx *c, lmm_field *rho, lmm_field *const gradp_
   }
   if (!rc && !stoch) launch_fill<double>(c->stream, stoch_fluxdiv->d, (long)ns * N, 0.0);
   return sp_status(c, rc, "compute_mass_fluxdiv");
+}
+
+// ---------------- inertial time step ----------------
+
+void lmm_inertial_params_default(lmm_inertial_params *p) {
+  if (!p) return;
+  memset(p, 0, sizeof(*p));
+  p->variance_coef_mom = 1.0;  // probin_common.f90:213
+  p->visc_coef = 1.0;          // :225
+  p->noise_seed = 1;
+}
+
+int lmm_mk_advective_s_fluxdiv(lmm_ctx *c, lmm_field *const umac[3], lmm_field *const s_fc[3], lmm_field *s_update,
+                               int start_comp, int num_comp) {
+  if (!c) return LMM_ERR_ARG;
+  if (int rc = check_stokes(c, umac, nullptr, "mk_advective_s_fluxdiv")) return rc;
+  if (!s_update || s_update->ctx != c || s_update->kind != LMM_CELL || !comps_ok(s_update, start_comp, num_comp))
+    return set_err(c, LMM_ERR_ARG, "mk_advective_s_fluxdiv: s_update components");
+  AdvArgs a{};
+  const long N = c->g.N;
+  for (int d = 0; d < c->g.dim; d++) {
+    if (!field_ok(c, s_fc[d], LMM_FACE_X + d) || !comps_ok(s_fc[d], start_comp, num_comp))
+      return set_err(c, LMM_ERR_ARG, "mk_advective_s_fluxdiv: s_fc[%d] components", d);
+    a.u[d] = umac[d]->d;
+    a.s[d] = s_fc[d]->d + (long)start_comp * N;
+  }
+  a.dxinv = 1.0 / c->g.dx;
+  RhoArgs r{};
+  r.upd = s_update->d + (long)start_comp * N;
+  launch_adv_s(c->stream, c->g, num_comp, a, r, 0);
+  return finish(c);
+}
+
+int lmm_mk_advective_m_fluxdiv(lmm_ctx *c, lmm_field *const umac[3], lmm_field *const m[3], lmm_field *m_update[3]) {
+  if (!c) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, umac, nullptr, "mk_advective_m_fluxdiv umac")) ||
+      (rc = check_stokes(c, m, nullptr, "mk_advective_m_fluxdiv m")) ||
+      (rc = check_stokes(c, m_update, nullptr, "mk_advective_m_fluxdiv m_update")))
+    return rc;
+  double *u[3] = {nullptr, nullptr, nullptr}, *mm[3] = {nullptr, nullptr, nullptr}, *up[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < c->g.dim; d++) {
+    u[d] = umac[d]->d;
+    mm[d] = m[d]->d;
+    up[d] = m_update[d]->d;
+  }
+  launch_adv_m(c->stream, c->g, u, mm, up);
+  return finish(c);
+}
+
+int lmm_mk_grav_force(lmm_ctx *c, lmm_field *m_force[3], lmm_field *const s_fc_old[3], lmm_field *const s_fc_new[3],
+                      const double grav[3]) {
+  if (!c || !grav) return LMM_ERR_ARG;
+  int rc;
+  if ((rc = check_stokes(c, m_force, nullptr, "mk_grav_force m_force")) ||
+      (rc = check_stokes(c, s_fc_old, nullptr, "mk_grav_force s_fc_old")) ||
+      (rc = check_stokes(c, s_fc_new, nullptr, "mk_grav_force s_fc_new")))
+    return rc;
+  double *f[3] = {nullptr, nullptr, nullptr}, *so[3] = {nullptr, nullptr, nullptr}, *sn[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < c->g.dim; d++) {
+    f[d] = m_force[d]->d;
+    so[d] = s_fc_old[d]->d;
+    sn[d] = s_fc_new[d]->d;
+  }
+  launch_grav(c->stream, c->g, grav, f, so, sn);
+  return finish(c);
+}
+
+// advance_timestep_inertial.f90:58-990 on a periodic box: the reference's
+// multifab sequence, with runs of element-wise calls fused into single
+// passes (lmm_inertial.hip) and the library's own routines for the
+// operators, fluxes and solves.  Temporaries live in the context (c->iw),
+// viewed as fields.
+int lmm_advance_timestep_inertial(lmm_ctx *c, lmm_field *umac[3], lmm_field *rho_old, lmm_field *rho_new,
+                                  lmm_field *rhotot_old, lmm_field *rhotot_new, lmm_field *gradp_baro[3],
+                                  lmm_field *pi, lmm_field *eta, lmm_field *eta_ed[3], lmm_field *kappa,
+                                  lmm_field *Temp, lmm_field *Temp_ed[3], lmm_field *diff_mass_fluxdiv,
+                                  lmm_field *stoch_mass_fluxdiv, double dt, double time, int64_t istep,
+                                  const lmm_inertial_params *ip, lmm_gmres_info info[2]) {
+  if (int rc = need_species(c, "advance_timestep_inertial")) return rc;
+  if (!ip || !(dt > 0)) return set_err(c, LMM_ERR_ARG, "advance_timestep_inertial: no parameters or dt <= 0");
+  if (ip->advection_type != 0)
+    return set_err(c, LMM_ERR_UNSUPPORTED, "advance_timestep_inertial: only centred advection (advection_type 0)");
+  if (ip->eta_model != 0 && ip->eta_model != 1)
+    return set_err(c, LMM_ERR_ARG, "advance_timestep_inertial: eta_model is 0 (constant) or 1 (linear in rhotot)");
+  const lmm_species_params &sp = c->species;
+  const int ns = sp.nspecies, dim = c->g.dim, ne = ne_of(dim);
+  const int baro = sp.barodiffusion_type;
+  if (baro != 0 && baro != 2)
+    return set_err(c, LMM_ERR_UNSUPPORTED, "advance_timestep_inertial: barodiffusion_type %d", baro);
+  const char *what = "advance_timestep_inertial";
+  int rc;
+  if ((rc = check_stokes(c, umac, pi, what))) return rc;
+  SP_FIELD(rho_old, LMM_CELL, ns, what);
+  SP_FIELD(rho_new, LMM_CELL, ns, what);
+  SP_FIELD(diff_mass_fluxdiv, LMM_CELL, ns, what);
+  SP_FIELD(stoch_mass_fluxdiv, LMM_CELL, ns, what);
+  SP_FIELD(rhotot_old, LMM_CELL, 1, what);
+  SP_FIELD(rhotot_new, LMM_CELL, 1, what);
+  SP_FIELD(eta, LMM_CELL, 1, what);
+  SP_FIELD(kappa, LMM_CELL, 1, what);
+  SP_FIELD(Temp, LMM_CELL, 1, what);
+  const int ek = dim == 2 ? LMM_NODE : LMM_EDGE_XY;
+  for (int e = 0; e < ne; e++)
+    if (!field_ok(c, eta_ed[e], ek + e) || !field_ok(c, Temp_ed[e], ek + e))
+      return set_err(c, LMM_ERR_ARG, "%s: eta_ed / Temp_ed[%d] missing or wrong kind", what, e);
+  if (baro == 2 && (rc = check_stokes(c, gradp_baro, nullptr, "advance_timestep_inertial gradp_baro"))) return rc;
+  const bool noise_m = ip->variance_coef_mom != 0.0, noise_mass = sp.variance_coef_mass != 0.0;
+  if (noise_m && c->n_rngs != 1)
+    return set_err(c, LMM_ERR_ARG, "%s: momentum noise needs lmm_init_m_stochastic(ctx, 1) and a first fill", what);
+  if (noise_mass && c->n_rngs_mass != 1)
+    return set_err(c, LMM_ERR_ARG, "%s: mass noise needs lmm_init_mass_stochastic(ctx, 1) and a first fill", what);
+
+  // temporaries: 10 face vectors, 2 face vectors of ns components, ns + 5
+  // cell arrays, ne node/edge arrays
+  const long N = c->g.N;
+  const long need = (long)(10 * dim + 2 * dim * ns + ns + 5 + ne) * N;
+  if (c->iw_n < need) {
+    dfree(c->iw);
+    c->iw = dalloc<double>(c->stream, need);
+    c->iw_n = c->iw ? need : 0;
+    if (!c->iw) return set_err(c, LMM_ERR_NOMEM, "%s: temporaries", what);
+  }
+  double *w = c->iw;
+  auto take = [&](long n) {
+    double *q = w;
+    w += n;
+    return q;
+  };
+  struct Vec {
+    lmm_field f[3];
+    lmm_field *h[3];
+    double *d[3];
+  };
+  auto faces = [&](Vec &v, int ncomp) {
+    for (int d = 0; d < 3; d++) {
+      v.d[d] = d < dim ? take((long)ncomp * N) : nullptr;
+      v.f[d] = lmm_field{c, LMM_FACE_X + d, ncomp, v.d[d]};
+      v.h[d] = d < dim ? &v.f[d] : nullptr;
+    }
+  };
+  Vec mold, mtemp, m_a, m_d, m_s, rv, dumac, rfo, gradpi, rf, rho_fc, flux_total;
+  for (Vec *v : {&mold, &mtemp, &m_a, &m_d, &m_s, &rv, &dumac, &rfo, &gradpi, &rf}) faces(*v, 1);
+  faces(rho_fc, ns);
+  faces(flux_total, ns);
+  lmm_field ru_f{c, LMM_CELL, ns, take((long)ns * N)};
+  lmm_field bp_f{c, LMM_CELL, 1, take(N)}, dpi_f{c, LMM_CELL, 1, take(N)}, divu_f{c, LMM_CELL, 1, take(N)};
+  lmm_field eh_f{c, LMM_CELL, 1, take(N)}, kh_f{c, LMM_CELL, 1, take(N)};
+  lmm_field eeh_f[3];
+  lmm_field *eeh[3] = {nullptr, nullptr, nullptr};
+  for (int e = 0; e < ne; e++) {
+    eeh_f[e] = lmm_field{c, ek + e, 1, take(N)};
+    eeh[e] = &eeh_f[e];
+  }
+  double *u[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < dim; d++) u[d] = umac[d]->d;
+  hipStream_t s = c->stream;
+  const double inv_dt = 1.0 / dt;
+  const double theta_alpha = 1.0 / dt;
+  const double weights[8] = {1.0, 0, 0, 0, 0, 0, 0, 0};
+  RhobarInv rbi{};
+  for (int q = 0; q < ns; q++) rbi.v[q] = -1.0 / sp.rhobar[q];
+  lmm_field *const *gb = baro ? gradp_baro : nullptr;
+  AdvArgs adv{};
+  for (int d = 0; d < dim; d++) {
+    adv.u[d] = u[d];
+    adv.s[d] = rho_fc.d[d];
+  }
+  adv.dxinv = 1.0 / c->g.dx;
+#define ICHK(x)                  \
+  do {                           \
+    if ((rc = (x)) != LMM_OK) return rc; \
+  } while (0)
+  // the viscosity and bulk viscosity at half weight for the solves, and the
+  // divergence of umac and gmres_rhs_p (:410-446, :883-902)
+  auto solve = [&](lmm_gmres_info *inf, double *npr) -> int {
+    launch_scale_copy(s, eh_f.d, eta->d, N, 0.5);
+    launch_scale_copy(s, kh_f.d, kappa->d, N, 0.5);
+    for (int e = 0; e < ne; e++) launch_scale_copy(s, eeh_f[e].d, eta_ed[e]->d, N, 0.5);
+    for (int d = 0; d < dim; d++) launch_fill<double>(s, dumac.d[d], N, 0.0);
+    launch_fill<double>(s, dpi_f.d, N, 0.0);
+    int r = lmm_gmres(c, rv.h, &bp_f, dumac.h, &dpi_f, rf.h, &eh_f, eeh, &kh_f, theta_alpha, npr, inf);
+    if (r) return r;
+    // v = v + dumac; pi = pi + dpi
+    for (int d = 0; d < dim; d++) launch_add(s, u[d], dumac.d[d], N);
+    launch_add(s, pi->d, dpi_f.d, N);
+    return LMM_OK;
+  };
+  // eta, eta_ed, kappa from the composition (compute_eta_kappa)
+  auto eta_kappa = [&]() -> int {
+    launch_eta(s, c->g, eta->d, kappa->d, rhotot_new->d, ip->visc_coef, ip->eta_model);
+    return dim == 2 ? lmm_average_cc_to_node(c, eta, eta_ed[0]) : lmm_average_cc_to_edge(c, eta, eta_ed);
+  };
+  // the mass fluxes at the new state, negated, and gmres_rhs_p (:376-408)
+  auto mass_and_rhs_p = [&]() -> int {
+    int r = lmm_compute_mass_fluxdiv(c, rho_new, gb, diff_mass_fluxdiv, stoch_mass_fluxdiv, Temp, flux_total.h, dt,
+                                     time, weights);
+    if (r) return r;
+    if ((r = lmm_compute_div(c, umac, &divu_f, 0))) return r;
+    launch_rhs_p(s, c->g, ns, bp_f.d, diff_mass_fluxdiv->d, noise_mass ? stoch_mass_fluxdiv->d : nullptr, divu_f.d,
+                 rbi);
+    return LMM_OK;
+  };
+  lmm_gmres_info loc[2];
+  if (!info) info = loc;
+  memset(info, 0, 2 * sizeof(lmm_gmres_info));
+
+  // ---- step 2: predictor Euler step for rho (:169-257)
+  ICHK(lmm_average_cc_to_face_comps(c, rho_old, rho_fc.h, 0, ns));
+  ICHK(lmm_average_cc_to_face(c, rhotot_old, rfo.h));
+  {
+    RhoArgs r{};
+    r.diff = diff_mass_fluxdiv->d;
+    r.stoch = noise_mass ? stoch_mass_fluxdiv->d : nullptr;
+    r.rho_old = rho_old->d;
+    r.rho_new = rho_new->d;
+    r.dt = dt;
+    launch_adv_s(s, c->g, ns, adv, r, 1);
+  }
+  launch_rhotot_conc(s, c->g, ns, rho_new->d, rhotot_new->d);
+  ICHK(lmm_average_cc_to_face_comps(c, rho_new, rho_fc.h, 0, ns));
+  ICHK(lmm_average_cc_to_face(c, rhotot_new, rf.h));
+
+  // ---- step 4: predictor Crank-Nicolson step (:264-496)
+  launch_mul_faces(s, c->g, mold.d, u, rfo.d);
+  ICHK(lmm_compute_grad(c, pi, gradpi.h));
+  if (baro == 2)
+    for (int d = 0; d < dim; d++)
+      hipMemcpyAsync(gradp_baro[d]->d, gradpi.d[d], sizeof(double) * N, hipMemcpyDeviceToDevice, s);
+  for (int d = 0; d < dim; d++) {
+    launch_fill<double>(s, m_a.d[d], N, 0.0);
+    launch_fill<double>(s, m_d.d[d], N, 0.0);
+    launch_fill<double>(s, m_s.d[d], N, 0.0);
+  }
+  launch_adv_m(s, c->g, u, mold.d, m_a.d);
+  ICHK(lmm_diffusive_m_fluxdiv(c, m_d.h, umac, eta, eta_ed, kappa));
+  if (noise_m)
+    ICHK(lmm_stochastic_m_fluxdiv(c, m_s.h, eta, eta_ed, Temp, Temp_ed, dt, weights, ip->variance_coef_mom, sp.k_B));
+  const bool grav = ip->grav[0] != 0.0 || ip->grav[1] != 0.0 || (dim == 3 && ip->grav[2] != 0.0);
+  RhsVArgs rva{};
+  for (int d = 0; d < dim; d++) {
+    rva.v[d] = rv.d[d];
+    rva.mold[d] = mold.d[d];
+    rva.gradpi[d] = gradpi.d[d];
+    rva.m_a[d] = m_a.d[d];
+    rva.m_d[d] = m_d.d[d];
+    rva.m_s[d] = m_s.d[d];
+    rva.rfo[d] = rfo.d[d];
+    rva.g[d] = ip->grav[d];
+  }
+  rva.inv_dt = inv_dt;
+  rva.grav = grav;
+  {
+    RhsVArgs r = rva;
+    for (int d = 0; d < dim; d++) r.rfn[d] = rfo.d[d];  // rho^n g at both ends
+    r.half_md = 1;
+    launch_rhs_v(s, c->g, r);
+  }
+  ICHK(eta_kappa());
+  ICHK(mass_and_rhs_p());
+  launch_v_sub_mt(s, c->g, rv.d, u, rf.d, inv_dt);
+  for (int d = 0; d < dim; d++) launch_fill<double>(s, mtemp.d[d], N, 0.0);
+  ICHK(lmm_diffusive_m_fluxdiv(c, mtemp.h, umac, eta, eta_ed, kappa));
+  launch_v_add_half(s, c->g, rv.d, mtemp.d);
+  double npr = 0.0;
+  ICHK(solve(&info[0], &npr));
+  launch_mul_faces(s, c->g, mtemp.d, u, rf.d);
+
+  // ---- step 5: trapezoidal scalar corrector (:585-606)
+  {
+    RhoArgs r{};
+    r.diff = diff_mass_fluxdiv->d;
+    r.stoch = noise_mass ? stoch_mass_fluxdiv->d : nullptr;
+    r.rho_old = rho_old->d;
+    r.rho_new = rho_new->d;
+    r.dt = dt / 2.0;
+    launch_adv_s(s, c->g, ns, adv, r, 2);
+  }
+  launch_rhotot_conc(s, c->g, ns, rho_new->d, rhotot_new->d);
+  ICHK(lmm_average_cc_to_face_comps(c, rho_new, rho_fc.h, 0, ns));
+  ICHK(lmm_average_cc_to_face(c, rhotot_new, rf.h));
+  ICHK(eta_kappa());
+
+  // ---- step 7: corrector Crank-Nicolson step (:641-957)
+  ICHK(lmm_compute_grad(c, pi, gradpi.h));
+  if (baro == 2)
+    for (int d = 0; d < dim; d++)
+      hipMemcpyAsync(gradp_baro[d]->d, gradpi.d[d], sizeof(double) * N, hipMemcpyDeviceToDevice, s);
+  launch_adv_m(s, c->g, u, mtemp.d, m_a.d);
+  if (noise_m)
+    ICHK(lmm_stochastic_m_fluxdiv(c, m_s.h, eta, eta_ed, Temp, Temp_ed, dt, weights, ip->variance_coef_mom, sp.k_B));
+  {
+    RhsVArgs r = rva;
+    for (int d = 0; d < dim; d++) r.rfn[d] = rf.d[d];
+    r.half_ma = 1;
+    r.half_ms = noise_m ? 1 : 0;
+    launch_rhs_v(s, c->g, r);
+  }
+  // fill_m_stochastic / fill_mass_stochastic: new noise for this stage and
+  // the next step's predictor
+  const uint64_t seed = ip->noise_seed, st = (uint64_t)istep;
+  if (noise_m)
+    ICHK(ip->noise ? lmm_fill_m_stochastic(c, (uint32_t)(seed + 2 * st))
+                   : lmm_fill_m_stochastic_device(c, seed, 2 * st));
+  if (noise_mass)
+    ICHK(ip->noise ? lmm_fill_mass_stochastic(c, (uint32_t)(seed + 2 * st + 1))
+                   : lmm_fill_mass_stochastic_device(c, seed, 2 * st + 1));
+  ICHK(mass_and_rhs_p());
+  launch_v_sub_mt(s, c->g, rv.d, u, rf.d, inv_dt);
+  for (int d = 0; d < dim; d++) launch_fill<double>(s, m_d.d[d], N, 0.0);
+  ICHK(lmm_diffusive_m_fluxdiv(c, m_d.h, umac, eta, eta_ed, kappa));
+  launch_v_add_half(s, c->g, rv.d, m_d.d);
+  // the corrector stops on the predictor's preconditioned right-hand side
+  // (:480-486, :946)
+  const double abs_in = c->prm.gmres_abs_tol;
+  c->prm.gmres_abs_tol = std::max(abs_in, npr * c->prm.gmres_rel_tol);
+  rc = solve(&info[1], nullptr);
+  c->prm.gmres_abs_tol = abs_in;
+  if (rc) return rc;
+#undef ICHK
+  return finish(c);
 }
 
 // ---------------- allocation-free species diffusion ----------------

@@ -86,6 +86,15 @@ class SpeciesParams(C.Structure):
     ]
 
 
+class InertialParams(C.Structure):
+    """lmm_inertial_params: the probin_common knobs of the inertial step."""
+    _fields_ = [
+        ("grav", C.c_double * 3), ("variance_coef_mom", C.c_double), ("visc_coef", C.c_double),
+        ("eta_model", C.c_int), ("advection_type", C.c_int), ("noise", C.c_int), ("reserved", C.c_int),
+        ("noise_seed", C.c_uint64),
+    ]
+
+
 class GmresMember(C.Structure):
     """lmm_gmres_member: one solve of a batch."""
     _fields_ = [
@@ -192,6 +201,13 @@ _SIGS = {
                                               C.POINTER(C.c_double), C.c_int]),
     "lmm_compute_mass_fluxdiv": (C.c_int, [C.c_void_p, _FP, _FP * 3, _FP, _FP, _FP, _FP * 3, C.c_double, C.c_double,
                                            C.POINTER(C.c_double)]),
+    "lmm_inertial_params_default": (None, [C.POINTER(InertialParams)]),
+    "lmm_advance_timestep_inertial": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP, _FP, _FP, _FP * 3, _FP, _FP, _FP * 3,
+                                                _FP, _FP, _FP * 3, _FP, _FP, C.c_double, C.c_double, C.c_int64,
+                                                C.POINTER(InertialParams), C.POINTER(GmresInfo * 2)]),
+    "lmm_mk_advective_s_fluxdiv": (C.c_int, [C.c_void_p, _FP * 3, _FP * 3, _FP, C.c_int, C.c_int]),
+    "lmm_mk_advective_m_fluxdiv": (C.c_int, [C.c_void_p, _FP * 3, _FP * 3, _FP * 3]),
+    "lmm_mk_grav_force": (C.c_int, [C.c_void_p, _FP * 3, _FP * 3, _FP * 3, C.POINTER(C.c_double)]),
 }
 
 
@@ -819,3 +835,54 @@ class Cnn:
 
     def detach(self):
         self.ctx.check(lib().lmm_ctx_set_cnn(self.ctx.h, None), "ctx_set_cnn")
+
+
+# ---------------------------------------------------------------------------
+# inertial time step (advance_timestep_inertial.f90)
+
+def inertial_params(**kw) -> InertialParams:
+    """probin_common defaults (lmm_inertial_params_default), then kw."""
+    p = InertialParams()
+    lib().lmm_inertial_params_default(C.byref(p))
+    for k, v in kw.items():
+        if k == "grav":
+            for i, x in enumerate(list(v) + [0.0] * (3 - len(v))):
+                p.grav[i] = float(x)
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def advance_timestep_inertial(ctx, umac, rho_old, rho_new, rhotot_old, rhotot_new, gradp_baro, pi, eta, eta_ed, kappa,
+                              Temp, Temp_ed, diff_mass_fluxdiv, stoch_mass_fluxdiv, dt, time, istep, params=None):
+    """advance_timestep_inertial.f90:58-79; returns the two solves' info dicts."""
+    p = params if params is not None else inertial_params()
+    info = (GmresInfo * 2)()
+    ctx.check(lib().lmm_advance_timestep_inertial(
+        ctx.h, _a3(umac), _h(rho_old), _h(rho_new), _h(rhotot_old), _h(rhotot_new), _a3(gradp_baro), _h(pi), _h(eta),
+        _a3(eta_ed), _h(kappa), _h(Temp), _a3(Temp_ed), _h(diff_mass_fluxdiv), _h(stoch_mass_fluxdiv),
+        C.c_double(dt), C.c_double(time), C.c_int64(int(istep)), C.byref(p), C.byref(info)),
+        "advance_timestep_inertial")
+    return info[0].as_dict(), info[1].as_dict()
+
+
+def mk_advective_s_fluxdiv(ctx, umac, s_fc, s_update, start_comp=0, num_comp=1):
+    """mk_advective_s_fluxdiv.f90:14-21 (0-based components)."""
+    ctx.check(lib().lmm_mk_advective_s_fluxdiv(ctx.h, _a3(umac), _a3(s_fc), _h(s_update), int(start_comp),
+                                               int(num_comp)), "mk_advective_s_fluxdiv")
+
+
+def mk_advective_m_fluxdiv(ctx, umac, m, m_update):
+    """mk_advective_m_fluxdiv.f90:16-22."""
+    ctx.check(lib().lmm_mk_advective_m_fluxdiv(ctx.h, _a3(umac), _a3(m), _a3(m_update)), "mk_advective_m_fluxdiv")
+
+
+def mk_grav_force(ctx, m_force, s_fc_old, s_fc_new, grav):
+    """mk_grav_force.f90:17-22."""
+    g = (C.c_double * 3)(*[float(x) for x in (list(grav) + [0.0, 0.0, 0.0])[:3]])
+    ctx.check(lib().lmm_mk_grav_force(ctx.h, _a3(m_force), _a3(s_fc_old), _a3(s_fc_new), g), "mk_grav_force")
+
+
+def field_copy(dst, src):
+    """lmm_field_copy: dst = src (same context, kind size and component count)."""
+    dst.ctx.check(lib().lmm_field_copy(dst.h, src.h), "field_copy")
