@@ -84,6 +84,7 @@ def parse():
     ap.add_argument("--ensemble", type=int, default=32, help="members per size of the batched 2D ensemble (0: skip)")
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--species-n", type=int, default=128, help="box of the species mass-flux line (0: skip)")
+    ap.add_argument("--inertial-n", type=int, default=128, help="box of the inertial time-step line (0: skip)")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -336,6 +337,121 @@ def species_chain(S, n, ns, steps, cpu, quiet):
     return out
 
 
+def _inertial_state(n, ns):
+    """C2-like inputs on an n^3 periodic box (prob_type 3 with smoothing_width 2,
+    init_lowmach.f90:702-711): ns species in two layers whose mass fractions
+    blend over a tanh of width 2 dx across y = 1/2, total density from the
+    EOS sum_i c_i / rhobar_i = 1 / rhotot, fluid at rest, uniform viscosity
+    (visc_type 2) and temperature."""
+    c1 = np.array([0.4, 0.3, 0.2, 0.1][:ns])
+    c2 = np.array([0.1, 0.2, 0.3, 0.4][:ns])
+    rhobar = np.array([1.0, 1.1, 1.2, 1.3][:ns])
+    y = (np.arange(n) + 0.5) / n - 0.5
+    blend = 0.5 * (np.tanh(y / (2.0 / n)) + 1.0)
+    c = c1[:, None] + (c2 - c1)[:, None] * blend[None, :]
+    rhotot_y = 1.0 / np.sum(c / rhobar[:, None], axis=0)
+    rho = np.broadcast_to((c * rhotot_y)[:, None, :, None], (ns, n, n, n)).copy()
+    knobs = dict(molmass=list(1e-12 * np.arange(1, ns + 1)), rhobar=list(rhobar),
+                 Dbar=[1e-3, 0.8e-3, 1.2e-3, 0.9e-3, 1.1e-3, 0.7e-3][:ns * (ns - 1) // 2], k_B=1e-12,
+                 variance_coef_mass=1.0)
+    return rho, knobs
+
+
+def inertial_step(S, n, ns, steps, cpu_n, quiet):
+    """advance_timestep_inertial on the C2-like n^3 box with ns species, FP64
+    and FP32-MG solves: time per step and the solves' iterations; species mass
+    conservation; the oracle's time for one step on a cpu_n^3 box of the same
+    set-up, and the relative difference of the GPU's step there."""
+    from lowmachmixtures_amd import problems as P  # noqa: F401  (package import check)
+
+    dt, grav = 1e-3, (0.0, -0.1, 0.0)
+    out = {"workload": f"advance_timestep_inertial, 3D periodic {n}^3, {ns} species, visc_type 2, gravity "
+                       f"{grav[1]} along y, momentum and mass noise (variance coefficients 1), centred advection, "
+                       f"dt {dt}; two gmres solves per step (reference solver settings)",
+           "cells": n ** 3}
+
+    def run(nn, precision, nsteps, noise, seed=3):
+        rho, knobs = _inertial_state(nn, ns)
+        shape = (nn, nn, nn)
+        rhotot = rho.sum(axis=0)
+        ones = np.ones(shape)
+        with S.Context(3, shape, 1.0 / nn, visc_type=2, stoch_stress_form=1, precision=precision) as ctx:
+            S.set_species(ctx, ns, **knobs)
+            S.init_m_stochastic(ctx, 1)
+            S.init_mass_stochastic(ctx, 1)
+            if noise:
+                S.fill_m_stochastic(ctx, 7)
+                S.fill_mass_stochastic(ctx, 8)
+            else:
+                S.fill_m_stochastic_device(ctx, seed, 1 << 40)
+                S.fill_mass_stochastic_device(ctx, seed, (1 << 40) + 1)
+            umac = ctx.faces()
+            r = [ctx.field(S.CELL, ns, rho), ctx.field(S.CELL, ns)]
+            rt = [ctx.field(S.CELL, 1, rhotot), ctx.field(S.CELL, 1)]
+            f_pi, f_eta, f_kap = ctx.field(S.CELL, 1), ctx.field(S.CELL, 1, 1e-2 * ones), ctx.field(S.CELL, 1, ones)
+            f_ed, f_T, f_Ted = ctx.edges([1e-2 * ones] * 3), ctx.field(S.CELL, 1, ones), ctx.edges([ones] * 3)
+            f_diff, f_stoch = ctx.field(S.CELL, ns), ctx.field(S.CELL, ns)
+            prm = S.inertial_params(grav=grav, visc_coef=1e-2, noise=noise, noise_seed=seed)
+            infos, times = [], []
+            for k in range(nsteps):
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                i1, i2 = S.advance_timestep_inertial(ctx, umac, r[0], r[1], rt[0], rt[1], None, f_pi, f_eta, f_ed,
+                                                     f_kap, f_T, f_Ted, f_diff, f_stoch, dt, k * dt, k, prm)
+                ctx.synchronize()
+                times.append(time.perf_counter() - t0)
+                infos.append((i1, i2))
+                S.field_copy(r[0], r[1])
+                S.field_copy(rt[0], rt[1])
+            res = dict(rho=r[0].download(), u=[f.download() for f in umac], pi=f_pi.download())
+        return times, infos, res, rho, knobs
+
+    forms = {}
+    for name, prec in (("fp64", S.FP64), ("fp32_mg", S.FP32_MG)):
+        times, infos, res, rho0, _ = run(n, prec, steps + 1, 0)
+        ms = float(np.mean(times[1:])) * 1e3
+        mass_err = max(abs(res["rho"][c].sum() - rho0[c].sum()) / rho0[c].sum() for c in range(ns))
+        forms[name] = {"ms_per_step": round(ms, 3), "steps_per_s": round(1e3 / ms, 3),
+                       "cells_per_s": round(n ** 3 / (ms * 1e-3), 1),
+                       "gmres_iters": [[i1["total_iter"], i2["total_iter"]] for i1, i2 in infos[1:]],
+                       "status": max(max(i1["status"], i2["status"]) for i1, i2 in infos),
+                       "species_mass_rel_change": float(mass_err),
+                       "max_speed": float(max(np.max(np.abs(u)) for u in res["u"]))}
+        log(f"inertial step {n}^3 x {ns} ({name}): {ms:.2f} ms per step, iters {forms[name]['gmres_iters']}", quiet)
+    out["forms"] = forms
+    if cpu_n > 0:
+        from oracle import oracle as O
+
+        times, infos, res, rho0, knobs = run(cpu_n, S.FP64, 1, 1)
+        shape = (cpu_n,) * 3
+        g = O.geom(3, shape, 1.0 / cpu_n)
+        ones = np.ones(shape)
+        st = dict(umac=[np.zeros(shape)] * 3, rho_old=rho0, rhotot_old=rho0.sum(axis=0), pi=np.zeros(shape),
+                  eta=1e-2 * ones, eta_ed=O._edges(g, 1e-2 * ones), kappa=ones, Temp=ones, Temp_ed=O._edges(g, ones),
+                  diff_mass_fluxdiv=np.zeros((ns,) + shape), stoch_mass_fluxdiv=np.zeros((ns,) + shape),
+                  gradp_baro=None)
+        N = cpu_n ** 3
+        mom = lambda s_: O.fill_m_stochastic(s_, 3, shape, 1, symmetric=True)[0]
+        mass = lambda s_: O.fill_normal(s_, 3 * ns * N).reshape((1, 3, ns) + shape)
+        t0 = time.perf_counter()
+        o, _, _ = O.advance_timestep_inertial(g, O.params(visc_type=2), O.species(ns, **knobs), st, dt, 0.0,
+                                              grav=grav, variance_coef_mom=1.0, visc_coef=1e-2, W_m=mom(7),
+                                              W_m_new=mom(3), W_mass=mass(8), W_mass_new=mass(4))
+        sec = time.perf_counter() - t0
+        rd = lambda a_, b_: float(np.linalg.norm(np.ravel(a_) - np.ravel(b_)) / max(np.linalg.norm(np.ravel(b_)),
+                                                                                    1e-300))
+        out["cpu_oracle"] = {"s_per_step": round(sec, 3), "gpu_ms_per_step_same_box": round(times[0] * 1e3, 3),
+                             "cores": O.lib().orc_max_threads(), "kind": "port",
+                             "sample": f"oracle advance_timestep_inertial on a {cpu_n}^3 box of the same set-up "
+                                       f"(host noise stream on both sides), one step"}
+        out["rel_diff_vs_oracle"] = {"rho_new": rd(res["rho"], o["rho_new"]),
+                                     "umac": rd(np.concatenate([np.ravel(u) for u in res["u"]]),
+                                                np.concatenate([np.ravel(u) for u in o["umac"]])),
+                                     "pi": rd(res["pi"], o["pi"])}
+        log(f"inertial step oracle {cpu_n}^3: {sec:.2f} s per step; rel diff {out['rel_diff_vs_oracle']}", quiet)
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -463,6 +579,8 @@ def main():
 
     if a.species_n > 0:
         out["species_mass_flux"] = species_chain(S, a.species_n, 4, 20, a.cpu_n > 0, a.quiet)
+    if a.inertial_n > 0:
+        out["inertial_step"] = inertial_step(S, a.inertial_n, 4, 3, 32 if a.cpu_n > 0 else 0, a.quiet)
 
     if a.cpu_n > 0:
         from oracle import oracle as O

@@ -75,16 +75,28 @@ module lmm_facade
      integer(c_int) :: correct_flux, barodiffusion_type
   end type lmm_species_params
 
+  ! lmm_inertial_params: probin_common's grav, variance_coef_mom, visc_coef,
+  ! advection_type (probin_common.f90:209-227), the eta model of
+  ! compute_eta_kappa and the noise stream of the time step
+  type, bind(c), public :: lmm_inertial_params
+     real(c_double) :: grav(3), variance_coef_mom, visc_coef
+     integer(c_int) :: eta_model, advection_type, noise, reserved
+     integer(c_int64_t) :: noise_seed
+  end type lmm_inertial_params
+
   public :: lmm_params_default, lmm_ctx_create, lmm_ctx_destroy, lmm_ctx_set_params, lmm_ctx_last_error, &
        lmm_field_create, lmm_field_destroy, lmm_field_upload_fab, lmm_field_download_fab, lmm_fill_normal_host, &
        lmm_gmres, lmm_apply_matrix, lmm_apply_precon, lmm_stag_mg_solver, lmm_macproject, &
        lmm_init_m_stochastic, lmm_fill_m_stochastic, lmm_stochastic_m_fluxdiv, lmm_diffusive_mass_fluxdiv, &
        lmm_diffusive_m_fluxdiv, lmm_species_params_default, lmm_ctx_set_species, lmm_init_mass_stochastic, &
-       lmm_fill_mass_stochastic, lmm_stochastic_mass_fluxdiv, lmm_compute_mass_fluxdiv, lmm_compute_div_comps
+       lmm_fill_mass_stochastic, lmm_stochastic_mass_fluxdiv, lmm_compute_mass_fluxdiv, lmm_compute_div_comps, &
+       lmm_inertial_params_default, lmm_advance_timestep_inertial, lmm_mk_advective_s_fluxdiv, &
+       lmm_mk_advective_m_fluxdiv, lmm_mk_grav_force
   public :: lmm_check, lmm_error_message, lmm_fab_in, lmm_fab_out
   public :: gmres, apply_matrix, apply_precon, stag_mg_solver, macproject, stochastic_m_fluxdiv, &
        diffusive_mass_fluxdiv, diffusive_m_fluxdiv, compute_mass_fluxdiv, stochastic_mass_fluxdiv, &
-       init_mass_stochastic, fill_mass_stochastic, compute_div
+       init_mass_stochastic, fill_mass_stochastic, compute_div, advance_timestep_inertial, &
+       mk_advective_s_fluxdiv, mk_advective_m_fluxdiv, mk_grav_force
 
   interface
      subroutine lmm_params_default(p) bind(c, name='lmm_params_default')
@@ -282,6 +294,47 @@ module lmm_facade
        integer(c_int), value :: start_incomp, start_outcomp, num_comp, increment
      end function lmm_compute_div_comps
 
+     subroutine lmm_inertial_params_default(p) bind(c, name='lmm_inertial_params_default')
+       import :: lmm_inertial_params
+       type(lmm_inertial_params), intent(out) :: p
+     end subroutine lmm_inertial_params_default
+
+     integer(c_int) function lmm_advance_timestep_inertial(ctx, umac, rho_old, rho_new, rhotot_old, rhotot_new, &
+          gradp_baro, pi, eta, eta_ed, kappa, temp, temp_ed, diff_mass_fluxdiv, stoch_mass_fluxdiv, dt, time, &
+          istep, p, info) bind(c, name='lmm_advance_timestep_inertial')
+       import :: c_int, c_int64_t, c_double, c_ptr, lmm_inertial_params, lmm_gmres_info
+       type(c_ptr), value :: ctx, rho_old, rho_new, rhotot_old, rhotot_new, pi, eta, kappa, temp
+       type(c_ptr), value :: diff_mass_fluxdiv, stoch_mass_fluxdiv
+       type(c_ptr), intent(in) :: umac(3), gradp_baro(3), eta_ed(3), temp_ed(3)
+       real(c_double), value :: dt, time
+       integer(c_int64_t), value :: istep
+       type(lmm_inertial_params), intent(in) :: p
+       type(lmm_gmres_info), intent(out) :: info(2)
+     end function lmm_advance_timestep_inertial
+
+     integer(c_int) function lmm_mk_advective_s_fluxdiv(ctx, umac, s_fc, s_update, start_comp, num_comp) &
+          bind(c, name='lmm_mk_advective_s_fluxdiv')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx, s_update
+       type(c_ptr), intent(in) :: umac(3), s_fc(3)
+       integer(c_int), value :: start_comp, num_comp
+     end function lmm_mk_advective_s_fluxdiv
+
+     integer(c_int) function lmm_mk_advective_m_fluxdiv(ctx, umac, m, m_update) &
+          bind(c, name='lmm_mk_advective_m_fluxdiv')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: ctx
+       type(c_ptr), intent(in) :: umac(3), m(3), m_update(3)
+     end function lmm_mk_advective_m_fluxdiv
+
+     integer(c_int) function lmm_mk_grav_force(ctx, m_force, s_fc_old, s_fc_new, grav) &
+          bind(c, name='lmm_mk_grav_force')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: ctx
+       type(c_ptr), intent(in) :: m_force(3), s_fc_old(3), s_fc_new(3)
+       real(c_double), intent(in) :: grav(3)
+     end function lmm_mk_grav_force
+
      integer(c_size_t) function c_strlen(s) bind(c, name='strlen')
        import :: c_size_t, c_ptr
        type(c_ptr), value :: s
@@ -461,6 +514,46 @@ contains
     call lmm_check(ctx, lmm_compute_div_comps(ctx, phi_fc, div, int(start_incomp - 1, c_int), &
          int(start_outcomp - 1, c_int), int(num_comp, c_int), inc), 'compute_div')
   end subroutine compute_div
+
+  ! advance_timestep_inertial.f90:58-79 advance_timestep_inertial(mla,umac,rho_old,rho_new,
+  ! rhotot_old,rhotot_new,gradp_baro,pi,eta,eta_ed,kappa,Temp,Temp_ed,diff_mass_fluxdiv,
+  ! stoch_mass_fluxdiv,dx,dt,time,the_bc_tower,istep); the probin_common knobs in p
+  subroutine advance_timestep_inertial(ctx, umac, rho_old, rho_new, rhotot_old, rhotot_new, gradp_baro, pi, eta, &
+       eta_ed, kappa, temp, temp_ed, diff_mass_fluxdiv, stoch_mass_fluxdiv, dt, time, istep, p, info)
+    type(c_ptr), intent(in) :: ctx, umac(3), rho_old, rho_new, rhotot_old, rhotot_new, gradp_baro(3), pi, eta, &
+         eta_ed(3), kappa, temp, temp_ed(3), diff_mass_fluxdiv, stoch_mass_fluxdiv
+    real(c_double), intent(in) :: dt, time
+    integer, intent(in) :: istep
+    type(lmm_inertial_params), intent(in) :: p
+    type(lmm_gmres_info), intent(out), optional :: info(2)
+    type(lmm_gmres_info) :: loc(2)
+    call lmm_check(ctx, lmm_advance_timestep_inertial(ctx, umac, rho_old, rho_new, rhotot_old, rhotot_new, &
+         gradp_baro, pi, eta, eta_ed, kappa, temp, temp_ed, diff_mass_fluxdiv, stoch_mass_fluxdiv, dt, time, &
+         int(istep, c_int64_t), p, loc), 'advance_timestep_inertial')
+    if (present(info)) info = loc
+  end subroutine advance_timestep_inertial
+
+  ! mk_advective_s_fluxdiv.f90:14 mk_advective_s_fluxdiv(mla,umac,s_fc,s_update,dx,start_comp,num_comp),
+  ! start_comp 1-based as in the reference
+  subroutine mk_advective_s_fluxdiv(ctx, umac, s_fc, s_update, start_comp, num_comp)
+    type(c_ptr), intent(in) :: ctx, umac(3), s_fc(3), s_update
+    integer, intent(in) :: start_comp, num_comp
+    call lmm_check(ctx, lmm_mk_advective_s_fluxdiv(ctx, umac, s_fc, s_update, int(start_comp - 1, c_int), &
+         int(num_comp, c_int)), 'mk_advective_s_fluxdiv')
+  end subroutine mk_advective_s_fluxdiv
+
+  ! mk_advective_m_fluxdiv.f90:16 mk_advective_m_fluxdiv(mla,umac,m,m_update,dx,the_bc_level)
+  subroutine mk_advective_m_fluxdiv(ctx, umac, m, m_update)
+    type(c_ptr), intent(in) :: ctx, umac(3), m(3), m_update(3)
+    call lmm_check(ctx, lmm_mk_advective_m_fluxdiv(ctx, umac, m, m_update), 'mk_advective_m_fluxdiv')
+  end subroutine mk_advective_m_fluxdiv
+
+  ! mk_grav_force.f90:17 mk_grav_force(mla,m_force,s_fc_old,s_fc_new,the_bc_tower)
+  subroutine mk_grav_force(ctx, m_force, s_fc_old, s_fc_new, grav)
+    type(c_ptr), intent(in) :: ctx, m_force(3), s_fc_old(3), s_fc_new(3)
+    real(c_double), intent(in) :: grav(3)
+    call lmm_check(ctx, lmm_mk_grav_force(ctx, m_force, s_fc_old, s_fc_new, grav), 'mk_grav_force')
+  end subroutine mk_grav_force
 
   ! diffusive_m_fluxdiv.f90:16-17
   subroutine diffusive_m_fluxdiv(ctx, m_update, umac, eta, eta_ed, kappa)

@@ -8,6 +8,9 @@ program facade_check
   implicit none
   type(lmm_params) :: p, bad
   type(lmm_species_params) :: sp
+  type(lmm_inertial_params) :: ip
+  type(lmm_gmres_info) :: info2(2)
+  type(c_ptr) :: f3(3)
   type(lmm_gmres_info) :: info
   type(c_ptr) :: ctx, b_u(3), x_u(3), a3(3), e3(3)
   integer(c_int) :: rc
@@ -60,6 +63,16 @@ program facade_check
   rc = lmm_compute_mass_fluxdiv(c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, a3, &
        1.0_c_double, 0.0_c_double, z)
   print '(a,1x,i0)', 'mass_fluxdiv_null_ctx', rc
+  ! inertial step knobs (probin_common defaults) and the null-context rejection
+  call lmm_inertial_params_default(ip)
+  print '(a,1x,es23.16,1x,es23.16)', 'inertial_coefs', ip%variance_coef_mom, ip%visc_coef
+  print '(a,1x,i0,1x,i0,1x,i0)', 'inertial_flags', ip%eta_model, ip%advection_type, ip%noise
+  print '(a,1x,i0)', 'sizeof_inertial', c_sizeof(ip)
+  f3 = c_null_ptr
+  rc = lmm_advance_timestep_inertial(c_null_ptr, f3, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, f3, &
+       c_null_ptr, c_null_ptr, f3, c_null_ptr, c_null_ptr, f3, c_null_ptr, c_null_ptr, 1.0_c_double, &
+       0.0_c_double, 0_c_int64_t, ip, info2)
+  print '(a,1x,i0)', 'inertial_null_ctx', rc
   ! the host Gaussian stream (MT19937 + Acklam)
   rc = lmm_fill_normal_host(99_c_int32_t, 4_c_int64_t, z)
   print '(a,1x,i0)', 'fill_rc', rc

@@ -60,6 +60,7 @@ def test_struct_layouts_match_the_c_abi(run_check):
     assert int(run_check["sizeof_params"][0]) == C.sizeof(S.Params)
     assert int(run_check["sizeof_info"][0]) == C.sizeof(S.GmresInfo)
     assert int(run_check["sizeof_species"][0]) == C.sizeof(S.SpeciesParams)
+    assert int(run_check["sizeof_inertial"][0]) == C.sizeof(S.InertialParams)
 
 
 def test_species_knobs_through_fortran(run_check):
@@ -83,3 +84,10 @@ def test_host_generator_through_fortran(run_check):
     assert run_check["fill_rc"] == ["0"]
     got = np.array([float(x) for x in run_check["normals"]])
     assert np.array_equal(got, O.fill_normal(99, 4))
+
+
+def test_inertial_knobs_through_fortran(run_check):
+    v = run_check
+    assert [float(x) for x in v["inertial_coefs"]] == [1.0, 1.0]  # variance_coef_mom, visc_coef
+    assert v["inertial_flags"] == ["0", "0", "0"]  # constant eta, centred advection, device noise
+    assert v["inertial_null_ctx"] == ["1"]
