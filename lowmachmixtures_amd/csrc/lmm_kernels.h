@@ -57,16 +57,18 @@ struct Faces {
 };
 
 // ---- staggered operator ----
-// Fused red-black sweep of one component (3D, visc_type -1 / 1): reads the
-// old field from pin, writes the new one to pout (lmm_smooth.hip); the form
-// and the z-slab target come from tuning().
+// Fused red-black sweep of one component (3D, visc_type -1, 1, 2, 3): reads
+// the old field from pin, writes the new one to pout (lmm_smooth.hip); the
+// form and the z-slab target come from tuning().  visc_type 2 and 3 read the
+// other components from cur (their state at this component's turn in the
+// colour order).
 // z-slab thickness for element size es (4 or 8 bytes) when one launch
 // sweeps ncomp components, 0: not eligible
 int fused_rb_tz(const Grid &g, int min_tiles, int es, int ncomp);
 bool fused_rb_ok(const Grid &g, int visc, int min_tiles);
 template <typename T>
 void launch_stag_rb_fused(hipStream_t s, const Grid &g, int visc, int a, T omega, const T *pin, T *pout,
-                          const T *rhs, const StagCoef<T> &K, int min_tiles);
+                          const T *rhs, const StagCoef<T> &K, int min_tiles, const T *const cur[3] = nullptr);
 // All three components of one fused sweep in one launch (3D, visc -1 / 1,
 // the paired plane-marching form); reads pin[a] (zero_in: a field of zeros,
 // pin not read), writes pout[a].
@@ -81,6 +83,13 @@ bool fused_rb2d_ok(const Grid &g, int visc, int min_tiles);
 template <typename T>
 void launch_stag_rb2d(hipStream_t s, const Grid &g, int visc, T omega, T *const pin[3], T *const pout[3],
                       T *const rhs[3], const StagCoef<T> &K, bool zero_in);
+// One face component of a 2D fused red-black sweep of a visc_type 2 / 3
+// level (the components couple, so they go one after the other): reads the
+// component from pin and the other component's current state from other.
+bool fused_rb2d_comp_ok(const Grid &g, int visc, int min_tiles);
+template <typename T>
+void launch_stag_rb2d_comp(hipStream_t s, const Grid &g, int visc, int a, T omega, const T *pin, T *pout,
+                           const T *rhs, const StagCoef<T> &K, const T *other);
 // The cell-centred red-black sweep of one 2D level in one launch (x a
 // multiple of 64, y of 8, at least min_tiles workgroups); a null pin is a
 // field of zeros.

@@ -292,17 +292,23 @@ __device__ __forceinline__ void st2(T *a, long p, T x0, T x1) {
 // its six neighbours in the same array (update), with the kernel's scalar
 // arguments in Args.
 
-// Component A of the staggered operator, visc_type -1 or 1.  The terms are
-// those of stag_applyop_3d (off-diagonal sum) and stag_mg_update_3d, in
-// their order.
+// Component A of the staggered operator, visc_type -1, 1, 2 or 3.  The
+// terms are those of stag_applyop_3d (off-diagonal sum) and
+// stag_mg_update_3d, in their order.  visc_type 2 and 3 couple the
+// components: the point reads the other two components (e1, e2 then hold
+// the arrays of t1 and t2, in the state the colour order gives them: already
+// swept components new, later ones old) and the constant coefficients
+// beta[0], gamma[0] (the tile corner).
 template <typename T, int A, int VISC>
 struct StagRB {
   struct Args {
     T omega, dxsqinv;
     const T *rhs, *alpha, *beta, *e1, *e2;
+    const T *gamma;  // visc_type 3
   };
   struct Coef {
     T av, rhs, B0, B1, E10, E11, E20, E21;
+    T G[8];  // visc_type 2: the eight cross-term values; 3: their sum in G[0]
   };
   // the array pointers in a fixed order, and back (the kernels receive them
   // as separate __restrict__ parameters)
@@ -342,6 +348,46 @@ struct StagRB {
       c.E11 = a.e1[p1];
       c.E20 = a.e2[p];
       c.E21 = a.e2[p2];
+    } else if (VISC >= 2) {
+      c.B0 = a.beta[0];
+      c.B1 = (VISC == 3) ? a.gamma[0] : (T)0;
+      c.E10 = c.E11 = c.E20 = c.E21 = (T)0;
+      const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
+      auto at = [&](int ii, int jj, int kk) { return (long)ii + sy * jj + sz * kk; };
+      // pma: -1 along A; p1p / p2p: +1 along t1 / t2; q1 / q2: both shifts
+      const int ia = (A == 0) ? wrapc(i - 1, nx) : i, ja = (A == 1) ? wrapc(j - 1, ny) : j;
+      const int ka = (A == 2) ? wrapc(k - 1, nz) : k;
+      const long pma = at(ia, ja, ka);
+      long p1p, q1, p2p, q2;
+      if (A == 0) {
+        p1p = at(i, wrapc(j + 1, ny), k);
+        q1 = at(ia, wrapc(j + 1, ny), k);
+        p2p = at(i, j, wrapc(k + 1, nz));
+        q2 = at(ia, j, wrapc(k + 1, nz));
+      } else if (A == 1) {
+        p1p = at(wrapc(i + 1, nx), j, k);
+        q1 = at(wrapc(i + 1, nx), ja, k);
+        p2p = at(i, j, wrapc(k + 1, nz));
+        q2 = at(i, ja, wrapc(k + 1, nz));
+      } else {
+        p1p = at(wrapc(i + 1, nx), j, k);
+        q1 = at(wrapc(i + 1, nx), j, ka);
+        p2p = at(i, wrapc(j + 1, ny), k);
+        q2 = at(i, wrapc(j + 1, ny), ka);
+      }
+      const T *G1 = a.e1, *G2 = a.e2;
+      if (VISC == 2) {
+        c.G[0] = G1[p1p];
+        c.G[1] = G1[p];
+        c.G[2] = G1[q1];
+        c.G[3] = G1[pma];
+        c.G[4] = G2[p2p];
+        c.G[5] = G2[p];
+        c.G[6] = G2[q2];
+        c.G[7] = G2[pma];
+      } else {
+        c.G[0] = G1[p1p] - G1[p] - G1[q1] + G1[pma] + G2[p2p] - G2[p] - G2[q2] + G2[pma];
+      }
     } else {
       c.B0 = a.beta[0];  // constant coefficients: beta at the tile corner
       c.B1 = c.E10 = c.E11 = c.E20 = c.E21 = (T)0;
@@ -352,6 +398,11 @@ struct StagRB {
   // vector load per array where the two points' values are adjacent.
   static __device__ __forceinline__ void load_pair(const Grid &g, int i, int j, int k, const Args &a, Coef &c0,
                                                    Coef &c1) {
+    if (VISC >= 2) {
+      c0 = load(g, i, j, k, a);
+      c1 = load(g, i + 1, j, k, a);
+      return;
+    }
     const long sy = g.s[1], sz = g.s[2];
     const long p = i + sy * j + sz * k;
     ld2(a.alpha, p, c0.av, c1.av);
@@ -412,6 +463,28 @@ struct StagRB {
       T s = xp + xm + yp + ym + zp + zm;
       T L = F * (c.av + (T)6.0 * b * dxsqinv) - s * b * dxsqinv;
       T fac = c.av + (T)6.0 * b * dxsqinv;
+      return F + gs_div(omega * (c.rhs - L), fac);
+    } else if (VISC >= 2) {
+      const T Fa = (A == 0) ? xp : (A == 1 ? yp : zp);
+      const T Fma = (A == 0) ? xm : (A == 1 ? ym : zm);
+      const T F1p = (A == 0) ? yp : xp;
+      const T F1m = (A == 0) ? ym : xm;
+      const T F2p = (A == 2) ? yp : zp;
+      const T F2m = (A == 2) ? ym : zm;
+      const T b = c.B0;
+      T L, fac;
+      if (VISC == 2) {
+        T s = (T)2.0 * Fa + (T)2.0 * Fma + F1p + F1m + F2p + F2m;
+        s = s + c.G[0] - c.G[1] - c.G[2] + c.G[3] + c.G[4] - c.G[5] - c.G[6] + c.G[7];
+        L = F * (c.av + (T)8.0 * b * dxsqinv) - s * b * dxsqinv;
+        fac = c.av + (T)8.0 * b * dxsqinv;
+      } else {
+        const T cg = c.B1, fourthirds = (T)(4.0 / 3.0), onethird = (T)(1.0 / 3.0);
+        L = F * (c.av + ((T)20.0 * b * onethird + (T)2.0 * cg) * dxsqinv) -
+            ((Fa + Fma) * (fourthirds * b + cg) + (F1p + F1m + F2p + F2m) * b + c.G[0] * (onethird * b + cg)) *
+                dxsqinv;
+        fac = c.av + ((T)(20.0 / 3.0) * b + (T)2.0 * cg) * dxsqinv;
+      }
       return F + gs_div(omega * (c.rhs - L), fac);
     } else {
       // along / transverse neighbours of component A

@@ -735,9 +735,11 @@ struct Stag2RB {
   struct Args {
     T omega, dxsqinv;
     const T *rhs, *alpha, *beta, *bnd;  // bnd: node beta (2D edges)
+    const T *g1, *gamma;                // visc_type 2, 3: the other component; gamma
   };
   struct Coef {
     T av, rhs, B0, B1, E10, E11;
+    T G[4];  // visc_type 2: the four cross-term values; 3: their sum in G[0]
   };
   static __device__ __forceinline__ Coef load(const Grid &g, int i, int j, const Args &a) {
     Coef c;
@@ -752,6 +754,25 @@ struct Stag2RB {
       c.B1 = a.beta[pma];
       c.E10 = a.bnd[p];
       c.E11 = a.bnd[p1p];
+    } else if (VISC >= 2) {
+      // pma: -1 along A; p1p: +1 along the other direction; q1: both shifts
+      const int nx = g.n[0], ny = g.n[1];
+      const int ia = (A == 0) ? wrapc(i - 1, nx) : i, ja = (A == 1) ? wrapc(j - 1, ny) : j;
+      const long pma = ia + sy * ja;
+      const long p1p = (A == 0) ? i + sy * wrapc(j + 1, ny) : wrapc(i + 1, nx) + sy * j;
+      const long q1 = (A == 0) ? ia + sy * wrapc(j + 1, ny) : wrapc(i + 1, nx) + sy * ja;
+      c.B0 = a.beta[0];
+      c.B1 = (VISC == 3) ? a.gamma[0] : (T)0;
+      c.E10 = c.E11 = (T)0;
+      const T *G1 = a.g1;
+      if (VISC == 2) {
+        c.G[0] = G1[p1p];
+        c.G[1] = G1[p];
+        c.G[2] = G1[q1];
+        c.G[3] = G1[pma];
+      } else {
+        c.G[0] = G1[p1p] - G1[p] - G1[q1] + G1[pma];
+      }
     } else {
       c.B0 = a.beta[0];
       c.B1 = c.E10 = c.E11 = (T)0;
@@ -770,6 +791,16 @@ struct Stag2RB {
       T s = fpa + fpma + fp1p + fp1m;
       L = F * (c.av + (T)4.0 * b * dxsqinv) - s * b * dxsqinv;
       fac = c.av + (T)4.0 * b * dxsqinv;
+    } else if (VISC == 2) {
+      const T b = c.B0;
+      T s = (T)2.0 * fpa + (T)2.0 * fpma + fp1p + fp1m + c.G[0] - c.G[1] - c.G[2] + c.G[3];
+      L = F * (c.av + (T)6.0 * b * dxsqinv) - s * b * dxsqinv;
+      fac = c.av + (T)6.0 * b * dxsqinv;
+    } else if (VISC == 3) {
+      const T b = c.B0, cg = c.B1, fourthirds = (T)(4.0 / 3.0), onethird = (T)(1.0 / 3.0);
+      L = F * (c.av + ((T)14.0 * b * onethird + (T)2.0 * cg) * dxsqinv) -
+          ((fpa + fpma) * (fourthirds * b + cg) + (fp1p + fp1m) * b + c.G[0] * (onethird * b + cg)) * dxsqinv;
+      fac = c.av + ((T)(14.0 / 3.0) * b + (T)2.0 * cg) * dxsqinv;
     } else {
       T dg = c.B0 + c.B1 + c.E11 + c.E10;
       T off = fpa * c.B0 + fpma * c.B1 + fp1p * c.E11 + fp1m * c.E10;
@@ -923,6 +954,52 @@ template void launch_stag_rb2d<double>(hipStream_t, const Grid &, int, double, d
 template void launch_stag_rb2d<float>(hipStream_t, const Grid &, int, float, float *const[3], float *const[3],
                                       float *const[3], const StagCoef<float> &, bool);
 
+bool fused_rb2d_comp_ok(const Grid &g, int visc, int min_tiles) {
+  return g.dim == 2 && (visc == 2 || visc == 3) && g.n[0] % TX2D == 0 && g.n[1] % TY2D == 0 &&
+         (g.n[0] / TX2D) * (g.n[1] / TY2D) >= min_tiles;
+}
+
+template <typename T, int A, int VISC>
+static void launch_rb2d_one(hipStream_t s, const Grid &g, T omega, const T *pin, T *pout, const T *rhs,
+                            const StagCoef<T> &K, const T *g1) {
+  using P = Stag2RB<T, A, VISC>;
+  typename P::Args a = {};
+  a.omega = omega;
+  a.dxsqinv = (T)(1.0 / (g.dx * g.dx));
+  a.rhs = rhs;
+  a.alpha = K.alpha[A];
+  a.beta = K.beta;
+  a.bnd = K.bed[0];
+  a.g1 = g1;
+  a.gamma = K.gamma;
+  const int ntx = g.n[0] / TX2D, ntiles = ntx * (g.n[1] / TY2D);
+  LaunchEvents *ev = launch_events();
+  hipExtLaunchKernelGGL((k_rb_tile2d1<T, P>), dim3(ntiles), dim3(256), 0, s, ev->start, ev->stop, 0, g, ntx, ntiles,
+                        pin, pout, a);
+  ev->start = ev->stop = nullptr;
+  post_launch(s, "k_rb_tile2d1");
+}
+
+template <typename T>
+void launch_stag_rb2d_comp(hipStream_t s, const Grid &g, int visc, int a, T omega, const T *pin, T *pout,
+                           const T *rhs, const StagCoef<T> &K, const T *other) {
+  if (!fused_rb2d_comp_ok(g, visc, 1) || !pin || !other) {
+    report_error("launch_stag_rb2d_comp: level not eligible");
+    return;
+  }
+  if (visc == 2) {
+    if (a == 0) launch_rb2d_one<T, 0, 2>(s, g, omega, pin, pout, rhs, K, other);
+    else launch_rb2d_one<T, 1, 2>(s, g, omega, pin, pout, rhs, K, other);
+  } else {
+    if (a == 0) launch_rb2d_one<T, 0, 3>(s, g, omega, pin, pout, rhs, K, other);
+    else launch_rb2d_one<T, 1, 3>(s, g, omega, pin, pout, rhs, K, other);
+  }
+}
+template void launch_stag_rb2d_comp<double>(hipStream_t, const Grid &, int, int, double, const double *, double *,
+                                            const double *, const StagCoef<double> &, const double *);
+template void launch_stag_rb2d_comp<float>(hipStream_t, const Grid &, int, int, float, const float *, float *,
+                                           const float *, const StagCoef<float> &, const float *);
+
 bool cc_rb2d_ok(const Grid &g, int min_tiles) {
   return g.dim == 2 && g.n[0] % TX2D == 0 && g.n[1] % TY2D == 0 && (g.n[0] / TX2D) * (g.n[1] / TY2D) >= min_tiles;
 }
@@ -954,16 +1031,23 @@ namespace {
 
 template <typename T, int A, int VISC>
 void launch_stag_one(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, const T *pin, T *pout,
-                     const T *rhs, const StagCoef<T> &K) {
+                     const T *rhs, const StagCoef<T> &K, const T *const cur[3]) {
   typename StagRB<T, A, VISC>::Args a = {};
   a.omega = omega;
   a.dxsqinv = dxsqinv;
   a.rhs = rhs;
   a.alpha = K.alpha[A];
   a.beta = K.beta;
-  // xy = 0, xz = 1, yz = 2; edges shared with (A, t1) and (A, t2)
-  a.e1 = (VISC == -1) ? K.bed[A == 2 ? 1 : 0] : nullptr;
-  a.e2 = (VISC == -1) ? K.bed[A == 0 ? 1 : 2] : nullptr;
+  a.gamma = K.gamma;
+  if (VISC >= 2) {
+    // the other components: t1 = y for A = 0, x otherwise; t2 = z for A < 2, y for A = 2
+    a.e1 = cur[A == 0 ? 1 : 0];
+    a.e2 = cur[A == 2 ? 1 : 2];
+  } else {
+    // xy = 0, xz = 1, yz = 2; edges shared with (A, t1) and (A, t2)
+    a.e1 = (VISC == -1) ? K.bed[A == 2 ? 1 : 0] : nullptr;
+    a.e2 = (VISC == -1) ? K.bed[A == 0 ? 1 : 2] : nullptr;
+  }
   launch_rb<T, StagRB<T, A, VISC>>(s, g, tz, pin, pout, a);
 }
 
@@ -1015,27 +1099,29 @@ int fused_rb_tz(const Grid &g, int min_tiles, int es, int ncomp) {
 }
 
 bool fused_rb_ok(const Grid &g, int visc, int min_tiles) {
-  return (visc == -1 || visc == 1) && fused_rb_tz(g, min_tiles, 4, 1) > 0;
+  return (visc == -1 || visc == 1 || visc == 2 || visc == 3) && fused_rb_tz(g, min_tiles, 4, 1) > 0;
 }
 
 template <typename T>
 void launch_stag_rb_fused(hipStream_t s, const Grid &g, int visc, int a, T omega, const T *pin, T *pout,
-                          const T *rhs, const StagCoef<T> &K, int min_tiles) {
+                          const T *rhs, const StagCoef<T> &K, int min_tiles, const T *const cur[3]) {
   const int tz = fused_rb_tz(g, min_tiles, sizeof(T), 1);
-  if (tz <= 0) {
+  if (tz <= 0 || !fused_rb_ok(g, visc, min_tiles) || ((visc == 2 || visc == 3) && !cur)) {
     report_error("launch_stag_rb_fused: level not eligible");
     return;
   }
   const T dxsqinv = (T)(1.0 / (g.dx * g.dx));
-  if (visc == -1) {
-    if (a == 0) launch_stag_one<T, 0, -1>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K);
-    if (a == 1) launch_stag_one<T, 1, -1>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K);
-    if (a == 2) launch_stag_one<T, 2, -1>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K);
-  } else {
-    if (a == 0) launch_stag_one<T, 0, 1>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K);
-    if (a == 1) launch_stag_one<T, 1, 1>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K);
-    if (a == 2) launch_stag_one<T, 2, 1>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K);
-  }
+#define ONE(V)                                                                         \
+  do {                                                                                 \
+    if (a == 0) launch_stag_one<T, 0, V>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K, cur); \
+    if (a == 1) launch_stag_one<T, 1, V>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K, cur); \
+    if (a == 2) launch_stag_one<T, 2, V>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K, cur); \
+  } while (0)
+  if (visc == -1) ONE(-1);
+  else if (visc == 1) ONE(1);
+  else if (visc == 2) ONE(2);
+  else ONE(3);
+#undef ONE
 }
 
 // tile width (64, or 32 on levels 32 points wide) and z-slab of the
@@ -1146,9 +1232,9 @@ template void launch_stag_rb_fused3<double>(hipStream_t, const Grid &, int, doub
 template void launch_stag_rb_fused3<float>(hipStream_t, const Grid &, int, float, float *const[3], float *const[3],
                                            float *const[3], const StagCoef<float> &, int, bool);
 template void launch_stag_rb_fused<double>(hipStream_t, const Grid &, int, int, double, const double *, double *,
-                                           const double *, const StagCoef<double> &, int);
+                                           const double *, const StagCoef<double> &, int, const double *const[3]);
 template void launch_stag_rb_fused<float>(hipStream_t, const Grid &, int, int, float, const float *, float *,
-                                          const float *, const StagCoef<float> &, int);
+                                          const float *, const StagCoef<float> &, int, const float *const[3]);
 template void launch_cc_rb_fused<double>(hipStream_t, const Grid &, const double *, double *, const double *,
                                           double *const[3], int);
 template bool launch_stag_resid_restrict<double>(hipStream_t, const Grid &, const Grid &, int, double *const[3],

@@ -511,7 +511,8 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
     }
     const bool x3 = c->fused && c->fused3 && fused_rb3_ok(L.g, p.visc_type, c->fused_min_tiles);
     const bool x2 = c->fused && fused_rb2d_ok(L.g, p.visc_type, c->fused_min_tiles);
-    if (x2 || x3 || (c->fused && fused_rb_ok(L.g, p.visc_type, c->fused_min_tiles))) {
+    const bool x2c = c->fused && fused_rb2d_comp_ok(L.g, p.visc_type, c->fused_min_tiles);
+    if (x2 || x2c || x3 || (c->fused && fused_rb_ok(L.g, p.visc_type, c->fused_min_tiles))) {
       // sweep m reads one buffer and writes the other (phi <-> L.res); an
       // odd sweep count ends with a copy back
       T *const *src = (m % 2 == 0) ? phi : L.res;
@@ -535,11 +536,20 @@ static void smooth(lmm_ctx *c, SLevel<T> &L, T *const phi[3], T *const rhs[3], i
         launch_stag_rb_fused3<T>(c->stream, L.g, p.visc_type, (T)p.stag_mg_omega, src, dst, rhs, K,
                                  c->fused_min_tiles, zin);
       } else {
+        // one component at a time; visc_type 2 and 3 read the other
+        // components in their state at this component's turn of the colour
+        // order: the ones before it already swept (dst), the later ones old
         for (int a = 0; a < dim; a++) {
+          const T *cur[3] = {nullptr, nullptr, nullptr};
+          for (int b = 0; b < dim; b++) cur[b] = (b < a) ? dst[b] : src[b];
           ProfScope pf(c, "stag_rb_fused", comp_sweep_bytes(L.g, p.visc_type, sizeof(T)));
           ProfScope ps(c, fam, comp_sweep_bytes(L.g, p.visc_type, sizeof(T)), true);
-          launch_stag_rb_fused<T>(c->stream, L.g, p.visc_type, a, (T)p.stag_mg_omega, src[a], dst[a], rhs[a], K,
-                                  c->fused_min_tiles);
+          if (x2c)
+            launch_stag_rb2d_comp<T>(c->stream, L.g, p.visc_type, a, (T)p.stag_mg_omega, src[a], dst[a], rhs[a], K,
+                                     cur[1 - a]);
+          else
+            launch_stag_rb_fused<T>(c->stream, L.g, p.visc_type, a, (T)p.stag_mg_omega, src[a], dst[a], rhs[a], K,
+                                    c->fused_min_tiles, cur);
         }
       }
       if (m == nsweeps - 1 && m % 2 == 0)

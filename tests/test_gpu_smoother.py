@@ -268,3 +268,48 @@ def test_fused_2d_cc_multigrid_matches_oracle_and_colour(n, vcycles, sweeps, mod
     assert out[mode][1] > 0 and out["colour"][1] == 0, "level 0 of the colour run is not fused; of the other run is"
     assert bit_equal(out[mode][0], out["colour"][0])
     assert bit_equal(out[mode][0], want)
+
+
+# visc_type 2 and 3: the components couple, so a fused sweep runs one
+# component per launch (red and black together), reading the other
+# components in their state at that point of the colour order
+
+
+@pytest.mark.parametrize("visc", [2, 3])
+@pytest.mark.parametrize("dim,n,vcycles,sweeps", [(3, (64, 16, 32), 2, (2, 2, 8)), (3, (64, 32, 16), 1, (1, 3, 3)),
+                                                  (2, (128, 64), 2, (2, 2, 8)), (2, (128, 128), 1, (1, 3, 3))])
+def test_fused_coupled_visc_mg_solver_bit_exact(dim, n, vcycles, sweeps, visc, monkeypatch):
+    monkeypatch.setenv("LMM_SMOOTHER", "fused")
+    pb = P.small_problem(dim, n, visc_type=visc, seed=137)
+    kw = dict(visc_type=visc, stag_mg_max_vcycles=vcycles, stag_mg_nsmooths_down=sweeps[0],
+              stag_mg_nsmooths_up=sweeps[1], stag_mg_nsmooths_bottom=sweeps[2])
+    g, p, cf = oracle_args(pb, **kw)
+    rhs = _rand_faces(pb.shape, 16, dim)
+    guess = _rand_faces(pb.shape, 17, dim)
+    want = O.stag_mg_solver(g, p, cf, 0.9, rhs, phi=guess)
+    got, fused = _mg_solve(pb, kw, rhs, guess)
+    assert fused > 0, "the fused smoother did not run"
+    assert bit_equal(got, want), relerr(got, want)
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_mg"])
+@pytest.mark.parametrize("dim,n,visc", [(3, (64, 32, 32), 2), (2, (256, 128), 2), (3, (64, 32, 32), 3)])
+def test_fused_coupled_visc_gmres_equals_colour(dim, n, visc, prec, monkeypatch):
+    """Whole solves at the default launch threshold, against the colour kernels."""
+    pb = P.small_problem(dim, n, visc_type=visc, seed=139, contrast=1.0)
+    b_u = P.rhs_from_noise_host(pb)
+    precision = {"fp64": S.FP64, "fp32_mg": S.FP32_MG}[prec]
+    out = {}
+    for mode in ("colour", None):
+        if mode is None:
+            monkeypatch.delenv("LMM_SMOOTHER", raising=False)
+        else:
+            monkeypatch.setenv("LMM_SMOOTHER", mode)
+        with S.Context(dim, pb.n, pb.dx, visc_type=visc, precision=precision, gmres_max_iter=12) as ctx:
+            dp = S.DeviceProblem(ctx, pb, rhs=b_u)
+            ctx.profiling(True)
+            _, info = dp.solve()
+            out[mode] = (info["total_iter"], dp.solution(), ctx.kernel_stats("stag_rb_fused")["launches"])
+    assert out[None][2] > 0 and out["colour"][2] == 0
+    assert out[None][0] == out["colour"][0]
+    assert bit_equal(out[None][1][0], out["colour"][1][0]) and bit_equal(out[None][1][1], out["colour"][1][1])
