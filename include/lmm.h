@@ -119,7 +119,8 @@ typedef struct {
                               one step ahead */
   int precon_graph;        /* 1: each FP32 preconditioner application of the flexible
                               GMRES is captured once per output slot into a HIP graph and
-                              replayed (same kernels, same bits; fewer host launches) */
+                              replayed (same kernels, same bits; fewer host launches);
+                              default 0: replays ran no faster than eager launches */
 } lmm_kernel_options;
 
 /* ---------------- context ---------------- */

@@ -169,7 +169,7 @@ struct lmm_ctx {
   // HIP graphs of the FP32 preconditioner application, one per output slot
   // (kernel option precon_graph); dropped whenever what they captured may
   // have changed (parameters, kernel options, storage)
-  bool precon_graph = true;
+  bool precon_graph = false;
   struct PGraph {
     hipGraphExec_t exec = nullptr;
     double theta = 0.0;
@@ -1503,6 +1503,7 @@ static lmm_kernel_options env_options() {
   if (pf && pf[0] == '1') o.sweep_prefetch = 1;
   const char *pg = getenv("LMM_PRECON_GRAPH");
   if (pg && pg[0] == '0') o.precon_graph = 0;
+  if (pg && pg[0] == '1') o.precon_graph = 1;
   return o;
 }
 
@@ -1716,7 +1717,7 @@ void lmm_kernel_options_default(lmm_kernel_options *o) {
   o->krylov_fused = 1;
   o->flex_cgs = 1;
   o->sweep_prefetch = 0;
-  o->precon_graph = 1;
+  o->precon_graph = 0;
 }
 
 int lmm_ctx_get_kernel_options(const lmm_ctx *c, lmm_kernel_options *o) {
