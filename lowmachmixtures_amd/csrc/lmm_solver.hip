@@ -116,6 +116,7 @@ struct lmm_ctx {
   // in one single-workgroup launch; LMM_COARSE=0 in the environment keeps
   // one launch per pass on every level (same bits)
   bool coarse_one_launch = true;
+  long coarse_max_cells = COARSE_MAX_CELLS;  // largest level inside the coarse launch
   // the flexible GMRES applies A to its FP32 preconditioned vectors and adds
   // them to x without widening them into FP64 buffers first (same bits);
   // LMM_Z32=0 in the environment widens them first
@@ -637,7 +638,7 @@ static void stag_vcycles(lmm_ctx *c, std::vector<SLevel<T>> &S, T *const phi0[3]
   int nc = nl;
   if (c->coarse_one_launch && p.stag_mg_smoother == 1)
     for (int n = 1; n < nl; n++)
-      if (S[n].g.N <= COARSE_MAX_CELLS && nl - n <= COARSE_MAX_LEVELS) {
+      if (S[n].g.N <= c->coarse_max_cells && nl - n <= COARSE_MAX_LEVELS) {
         nc = n;
         break;
       }
@@ -801,7 +802,7 @@ static void cc_vcycles(lmm_ctx *c, std::vector<CLevel<T>> &C, T *phi0, const T *
   int nc = nl;
   if (c->coarse_one_launch)
     for (int n = 1; n < nl; n++)
-      if (C[n].g.N <= COARSE_MAX_CELLS && nl - n <= COARSE_MAX_LEVELS) {
+      if (C[n].g.N <= c->coarse_max_cells && nl - n <= COARSE_MAX_LEVELS) {
         nc = n;
         break;
       }
@@ -1583,6 +1584,11 @@ int lmm_ctx_create(int dim, const int n_cells[3], double dx, const lmm_params *p
   if (!p) p = &dp;
   lmm_ctx *c = new lmm_ctx();
   apply_options(c, env_options());
+  // LMM_COARSE_MAX_CELLS: a smaller coarse-launch threshold, for measurement
+  if (const char *cm = getenv("LMM_COARSE_MAX_CELLS")) {
+    const long v = atol(cm);
+    if (v >= 8 && v <= COARSE_MAX_CELLS) c->coarse_max_cells = v;
+  }
   c->g = make_grid(dim, n_cells, dx);
   int rc = check_params(c, p);
   if (rc) {
