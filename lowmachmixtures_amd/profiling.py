@@ -12,7 +12,9 @@ same, so their Dispatch_Ids match):
     python -m lowmachmixtures_amd.profiling <dir> <n> "<command text>" <out_trace.txt> <out_counters.txt>
 
 writes a kernel-trace summary (top kernels, level-0 fused sweeps) and a counter
-summary: per level-0 kernel its mean duration, HBM read (FETCH_SIZE scaled by
+summary (``python -m lowmachmixtures_amd.profiling steps <dir> <n> "<command>" <out.txt>``
+instead summarises the kernel trace of a run of the bench's inertial_step line per time
+step): per level-0 kernel its mean duration, HBM read (FETCH_SIZE scaled by
 the factor that makes a plain conversion kernel read its true byte count) and
 write bytes, algorithmic bytes and the two bandwidths, and the
 ``traffic <form> <n> <bytes>`` lines bench.py reads.
@@ -162,7 +164,58 @@ def summarize(d, n, command, out_trace, out_counters):
         f.write("\n".join(t) + "\n")
 
 
+def inertial_steps(d, n, command, out_trace):
+    """Kernel-trace summary of the bench's inertial_step line: its steps are
+    delimited by their first k_adv_s launch (two per step: predictor and
+    corrector); the bench runs 4 FP64 steps then 4 FP32-MG steps, the first of
+    each a warm-up.  Per-step wall time (first to last dispatch), kernel busy
+    time and the per-step kernel breakdown of the timed steps."""
+    rows = sorted(((s_, e_, name) for name, s_, e_ in _trace_rows(d)), key=lambda r: r[0])
+    adv = [i for i, r in enumerate(rows) if "k_adv_s" in r[2]]
+    starts = adv[0::2]
+    # a step starts a few dispatches before its first k_adv_s (the face averages of rho)
+    bounds = [max(0, i - 8) for i in starts] + [len(rows)]
+    steps = [rows[bounds[k]:bounds[k + 1]] for k in range(len(starts))]
+    t = [f"rocprofv3 kernel trace: the inertial time step, 3D periodic {n}^3, 4 species (bench.py inertial_step line)",
+         "", "Command (from the repository root on an MI355X, gfx950):",
+         f"  cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv -d <dir> -o run -- {command}",
+         "Reduced by python -m lowmachmixtures_amd.profiling steps <dir> <n> <command> <out> from run_kernel_trace.csv:",
+         "steps delimited by their first k_adv_s dispatch; FP64 steps first, then FP32-MG, the first of each a warm-up;",
+         "a form's last step runs to the next form's first dispatch (downloads and the next context's set-up).",
+         ""]
+    for k, st in enumerate(steps):
+        wall = (st[-1][1] - st[0][0]) * 1e-6
+        busy = sum(e_ - s_ for s_, e_, _ in st) * 1e-6
+        t.append(f"  step {k} ({'fp64' if k < len(steps) // 2 else 'fp32_mg'}): wall {wall:7.2f} ms, kernel busy "
+                 f"{busy:7.2f} ms, {len(st)} dispatches")
+    half = len(steps) // 2
+    for label, sel in (("FP64 solves", steps[1:half]), ("FP32-MG solves", steps[half + 1:])):
+        if not sel:
+            continue
+        tot = defaultdict(lambda: [0, 0])
+        for st in sel:
+            for s_, e_, name in st:
+                tot[name][0] += 1
+                tot[name][1] += e_ - s_
+        busy = sum(v[1] for v in tot.values()) / len(sel) * 1e-6
+        t += ["", f"{label}: per step ({len(sel)} timed steps), kernel busy {busy:.2f} ms:",
+              f"  {'kernel':60s} {'calls':>7s} {'ms':>8s} {'pct':>6s}"]
+        for name, (c, dsum) in sorted(tot.items(), key=lambda kv: -kv[1][1])[:24]:
+            ms = dsum / len(sel) * 1e-6
+            t.append(f"  {short(name, 60):60s} {c / len(sel):7.1f} {ms:8.3f} {ms / busy * 100:6.2f}")
+    with open(out_trace, "w") as f:
+        f.write("\n".join(t) + "\n")
+
+
+def _trace_rows(d):
+    for r in _rows(_find(d, "kernel_trace.csv")):
+        yield r["Kernel_Name"], int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+
+
 if __name__ == "__main__":
+    if len(sys.argv) == 6 and sys.argv[1] == "steps":
+        inertial_steps(sys.argv[2], int(sys.argv[3]), sys.argv[4], sys.argv[5])
+        sys.exit(0)
     if len(sys.argv) != 6:
         print(__doc__)
         sys.exit(2)
