@@ -353,18 +353,18 @@ __device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int 
   }
 }
 
-template <typename T, typename P, bool PF>
+template <typename T, typename P, bool PF, int TX>
 __global__ void __launch_bounds__(256) k_rb_march2(Grid g, int tz, int ntx, int nty, int ntiles,
                                                    const T *__restrict__ pin, T *__restrict__ pout,
                                                    typename P::Args args0, const T *__restrict__ q0,
                                                    const T *__restrict__ q1, const T *__restrict__ q2,
                                                    const T *__restrict__ q3, const T *__restrict__ q4) {
   const typename P::Args args = P::bind(args0, q0, q1, q2, q3, q4);
-  __shared__ T sp[4][HY][HX];
-  __shared__ T sr[4][RY][RX];
+  __shared__ T sp[4][512 / TX + 4][TX + 4];
+  __shared__ T sr[4][512 / TX + 2][TX + 2];
   int w = blockIdx.x, tile = w;
   if (ntiles % 8 == 0) tile = (w % 8) * (ntiles / 8) + w / 8;
-  march2_tile<T, P, FX, PF>(g, tz, ntx, nty, tile, pin, pout, args, sp, sr);
+  march2_tile<T, P, TX, PF>(g, tz, ntx, nty, tile, pin, pout, args, sp, sr);
 }
 
 // The three face components of one sweep in one launch (visc_type -1 / 1:
@@ -503,8 +503,9 @@ __global__ void __launch_bounds__(256) k_rb_block(Grid g, int nbx, int nby, int 
   }
 }
 
+// tx: the march's tile width (64, or 32 with 32 x 16 tiles)
 template <typename T, typename P>
-void launch_rb(hipStream_t s, const Grid &g, int tz, const T *pin, T *pout, const typename P::Args &args) {
+void launch_rb(hipStream_t s, const Grid &g, int tz, int tx, const T *pin, T *pout, const typename P::Args &args) {
   const T *q[5];
   P::ptrs(args, q);
   if (tuning().rb_variant == 1) {
@@ -515,13 +516,15 @@ void launch_rb(hipStream_t s, const Grid &g, int tz, const T *pin, T *pout, cons
     post_launch(s, kBlockName);
     return;
   }
-  const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
+  const int fy = (tx == FX2) ? FY2 : FY;
+  const int ntx = g.n[0] / tx, nty = g.n[1] / fy, ntz = g.n[2] / tz;
   const int ntiles = ntx * nty * ntz;
   if (tuning().rb_variant == 2) {
     LaunchEvents *ev = launch_events();
     // FP32 prefetches the next plane's coefficients (FP64 would lose occupancy)
     const bool pf = sizeof(T) == 4 && tuning().rb_prefetch;
-    auto kern = pf ? k_rb_march2<T, P, true> : k_rb_march2<T, P, false>;
+    auto kern = (tx == FX2) ? (pf ? k_rb_march2<T, P, true, FX2> : k_rb_march2<T, P, false, FX2>)
+                            : (pf ? k_rb_march2<T, P, true, FX> : k_rb_march2<T, P, false, FX>);
     if (ev->start) {
       hipExtLaunchKernelGGL(kern, dim3(ntiles), dim3(256), 0, s, ev->start, ev->stop, 0, g, tz, ntx, nty, ntiles, pin,
                             pout, args, q[0], q[1], q[2], q[3], q[4]);
@@ -916,9 +919,13 @@ __global__ void __launch_bounds__(256) k_rb_tile2d1(Grid g, int ntx, int ntiles,
 
 }  // namespace
 
+// 2D levels are fused from 16 workgroups on (a lower threshold than 3D's:
+// a 2D level has no z-slabs to spread, and its colour passes are as short)
+static int min_tiles_2d(int min_tiles) { return min_tiles < 16 ? min_tiles : 16; }
+
 bool fused_rb2d_ok(const Grid &g, int visc, int min_tiles) {
   return g.dim == 2 && (visc == -1 || visc == 1) && g.n[0] % TX2D == 0 && g.n[1] % TY2D == 0 &&
-         2 * (g.n[0] / TX2D) * (g.n[1] / TY2D) >= min_tiles;
+         2 * (g.n[0] / TX2D) * (g.n[1] / TY2D) >= min_tiles_2d(min_tiles);
 }
 
 template <typename T, int VISC>
@@ -956,7 +963,7 @@ template void launch_stag_rb2d<float>(hipStream_t, const Grid &, int, float, flo
 
 bool fused_rb2d_comp_ok(const Grid &g, int visc, int min_tiles) {
   return g.dim == 2 && (visc == 2 || visc == 3) && g.n[0] % TX2D == 0 && g.n[1] % TY2D == 0 &&
-         (g.n[0] / TX2D) * (g.n[1] / TY2D) >= min_tiles;
+         2 * (g.n[0] / TX2D) * (g.n[1] / TY2D) >= min_tiles_2d(min_tiles);
 }
 
 template <typename T, int A, int VISC>
@@ -1001,7 +1008,8 @@ template void launch_stag_rb2d_comp<float>(hipStream_t, const Grid &, int, int, 
                                            const float *, const StagCoef<float> &, const float *);
 
 bool cc_rb2d_ok(const Grid &g, int min_tiles) {
-  return g.dim == 2 && g.n[0] % TX2D == 0 && g.n[1] % TY2D == 0 && (g.n[0] / TX2D) * (g.n[1] / TY2D) >= min_tiles;
+  return g.dim == 2 && g.n[0] % TX2D == 0 && g.n[1] % TY2D == 0 &&
+         (g.n[0] / TX2D) * (g.n[1] / TY2D) >= min_tiles_2d(min_tiles);
 }
 
 template <typename T>
@@ -1030,7 +1038,7 @@ template void launch_cc_rb2d<float>(hipStream_t, const Grid &, const float *, fl
 namespace {
 
 template <typename T, int A, int VISC>
-void launch_stag_one(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, const T *pin, T *pout,
+void launch_stag_one(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, int tx, const T *pin, T *pout,
                      const T *rhs, const StagCoef<T> &K, const T *const cur[3]) {
   typename StagRB<T, A, VISC>::Args a = {};
   a.omega = omega;
@@ -1048,7 +1056,7 @@ void launch_stag_one(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, c
     a.e1 = (VISC == -1) ? K.bed[A == 2 ? 1 : 0] : nullptr;
     a.e2 = (VISC == -1) ? K.bed[A == 0 ? 1 : 2] : nullptr;
   }
-  launch_rb<T, StagRB<T, A, VISC>>(s, g, tz, pin, pout, a);
+  launch_rb<T, StagRB<T, A, VISC>>(s, g, tz, tx, pin, pout, a);
 }
 
 }  // namespace
@@ -1075,7 +1083,9 @@ static int rb_target(int es) {
 // components one launch sweeps, down to 2 planes for the paired form (8 for
 // the first form); a launch with fewer than min_tiles workgroups is left to
 // the colour kernels.  rb_tz is the rule for fx x fy tiles.
-static int rb_tz(const Grid &g, int min_tiles, int es, int ncomp, int fx, int fy) {
+// nsweep: the launches one sweep makes of this shape (the components swept
+// one after another), counted against min_tiles with the launch's own tiles
+static int rb_tz(const Grid &g, int min_tiles, int es, int ncomp, int fx, int fy, int nsweep = 1) {
   if (g.dim != 3 || g.n[0] % fx != 0 || g.n[1] % fy != 0) return 0;
   const int per_plane_set = (g.n[0] / fx) * (g.n[1] / fy);
   const int tz_min = (tuning().rb_variant == 2) ? 2 : 8;
@@ -1085,37 +1095,54 @@ static int rb_tz(const Grid &g, int min_tiles, int es, int ncomp, int fx, int fy
   auto fits = [&](int z) { return g.n[2] % z == 0 && ncomp * per_plane_set * (g.n[2] / z) >= target; };
   while (tz > tz_min && !fits(tz)) tz /= 2;
   // at the thinnest slab only the minimum (and divisibility) must hold
-  if (g.n[2] % tz != 0 || ncomp * per_plane_set * (g.n[2] / tz) < min_tiles) return 0;
+  if (g.n[2] % tz != 0 || nsweep * ncomp * per_plane_set * (g.n[2] / tz) < min_tiles) return 0;
   return tz;
 }
 
-int fused_rb_tz(const Grid &g, int min_tiles, int es, int ncomp) {
+// z-slab and tile width of a one-component fused sweep: 64 x 8 tiles, or (the
+// plane-marching form only) 32 x 16 tiles on levels 32 points wide.  nsweep
+// launches of this shape make one sweep (the staggered components one after
+// another, 1 for the cell-centred field); together they must reach min_tiles.
+static int rb1_shape(const Grid &g, int min_tiles, int es, int &tx, int nsweep = 1) {
+  tx = FX;
   if (tuning().rb_variant == 1) {
     if (g.dim != 3 || g.n[0] % BX != 0 || g.n[1] % BY != 0 || g.n[2] % BZ != 0) return 0;
     const int nb = (g.n[0] / BX) * (g.n[1] / BY) * (g.n[2] / BZ);
-    return nb >= min_tiles ? BZ : 0;
+    return nsweep * nb >= min_tiles ? BZ : 0;
   }
+  const int tz = rb_tz(g, min_tiles, es, 1, FX, FY, nsweep);
+  if (tz > 0 || tuning().rb_variant != 2) return tz;
+  tx = FX2;
+  return rb_tz(g, min_tiles, es, 1, FX2, FY2, nsweep);
+}
+
+int fused_rb_tz(const Grid &g, int min_tiles, int es, int ncomp) {
+  int tx;
+  if (ncomp == 1) return rb1_shape(g, min_tiles, es, tx);
+  if (tuning().rb_variant == 1) return rb1_shape(g, min_tiles, es, tx);
   return rb_tz(g, min_tiles, es, ncomp, FX, FY);
 }
 
 bool fused_rb_ok(const Grid &g, int visc, int min_tiles) {
-  return (visc == -1 || visc == 1 || visc == 2 || visc == 3) && fused_rb_tz(g, min_tiles, 4, 1) > 0;
+  int tx;
+  return (visc == -1 || visc == 1 || visc == 2 || visc == 3) && rb1_shape(g, min_tiles, 4, tx, g.dim) > 0;
 }
 
 template <typename T>
 void launch_stag_rb_fused(hipStream_t s, const Grid &g, int visc, int a, T omega, const T *pin, T *pout,
                           const T *rhs, const StagCoef<T> &K, int min_tiles, const T *const cur[3]) {
-  const int tz = fused_rb_tz(g, min_tiles, sizeof(T), 1);
+  int tx = FX;
+  const int tz = rb1_shape(g, min_tiles, sizeof(T), tx, g.dim);
   if (tz <= 0 || !fused_rb_ok(g, visc, min_tiles) || ((visc == 2 || visc == 3) && !cur)) {
     report_error("launch_stag_rb_fused: level not eligible");
     return;
   }
   const T dxsqinv = (T)(1.0 / (g.dx * g.dx));
-#define ONE(V)                                                                         \
-  do {                                                                                 \
-    if (a == 0) launch_stag_one<T, 0, V>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K, cur); \
-    if (a == 1) launch_stag_one<T, 1, V>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K, cur); \
-    if (a == 2) launch_stag_one<T, 2, V>(s, g, omega, dxsqinv, tz, pin, pout, rhs, K, cur); \
+#define ONE(V)                                                                             \
+  do {                                                                                     \
+    if (a == 0) launch_stag_one<T, 0, V>(s, g, omega, dxsqinv, tz, tx, pin, pout, rhs, K, cur); \
+    if (a == 1) launch_stag_one<T, 1, V>(s, g, omega, dxsqinv, tz, tx, pin, pout, rhs, K, cur); \
+    if (a == 2) launch_stag_one<T, 2, V>(s, g, omega, dxsqinv, tz, tx, pin, pout, rhs, K, cur); \
   } while (0)
   if (visc == -1) ONE(-1);
   else if (visc == 1) ONE(1);
@@ -1162,12 +1189,13 @@ void launch_cc_rb_fused(hipStream_t s, const Grid &g, const T *pin, T *pout, con
   args.ax = a[0];
   args.ay = a[1];
   args.az = a[2];
-  const int tz = fused_rb_tz(g, min_tiles, sizeof(T), 1);
+  int tx = FX;
+  const int tz = rb1_shape(g, min_tiles, sizeof(T), tx);
   if (tz <= 0) {
     report_error("launch_cc_rb_fused: level not eligible");
     return;
   }
-  launch_rb<T, CCRB<T>>(s, g, tz, pin, pout, args);
+  launch_rb<T, CCRB<T>>(s, g, tz, tx, pin, pout, args);
 }
 
 // z-slab thickness of the fused residual + restriction, or 0 when the level

@@ -99,7 +99,8 @@ def test_fused_gmres_parity(monkeypatch):
 
 
 @pytest.mark.parametrize("kernel,target", [("march2", "1024"), ("march2", "1"), ("block", "1024"), ("march", "1")])
-@pytest.mark.parametrize("n,vcycles,sweeps", [((64, 16, 32), 1, (2, 2)), ((128, 32, 16), 2, (3, 1))])
+@pytest.mark.parametrize("n,vcycles,sweeps", [((64, 16, 32), 1, (2, 2)), ((128, 32, 16), 2, (3, 1)),
+                                              ((64, 32, 32), 1, (1, 2))])
 def test_fused_cc_multigrid_matches_oracle_and_colour(n, vcycles, sweeps, kernel, target, monkeypatch):
     monkeypatch.setenv("LMM_FUSED_KERNEL", kernel)
     monkeypatch.setenv("LMM_FUSED_TARGET", target)
@@ -128,9 +129,9 @@ def test_fused_cc_multigrid_matches_oracle_and_colour(n, vcycles, sweeps, kernel
                                       ((32, 32, 16), "768")])
 def test_three_components_in_one_launch_equal_separate(visc, n, target, monkeypatch):
     """The default fused sweep runs the three face components in one launch
-    (64 x 8 tiles, or 32 x 16 on levels 32 points wide, which the
-    one-component launches leave to the colour kernels); LMM_FUSED_X3=0
-    launches them one by one.  Same point code, so the same bits."""
+    (64 x 8 tiles, or 32 x 16 on levels 32 points wide, as the plane-marching
+    one-component launches do too); LMM_FUSED_X3=0 launches them one by one.
+    Same point code, so the same bits."""
     monkeypatch.setenv("LMM_SMOOTHER", "fused")
     monkeypatch.setenv("LMM_FUSED_TARGET", target)
     pb = P.small_problem(3, n, visc_type=visc, seed=101)
@@ -205,7 +206,7 @@ def test_fused_2d_mg_solver_bit_exact(n, visc, vcycles, sweeps, monkeypatch):
 
 @pytest.mark.parametrize("visc", [-1, 1])
 def test_fused_2d_equals_colour_kernels(visc, monkeypatch):
-    """At the default launch threshold (64 workgroups: 128^2 and up)."""
+    """At the default launch threshold (16 workgroups on 2D levels: 64^2 and up)."""
     pb = P.small_problem(2, (512, 256), visc_type=visc, seed=127)
     kw = dict(visc_type=visc, stag_mg_max_vcycles=2)
     rhs, guess = _rand_faces(pb.shape, 14, 2), _rand_faces(pb.shape, 15, 2)
@@ -244,7 +245,7 @@ def test_fused_2d_gmres_equals_colour(prec, visc, monkeypatch):
 def test_fused_2d_cc_multigrid_matches_oracle_and_colour(n, vcycles, sweeps, mode, monkeypatch):
     """The 2D cell-centred red-black sweep in one launch per sweep (64 x 8
     tiles): with LMM_SMOOTHER=fused on every eligible level, and at the
-    default threshold (64 workgroups: 32768 cells and up, e.g. 256 x 128)."""
+    default threshold (16 workgroups on 2D levels: 8192 cells and up)."""
     rng = np.random.default_rng(93)
     shape = tuple(reversed(n))
     ainv = [0.5 + rng.random(shape) for _ in range(2)]
