@@ -98,7 +98,7 @@ template <typename T>
 void launch_cc_rb2d(hipStream_t s, const Grid &g, const T *pin, T *pout, const T *rhs, T *const a[3]);
 // Residual of every face component and its full-weighting restriction into
 // the coarse right-hand side crhs in one pass per component (3D, visc_type
-// -1 / 1, x a multiple of 64, y of 8); false (nothing launched) elsewhere.
+// -1, 1, 2, 3, x a multiple of 64, y of 8); false (nothing launched) elsewhere.
 bool stag_resid_restrict_ok(const Grid &g, const Grid &gc, int visc);
 template <typename T>
 bool launch_stag_resid_restrict(hipStream_t s, const Grid &g, const Grid &gc, int visc, T *const phi[3],

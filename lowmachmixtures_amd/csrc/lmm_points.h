@@ -442,6 +442,23 @@ struct StagRB {
       const T b = c.B0;
       T s = xp + xm + yp + ym + zp + zm;
       return F * (c.av + (T)6.0 * b * dxsqinv) - s * b * dxsqinv;
+    } else if (VISC >= 2) {
+      const T Fa = (A == 0) ? xp : (A == 1 ? yp : zp);
+      const T Fma = (A == 0) ? xm : (A == 1 ? ym : zm);
+      const T F1p = (A == 0) ? yp : xp;
+      const T F1m = (A == 0) ? ym : xm;
+      const T F2p = (A == 2) ? yp : zp;
+      const T F2m = (A == 2) ? ym : zm;
+      const T b = c.B0;
+      if (VISC == 2) {
+        T s = (T)2.0 * Fa + (T)2.0 * Fma + F1p + F1m + F2p + F2m;
+        s = s + c.G[0] - c.G[1] - c.G[2] + c.G[3] + c.G[4] - c.G[5] - c.G[6] + c.G[7];
+        return F * (c.av + (T)8.0 * b * dxsqinv) - s * b * dxsqinv;
+      }
+      const T cg = c.B1, fourthirds = (T)(4.0 / 3.0), onethird = (T)(1.0 / 3.0);
+      return F * (c.av + ((T)20.0 * b * onethird + (T)2.0 * cg) * dxsqinv) -
+             ((Fa + Fma) * (fourthirds * b + cg) + (F1p + F1m + F2p + F2m) * b + c.G[0] * (onethird * b + cg)) *
+                 dxsqinv;
     } else {
       const T Fa = (A == 0) ? xp : (A == 1 ? yp : zp);
       const T Fma = (A == 0) ? xm : (A == 1 ? ym : zm);

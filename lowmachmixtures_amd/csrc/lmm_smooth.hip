@@ -1212,17 +1212,25 @@ static int resid_restrict_tz(const Grid &g, const Grid &gc, int es) {
 
 namespace {
 template <typename T, int A, int VISC>
-void launch_rr_one(hipStream_t s, const Grid &g, const Grid &gc, int tz, const T *phi, const T *rhs, T *crhs,
+void launch_rr_one(hipStream_t s, const Grid &g, const Grid &gc, int tz, T *const phi_all[3], const T *rhs, T *crhs,
                    const StagCoef<T> &K) {
   using P = StagRB<T, A, VISC>;
-  typename P::Args a;
+  const T *phi = phi_all[A];
+  typename P::Args a = {};
   a.omega = (T)0;
   a.dxsqinv = (T)(1.0 / (g.dx * g.dx));
   a.rhs = rhs;
   a.alpha = K.alpha[A];
   a.beta = K.beta;
-  a.e1 = (VISC == -1) ? K.bed[A == 2 ? 1 : 0] : nullptr;
-  a.e2 = (VISC == -1) ? K.bed[A == 0 ? 1 : 2] : nullptr;
+  a.gamma = K.gamma;
+  if (VISC >= 2) {
+    // the other components, all in the state the residual sees
+    a.e1 = phi_all[A == 0 ? 1 : 0];
+    a.e2 = phi_all[A == 2 ? 1 : 2];
+  } else {
+    a.e1 = (VISC == -1) ? K.bed[A == 2 ? 1 : 0] : nullptr;
+    a.e2 = (VISC == -1) ? K.bed[A == 0 ? 1 : 2] : nullptr;
+  }
   const T *q[5];
   P::ptrs(a, q);
   const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
@@ -1234,7 +1242,7 @@ void launch_rr_one(hipStream_t s, const Grid &g, const Grid &gc, int tz, const T
 }  // namespace
 
 bool stag_resid_restrict_ok(const Grid &g, const Grid &gc, int visc) {
-  return (visc == -1 || visc == 1) && resid_restrict_tz(g, gc, 4) > 0;
+  return (visc == -1 || visc == 1 || visc == 2 || visc == 3) && resid_restrict_tz(g, gc, 4) > 0;
 }
 
 template <typename T>
@@ -1242,15 +1250,17 @@ bool launch_stag_resid_restrict(hipStream_t s, const Grid &g, const Grid &gc, in
                                 T *const rhs[3], T *const crhs[3], const StagCoef<T> &K) {
   if (!stag_resid_restrict_ok(g, gc, visc)) return false;
   const int tz = resid_restrict_tz(g, gc, sizeof(T));
-  if (visc == -1) {
-    launch_rr_one<T, 0, -1>(s, g, gc, tz, phi[0], rhs[0], crhs[0], K);
-    launch_rr_one<T, 1, -1>(s, g, gc, tz, phi[1], rhs[1], crhs[1], K);
-    launch_rr_one<T, 2, -1>(s, g, gc, tz, phi[2], rhs[2], crhs[2], K);
-  } else {
-    launch_rr_one<T, 0, 1>(s, g, gc, tz, phi[0], rhs[0], crhs[0], K);
-    launch_rr_one<T, 1, 1>(s, g, gc, tz, phi[1], rhs[1], crhs[1], K);
-    launch_rr_one<T, 2, 1>(s, g, gc, tz, phi[2], rhs[2], crhs[2], K);
-  }
+#define RR(V)                                                         \
+  do {                                                                \
+    launch_rr_one<T, 0, V>(s, g, gc, tz, phi, rhs[0], crhs[0], K);    \
+    launch_rr_one<T, 1, V>(s, g, gc, tz, phi, rhs[1], crhs[1], K);    \
+    launch_rr_one<T, 2, V>(s, g, gc, tz, phi, rhs[2], crhs[2], K);    \
+  } while (0)
+  if (visc == -1) RR(-1);
+  else if (visc == 1) RR(1);
+  else if (visc == 2) RR(2);
+  else RR(3);
+#undef RR
   return true;
 }
 

@@ -165,6 +165,10 @@ struct lmm_ctx {
   long sp_work_n = 0;
   double *iw = nullptr;       // inertial-step temporaries, grown on demand
   long iw_n = 0;
+  // set while a composite entry point (the inertial step) calls other entry
+  // points: their completion checks skip the stream synchronisation, the
+  // composite one synchronises once at its end
+  bool defer_sync = false;
   double hweights_mass[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // weights last copied to dweights_mass
   bool weights_mass_valid = false;
   // HIP graphs of the FP32 preconditioner application, one per output slot
@@ -1923,6 +1927,12 @@ double *lmm_field_device_ptr(lmm_field *f) { return f ? f->d : nullptr; }
 // ---------------- operators ----------------
 
 static int finish(lmm_ctx *c) {
+  if (c->defer_sync) {
+    if (first_error()[0]) return set_err(c, LMM_ERR_DEVICE, "kernel %s", first_error());
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return set_err(c, LMM_ERR_DEVICE, "launch: %s", hipGetErrorString(e));
+    return LMM_OK;
+  }
   hipError_t e = hipStreamSynchronize(c->stream);
   if (first_error()[0]) return set_err(c, LMM_ERR_DEVICE, "kernel %s", first_error());
   if (e != hipSuccess) return set_err(c, LMM_ERR_DEVICE, "stream: %s", hipGetErrorString(e));
@@ -3166,6 +3176,12 @@ int lmm_advance_timestep_inertial(lmm_ctx *c, lmm_field *umac[3], lmm_field *rho
   lmm_gmres_info loc[2];
   if (!info) info = loc;
   memset(info, 0, 2 * sizeof(lmm_gmres_info));
+  // one synchronisation at the end instead of one per inner call
+  struct DeferSync {
+    lmm_ctx *c;
+    explicit DeferSync(lmm_ctx *x) : c(x) { c->defer_sync = true; }
+    ~DeferSync() { c->defer_sync = false; }
+  } defer(c);
 
   // ---- step 2: predictor Euler step for rho (:169-257)
   ICHK(lmm_average_cc_to_face_comps(c, rho_old, rho_fc.h, 0, ns));
@@ -3280,6 +3296,7 @@ int lmm_advance_timestep_inertial(lmm_ctx *c, lmm_field *umac[3], lmm_field *rho
   c->prm.gmres_abs_tol = abs_in;
   if (rc) return rc;
 #undef ICHK
+  c->defer_sync = false;
   return finish(c);
 }
 

@@ -1,7 +1,7 @@
 """The fused residual + restriction of the staggered multigrid's down-leg
 against the two separate passes (residual into a fine buffer, then the
 full-weighting restriction) and against the oracle.  Levels with x a
-multiple of 64 and y of 8 (3D, visc_type -1 / 1) use the fused pass;
+multiple of 64 and y of 8 (3D, visc_type -1, 1, 2, 3) use the fused pass;
 LMM_RESID_RESTRICT=0 keeps the two passes everywhere.  Every coarse value
 sums the same fine residuals in the same order, so the bits agree."""
 import numpy as np
@@ -31,7 +31,8 @@ def _mg(pb, kw, rhs, guess):
 
 @pytest.mark.parametrize("n,visc,target", [((128, 32, 32), -1, "1024"), ((128, 32, 32), 1, "1024"),
                                            ((64, 64, 64), -1, "1"), ((64, 16, 32), -1, "1"),
-                                           ((64, 8, 16), 1, "1024")])
+                                           ((64, 8, 16), 1, "1024"), ((128, 32, 32), 2, "1024"),
+                                           ((64, 16, 32), 3, "1")])
 def test_resid_restrict_fused_equals_two_passes(n, visc, target, monkeypatch):
     monkeypatch.setenv("LMM_FUSED_TARGET", target)
     pb = P.small_problem(3, n, visc_type=visc, seed=91)
@@ -45,10 +46,11 @@ def test_resid_restrict_fused_equals_two_passes(n, visc, target, monkeypatch):
     assert bit_equal(got, ref)
 
 
-def test_resid_restrict_fused_matches_oracle(monkeypatch):
+@pytest.mark.parametrize("visc", [-1, 2, 3])
+def test_resid_restrict_fused_matches_oracle(visc, monkeypatch):
     monkeypatch.setenv("LMM_RESID_RESTRICT", "1")
-    pb = P.small_problem(3, (128, 16, 32), visc_type=-1, seed=93)
-    kw = dict(visc_type=-1)
+    pb = P.small_problem(3, (128, 16, 32), visc_type=visc, seed=93)
+    kw = dict(visc_type=visc)
     g, p, cf = oracle_args(pb, **kw)
     rhs, guess = _faces(pb.shape, 7), _faces(pb.shape, 8)
     want = O.stag_mg_solver(g, p, cf, 0.8, rhs, phi=guess)
