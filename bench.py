@@ -85,6 +85,7 @@ def parse():
     ap.add_argument("--cpu-runs", type=int, default=3)
     ap.add_argument("--species-n", type=int, default=128, help="box of the species mass-flux line (0: skip)")
     ap.add_argument("--inertial-n", type=int, default=128, help="box of the inertial time-step line (0: skip)")
+    ap.add_argument("--autotune", type=int, default=0, help="1: lmm_ctx_autotune before each form's solves")
     ap.add_argument("--quiet", action="store_true")
     return ap.parse_args()
 
@@ -170,9 +171,12 @@ def train_learned(P, S, a, quiet):
     return params, stats
 
 
-def time_form(S, ctx, dp, form, steps, warmup, quiet):
+def time_form(S, ctx, dp, form, steps, warmup, quiet, autotune=False):
     prec = {"fp64": S.FP64, "fp32_mg": S.FP32_MG, "learned": S.LEARNED}[form]
     ctx.set_params(precision=prec)
+    if autotune:
+        t = ctx.autotune()
+        log(f"{form}: autotuned z-slab target {t} workgroups", quiet)
     info = None
     for w in range(warmup):
         dp.zero_solution()
@@ -490,7 +494,7 @@ def main():
             learned_stats["ablation_err_learned"] = round(e1, 5)
             log(f"learned ablation at {n}^3: preconditioner error {e0:.4f} plain, {e1:.4f} with the stage", a.quiet)
         for form in forms:
-            ms, info, iters, sm, fams = time_form(S, ctx, dp, form, a.steps, a.warmup, a.quiet)
+            ms, info, iters, sm, fams = time_form(S, ctx, dp, form, a.steps, a.warmup, a.quiet, a.autotune > 0)
             res = rel_true_residual(S, ctx, dp, tmp_u, tmp_p)
             sols[form] = dp.solution()
             achieved = sm["bytes"] / (sm["ms"] * 1e-3) / 1e9 if sm["ms"] > 0 else 0.0
@@ -510,6 +514,7 @@ def main():
                     "frac_hbm": round(achieved / HBM_PEAK_GBS, 4),
                 },
                 "time_by_family_ms": {k: round(v["ms"], 3) for k, v in fams.items()},
+                "fused_target": ctx.kernel_options()["fused_target"],
             }
             # whole-solve bytes of the families that account them (all but the
             # coarse single-workgroup V-cycles, prolongation and the learned stage)

@@ -123,6 +123,14 @@ typedef struct {
                               default 0: replays ran no faster than eager launches */
 } lmm_kernel_options;
 
+/* No counterpart in the reference: times the context's fine-level fused
+ * sweeps (staggered and cell-centred, in the context's precision) for a set
+ * of z-slab workgroup targets, 256 .. 2048, on this GPU, and keeps the
+ * fastest as the context's fused_target (returned in *fused_target; may be
+ * null).  The slab shape never changes results.  The level buffers it runs
+ * on are scratch between calls; the call synchronises the context. */
+int lmm_ctx_autotune(lmm_ctx *ctx, int *fused_target);
+
 /* ---------------- context ---------------- */
 
 /* Fills p with the reference defaults (probin_gmres.f90:119-157, visc_type 1). */

@@ -1745,6 +1745,100 @@ int lmm_ctx_set_kernel_options(lmm_ctx *c, const lmm_kernel_options *o) {
   return LMM_OK;
 }
 
+}  // extern "C"
+
+// Device time of `reps` smoothing sweeps of the finest staggered level plus
+// as many fused sweeps of the finest cell-centred level, on the context's own
+// level storage (scratch arrays where level 0 borrows the caller's fields).
+// The values are whatever the buffers hold: the kernels have no
+// data-dependent control flow, so the time does not depend on them, and
+// every buffer written is rewritten before its next use by a solve.
+template <typename T>
+static int time_fine_sweeps(lmm_ctx *c, std::vector<SLevel<T>> &S, std::vector<CLevel<T>> &C, int reps,
+                            float *ms) {
+  const long N = c->g.N;
+  const int dim = c->g.dim, ne = ne_of(dim);
+  std::vector<void *> tmp;
+  auto need = [&](T *p) -> T * {
+    if (p) return p;
+    T *q = dalloc<T>(c->stream, N);
+    tmp.push_back(q);
+    return q;
+  };
+  SLevel<T> L = S[0];
+  T *phi[3] = {nullptr, nullptr, nullptr}, *rhs[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < dim; d++) {
+    phi[d] = need(L.phi[d]);
+    rhs[d] = need(L.rhs[d]);
+  }
+  L.beta = need(L.beta);
+  L.gamma = need(L.gamma);
+  for (int e = 0; e < ne; e++) L.bed[e] = need(L.bed[e]);
+  CLevel<T> &Z = C[0];
+  T *cin = need(Z.phi), *crhs = need(Z.rhs);
+  for (void *q : tmp)
+    if (!q) {
+      for (void *r : tmp) dfree(r);
+      return LMM_ERR_NOMEM;
+    }
+  const bool cc_fused = c->fused && fused_rb_tz(Z.g, c->fused_min_tiles, 4, 1) > 0;
+  auto run = [&]() {
+    smooth<T>(c, L, phi, rhs, reps);
+    if (cc_fused)
+      for (int m = 0; m < reps; m++) launch_cc_rb_fused<T>(c->stream, Z.g, cin, Z.res, crhs, Z.a, c->fused_min_tiles);
+  };
+  run();  // warm-up
+  hipEvent_t a, b;
+  hipEventCreate(&a);
+  hipEventCreate(&b);
+  hipEventRecord(a, c->stream);
+  run();
+  hipEventRecord(b, c->stream);
+  hipEventSynchronize(b);
+  hipEventElapsedTime(ms, a, b);
+  hipEventDestroy(a);
+  hipEventDestroy(b);
+  for (void *q : tmp) dfree(q);
+  if (first_error()[0]) return set_err(c, LMM_ERR_DEVICE, "kernel %s", first_error());
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return set_err(c, LMM_ERR_DEVICE, "autotune: %s", hipGetErrorString(e));
+  return LMM_OK;
+}
+
+extern "C" {
+
+int lmm_ctx_autotune(lmm_ctx *c, int *fused_target) {
+  if (!c) return LMM_ERR_ARG;
+  TuningScope ts(&c->tune);
+  hipStreamSynchronize(c->stream);
+  const bool mixed = c->prm.precision != LMM_FP64;
+  if (mixed)
+    if (int rc = ensure_lvl32(c)) return rc;
+  static const int cand[] = {256, 384, 512, 768, 1024, 1536, 2048};
+  const int keep = c->tune.rb_target;
+  int best = keep;
+  float best_ms = 0.f;
+  for (int t : cand) {
+    c->tune.rb_target = t;
+    float ms = 0.f;
+    int rc = mixed ? time_fine_sweeps<float>(c, c->s32, c->c32, 4, &ms)
+                   : time_fine_sweeps<double>(c, c->s64, c->c64, 4, &ms);
+    if (rc) {
+      c->tune.rb_target = keep;
+      return rc;
+    }
+    if (best_ms == 0.f || ms < best_ms) {
+      best_ms = ms;
+      best = t;
+    }
+  }
+  lmm_kernel_options o = c->kopt;
+  o.fused_target = best;
+  apply_options(c, o);  // drops any captured graph: their slab shapes change
+  if (fused_target) *fused_target = best;
+  return LMM_OK;
+}
+
 int lmm_ctx_synchronize(lmm_ctx *c) {
   if (!c) return LMM_ERR_ARG;
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -117,6 +117,7 @@ _SIGS = {
     "lmm_kernel_options_default": (None, [C.POINTER(KernelOptions)]),
     "lmm_ctx_get_kernel_options": (C.c_int, [C.c_void_p, C.POINTER(KernelOptions)]),
     "lmm_ctx_set_kernel_options": (C.c_int, [C.c_void_p, C.POINTER(KernelOptions)]),
+    "lmm_ctx_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int)]),
     "lmm_ctx_nlevels": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "lmm_ctx_last_error": (C.c_char_p, [C.c_void_p]),
     "lmm_ctx_set_profiling": (C.c_int, [C.c_void_p, C.c_int]),
@@ -281,6 +282,13 @@ class Context:
                 setattr(o, k, int(v))
             self.check(lib().lmm_ctx_set_kernel_options(self.h, C.byref(o)), "set_kernel_options")
         return o.as_dict()
+
+    def autotune(self):
+        """lmm_ctx_autotune: times the fine-level fused sweeps for a set of z-slab
+        targets on this GPU and keeps the fastest; returns it."""
+        t = C.c_int()
+        self.check(lib().lmm_ctx_autotune(self.h, C.byref(t)), "autotune")
+        return t.value
 
     def nlevels(self):
         a, b = C.c_int(), C.c_int()

@@ -314,3 +314,26 @@ def test_fused_coupled_visc_gmres_equals_colour(dim, n, visc, prec, monkeypatch)
     assert out[None][2] > 0 and out["colour"][2] == 0
     assert out[None][0] == out["colour"][0]
     assert bit_equal(out[None][1][0], out["colour"][1][0]) and bit_equal(out[None][1][1], out["colour"][1][1])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32_mg"])
+def test_autotune_keeps_the_bits(prec):
+    """lmm_ctx_autotune picks the z-slab target among its candidates by timing
+    the fine-level sweeps; a solve after it equals one before it bit for bit
+    (the slab shape never changes the arithmetic), and the target stays in the
+    context's kernel options."""
+    pb = P.small_problem(3, (128, 64, 64), visc_type=-1, seed=149, contrast=10.0)
+    b_u = P.rhs_from_noise_host(pb)
+    precision = {"fp64": S.FP64, "fp32_mg": S.FP32_MG}[prec]
+    with S.Context(3, pb.n, pb.dx, visc_type=-1, precision=precision, gmres_max_iter=8) as ctx:
+        dp = S.DeviceProblem(ctx, pb, rhs=b_u)
+        _, i0 = dp.solve()
+        before = dp.solution()
+        t = ctx.autotune()
+        assert t in (256, 384, 512, 768, 1024, 1536, 2048)
+        assert ctx.kernel_options()["fused_target"] == t
+        dp.zero_solution()
+        _, i1 = dp.solve()
+        after = dp.solution()
+    assert i0["total_iter"] == i1["total_iter"]
+    assert bit_equal(before[0], after[0]) and bit_equal(before[1], after[1])
