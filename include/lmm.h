@@ -477,16 +477,20 @@ typedef struct {
                                0 = visc_coef (the default prob_type branch),
                                1 = visc_coef*(-5 + 6 rhotot) (prob_type -12) */
   int advection_type;       /* :227; only 0 */
-  int noise;                /* the fills of step 7: 0 = device counter-based stream
+  int noise;                /* the step's noise fills: 0 = device counter-based stream
                                (seed, 2*istep) for the momentum noise and (seed,
                                2*istep+1) for the mass noise; 1 = host MT19937 seeded
                                with seed + 2*istep and seed + 2*istep + 1 */
-  int reserved;
+  int algorithm_type;       /* probin_common.f90:218; the overdamped step takes 1 or 2 */
   uint64_t noise_seed;
+  int restart;              /* probin_common.f90:205; the overdamped step draws no new
+                               noise at istep 1 and restart+1 */
+  int reserved;
 } lmm_inertial_params;
 
 /* probin_common defaults: no gravity, variance_coef_mom 1, visc_coef 1,
- * centred advection, device noise with seed 1. */
+ * centred advection, algorithm_type 0, restart -1, device noise with seed 1.
+ * The same knobs drive lmm_advance_timestep_overdamped. */
 void lmm_inertial_params_default(lmm_inertial_params *p);
 
 /* advance_timestep_inertial.f90:58-79 advance_timestep_inertial(mla,umac,
@@ -509,6 +513,25 @@ int lmm_advance_timestep_inertial(lmm_ctx *ctx, lmm_field *umac[3], lmm_field *r
                                   lmm_field *stoch_mass_fluxdiv, double dt, double time,
                                   int64_t istep, const lmm_inertial_params *params,
                                   lmm_gmres_info info[2]);
+/* advance_timestep_overdamped.f90:60-66 advance_timestep_overdamped(mla,umac,rho_old,
+ * rho_new,rhotot_old,rhotot_new,gradp_baro,pi,eta,eta_ed,kappa,Temp,Temp_ed,
+ * diff_mass_fluxdiv,stoch_mass_fluxdiv,dx,dt,time,the_bc_tower,istep): the overdamped
+ * (theta_alpha = 0) midpoint scheme, algorithm_type 1 or 2 (params->algorithm_type):
+ * new noise at the start of every step but the first (istep 1 or restart+1), a
+ * predictor Stokes solve with the fluxes at rho^n (dt/2 for algorithm_type 2, noise
+ * weights (1, 0)), a half step for rho, a corrector solve with the fluxes at the
+ * midpoint state (dt, weights 1 or (1/sqrt 2, 1/sqrt 2)) and the full step for rho.
+ * The context's noise needs algorithm_type stages (lmm_init_m_stochastic /
+ * lmm_init_mass_stochastic).  Same field arguments and limits as the inertial
+ * step; diff/stoch_mass_fluxdiv are outputs. */
+int lmm_advance_timestep_overdamped(lmm_ctx *ctx, lmm_field *umac[3], lmm_field *rho_old,
+                                    lmm_field *rho_new, lmm_field *rhotot_old, lmm_field *rhotot_new,
+                                    lmm_field *gradp_baro[3], lmm_field *pi, lmm_field *eta,
+                                    lmm_field *eta_ed[3], lmm_field *kappa, lmm_field *Temp,
+                                    lmm_field *Temp_ed[3], lmm_field *diff_mass_fluxdiv,
+                                    lmm_field *stoch_mass_fluxdiv, double dt, double time,
+                                    int64_t istep, const lmm_inertial_params *params,
+                                    lmm_gmres_info info[2]);
 /* mk_advective_s_fluxdiv.f90:14-21 mk_advective_s_fluxdiv(mla,umac,s_fc,s_update,dx,
  * start_comp,num_comp): s_update -= div(umac s_fc) for components
  * start_comp .. start_comp+num_comp-1 (0-based). */

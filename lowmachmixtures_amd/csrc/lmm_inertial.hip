@@ -218,6 +218,15 @@ __global__ void __launch_bounds__(ITPB) k_v_add_half(long N, int dim, Ptr3<doubl
   v.c[d][p] = v.c[d][p] + w.c[d][p] * 0.5;
 }
 
+// v = 0 - w on every face (a zeroed right-hand side minus grad pi)
+__global__ void __launch_bounds__(ITPB) k_zero_minus(long N, int dim, Ptr3<double> v, Ptr3<double> w) {
+  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long)dim * N) return;
+  const int d = (int)(t / N);
+  const long p = t - (long)d * N;
+  v.c[d][p] = 0.0 - w.c[d][p];
+}
+
 // a = a + b
 __global__ void __launch_bounds__(ITPB) k_add(long n, double *a, const double *__restrict__ b) {
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -310,6 +319,14 @@ void launch_v_add_half(hipStream_t s, const Grid &g, double *const v[3], double 
     W.c[d] = d < g.dim ? w[d] : nullptr;
   }
   ILAUNCH(k_v_add_half, iblk((long)g.dim * g.N), s, g.N, g.dim, V, W);
+}
+void launch_zero_minus(hipStream_t s, const Grid &g, double *const v[3], double *const w[3]) {
+  Ptr3<double> V, W;
+  for (int d = 0; d < 3; d++) {
+    V.c[d] = d < g.dim ? v[d] : nullptr;
+    W.c[d] = d < g.dim ? w[d] : nullptr;
+  }
+  ILAUNCH(k_zero_minus, iblk((long)g.dim * g.N), s, g.N, g.dim, V, W);
 }
 void launch_add(hipStream_t s, double *a, const double *b, long n) {
   if (n > 0) ILAUNCH(k_add, iblk(n), s, n, a, b);

@@ -296,6 +296,7 @@ void launch_rhs_v(hipStream_t s, const Grid &g, const RhsVArgs &r);
 void launch_v_sub_mt(hipStream_t s, const Grid &g, double *const v[3], double *const u[3], double *const rf[3],
                      double inv_dt);
 void launch_v_add_half(hipStream_t s, const Grid &g, double *const v[3], double *const w[3]);
+void launch_zero_minus(hipStream_t s, const Grid &g, double *const v[3], double *const w[3]);
 void launch_add(hipStream_t s, double *a, const double *b, long n);
 void launch_rhs_p(hipStream_t s, const Grid &g, int ns, double *bp, double *diff, double *stoch, const double *divu,
                   const RhobarInv &a);

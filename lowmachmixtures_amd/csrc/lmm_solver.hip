@@ -3074,6 +3074,7 @@ void lmm_inertial_params_default(lmm_inertial_params *p) {
   p->variance_coef_mom = 1.0;  // probin_common.f90:213
   p->visc_coef = 1.0;          // :225
   p->noise_seed = 1;
+  p->restart = -1;             // :205
 }
 
 int lmm_mk_advective_s_fluxdiv(lmm_ctx *c, lmm_field *const umac[3], lmm_field *const s_fc[3], lmm_field *s_update,
@@ -3132,29 +3133,23 @@ int lmm_mk_grav_force(lmm_ctx *c, lmm_field *m_force[3], lmm_field *const s_fc_o
   return finish(c);
 }
 
-// advance_timestep_inertial.f90:58-990 on a periodic box: the reference's
-// multifab sequence, with runs of element-wise calls fused into single
-// passes (lmm_inertial.hip) and the library's own routines for the
-// operators, fluxes and solves.  Temporaries live in the context (c->iw),
-// viewed as fields.
-int lmm_advance_timestep_inertial(lmm_ctx *c, lmm_field *umac[3], lmm_field *rho_old, lmm_field *rho_new,
-                                  lmm_field *rhotot_old, lmm_field *rhotot_new, lmm_field *gradp_baro[3],
-                                  lmm_field *pi, lmm_field *eta, lmm_field *eta_ed[3], lmm_field *kappa,
-                                  lmm_field *Temp, lmm_field *Temp_ed[3], lmm_field *diff_mass_fluxdiv,
-                                  lmm_field *stoch_mass_fluxdiv, double dt, double time, int64_t istep,
-                                  const lmm_inertial_params *ip, lmm_gmres_info info[2]) {
-  if (int rc = need_species(c, "advance_timestep_inertial")) return rc;
-  if (!ip || !(dt > 0)) return set_err(c, LMM_ERR_ARG, "advance_timestep_inertial: no parameters or dt <= 0");
+// Argument checks shared by the time steps (advance_timestep_inertial /
+// _overdamped take the same fields); nrng: the noise stages the step uses.
+static int check_step(lmm_ctx *c, const char *what, lmm_field *umac[3], lmm_field *rho_old, lmm_field *rho_new,
+                      lmm_field *rhotot_old, lmm_field *rhotot_new, lmm_field *gradp_baro[3], lmm_field *pi,
+                      lmm_field *eta, lmm_field *eta_ed[3], lmm_field *kappa, lmm_field *Temp,
+                      lmm_field *Temp_ed[3], lmm_field *diff_mass_fluxdiv, lmm_field *stoch_mass_fluxdiv, double dt,
+                      const lmm_inertial_params *ip, int nrng) {
+  if (int rc = need_species(c, what)) return rc;
+  if (!ip || !(dt > 0)) return set_err(c, LMM_ERR_ARG, "%s: no parameters or dt <= 0", what);
   if (ip->advection_type != 0)
-    return set_err(c, LMM_ERR_UNSUPPORTED, "advance_timestep_inertial: only centred advection (advection_type 0)");
+    return set_err(c, LMM_ERR_UNSUPPORTED, "%s: only centred advection (advection_type 0)", what);
   if (ip->eta_model != 0 && ip->eta_model != 1)
-    return set_err(c, LMM_ERR_ARG, "advance_timestep_inertial: eta_model is 0 (constant) or 1 (linear in rhotot)");
+    return set_err(c, LMM_ERR_ARG, "%s: eta_model is 0 (constant) or 1 (linear in rhotot)", what);
   const lmm_species_params &sp = c->species;
   const int ns = sp.nspecies, dim = c->g.dim, ne = ne_of(dim);
   const int baro = sp.barodiffusion_type;
-  if (baro != 0 && baro != 2)
-    return set_err(c, LMM_ERR_UNSUPPORTED, "advance_timestep_inertial: barodiffusion_type %d", baro);
-  const char *what = "advance_timestep_inertial";
+  if (baro != 0 && baro != 2) return set_err(c, LMM_ERR_UNSUPPORTED, "%s: barodiffusion_type %d", what, baro);
   int rc;
   if ((rc = check_stokes(c, umac, pi, what))) return rc;
   SP_FIELD(rho_old, LMM_CELL, ns, what);
@@ -3170,15 +3165,34 @@ int lmm_advance_timestep_inertial(lmm_ctx *c, lmm_field *umac[3], lmm_field *rho
   for (int e = 0; e < ne; e++)
     if (!field_ok(c, eta_ed[e], ek + e) || !field_ok(c, Temp_ed[e], ek + e))
       return set_err(c, LMM_ERR_ARG, "%s: eta_ed / Temp_ed[%d] missing or wrong kind", what, e);
-  if (baro == 2 && (rc = check_stokes(c, gradp_baro, nullptr, "advance_timestep_inertial gradp_baro"))) return rc;
-  const bool noise_m = ip->variance_coef_mom != 0.0, noise_mass = sp.variance_coef_mass != 0.0;
-  if (noise_m && c->n_rngs != 1)
-    return set_err(c, LMM_ERR_ARG, "%s: momentum noise needs lmm_init_m_stochastic(ctx, 1) and a first fill", what);
-  if (noise_mass && c->n_rngs_mass != 1)
-    return set_err(c, LMM_ERR_ARG, "%s: mass noise needs lmm_init_mass_stochastic(ctx, 1) and a first fill", what);
+  if (baro == 2 && (rc = check_stokes(c, gradp_baro, nullptr, what))) return rc;
+  if (ip->variance_coef_mom != 0.0 && c->n_rngs != nrng)
+    return set_err(c, LMM_ERR_ARG, "%s: momentum noise needs lmm_init_m_stochastic(ctx, %d) and a first fill", what,
+                   nrng);
+  if (sp.variance_coef_mass != 0.0 && c->n_rngs_mass != nrng)
+    return set_err(c, LMM_ERR_ARG, "%s: mass noise needs lmm_init_mass_stochastic(ctx, %d) and a first fill", what,
+                   nrng);
+  return LMM_OK;
+}
 
-  // temporaries: 10 face vectors, 2 face vectors of ns components, ns + 5
-  // cell arrays, ne node/edge arrays
+// A time step's temporaries, views of the context's scratch c->iw: 10 face
+// vectors, 2 face vectors of ns components, ns + 5 cell arrays, ne
+// node/edge arrays.
+struct StepVec {
+  lmm_field f[3];
+  lmm_field *h[3];
+  double *d[3];
+};
+struct StepWork {
+  StepVec mold, mtemp, m_a, m_d, m_s, rv, dumac, rfo, gradpi, rf, rho_fc, flux_total;
+  lmm_field ru, bp, dpi, divu, eh, kh;
+  lmm_field eeh[3];
+  lmm_field *eehp[3];
+};
+
+static int step_work(lmm_ctx *c, const char *what, StepWork &W) {
+  const int ns = c->species.nspecies, dim = c->g.dim, ne = ne_of(dim);
+  const int ek = dim == 2 ? LMM_NODE : LMM_EDGE_XY;
   const long N = c->g.N;
   const long need = (long)(10 * dim + 2 * dim * ns + ns + 5 + ne) * N;
   if (c->iw_n < need) {
@@ -3193,31 +3207,59 @@ int lmm_advance_timestep_inertial(lmm_ctx *c, lmm_field *umac[3], lmm_field *rho
     w += n;
     return q;
   };
-  struct Vec {
-    lmm_field f[3];
-    lmm_field *h[3];
-    double *d[3];
-  };
-  auto faces = [&](Vec &v, int ncomp) {
+  auto faces = [&](StepVec &v, int ncomp) {
     for (int d = 0; d < 3; d++) {
       v.d[d] = d < dim ? take((long)ncomp * N) : nullptr;
       v.f[d] = lmm_field{c, LMM_FACE_X + d, ncomp, v.d[d]};
       v.h[d] = d < dim ? &v.f[d] : nullptr;
     }
   };
-  Vec mold, mtemp, m_a, m_d, m_s, rv, dumac, rfo, gradpi, rf, rho_fc, flux_total;
-  for (Vec *v : {&mold, &mtemp, &m_a, &m_d, &m_s, &rv, &dumac, &rfo, &gradpi, &rf}) faces(*v, 1);
-  faces(rho_fc, ns);
-  faces(flux_total, ns);
-  lmm_field ru_f{c, LMM_CELL, ns, take((long)ns * N)};
-  lmm_field bp_f{c, LMM_CELL, 1, take(N)}, dpi_f{c, LMM_CELL, 1, take(N)}, divu_f{c, LMM_CELL, 1, take(N)};
-  lmm_field eh_f{c, LMM_CELL, 1, take(N)}, kh_f{c, LMM_CELL, 1, take(N)};
-  lmm_field eeh_f[3];
-  lmm_field *eeh[3] = {nullptr, nullptr, nullptr};
-  for (int e = 0; e < ne; e++) {
-    eeh_f[e] = lmm_field{c, ek + e, 1, take(N)};
-    eeh[e] = &eeh_f[e];
+  for (StepVec *v : {&W.mold, &W.mtemp, &W.m_a, &W.m_d, &W.m_s, &W.rv, &W.dumac, &W.rfo, &W.gradpi, &W.rf})
+    faces(*v, 1);
+  faces(W.rho_fc, ns);
+  faces(W.flux_total, ns);
+  W.ru = lmm_field{c, LMM_CELL, ns, take((long)ns * N)};
+  W.bp = lmm_field{c, LMM_CELL, 1, take(N)};
+  W.dpi = lmm_field{c, LMM_CELL, 1, take(N)};
+  W.divu = lmm_field{c, LMM_CELL, 1, take(N)};
+  W.eh = lmm_field{c, LMM_CELL, 1, take(N)};
+  W.kh = lmm_field{c, LMM_CELL, 1, take(N)};
+  for (int e = 0; e < 3; e++) {
+    W.eeh[e] = lmm_field{c, ek + e, 1, e < ne ? take(N) : nullptr};
+    W.eehp[e] = e < ne ? &W.eeh[e] : nullptr;
   }
+  return LMM_OK;
+}
+
+// advance_timestep_inertial.f90:58-990 on a periodic box: the reference's
+// multifab sequence, with runs of element-wise calls fused into single
+// passes (lmm_inertial.hip) and the library's own routines for the
+// operators, fluxes and solves.  Temporaries live in the context (c->iw),
+// viewed as fields.
+int lmm_advance_timestep_inertial(lmm_ctx *c, lmm_field *umac[3], lmm_field *rho_old, lmm_field *rho_new,
+                                  lmm_field *rhotot_old, lmm_field *rhotot_new, lmm_field *gradp_baro[3],
+                                  lmm_field *pi, lmm_field *eta, lmm_field *eta_ed[3], lmm_field *kappa,
+                                  lmm_field *Temp, lmm_field *Temp_ed[3], lmm_field *diff_mass_fluxdiv,
+                                  lmm_field *stoch_mass_fluxdiv, double dt, double time, int64_t istep,
+                                  const lmm_inertial_params *ip, lmm_gmres_info info[2]) {
+  const char *what = "advance_timestep_inertial";
+  int rc;
+  if ((rc = check_step(c, what, umac, rho_old, rho_new, rhotot_old, rhotot_new, gradp_baro, pi, eta, eta_ed, kappa,
+                       Temp, Temp_ed, diff_mass_fluxdiv, stoch_mass_fluxdiv, dt, ip, 1)))
+    return rc;
+  const lmm_species_params &sp = c->species;
+  const int ns = sp.nspecies, dim = c->g.dim, ne = ne_of(dim);
+  const int baro = sp.barodiffusion_type;
+  const bool noise_m = ip->variance_coef_mom != 0.0, noise_mass = sp.variance_coef_mass != 0.0;
+  const long N = c->g.N;
+  StepWork W;
+  if ((rc = step_work(c, what, W))) return rc;
+  StepVec &mold = W.mold, &mtemp = W.mtemp, &m_a = W.m_a, &m_d = W.m_d, &m_s = W.m_s, &rv = W.rv, &dumac = W.dumac,
+          &rfo = W.rfo, &gradpi = W.gradpi, &rf = W.rf, &rho_fc = W.rho_fc, &flux_total = W.flux_total;
+  lmm_field &bp_f = W.bp, &dpi_f = W.dpi, &divu_f = W.divu, &eh_f = W.eh, &kh_f = W.kh;
+  lmm_field *eeh_f = W.eeh;
+  lmm_field **eeh = W.eehp;
+  (void)mtemp;
   double *u[3] = {nullptr, nullptr, nullptr};
   for (int d = 0; d < dim; d++) u[d] = umac[d]->d;
   hipStream_t s = c->stream;
@@ -3389,6 +3431,148 @@ int lmm_advance_timestep_inertial(lmm_ctx *c, lmm_field *umac[3], lmm_field *rho
   rc = solve(&info[1], nullptr);
   c->prm.gmres_abs_tol = abs_in;
   if (rc) return rc;
+#undef ICHK
+  c->defer_sync = false;
+  return finish(c);
+}
+
+// advance_timestep_overdamped.f90:60-700 on a periodic box: the midpoint
+// scheme with theta_alpha = 0, the full eta in the solves and the explicit
+// viscous term, centred advection; the same temporaries as the inertial step.
+int lmm_advance_timestep_overdamped(lmm_ctx *c, lmm_field *umac[3], lmm_field *rho_old, lmm_field *rho_new,
+                                    lmm_field *rhotot_old, lmm_field *rhotot_new, lmm_field *gradp_baro[3],
+                                    lmm_field *pi, lmm_field *eta, lmm_field *eta_ed[3], lmm_field *kappa,
+                                    lmm_field *Temp, lmm_field *Temp_ed[3], lmm_field *diff_mass_fluxdiv,
+                                    lmm_field *stoch_mass_fluxdiv, double dt, double time, int64_t istep,
+                                    const lmm_inertial_params *ip, lmm_gmres_info info[2]) {
+  const char *what = "advance_timestep_overdamped";
+  if (!c) return LMM_ERR_ARG;
+  if (!ip) return set_err(c, LMM_ERR_ARG, "%s: no parameters", what);
+  const int alg = ip->algorithm_type;
+  if (alg != 1 && alg != 2)
+    return set_err(c, LMM_ERR_ARG, "%s: algorithm_type %d (the midpoint scheme takes 1 or 2)", what, alg);
+  int rc;
+  if ((rc = check_step(c, what, umac, rho_old, rho_new, rhotot_old, rhotot_new, gradp_baro, pi, eta, eta_ed, kappa,
+                       Temp, Temp_ed, diff_mass_fluxdiv, stoch_mass_fluxdiv, dt, ip, alg)))
+    return rc;
+  const lmm_species_params &sp = c->species;
+  const int ns = sp.nspecies, dim = c->g.dim;
+  const int baro = sp.barodiffusion_type;
+  const bool noise_m = ip->variance_coef_mom != 0.0, noise_mass = sp.variance_coef_mass != 0.0;
+  const long N = c->g.N;
+  StepWork W;
+  if ((rc = step_work(c, what, W))) return rc;
+  StepVec &rv = W.rv, &dumac = W.dumac, &gradpi = W.gradpi, &rf = W.rf, &rho_fc = W.rho_fc,
+          &flux_total = W.flux_total;
+  lmm_field &bp_f = W.bp, &dpi_f = W.dpi, &divu_f = W.divu;
+  double *u[3] = {nullptr, nullptr, nullptr};
+  for (int d = 0; d < dim; d++) u[d] = umac[d]->d;
+  hipStream_t s = c->stream;
+  // weights(1:algorithm_type) = (1, 0), then 1/sqrt(2) each for the
+  // corrector of algorithm_type 2 (:115-116, :459-461)
+  double weights[8] = {1.0, 0, 0, 0, 0, 0, 0, 0};
+  RhobarInv rbi{};
+  for (int q = 0; q < ns; q++) rbi.v[q] = -1.0 / sp.rhobar[q];
+  lmm_field *const *gb = baro ? gradp_baro : nullptr;
+  AdvArgs adv{};
+  for (int d = 0; d < dim; d++) {
+    adv.u[d] = u[d];
+    adv.s[d] = rho_fc.d[d];
+  }
+  adv.dxinv = 1.0 / c->g.dx;
+  const bool grav = ip->grav[0] != 0.0 || ip->grav[1] != 0.0 || (dim == 3 && ip->grav[2] != 0.0);
+#define ICHK(x)                          \
+  do {                                   \
+    if ((rc = (x)) != LMM_OK) return rc; \
+  } while (0)
+  lmm_gmres_info loc[2];
+  if (!info) info = loc;
+  memset(info, 0, 2 * sizeof(lmm_gmres_info));
+  struct DeferSync {
+    lmm_ctx *c;
+    explicit DeferSync(lmm_ctx *x) : c(x) { c->defer_sync = true; }
+    ~DeferSync() { c->defer_sync = false; }
+  } defer(c);
+
+  // one Stokes stage: gmres_rhs_v = -grad pi + div(Sigma) + rho g + A_0 v,
+  // gmres_rhs_p = div v - S from the mass fluxes at rho_s, then the solve
+  // and v += dv, pi += dpi (:166-336, :420-582)
+  auto stage = [&](lmm_field *rho_s, double dt_s, lmm_gmres_info *inf, double *npr) -> int {
+    ICHK(lmm_compute_grad(c, pi, gradpi.h));
+    if (baro == 2)
+      for (int d = 0; d < dim; d++)
+        hipMemcpyAsync(gradp_baro[d]->d, gradpi.d[d], sizeof(double) * N, hipMemcpyDeviceToDevice, s);
+    launch_zero_minus(s, c->g, rv.d, gradpi.d);
+    if (noise_m)
+      ICHK(lmm_stochastic_m_fluxdiv(c, rv.h, eta, eta_ed, Temp, Temp_ed, dt_s, weights, ip->variance_coef_mom,
+                                    sp.k_B));
+    if (grav) launch_grav(s, c->g, ip->grav, rv.d, rf.d, rf.d);
+    ICHK(lmm_compute_mass_fluxdiv(c, rho_s, gb, diff_mass_fluxdiv, stoch_mass_fluxdiv, Temp, flux_total.h, dt_s,
+                                  time, weights));
+    ICHK(lmm_diffusive_m_fluxdiv(c, rv.h, umac, eta, eta_ed, kappa));
+    ICHK(lmm_compute_div(c, umac, &divu_f, 0));
+    launch_rhs_p(s, c->g, ns, bp_f.d, diff_mass_fluxdiv->d, noise_mass ? stoch_mass_fluxdiv->d : nullptr, divu_f.d,
+                 rbi);
+    for (int d = 0; d < dim; d++) launch_fill<double>(s, dumac.d[d], N, 0.0);
+    launch_fill<double>(s, dpi_f.d, N, 0.0);
+    ICHK(lmm_gmres(c, rv.h, &bp_f, dumac.h, &dpi_f, rf.h, eta, eta_ed, kappa, 0.0, npr, inf));
+    for (int d = 0; d < dim; d++) launch_add(s, u[d], dumac.d[d], N);
+    launch_add(s, pi->d, dpi_f.d, N);
+    return LMM_OK;
+  };
+  // rho_new = rho_old + dt_s (F + A) with A from rho_fc, then rhotot, the
+  // concentration round trip and the new faces (:340-418, :586-645)
+  auto scalar = [&](double dt_s) {
+    RhoArgs r{};
+    r.diff = diff_mass_fluxdiv->d;
+    r.stoch = noise_mass ? stoch_mass_fluxdiv->d : nullptr;
+    r.rho_old = rho_old->d;
+    r.rho_new = rho_new->d;
+    r.dt = dt_s;
+    launch_adv_s(s, c->g, ns, adv, r, 1);
+    launch_rhotot_conc(s, c->g, ns, rho_new->d, rhotot_new->d);
+  };
+  auto eta_kappa = [&]() -> int {
+    launch_eta(s, c->g, eta->d, kappa->d, rhotot_new->d, ip->visc_coef, ip->eta_model);
+    return dim == 2 ? lmm_average_cc_to_node(c, eta, eta_ed[0]) : lmm_average_cc_to_edge(c, eta, eta_ed);
+  };
+
+  ICHK(lmm_average_cc_to_face_comps(c, rho_old, rho_fc.h, 0, ns));
+  ICHK(lmm_average_cc_to_face(c, rhotot_old, rf.h));
+  // new noise, except at the first step after initialisation or a restart
+  // (:158-165)
+  if (istep != 1 && istep != (int64_t)ip->restart + 1) {
+    const uint64_t seed = ip->noise_seed, st = (uint64_t)istep;
+    if (noise_mass)
+      ICHK(ip->noise ? lmm_fill_mass_stochastic(c, (uint32_t)(seed + 2 * st + 1))
+                     : lmm_fill_mass_stochastic_device(c, seed, 2 * st + 1));
+    if (noise_m)
+      ICHK(ip->noise ? lmm_fill_m_stochastic(c, (uint32_t)(seed + 2 * st))
+                     : lmm_fill_m_stochastic_device(c, seed, 2 * st));
+  }
+  const double dt_pred = alg == 1 ? dt : 0.5 * dt;
+
+  // ---- steps 1-2: predictor fluxes and Stokes solve (:166-336)
+  double npr = 0.0;
+  ICHK(stage(rho_old, dt_pred, &info[0], &npr));
+  const double abs_in = c->prm.gmres_abs_tol;
+
+  // ---- step 3: scalar predictor, half a step (:340-418)
+  scalar(0.5 * dt);
+  ICHK(lmm_average_cc_to_face_comps(c, rho_new, rho_fc.h, 0, ns));
+  ICHK(lmm_average_cc_to_face(c, rhotot_new, rf.h));
+  ICHK(eta_kappa());
+
+  // ---- steps 4-5: corrector fluxes and Stokes solve (:420-582)
+  if (alg == 2) weights[0] = weights[1] = 1.0 / sqrt(2.0);
+  c->prm.gmres_abs_tol = std::max(abs_in, npr * c->prm.gmres_rel_tol);
+  rc = stage(rho_new, dt, &info[1], nullptr);
+  c->prm.gmres_abs_tol = abs_in;
+  if (rc) return rc;
+
+  // ---- step 6: midpoint scalar corrector (:586-648)
+  scalar(dt);
+  ICHK(eta_kappa());
 #undef ICHK
   c->defer_sync = false;
   return finish(c);

@@ -80,8 +80,9 @@ module lmm_facade
   ! compute_eta_kappa and the noise stream of the time step
   type, bind(c), public :: lmm_inertial_params
      real(c_double) :: grav(3), variance_coef_mom, visc_coef
-     integer(c_int) :: eta_model, advection_type, noise, reserved
+     integer(c_int) :: eta_model, advection_type, noise, algorithm_type
      integer(c_int64_t) :: noise_seed
+     integer(c_int) :: restart, reserved
   end type lmm_inertial_params
 
   public :: lmm_params_default, lmm_ctx_create, lmm_ctx_destroy, lmm_ctx_set_params, lmm_ctx_last_error, &
@@ -91,12 +92,12 @@ module lmm_facade
        lmm_diffusive_m_fluxdiv, lmm_species_params_default, lmm_ctx_set_species, lmm_init_mass_stochastic, &
        lmm_fill_mass_stochastic, lmm_stochastic_mass_fluxdiv, lmm_compute_mass_fluxdiv, lmm_compute_div_comps, &
        lmm_inertial_params_default, lmm_advance_timestep_inertial, lmm_mk_advective_s_fluxdiv, &
-       lmm_mk_advective_m_fluxdiv, lmm_mk_grav_force
+       lmm_mk_advective_m_fluxdiv, lmm_mk_grav_force, lmm_advance_timestep_overdamped
   public :: lmm_check, lmm_error_message, lmm_fab_in, lmm_fab_out
   public :: gmres, apply_matrix, apply_precon, stag_mg_solver, macproject, stochastic_m_fluxdiv, &
        diffusive_mass_fluxdiv, diffusive_m_fluxdiv, compute_mass_fluxdiv, stochastic_mass_fluxdiv, &
        init_mass_stochastic, fill_mass_stochastic, compute_div, advance_timestep_inertial, &
-       mk_advective_s_fluxdiv, mk_advective_m_fluxdiv, mk_grav_force
+       mk_advective_s_fluxdiv, mk_advective_m_fluxdiv, mk_grav_force, advance_timestep_overdamped
 
   interface
      subroutine lmm_params_default(p) bind(c, name='lmm_params_default')
@@ -311,6 +312,19 @@ module lmm_facade
        type(lmm_inertial_params), intent(in) :: p
        type(lmm_gmres_info), intent(out) :: info(2)
      end function lmm_advance_timestep_inertial
+
+     integer(c_int) function lmm_advance_timestep_overdamped(ctx, umac, rho_old, rho_new, rhotot_old, rhotot_new, &
+          gradp_baro, pi, eta, eta_ed, kappa, temp, temp_ed, diff_mass_fluxdiv, stoch_mass_fluxdiv, dt, time, &
+          istep, p, info) bind(c, name='lmm_advance_timestep_overdamped')
+       import :: c_int, c_int64_t, c_double, c_ptr, lmm_inertial_params, lmm_gmres_info
+       type(c_ptr), value :: ctx, rho_old, rho_new, rhotot_old, rhotot_new, pi, eta, kappa, temp
+       type(c_ptr), value :: diff_mass_fluxdiv, stoch_mass_fluxdiv
+       type(c_ptr), intent(in) :: umac(3), gradp_baro(3), eta_ed(3), temp_ed(3)
+       real(c_double), value :: dt, time
+       integer(c_int64_t), value :: istep
+       type(lmm_inertial_params), intent(in) :: p
+       type(lmm_gmres_info), intent(out) :: info(2)
+     end function lmm_advance_timestep_overdamped
 
      integer(c_int) function lmm_mk_advective_s_fluxdiv(ctx, umac, s_fc, s_update, start_comp, num_comp) &
           bind(c, name='lmm_mk_advective_s_fluxdiv')
@@ -532,6 +546,24 @@ contains
          int(istep, c_int64_t), p, loc), 'advance_timestep_inertial')
     if (present(info)) info = loc
   end subroutine advance_timestep_inertial
+
+  ! advance_timestep_overdamped.f90:60-66 advance_timestep_overdamped(mla,umac,rho_old,rho_new,
+  ! rhotot_old,rhotot_new,gradp_baro,pi,eta,eta_ed,kappa,Temp,Temp_ed,diff_mass_fluxdiv,
+  ! stoch_mass_fluxdiv,dx,dt,time,the_bc_tower,istep); p%algorithm_type is 1 or 2
+  subroutine advance_timestep_overdamped(ctx, umac, rho_old, rho_new, rhotot_old, rhotot_new, gradp_baro, pi, eta, &
+       eta_ed, kappa, temp, temp_ed, diff_mass_fluxdiv, stoch_mass_fluxdiv, dt, time, istep, p, info)
+    type(c_ptr), intent(in) :: ctx, umac(3), rho_old, rho_new, rhotot_old, rhotot_new, gradp_baro(3), pi, eta, &
+         eta_ed(3), kappa, temp, temp_ed(3), diff_mass_fluxdiv, stoch_mass_fluxdiv
+    real(c_double), intent(in) :: dt, time
+    integer, intent(in) :: istep
+    type(lmm_inertial_params), intent(in) :: p
+    type(lmm_gmres_info), intent(out), optional :: info(2)
+    type(lmm_gmres_info) :: loc(2)
+    call lmm_check(ctx, lmm_advance_timestep_overdamped(ctx, umac, rho_old, rho_new, rhotot_old, rhotot_new, &
+         gradp_baro, pi, eta, eta_ed, kappa, temp, temp_ed, diff_mass_fluxdiv, stoch_mass_fluxdiv, dt, time, &
+         int(istep, c_int64_t), p, loc), 'advance_timestep_overdamped')
+    if (present(info)) info = loc
+  end subroutine advance_timestep_overdamped
 
   ! mk_advective_s_fluxdiv.f90:14 mk_advective_s_fluxdiv(mla,umac,s_fc,s_update,dx,start_comp,num_comp),
   ! start_comp 1-based as in the reference

@@ -90,8 +90,8 @@ class InertialParams(C.Structure):
     """lmm_inertial_params: the probin_common knobs of the inertial step."""
     _fields_ = [
         ("grav", C.c_double * 3), ("variance_coef_mom", C.c_double), ("visc_coef", C.c_double),
-        ("eta_model", C.c_int), ("advection_type", C.c_int), ("noise", C.c_int), ("reserved", C.c_int),
-        ("noise_seed", C.c_uint64),
+        ("eta_model", C.c_int), ("advection_type", C.c_int), ("noise", C.c_int), ("algorithm_type", C.c_int),
+        ("noise_seed", C.c_uint64), ("restart", C.c_int), ("reserved", C.c_int),
     ]
 
 
@@ -206,6 +206,9 @@ _SIGS = {
     "lmm_advance_timestep_inertial": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP, _FP, _FP, _FP * 3, _FP, _FP, _FP * 3,
                                                 _FP, _FP, _FP * 3, _FP, _FP, C.c_double, C.c_double, C.c_int64,
                                                 C.POINTER(InertialParams), C.POINTER(GmresInfo * 2)]),
+    "lmm_advance_timestep_overdamped": (C.c_int, [C.c_void_p, _FP * 3, _FP, _FP, _FP, _FP, _FP * 3, _FP, _FP,
+                                                  _FP * 3, _FP, _FP, _FP * 3, _FP, _FP, C.c_double, C.c_double,
+                                                  C.c_int64, C.POINTER(InertialParams), C.POINTER(GmresInfo * 2)]),
     "lmm_mk_advective_s_fluxdiv": (C.c_int, [C.c_void_p, _FP * 3, _FP * 3, _FP, C.c_int, C.c_int]),
     "lmm_mk_advective_m_fluxdiv": (C.c_int, [C.c_void_p, _FP * 3, _FP * 3, _FP * 3]),
     "lmm_mk_grav_force": (C.c_int, [C.c_void_p, _FP * 3, _FP * 3, _FP * 3, C.POINTER(C.c_double)]),
@@ -871,6 +874,20 @@ def advance_timestep_inertial(ctx, umac, rho_old, rho_new, rhotot_old, rhotot_ne
         _a3(eta_ed), _h(kappa), _h(Temp), _a3(Temp_ed), _h(diff_mass_fluxdiv), _h(stoch_mass_fluxdiv),
         C.c_double(dt), C.c_double(time), C.c_int64(int(istep)), C.byref(p), C.byref(info)),
         "advance_timestep_inertial")
+    return info[0].as_dict(), info[1].as_dict()
+
+
+def advance_timestep_overdamped(ctx, umac, rho_old, rho_new, rhotot_old, rhotot_new, gradp_baro, pi, eta, eta_ed,
+                                kappa, Temp, Temp_ed, diff_mass_fluxdiv, stoch_mass_fluxdiv, dt, time, istep,
+                                params=None):
+    """advance_timestep_overdamped.f90:60-66; returns the two solves' info dicts."""
+    p = params if params is not None else inertial_params(algorithm_type=1)
+    info = (GmresInfo * 2)()
+    ctx.check(lib().lmm_advance_timestep_overdamped(
+        ctx.h, _a3(umac), _h(rho_old), _h(rho_new), _h(rhotot_old), _h(rhotot_new), _a3(gradp_baro), _h(pi), _h(eta),
+        _a3(eta_ed), _h(kappa), _h(Temp), _a3(Temp_ed), _h(diff_mass_fluxdiv), _h(stoch_mass_fluxdiv),
+        C.c_double(dt), C.c_double(time), C.c_int64(int(istep)), C.byref(p), C.byref(info)),
+        "advance_timestep_overdamped")
     return info[0].as_dict(), info[1].as_dict()
 
 

@@ -886,3 +886,110 @@ def advance_timestep_inertial(g, p, sp, st, dt, time, grav=(0.0, 0.0, 0.0), vari
     out = dict(umac=umac, rho_new=rho_new, rhotot_new=rhotot_new, pi=pi, eta=eta, eta_ed=eta_ed, kappa=kappa,
                diff_mass_fluxdiv=diff, stoch_mass_fluxdiv=stoch, gradp_baro=gradp_baro, W_m=W_m, W_mass=W_mass)
     return out, info1, info2
+
+
+def advance_timestep_overdamped(g, p, sp, st, dt, time, istep, algorithm_type, grav=(0.0, 0.0, 0.0),
+                                variance_coef_mom=1.0, visc_coef=1.0, eta_model=0, restart=-1, W_m=None,
+                                W_mass=None, W_m_new=None, W_mass_new=None):
+    """One overdamped midpoint step (advance_timestep_overdamped.f90:60-700)
+    in the reference's order of operations, algorithm_type 1 or 2.  st holds
+    the state as for advance_timestep_inertial (its diff/stoch_mass_fluxdiv
+    are not read).  W_m: the momentum noise held on entry, a list over stages
+    of (W_cc, [W_ed]); W_mass: the mass noise (n_rngs, dim, ns, *shape);
+    W_m_new / W_mass_new: what the fills at the start of the step draw
+    (:158-165), used unless istep is 1 or restart+1.  Returns the new state
+    (with the noise held at the end) and the two solves' info dicts."""
+    alg = int(algorithm_type)
+    if alg not in (1, 2):
+        raise ValueError("algorithm_type is 1 or 2")
+    dim = g.dim
+    ns = sp.nspecies
+    var_mass = sp.variance_coef_mass
+    baro = sp.barodiffusion_type
+    umac = _copy(st["umac"])
+    rho_old = _c(st["rho_old"]).copy()
+    rhotot_old = _c(st["rhotot_old"])
+    pi = _c(st["pi"]).copy()
+    eta, eta_ed, kappa = _c(st["eta"]), _copy(st["eta_ed"]), _c(st["kappa"])
+    Temp, Temp_ed = _c(st["Temp"]), _copy(st["Temp_ed"])
+    gradp_baro = _copy(st["gradp_baro"]) if st.get("gradp_baro") is not None else [np.zeros_like(pi)
+                                                                                     for _ in range(dim)]
+    shape = pi.shape
+    if istep != 1 and istep != restart + 1:
+        W_m, W_mass = W_m_new, W_mass_new
+    if W_mass is None:
+        W_mass = np.zeros((alg, dim, ns) + shape)
+    dv = 1.0
+    for _ in range(dim):
+        dv *= g.dx
+    weights = [1.0] + [0.0] * (alg - 1)
+
+    def stoch_m(force, dt_s, w):
+        variance = math.sqrt(variance_coef_mom * 2.0 * sp.k_B / (dv * dt_s))
+        wcc = 0.0 + w[0] * W_m[0][0]
+        wed = [0.0 + w[0] * x for x in W_m[0][1]]
+        for r in range(1, len(w)):
+            wcc = wcc + w[r] * W_m[r][0]
+            wed = [a + w[r] * x for a, x in zip(wed, W_m[r][1])]
+        return stochastic_m_fluxdiv(g, wcc, wed, eta, eta_ed, Temp, Temp_ed, variance, force)
+
+    def stage(rho_s, dt_s, w, pp):
+        nonlocal umac, pi, gradp_baro
+        gradpi = compute_grad(g, pi)
+        if baro == 2:
+            gradp_baro = _copy(gradpi)
+        v = [0.0 - x for x in gradpi]
+        if variance_coef_mom != 0.0:
+            v = stoch_m(v, dt_s, w)
+        if any(x != 0.0 for x in grav[:dim]):
+            v = mk_grav_force(g, grav, rf, rf, v)
+        rho_s, diff, stoch, _ = compute_mass_fluxdiv(g, sp, rho_s, Temp, dt_s, W_mass, w,
+                                                     gradp_baro if baro != 0 else None)
+        diff = diff * -1.0
+        if var_mass != 0.0:
+            stoch = stoch * -1.0
+        bp = np.zeros(shape)
+        for c in range(ns):
+            a = -1.0 / sp.rhobar[c]
+            bp = bp + a * diff[c]
+            if var_mass != 0.0:
+                bp = bp + a * stoch[c]
+        v = diffusive_m_fluxdiv(g, p.visc_type, v, umac, eta, eta_ed, kappa)
+        bp = bp + compute_div(g, umac)
+        du, dpi, info = gmres(g, pp, coef(rf, eta, eta_ed, kappa), 0.0, v, bp)
+        umac = [u + d for u, d in zip(umac, du)]
+        pi = pi + dpi
+        return rho_s, diff, stoch, info
+
+    def scalar(diff, stoch, dt_s):
+        ru = 0.0 + diff
+        if var_mass != 0.0:
+            ru = ru + stoch
+        ru = mk_advective_s_fluxdiv(g, umac, rho_fc, ru)
+        ru = ru * dt_s
+        rn = rho_old + ru
+        rt = compute_rhotot(rn)
+        return (rn / rt) * rt, rt
+
+    rho_fc = _faces_comps(g, rho_old)
+    rf = average_cc_to_face(g, rhotot_old)
+    # ---- steps 1-2: predictor fluxes and Stokes solve (:166-336)
+    rho_old, diff, stoch, info1 = stage(rho_old, dt if alg == 1 else 0.5 * dt, weights, p)
+    p2 = Params.from_buffer_copy(p)
+    p2.gmres_abs_tol = max(p.gmres_abs_tol, info1["norm_pre_rhs"] * p.gmres_rel_tol)
+    # ---- step 3: scalar predictor (:340-418)
+    rho_new, rhotot_new = scalar(diff, stoch, 0.5 * dt)
+    rho_fc = _faces_comps(g, rho_new)
+    rf = average_cc_to_face(g, rhotot_new)
+    eta, eta_ed, kappa = compute_eta_kappa(g, rho_new, rhotot_new, visc_coef, eta_model)
+    # ---- steps 4-5: corrector fluxes and Stokes solve (:420-582)
+    if alg == 2:
+        weights = [1.0 / math.sqrt(2.0)] * 2
+    rho_new, diff, stoch, info2 = stage(rho_new, dt, weights, p2)
+    # ---- step 6: midpoint scalar corrector (:586-648)
+    rho_new, rhotot_new = scalar(diff, stoch, dt)
+    eta, eta_ed, kappa = compute_eta_kappa(g, rho_new, rhotot_new, visc_coef, eta_model)
+    out = dict(umac=umac, rho_old=rho_old, rho_new=rho_new, rhotot_new=rhotot_new, pi=pi, eta=eta, eta_ed=eta_ed,
+               kappa=kappa, diff_mass_fluxdiv=diff, stoch_mass_fluxdiv=stoch, gradp_baro=gradp_baro, W_m=W_m,
+               W_mass=W_mass)
+    return out, info1, info2

@@ -73,6 +73,11 @@ program facade_check
        c_null_ptr, c_null_ptr, f3, c_null_ptr, c_null_ptr, f3, c_null_ptr, c_null_ptr, 1.0_c_double, &
        0.0_c_double, 0_c_int64_t, ip, info2)
   print '(a,1x,i0)', 'inertial_null_ctx', rc
+  print '(a,1x,i0,1x,i0)', 'overdamped_knobs', ip%algorithm_type, ip%restart
+  rc = lmm_advance_timestep_overdamped(c_null_ptr, f3, c_null_ptr, c_null_ptr, c_null_ptr, c_null_ptr, f3, &
+       c_null_ptr, c_null_ptr, f3, c_null_ptr, c_null_ptr, f3, c_null_ptr, c_null_ptr, 1.0_c_double, &
+       0.0_c_double, 0_c_int64_t, ip, info2)
+  print '(a,1x,i0)', 'overdamped_null_ctx', rc
   ! the host Gaussian stream (MT19937 + Acklam)
   rc = lmm_fill_normal_host(99_c_int32_t, 4_c_int64_t, z)
   print '(a,1x,i0)', 'fill_rc', rc

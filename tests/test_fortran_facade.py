@@ -91,3 +91,5 @@ def test_inertial_knobs_through_fortran(run_check):
     assert [float(x) for x in v["inertial_coefs"]] == [1.0, 1.0]  # variance_coef_mom, visc_coef
     assert v["inertial_flags"] == ["0", "0", "0"]  # constant eta, centred advection, device noise
     assert v["inertial_null_ctx"] == ["1"]
+    assert v["overdamped_knobs"] == ["0", "-1"]  # algorithm_type, restart (probin_common.f90:205, :218)
+    assert v["overdamped_null_ctx"] == ["1"]

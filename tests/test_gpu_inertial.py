@@ -182,3 +182,109 @@ def test_inertial_step_64cubed_device_noise_conserves_mass():
         assert abs(out[c].sum() - rho[c].sum()) < 1e-11 * rho[c].sum()
     assert all(np.isfinite(x).all() for x in u) and np.isfinite(out).all()
     assert np.max(np.abs(out - rho)) > 0
+
+
+OD_CASES = [
+    dict(dim=2, n=(32, 16), ns=3, visc=2, eta_model=0, alg=1),
+    dict(dim=2, n=(16, 16), ns=2, visc=-1, eta_model=1, alg=2),
+    dict(dim=3, n=(16, 8, 8), ns=4, visc=1, eta_model=0, alg=2),
+    dict(dim=3, n=(8, 8, 16), ns=2, visc=-2, eta_model=1, alg=1),
+]
+
+
+@pytest.mark.parametrize("case", range(len(OD_CASES)))
+def test_overdamped_step_matches_oracle(case):
+    """advance_timestep_overdamped.f90:60-700 over istep 1 (the noise of the
+    first fill) and istep 2 (new noise drawn at the start), host noise on
+    both sides.  No gravity: a uniform body force has no steady Stokes
+    solution on a periodic box."""
+    c = OD_CASES[case]
+    dim, n, ns, alg = c["dim"], c["n"], c["ns"], c["alg"]
+    shape = _shape(n)
+    rng = np.random.default_rng(500 + case)
+    dx = 1.0 / n[0]
+    g = O.geom(dim, n, dx)
+    kn = _knobs(ns)
+    sp = O.species(ns, **kn)
+    p = O.params(visc_type=c["visc"])
+    rho = np.stack([0.5 + 0.5 * rng.random(shape) for _ in range(ns)]) / ns
+    if c["eta_model"] == 1:
+        rho = rho / O.compute_rhotot(rho) * (0.9 + 0.05 * rng.random(shape))
+    rhotot = O.compute_rhotot(rho)
+    temp = np.ones(shape)
+    eta = np.full(shape, 0.8) if c["eta_model"] == 0 else 0.8 * (-5.0 + 6.0 * rhotot)
+    ed = O._edges(g, eta)
+    st = dict(umac=[0.05 * rng.standard_normal(shape) for _ in range(dim)], rho_old=rho, rhotot_old=rhotot,
+              pi=0.1 * rng.standard_normal(shape), eta=eta, eta_ed=ed, kappa=np.ones(shape), Temp=temp,
+              Temp_ed=O._edges(g, temp), diff_mass_fluxdiv=np.zeros((ns,) + shape),
+              stoch_mass_fluxdiv=np.zeros((ns,) + shape), gradp_baro=None)
+    dt, seed = 2e-4, 60
+    N = int(np.prod(shape))
+    mom = lambda s_: O.fill_m_stochastic(s_, dim, shape, alg, symmetric=True)
+    mass = lambda s_: O.fill_normal(s_, alg * dim * ns * N).reshape((alg, dim, ns) + shape)
+    W_m, W_mass = mom(9), mass(10)
+    want = []
+    s_o = dict(st)
+    for istep in (1, 2):
+        out, i1, i2 = O.advance_timestep_overdamped(
+            g, p, sp, s_o, dt, (istep - 1) * dt, istep, alg, variance_coef_mom=1.0, visc_coef=0.8,
+            eta_model=c["eta_model"], W_m=W_m, W_mass=W_mass, W_m_new=mom(seed + 2 * istep),
+            W_mass_new=mass(seed + 2 * istep + 1))
+        want.append((out, i1, i2))
+        W_m, W_mass = out["W_m"], out["W_mass"]
+        s_o = dict(st, umac=out["umac"], rho_old=out["rho_new"], rhotot_old=out["rhotot_new"], pi=out["pi"],
+                   eta=out["eta"], eta_ed=out["eta_ed"], kappa=out["kappa"])
+    with S.Context(dim, n, dx, visc_type=c["visc"], stoch_stress_form=1) as ctx:
+        S.set_species(ctx, ns, **kn)
+        S.init_m_stochastic(ctx, alg)
+        S.fill_m_stochastic(ctx, 9)
+        S.init_mass_stochastic(ctx, alg)
+        S.fill_mass_stochastic(ctx, 10)
+        umac = ctx.faces(st["umac"])
+        r_old, r_new = ctx.field(S.CELL, ns, rho), ctx.field(S.CELL, ns)
+        rt_old, rt_new = ctx.field(S.CELL, 1, rhotot), ctx.field(S.CELL, 1)
+        f_pi, f_eta, f_kap = ctx.field(S.CELL, 1, st["pi"]), ctx.field(S.CELL, 1, eta), ctx.field(S.CELL, 1, st["kappa"])
+        f_ed, f_T, f_Ted = ctx.edges(ed), ctx.field(S.CELL, 1, temp), ctx.edges(st["Temp_ed"])
+        f_diff, f_stoch = ctx.field(S.CELL, ns), ctx.field(S.CELL, ns)
+        prm = S.inertial_params(noise=1, noise_seed=seed, algorithm_type=alg, variance_coef_mom=1.0, visc_coef=0.8,
+                                eta_model=c["eta_model"])
+        for k, istep in enumerate((1, 2)):
+            i1, i2 = S.advance_timestep_overdamped(ctx, umac, r_old, r_new, rt_old, rt_new, None, f_pi, f_eta, f_ed,
+                                                   f_kap, f_T, f_Ted, f_diff, f_stoch, dt, (istep - 1) * dt, istep,
+                                                   prm)
+            out, o1, o2 = want[k]
+            assert i1["status"] == 0 and i2["status"] == 0
+            assert (i1["total_iter"], i2["total_iter"]) == (o1["total_iter"], o2["total_iter"]), istep
+            got = dict(umac=[f.download() for f in umac], rho_old=r_old.download(), rho_new=r_new.download(),
+                       rhotot_new=rt_new.download(), pi=f_pi.download(), eta=f_eta.download(),
+                       diff_mass_fluxdiv=f_diff.download(), stoch_mass_fluxdiv=f_stoch.download())
+            for key, v in got.items():
+                assert relerr(v, out[key]) < 1e-11, (istep, key, relerr(v, out[key]))
+            for src, dst in ((r_new, r_old), (rt_new, rt_old)):
+                S.field_copy(dst, src)
+
+
+def test_overdamped_step_checks_its_arguments():
+    with S.Context(2, (16, 16), 1.0 / 16) as ctx:
+        S.set_species(ctx, 2, **_knobs(2))
+        S.init_m_stochastic(ctx, 1)
+        S.fill_m_stochastic(ctx, 1)
+        S.init_mass_stochastic(ctx, 1)
+        S.fill_mass_stochastic(ctx, 2)
+        z = np.zeros((16, 16))
+        umac = ctx.faces([z, z])
+        fs = [ctx.field(S.CELL, 2, np.full((2, 16, 16), 0.5)), ctx.field(S.CELL, 2), ctx.field(S.CELL, 1, z + 1.0),
+              ctx.field(S.CELL, 1)]
+        rest = [None, ctx.field(S.CELL, 1), ctx.field(S.CELL, 1, z + 1.0), ctx.edges([z + 1.0]),
+                ctx.field(S.CELL, 1, z + 1.0), ctx.field(S.CELL, 1, z + 1.0), ctx.edges([z + 1.0]),
+                ctx.field(S.CELL, 2), ctx.field(S.CELL, 2)]
+        for alg in (0, 3):
+            with pytest.raises(S.LmmError):
+                S.advance_timestep_overdamped(ctx, umac, *fs, *rest, 1e-3, 0.0, 1,
+                                              S.inertial_params(algorithm_type=alg))
+        # two noise stages wanted, one initialised
+        with pytest.raises(S.LmmError):
+            S.advance_timestep_overdamped(ctx, umac, *fs, *rest, 1e-3, 0.0, 1, S.inertial_params(algorithm_type=2))
+        i1, i2 = S.advance_timestep_overdamped(ctx, umac, *fs, *rest, 1e-3, 0.0, 1,
+                                               S.inertial_params(algorithm_type=1))
+        assert i1["status"] == 0 and i2["status"] == 0
