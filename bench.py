@@ -374,15 +374,18 @@ def inertial_step(S, n, ns, steps, cpu_n, quiet):
                        f"dt {dt}; two gmres solves per step (reference solver settings)",
            "cells": n ** 3}
 
-    def run(nn, precision, nsteps, noise, seed=3):
+    def run(nn, precision, nsteps, noise, seed=3, overdamped=False):
+        # the overdamped step: algorithm_type 2 (two noise stages), no gravity
+        # (a uniform body force has no steady Stokes solution on a periodic box)
+        nrng = 2 if overdamped else 1
         rho, knobs = _inertial_state(nn, ns)
         shape = (nn, nn, nn)
         rhotot = rho.sum(axis=0)
         ones = np.ones(shape)
         with S.Context(3, shape, 1.0 / nn, visc_type=2, stoch_stress_form=1, precision=precision) as ctx:
             S.set_species(ctx, ns, **knobs)
-            S.init_m_stochastic(ctx, 1)
-            S.init_mass_stochastic(ctx, 1)
+            S.init_m_stochastic(ctx, nrng)
+            S.init_mass_stochastic(ctx, nrng)
             if noise:
                 S.fill_m_stochastic(ctx, 7)
                 S.fill_mass_stochastic(ctx, 8)
@@ -395,13 +398,15 @@ def inertial_step(S, n, ns, steps, cpu_n, quiet):
             f_pi, f_eta, f_kap = ctx.field(S.CELL, 1), ctx.field(S.CELL, 1, 1e-2 * ones), ctx.field(S.CELL, 1, ones)
             f_ed, f_T, f_Ted = ctx.edges([1e-2 * ones] * 3), ctx.field(S.CELL, 1, ones), ctx.edges([ones] * 3)
             f_diff, f_stoch = ctx.field(S.CELL, ns), ctx.field(S.CELL, ns)
-            prm = S.inertial_params(grav=grav, visc_coef=1e-2, noise=noise, noise_seed=seed)
+            prm = S.inertial_params(grav=(0.0, 0.0, 0.0) if overdamped else grav, visc_coef=1e-2, noise=noise,
+                                    noise_seed=seed, algorithm_type=nrng if overdamped else 0)
+            step = S.advance_timestep_overdamped if overdamped else S.advance_timestep_inertial
             infos, times = [], []
             for k in range(nsteps):
                 ctx.synchronize()
                 t0 = time.perf_counter()
-                i1, i2 = S.advance_timestep_inertial(ctx, umac, r[0], r[1], rt[0], rt[1], None, f_pi, f_eta, f_ed,
-                                                     f_kap, f_T, f_Ted, f_diff, f_stoch, dt, k * dt, k, prm)
+                i1, i2 = step(ctx, umac, r[0], r[1], rt[0], rt[1], None, f_pi, f_eta, f_ed, f_kap, f_T, f_Ted, f_diff,
+                              f_stoch, dt, k * dt, k + 1 if overdamped else k, prm)
                 ctx.synchronize()
                 times.append(time.perf_counter() - t0)
                 infos.append((i1, i2))
@@ -423,6 +428,16 @@ def inertial_step(S, n, ns, steps, cpu_n, quiet):
                        "max_speed": float(max(np.max(np.abs(u)) for u in res["u"]))}
         log(f"inertial step {n}^3 x {ns} ({name}): {ms:.2f} ms per step, iters {forms[name]['gmres_iters']}", quiet)
     out["forms"] = forms
+    od = {}
+    for name, prec in (("fp64", S.FP64), ("fp32_mg", S.FP32_MG)):
+        times, infos, res, rho0, _ = run(n, prec, steps + 1, 0, overdamped=True)
+        ms = float(np.mean(times[1:])) * 1e3
+        od[name] = {"ms_per_step": round(ms, 3), "gmres_iters": [[i1["total_iter"], i2["total_iter"]]
+                                                                 for i1, i2 in infos[1:]],
+                    "status": max(max(i1["status"], i2["status"]) for i1, i2 in infos)}
+        log(f"overdamped step {n}^3 x {ns} ({name}): {ms:.2f} ms per step, iters {od[name]['gmres_iters']}", quiet)
+    out["overdamped"] = {"workload": f"advance_timestep_overdamped, algorithm_type 2, the same {n}^3 box without "
+                                     f"gravity; two gmres solves per step (theta_alpha 0)", "forms": od}
     if cpu_n > 0:
         from oracle import oracle as O
 
@@ -453,6 +468,21 @@ def inertial_step(S, n, ns, steps, cpu_n, quiet):
                                                 np.concatenate([np.ravel(u) for u in o["umac"]])),
                                      "pi": rd(res["pi"], o["pi"])}
         log(f"inertial step oracle {cpu_n}^3: {sec:.2f} s per step; rel diff {out['rel_diff_vs_oracle']}", quiet)
+        # the overdamped step on the same small box (istep 1: the first fill's noise)
+        times, infos, res, rho0, knobs = run(cpu_n, S.FP64, 1, 1, overdamped=True)
+        mom2 = lambda s_: O.fill_m_stochastic(s_, 3, shape, 2, symmetric=True)
+        mass2 = lambda s_: O.fill_normal(s_, 2 * 3 * ns * N).reshape((2, 3, ns) + shape)
+        t0 = time.perf_counter()
+        o, _, _ = O.advance_timestep_overdamped(g, O.params(visc_type=2), O.species(ns, **knobs), st, dt, 0.0, 1, 2,
+                                                variance_coef_mom=1.0, visc_coef=1e-2, W_m=mom2(7), W_mass=mass2(8))
+        sec = time.perf_counter() - t0
+        out["overdamped"]["cpu_oracle_s_per_step"] = round(sec, 3)
+        out["overdamped"]["rel_diff_vs_oracle"] = {
+            "rho_new": rd(res["rho"], o["rho_new"]),
+            "umac": rd(np.concatenate([np.ravel(u) for u in res["u"]]),
+                       np.concatenate([np.ravel(u) for u in o["umac"]])), "pi": rd(res["pi"], o["pi"])}
+        log(f"overdamped step oracle {cpu_n}^3: {sec:.2f} s; rel diff {out['overdamped']['rel_diff_vs_oracle']}",
+            quiet)
     return out
 
 
