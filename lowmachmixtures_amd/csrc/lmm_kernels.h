@@ -26,6 +26,7 @@ struct Tuning {
   int rb_target = 768;  // workgroups the z-slab choice aims for (FP32; FP64 twice)
   int am_dz = 32;       // z depth of the operator's brick order (0: row order)
   int rb_prefetch = 0;  // FP32 fused sweep: load the next plane's coefficients one step ahead
+  int rb_pairs = 2;     // FP32 three-component sweep: point pairs per thread (2: 64 x 16 tiles)
 };
 // The settings the launchers on this thread use: those of the context whose
 // library call is running on it (TuningScope), else the defaults.

@@ -18,7 +18,8 @@
 // that of stag_applyop_3d + stag_mg_update_3d term for term, so the result
 // equals the reference's two colour passes bit for bit.
 //
-// Tiling: a workgroup owns a 64 x 8 column of points in (x, y) and marches
+// Tiling: a workgroup owns a 64 x 8 column of points in (x, y) (64 x 16 in
+// the FP32 three-component sweep, two point pairs per thread) and marches
 // through tz planes in z.  Per plane it stages the old field with a 2-point
 // halo in LDS, computes red on the 66 x 10 region (interior + 1-point halo)
 // into a second LDS ring, then black on the interior of the previous plane
@@ -204,14 +205,16 @@ __global__ void __launch_bounds__(256) k_rb_fused(Grid g, int tz, int ntx, int n
 // (the thread's pair and its halo-ring point) are loaded one step ahead, so
 // a step's global loads are consumed by the following step instead of
 // stalling the one that issued them.  Same arithmetic, same bits.
-template <typename T, typename P, int TX, bool PF>
+template <typename T, typename P, int TX, bool PF, int NP = 1>
 __device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int nty, int tile,
                                             const T *__restrict__ pin, T *__restrict__ pout,
-                                            const typename P::Args &args, T (*sp)[512 / TX + 4][TX + 4],
-                                            T (*sr)[512 / TX + 2][TX + 2]) {
+                                            const typename P::Args &args, T (*sp)[512 * NP / TX + 4][TX + 4],
+                                            T (*sr)[512 * NP / TX + 2][TX + 2]) {
   using Coef = typename P::Coef;
-  // tile TX x (512 / TX): one point pair per thread of the 256
-  constexpr int FX = TX, FY = 512 / TX, HX = FX + 4, HY = FY + 4, RX = FX + 2, RY = FY + 2;
+  // tile TX x (512 NP / TX): NP point pairs per thread of the 256, rows
+  // 512 / TX apart
+  constexpr int FX = TX, FY = 512 * NP / TX, HX = FX + 4, HY = FY + 4, RX = FX + 2, RY = FY + 2;
+  constexpr int RPP = 512 / TX;
   constexpr int NRED = RY * (RX / 2);
   const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
   const int i0 = bx * FX, j0 = by * FY, k0 = bz * tz;
@@ -219,7 +222,7 @@ __device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int 
   const long sy = g.s[1], sz = g.s[2];
   const int tid = threadIdx.x;
   constexpr int NLD = (HY * HX + 255) / 256;
-  // this thread's pair: columns 2m, 2m+1 of tile row r
+  // this thread's pairs: columns 2m, 2m+1 of tile rows r + RPP q, q < NP
   const int pr = tid / (FX / 2), pm = tid - pr * (FX / 2);
   const int pj = j0 + pr;
 
@@ -276,8 +279,9 @@ __device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int 
   auto interior = [&](int kk) { return kk >= k0 && kk < k0 + tz; };
   // the coefficients an interior plane's red step needs: the thread's pair
   // and, for tid < NHALO, its halo-ring point
-  auto load_red = [&](int kk, Coef &c0, Coef &c1, Coef &ch) {
-    P::load_pair(g, i0 + 2 * pm, pj, kk, args, c0, c1);
+  auto load_red = [&](int kk, Coef (&c0)[NP], Coef (&c1)[NP], Coef &ch) {
+#pragma unroll
+    for (int q = 0; q < NP; q++) P::load_pair(g, i0 + 2 * pm, pj + RPP * q, kk, args, c0[q], c1[q]);
     if (tid < NHALO) {
       int row2, rr2;
       halo_pos(kk, tid, row2, rr2);
@@ -286,20 +290,27 @@ __device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int 
   };
   // red on plane kk; for interior planes returns the black half of the pair.
   // PF: c0, c1, ch hold the plane's coefficients, loaded a step earlier.
-  auto red_plane = [&](int kk, Coef &cb, const Coef &c0, const Coef &c1, const Coef &ch) {
+  auto red_plane = [&](int kk, Coef (&cb)[NP], const Coef (&c0)[NP], const Coef (&c1)[NP], const Coef &ch) {
     if (interior(kk)) {
-      Coef x0, x1, xh;
+      Coef x0[NP], x1[NP], xh;
       if constexpr (PF) {
-        x0 = c0;
-        x1 = c1;
+#pragma unroll
+        for (int q = 0; q < NP; q++) {
+          x0[q] = c0[q];
+          x1[q] = c1[q];
+        }
         xh = ch;
       } else {
         load_red(kk, x0, x1, xh);
       }
+      // rows RPP apart have the same parity
       const bool first_red = ((pj + kk) & 1) == 0;  // column 2m is red when j + k is even
-      const Coef cr = first_red ? x0 : x1;
-      cb = first_red ? x1 : x0;
-      red_with(kk, pr + 1, 2 * pm + (first_red ? 0 : 1) + 1, cr);
+#pragma unroll
+      for (int q = 0; q < NP; q++) {
+        const Coef cr = first_red ? x0[q] : x1[q];
+        cb[q] = first_red ? x1[q] : x0[q];
+        red_with(kk, pr + RPP * q + 1, 2 * pm + (first_red ? 0 : 1) + 1, cr);
+      }
       if (tid < NHALO) {
         int row2, rr2;
         halo_pos(kk, tid, row2, rr2);
@@ -313,23 +324,27 @@ __device__ __forceinline__ void march2_tile(const Grid &g, int tz, int ntx, int 
       }
     }
   };
-  auto black_plane = [&](int kk, const Coef &cb) {
+  auto black_plane = [&](int kk, const Coef (&cb)[NP]) {
     const int s = kk & 3, sm = (kk - 1) & 3, spl = (kk + 1) & 3;
     const bool first_red = ((pj + kk) & 1) == 0;
     const int il = 2 * pm + (first_red ? 1 : 0);
-    const int rrow = pr + 1, rc0 = il + 1;
-    const T v = P::update(cb, sp[s][pr + 2][il + 2], sr[s][rrow][rc0 + 1], sr[s][rrow][rc0 - 1], sr[s][rrow + 1][rc0],
-                          sr[s][rrow - 1][rc0], sr[spl][rrow][rc0], sr[sm][rrow][rc0], args);
-    const T r = sr[s][rrow][first_red ? rc0 - 1 : rc0 + 1];
-    st2(pout, i0 + 2 * pm + sy * pj + sz * (long)kk, first_red ? r : v, first_red ? v : r);
+    const int rc0 = il + 1;
+#pragma unroll
+    for (int q = 0; q < NP; q++) {
+      const int rrow = pr + RPP * q + 1;
+      const T v = P::update(cb[q], sp[s][rrow + 1][il + 2], sr[s][rrow][rc0 + 1], sr[s][rrow][rc0 - 1],
+                            sr[s][rrow + 1][rc0], sr[s][rrow - 1][rc0], sr[spl][rrow][rc0], sr[sm][rrow][rc0], args);
+      const T r = sr[s][rrow][first_red ? rc0 - 1 : rc0 + 1];
+      st2(pout, i0 + 2 * pm + sy * (pj + RPP * q) + sz * (long)kk, first_red ? r : v, first_red ? v : r);
+    }
   };
 
   T pva[NLD] = {}, pvb[NLD] = {};
-  Coef cba = {}, cbb = {};
+  Coef cba[NP] = {}, cbb[NP] = {};
   // red coefficients of the current plane (a) and the next one (b)
-  Coef ra0 = {}, ra1 = {}, rah = {}, rb0 = {}, rb1 = {}, rbh = {};
-  auto step = [&](int kk, const T *pv, T *pvn, const Coef &cb_prev, Coef &cb_cur, Coef &c0, Coef &c1, Coef &ch,
-                  Coef &n0, Coef &n1, Coef &nh) {
+  Coef ra0[NP] = {}, ra1[NP] = {}, rah = {}, rb0[NP] = {}, rb1[NP] = {}, rbh = {};
+  auto step = [&](int kk, const T *pv, T *pvn, const Coef (&cb_prev)[NP], Coef (&cb_cur)[NP], Coef (&c0)[NP],
+                  Coef (&c1)[NP], Coef &ch, Coef (&n0)[NP], Coef (&n1)[NP], Coef &nh) {
     store_plane(kk + 1, pv);
     __syncthreads();
     if (kk + 2 <= k0 + tz + 1) fetch_plane(kk + 2, pvn);
@@ -375,7 +390,7 @@ __global__ void __launch_bounds__(256) k_rb_march2(Grid g, int tz, int ntx, int 
 // separate launches stream those arrays from memory once per component.
 // Each workgroup runs the single-component tile code, so every point is
 // computed exactly as in k_rb_march2.
-template <typename T, typename P0, typename P1, typename P2, int TX, bool PF>
+template <typename T, typename P0, typename P1, typename P2, int TX, bool PF, int NP = 1>
 __global__ void __launch_bounds__(256)
     k_rb_march2x3(Grid g, int tz, int ntx, int nty, int ntiles, const T *__restrict__ pin0, T *__restrict__ pout0,
                   const T *__restrict__ pin1, T *__restrict__ pout1, const T *__restrict__ pin2,
@@ -385,18 +400,18 @@ __global__ void __launch_bounds__(256)
                   const T *__restrict__ q11, const T *__restrict__ q12, const T *__restrict__ q13,
                   const T *__restrict__ q14, const T *__restrict__ q20, const T *__restrict__ q21,
                   const T *__restrict__ q22, const T *__restrict__ q23, const T *__restrict__ q24) {
-  __shared__ T sp[4][512 / TX + 4][TX + 4];
-  __shared__ T sr[4][512 / TX + 2][TX + 2];
+  __shared__ T sp[4][512 * NP / TX + 4][TX + 4];
+  __shared__ T sr[4][512 * NP / TX + 2][TX + 2];
   const int nslots = 3 * ntiles, w = blockIdx.x;
   int slot = w;
   if (nslots % 8 == 0) slot = (w % 8) * (nslots / 8) + w / 8;
   const int tile = slot / 3, comp = slot - 3 * tile;
   if (comp == 0) {
-    march2_tile<T, P0, TX, PF>(g, tz, ntx, nty, tile, pin0, pout0, P0::bind(a0, q00, q01, q02, q03, q04), sp, sr);
+    march2_tile<T, P0, TX, PF, NP>(g, tz, ntx, nty, tile, pin0, pout0, P0::bind(a0, q00, q01, q02, q03, q04), sp, sr);
   } else if (comp == 1) {
-    march2_tile<T, P1, TX, PF>(g, tz, ntx, nty, tile, pin1, pout1, P1::bind(a1, q10, q11, q12, q13, q14), sp, sr);
+    march2_tile<T, P1, TX, PF, NP>(g, tz, ntx, nty, tile, pin1, pout1, P1::bind(a1, q10, q11, q12, q13, q14), sp, sr);
   } else {
-    march2_tile<T, P2, TX, PF>(g, tz, ntx, nty, tile, pin2, pout2, P2::bind(a2, q20, q21, q22, q23, q24), sp, sr);
+    march2_tile<T, P2, TX, PF, NP>(g, tz, ntx, nty, tile, pin2, pout2, P2::bind(a2, q20, q21, q22, q23, q24), sp, sr);
   }
 }
 
@@ -687,7 +702,7 @@ typename StagRB<T, A, VISC>::Args stag_args(T omega, T dxsqinv, const T *rhs, co
 }
 
 template <typename T, int VISC>
-void launch_stag_three(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, int tx, T *const pin_[3],
+void launch_stag_three(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz, int tx, int np, T *const pin_[3],
                        T *const pout[3], T *const rhs[3], const StagCoef<T> &K, bool zero_in) {
   T *const zeros[3] = {nullptr, nullptr, nullptr};
   T *const *pin = zero_in ? zeros : pin_;
@@ -704,7 +719,15 @@ void launch_stag_three(hipStream_t s, const Grid &g, T omega, T dxsqinv, int tz,
   LaunchEvents *ev = launch_events();
   // FP32 prefetches the next plane's coefficients (FP64 would lose occupancy)
   const bool pf = sizeof(T) == 4 && tuning().rb_prefetch;
-  if (tx == FX) {
+  if (tx == FX && np == 2) {
+    // 64 x 16 tiles, two point pairs per thread
+    const int ntx = g.n[0] / FX, nty = g.n[1] / (2 * FY), ntz = g.n[2] / tz;
+    const int ntiles = ntx * nty * ntz;
+    auto kern = pf ? k_rb_march2x3<T, P0, P1, P2, FX, true, 2> : k_rb_march2x3<T, P0, P1, P2, FX, false, 2>;
+    hipExtLaunchKernelGGL(kern, dim3(3 * ntiles), dim3(256), 0, s, ev->start, ev->stop, 0, g, tz, ntx, nty, ntiles,
+                          pin[0], pout[0], pin[1], pout[1], pin[2], pout[2], a0, a1, a2, q0[0], q0[1], q0[2], q0[3],
+                          q0[4], q1[0], q1[1], q1[2], q1[3], q1[4], q2[0], q2[1], q2[2], q2[3], q2[4]);
+  } else if (tx == FX) {
     const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
     const int ntiles = ntx * nty * ntz;
     auto kern = pf ? k_rb_march2x3<T, P0, P1, P2, FX, true> : k_rb_march2x3<T, P0, P1, P2, FX, false>;
@@ -1151,10 +1174,17 @@ void launch_stag_rb_fused(hipStream_t s, const Grid &g, int visc, int a, T omega
 #undef ONE
 }
 
-// tile width (64, or 32 on levels 32 points wide) and z-slab of the
+// tile width (64, or 32 on levels 32 points wide), pairs per thread (2:
+// 64 x 16 tiles, where tuning asks for them) and z-slab of the
 // three-component sweep; 0 when the level cannot use it
-static int rb3_shape(const Grid &g, int min_tiles, int es, int &tx) {
+static int rb3_shape(const Grid &g, int min_tiles, int es, int &tx, int &np) {
   tx = FX;
+  np = 2;
+  if (tuning().rb_pairs == 2 && es == 4) {  // FP64 would need 80 KB of LDS per workgroup
+    const int tz2 = rb_tz(g, min_tiles, es, 3, FX, 2 * FY);
+    if (tz2 > 0) return tz2;
+  }
+  np = 1;
   const int tz = rb_tz(g, min_tiles, es, 3, FX, FY);
   if (tz > 0) return tz;
   tx = FX2;
@@ -1162,22 +1192,22 @@ static int rb3_shape(const Grid &g, int min_tiles, int es, int &tx) {
 }
 
 bool fused_rb3_ok(const Grid &g, int visc, int min_tiles) {
-  int tx;
-  return tuning().rb_variant == 2 && (visc == -1 || visc == 1) && rb3_shape(g, min_tiles, 4, tx) > 0;
+  int tx, np;
+  return tuning().rb_variant == 2 && (visc == -1 || visc == 1) && rb3_shape(g, min_tiles, 4, tx, np) > 0;
 }
 
 template <typename T>
 void launch_stag_rb_fused3(hipStream_t s, const Grid &g, int visc, T omega, T *const pin[3], T *const pout[3],
                            T *const rhs[3], const StagCoef<T> &K, int min_tiles, bool zero_in) {
-  int tx = FX;
-  const int tz = rb3_shape(g, min_tiles, sizeof(T), tx);
+  int tx = FX, np = 1;
+  const int tz = rb3_shape(g, min_tiles, sizeof(T), tx, np);
   if (tz <= 0) {
     report_error("launch_stag_rb_fused3: level not eligible");
     return;
   }
   const T dxsqinv = (T)(1.0 / (g.dx * g.dx));
-  if (visc == -1) launch_stag_three<T, -1>(s, g, omega, dxsqinv, tz, tx, pin, pout, rhs, K, zero_in);
-  else launch_stag_three<T, 1>(s, g, omega, dxsqinv, tz, tx, pin, pout, rhs, K, zero_in);
+  if (visc == -1) launch_stag_three<T, -1>(s, g, omega, dxsqinv, tz, tx, np, pin, pout, rhs, K, zero_in);
+  else launch_stag_three<T, 1>(s, g, omega, dxsqinv, tz, tx, np, pin, pout, rhs, K, zero_in);
 }
 
 template <typename T>
