@@ -27,6 +27,7 @@ struct Tuning {
   int am_dz = 32;       // z depth of the operator's brick order (0: row order)
   int rb_prefetch = 0;  // FP32 fused sweep: load the next plane's coefficients one step ahead
   int rb_pairs = 2;     // FP32 three-component sweep: point pairs per thread (2: 64 x 16 tiles)
+  int prolong_tiled = 1;  // 3D staggered prolongation from coarse values staged in LDS
 };
 // The settings the launchers on this thread use: those of the context whose
 // library call is running on it (TuningScope), else the defaults.

@@ -192,10 +192,82 @@ __global__ void __launch_bounds__(256) k_stag_prolong2(Grid gf, Grid gc, T *fine
   st2(fine, p, f0, f1);
 }
 
+// Tiled prolongation of the three face components (3D; fine level with x a
+// multiple of 64, y and z of 4): a workgroup owns a 64 x 4 x 4 block of fine
+// points of one component, stages the coarse values the block reads -- its
+// 32 x 2 x 2 coarse points, one more along the component's direction and one
+// on each side across it -- in LDS once, and each thread prolongs two pairs
+// of fine points from there with prolong_combine, the arithmetic of the
+// one-point kernel, so the same bits with one coarse load per staged value
+// instead of 4 or 8 per fine point.
+constexpr int PTX = 64, PTY = 4, PTZ = 4;
+
+template <typename T, int A>
+__device__ __forceinline__ void prolong_tile(const Grid &gf, const Grid &gc, T *fine, const T *__restrict__ C,
+                                             long tile, int ntx, int nty, T (*S)[PTY][PTX / 2 + 2]) {
+  constexpr int T1 = (A == 0 ? 1 : 0), T2 = (A == 2 ? 1 : 2);
+  // the box: along A from the tile's first coarse point, one extra above;
+  // across A one extra on each side
+  constexpr int LX0 = (A == 0) ? 0 : -1, LY0 = (A == 1) ? 0 : -1, LZ0 = (A == 2) ? 0 : -1;
+  constexpr int SX = PTX / 2 + (A == 0 ? 1 : 2), SY = PTY / 2 + (A == 1 ? 1 : 2), SZ = PTZ / 2 + (A == 2 ? 1 : 2);
+  const int bx = (int)(tile % ntx), by = (int)((tile / ntx) % nty), bz = (int)(tile / ((long)ntx * nty));
+  const int i0 = bx * PTX, j0 = by * PTY, k0 = bz * PTZ;
+  const int ox = (i0 >> 1) + LX0, oy = (j0 >> 1) + LY0, oz = (k0 >> 1) + LZ0;
+  const int tid = threadIdx.x;
+  for (int q = tid; q < SX * SY * SZ; q += 256) {
+    const int x = q % SX, y = (q / SX) % SY, z = q / (SX * SY);
+    S[z][y][x] = C[wrapc(ox + x, gc.n[0]) + gc.s[1] * wrapc(oy + y, gc.n[1]) + gc.s[2] * wrapc(oz + z, gc.n[2])];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    // 512 pairs per block: 32 per row, 4 rows, 4 planes
+    const int pidx = tid + 256 * r;
+    const int m = pidx & 31, yy = (pidx >> 5) & 3, zz = pidx >> 7;
+    const long p = (i0 + 2 * m) + gf.s[1] * (j0 + yy) + gf.s[2] * (k0 + zz);
+    T f[2];
+    ld2(fine, p, f[0], f[1]);
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int c[3] = {i0 + 2 * m + h, j0 + yy, k0 + zz};
+      const int lq[3] = {(c[0] >> 1) - ox, (c[1] >> 1) - oy, (c[2] >> 1) - oz};
+      f[h] = prolong_combine<T, 3, A>(c, f[h], [&](int sa, int s1, int s2) -> T {
+        int q[3] = {lq[0], lq[1], lq[2]};
+        q[A] += sa;
+        q[T1] += s1;
+        q[T2] += s2;
+        return S[q[2]][q[1]][q[0]];
+      });
+    }
+    st2(fine, p, f[0], f[1]);
+  }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_stag_prolong_tiled(Grid gf, Grid gc, Ptr3<T> fine, Ptr3<T> coarse, int ntx,
+                                                            int nty) {
+  __shared__ T S[PTZ][PTY][PTX / 2 + 2];
+  const long tile = xcd_block();
+  const int a = (int)blockIdx.y;
+  if (a == 0) prolong_tile<T, 0>(gf, gc, fine.c[0], coarse.c[0], tile, ntx, nty, S);
+  else if (a == 1) prolong_tile<T, 1>(gf, gc, fine.c[1], coarse.c[1], tile, ntx, nty, S);
+  else prolong_tile<T, 2>(gf, gc, fine.c[2], coarse.c[2], tile, ntx, nty, S);
+}
+
 template <typename T>
 void launch_stag_prolong(hipStream_t s, const Grid &gf, T *const fine[3], T *const coarse[3]) {
   Grid gc = coarsen(gf);
   dim3 grid(nblk(gf.N));
+  if (gf.dim == 3 && tuning().prolong_tiled && gf.n[0] % PTX == 0 && gf.n[1] % PTY == 0 && gf.n[2] % PTZ == 0) {
+    const int ntx = gf.n[0] / PTX, nty = gf.n[1] / PTY, ntz = gf.n[2] / PTZ;
+    Ptr3<T> F, Cc;
+    for (int d = 0; d < 3; d++) {
+      F.c[d] = fine[d];
+      Cc.c[d] = coarse[d];
+    }
+    LAUNCH(k_stag_prolong_tiled<T>, dim3(ntx * nty * ntz, 3), dim3(256), 0, s, gf, gc, F, Cc, ntx, nty);
+    return;
+  }
   if (gf.dim == 3 && gf.n[0] % 2 == 0) {
     dim3 grid2(nblk(gf.N / 2));
     LAUNCH((k_stag_prolong2<T, 0>), grid2, dim3(TPB), 0, s, gf, gc, fine[0], coarse[0]);

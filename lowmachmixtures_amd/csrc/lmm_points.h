@@ -144,14 +144,43 @@ __device__ __forceinline__ void restrict_at(const Grid &gc, const Grid &gf, int 
 // phi_f += P phi_c for face component A in dimension D (stag_prolongation_2d/3d,
 // stag_mg_solver.f90:1370-1633): bilinear in the transverse directions,
 // linear along A.  The component is a template argument so every index is static.
+// The prolongation arithmetic of one fine point c of component A
+// (stag_prolongation_2d/3d, stag_mg_solver.f90:1370-1633): at(sa, s1, s2) is
+// the coarse value at Q + sa along A, + s1 along the first transverse
+// direction and + s2 along the second (s1, s2 in {0, o1 / o2}).
+template <typename T, int D, int A, typename F>
+__device__ __forceinline__ T prolong_combine(const int c[3], T v, F &&at) {
+  constexpr int T1 = (D == 2) ? 1 - A : (A == 0 ? 1 : 0);
+  constexpr int T2 = (D == 2) ? -1 : (A == 2 ? 1 : 2);
+  const int o1 = (c[T1] & 1) ? 1 : -1;
+  const int o2 = (T2 >= 0 && (c[T2 >= 0 ? T2 : 0] & 1)) ? 1 : -1;
+  const bool even = (c[A] & 1) == 0;
+  if (D == 2) {
+    if (even) {
+      v = v + (T)0.75 * at(0, 0, 0) + (T)0.25 * at(0, o1, 0);
+    } else {
+      v = v + (T)0.375 * at(0, 0, 0) + (T)0.125 * at(0, o1, 0) + (T)0.375 * at(1, 0, 0) +
+          (T)0.125 * at(1, o1, 0);
+    }
+  } else {
+    const T nine16 = (T)0.5625, three16 = (T)0.1875, one16 = (T)0.0625;
+    const T nine32 = (T)0.28125, three32 = (T)0.09375, one32 = (T)0.03125;
+    if (even) {
+      v = v + nine16 * at(0, 0, 0) + three16 * at(0, o1, 0) + three16 * at(0, 0, o2) + one16 * at(0, o1, o2);
+    } else {
+      v = v + nine32 * at(0, 0, 0) + three32 * at(0, o1, 0) + three32 * at(0, 0, o2) + one32 * at(0, o1, o2) +
+          nine32 * at(1, 0, 0) + three32 * at(1, o1, 0) + three32 * at(1, 0, o2) + one32 * at(1, o1, o2);
+    }
+  }
+  return v;
+}
+
 template <typename T, int D, int A>
 __device__ __forceinline__ T prolong_value(const Grid &gf, const Grid &gc, const T *C, const int c[3], T v) {
   constexpr int T1 = (D == 2) ? 1 - A : (A == 0 ? 1 : 0);
   constexpr int T2 = (D == 2) ? -1 : (A == 2 ? 1 : 2);
   const int Q[3] = {c[0] >> 1, c[1] >> 1, c[2] >> 1};
-  const int o1 = (c[T1] & 1) ? 1 : -1;
-  const int o2 = (T2 >= 0 && (c[T2 >= 0 ? T2 : 0] & 1)) ? 1 : -1;
-  auto cidx = [&](int sa, int s1, int s2) -> long {
+  return prolong_combine<T, D, A>(c, v, [&](int sa, int s1, int s2) -> T {
     int q[3] = {Q[0], Q[1], Q[2]};
     q[A] += sa;
     q[T1] += s1;
@@ -161,29 +190,8 @@ __device__ __forceinline__ T prolong_value(const Grid &gf, const Grid &gc, const
       if (q[d] < 0) q[d] += gc.n[d];
       if (q[d] >= gc.n[d]) q[d] -= gc.n[d];
     }
-    return q[0] + gc.s[1] * q[1] + gc.s[2] * q[2];
-  };
-  const bool even = (c[A] & 1) == 0;
-  if (D == 2) {
-    if (even) {
-      v = v + (T)0.75 * C[cidx(0, 0, 0)] + (T)0.25 * C[cidx(0, o1, 0)];
-    } else {
-      v = v + (T)0.375 * C[cidx(0, 0, 0)] + (T)0.125 * C[cidx(0, o1, 0)] + (T)0.375 * C[cidx(1, 0, 0)] +
-          (T)0.125 * C[cidx(1, o1, 0)];
-    }
-  } else {
-    const T nine16 = (T)0.5625, three16 = (T)0.1875, one16 = (T)0.0625;
-    const T nine32 = (T)0.28125, three32 = (T)0.09375, one32 = (T)0.03125;
-    if (even) {
-      v = v + nine16 * C[cidx(0, 0, 0)] + three16 * C[cidx(0, o1, 0)] + three16 * C[cidx(0, 0, o2)] +
-          one16 * C[cidx(0, o1, o2)];
-    } else {
-      v = v + nine32 * C[cidx(0, 0, 0)] + three32 * C[cidx(0, o1, 0)] + three32 * C[cidx(0, 0, o2)] +
-          one32 * C[cidx(0, o1, o2)] + nine32 * C[cidx(1, 0, 0)] + three32 * C[cidx(1, o1, 0)] +
-          three32 * C[cidx(1, 0, o2)] + one32 * C[cidx(1, o1, o2)];
-    }
-  }
-  return v;
+    return C[q[0] + gc.s[1] * q[1] + gc.s[2] * q[2]];
+  });
 }
 
 template <typename T, int D, int A>

@@ -1527,6 +1527,8 @@ static void apply_options(lmm_ctx *c, const lmm_kernel_options &o) {
   c->tune.rb_prefetch = o.sweep_prefetch;
   const char *pairs = getenv("LMM_SWEEP_PAIRS");
   c->tune.rb_pairs = (pairs && pairs[0] == '1') ? 1 : 2;
+  const char *pt = getenv("LMM_PROLONG_TILED");
+  c->tune.prolong_tiled = (pt && pt[0] == '0') ? 0 : 1;
   c->precon_graph = o.precon_graph != 0;
   c->kopt = o;
   drop_graphs(c);
