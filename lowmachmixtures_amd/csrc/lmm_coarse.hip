@@ -10,7 +10,11 @@
 // workgroup barrier between dependent passes.  Every point is computed by the
 // same device function as in the one-launch-per-pass kernels, in the same
 // order of passes, so the result is bit for bit that of the separate launches
-// (and of the reference's stag_mg_solver / F_MG level loops).
+// (and of the reference's stag_mg_solver / F_MG level loops).  For
+// visc_type -1 and 1 a face component's operator reads only that component,
+// so the staggered V-cycle runs one workgroup per component side by side
+// (the serial chain of barriers a third as long); the coupled viscosities
+// keep all components in one workgroup.
 #include "lmm_kernels.h"
 #include "lmm_points.h"
 
@@ -34,6 +38,7 @@ struct SCoarseArgs {
   SLevelDesc<T> L[COARSE_MAX_LEVELS];
   int m;                 // coarse levels held in L[0..m-1], finest first
   int visc, nd, nu, nb;  // visc_type, sweeps down / up / bottom
+  int split;             // 1: workgroup a handles component a only
   T omega;
 };
 
@@ -46,13 +51,14 @@ __device__ void blk_fill(T *x, long n, T v) {
 // every colour is a constant after inlining (no per-lane stack for the
 // per-component pointer tables)
 template <typename T, int DIM>
-__device__ void blk_stag_smooth_d(const SLevelDesc<T> &L, int visc, T omega, int nsweeps) {
+__device__ void blk_stag_smooth_d(const SLevelDesc<T> &L, int visc, T omega, int nsweeps, int alo, int ahi) {
   const long half = (long)(L.g.n[0] >> 1) * L.g.n[1] * L.g.n[2];
   const T dxsqinv = (T)(1.0 / (L.g.dx * L.g.dx));
   for (int m = 0; m < nsweeps; m++) {
 #pragma unroll
     for (int color = 1; color <= 2 * DIM; color++) {
       const int a = (color - 1) / 2, par = (color + 1) % 2;
+      if (a < alo || a >= ahi) continue;  // uniform over the workgroup
       for (long t = threadIdx.x; t < half; t += CTPB)
         gs_colour_at<T>(L.g, visc, a, par, omega, dxsqinv, L.phi, L.K, L.rhs.c[a], t);
       __syncthreads();
@@ -61,47 +67,54 @@ __device__ void blk_stag_smooth_d(const SLevelDesc<T> &L, int visc, T omega, int
 }
 
 template <typename T>
-__device__ void blk_stag_smooth(const SLevelDesc<T> &L, int visc, T omega, int nsweeps) {
-  if (L.g.dim == 3) blk_stag_smooth_d<T, 3>(L, visc, omega, nsweeps);
-  else blk_stag_smooth_d<T, 2>(L, visc, omega, nsweeps);
+__device__ void blk_stag_smooth(const SLevelDesc<T> &L, int visc, T omega, int nsweeps, int alo, int ahi) {
+  if (L.g.dim == 3) blk_stag_smooth_d<T, 3>(L, visc, omega, nsweeps, alo, ahi);
+  else blk_stag_smooth_d<T, 2>(L, visc, omega, nsweeps, alo, ahi);
 }
 
 template <typename T>
-__device__ void blk_prolong(const SLevelDesc<T> &F, const SLevelDesc<T> &C) {
+__device__ void blk_prolong(const SLevelDesc<T> &F, const SLevelDesc<T> &C, int alo, int ahi) {
   const long N = F.g.N;
   if (F.g.dim == 2) {
-    for (long t = threadIdx.x; t < N; t += CTPB) prolong_at<T, 2, 0>(F.g, C.g, F.phi.c[0], C.phi.c[0], t);
-    for (long t = threadIdx.x; t < N; t += CTPB) prolong_at<T, 2, 1>(F.g, C.g, F.phi.c[1], C.phi.c[1], t);
+    if (alo <= 0 && 0 < ahi)
+      for (long t = threadIdx.x; t < N; t += CTPB) prolong_at<T, 2, 0>(F.g, C.g, F.phi.c[0], C.phi.c[0], t);
+    if (alo <= 1 && 1 < ahi)
+      for (long t = threadIdx.x; t < N; t += CTPB) prolong_at<T, 2, 1>(F.g, C.g, F.phi.c[1], C.phi.c[1], t);
   } else {
-    for (long t = threadIdx.x; t < N; t += CTPB) prolong_at<T, 3, 0>(F.g, C.g, F.phi.c[0], C.phi.c[0], t);
-    for (long t = threadIdx.x; t < N; t += CTPB) prolong_at<T, 3, 1>(F.g, C.g, F.phi.c[1], C.phi.c[1], t);
-    for (long t = threadIdx.x; t < N; t += CTPB) prolong_at<T, 3, 2>(F.g, C.g, F.phi.c[2], C.phi.c[2], t);
+    if (alo <= 0 && 0 < ahi)
+      for (long t = threadIdx.x; t < N; t += CTPB) prolong_at<T, 3, 0>(F.g, C.g, F.phi.c[0], C.phi.c[0], t);
+    if (alo <= 1 && 1 < ahi)
+      for (long t = threadIdx.x; t < N; t += CTPB) prolong_at<T, 3, 1>(F.g, C.g, F.phi.c[1], C.phi.c[1], t);
+    if (alo <= 2 && 2 < ahi)
+      for (long t = threadIdx.x; t < N; t += CTPB) prolong_at<T, 3, 2>(F.g, C.g, F.phi.c[2], C.phi.c[2], t);
   }
   __syncthreads();
 }
 
 template <typename T>
 __global__ void __launch_bounds__(CTPB) k_stag_coarse_vcycle(SCoarseArgs<T> A) {
-  const int m = A.m;
+  const int m = A.m, dim = A.L[0].g.dim;
+  // the components this workgroup handles
+  const int alo = A.split ? (int)blockIdx.x : 0, ahi = A.split ? alo + 1 : dim;
   for (int n = 0; n < m; n++)
-    for (int d = 0; d < A.L[n].g.dim; d++) blk_fill<T>(A.L[n].phi.c[d], A.L[n].g.N, (T)0);
+    for (int d = alo; d < ahi; d++) blk_fill<T>(A.L[n].phi.c[d], A.L[n].g.N, (T)0);
   __syncthreads();
   for (int n = 0; n < m - 1; n++) {
     const SLevelDesc<T> &L = A.L[n];
-    blk_stag_smooth<T>(L, A.visc, A.omega, A.nd);
+    blk_stag_smooth<T>(L, A.visc, A.omega, A.nd, alo, ahi);
     const T dxsqinv = (T)(1.0 / (L.g.dx * L.g.dx));
-    for (int a = 0; a < L.g.dim; a++)
+    for (int a = alo; a < ahi; a++)
       for (long t = threadIdx.x; t < L.g.N; t += CTPB) residual_at<T>(L.g, A.visc, dxsqinv, L.phi, L.rhs, L.res, L.K, a, t);
     __syncthreads();
     const SLevelDesc<T> &Cn = A.L[n + 1];
-    for (int a = 0; a < Cn.g.dim; a++)
+    for (int a = alo; a < ahi; a++)
       for (long t = threadIdx.x; t < Cn.g.N; t += CTPB) restrict_at<T>(Cn.g, L.g, 0, L.res, Cn.rhs, a, t);
     __syncthreads();
   }
-  blk_stag_smooth<T>(A.L[m - 1], A.visc, A.omega, A.nb);
+  blk_stag_smooth<T>(A.L[m - 1], A.visc, A.omega, A.nb, alo, ahi);
   for (int n = m - 2; n >= 0; n--) {
-    blk_prolong<T>(A.L[n], A.L[n + 1]);
-    blk_stag_smooth<T>(A.L[n], A.visc, A.omega, A.nu);
+    blk_prolong<T>(A.L[n], A.L[n + 1], alo, ahi);
+    blk_stag_smooth<T>(A.L[n], A.visc, A.omega, A.nu, alo, ahi);
   }
 }
 
@@ -172,7 +185,10 @@ void launch_stag_coarse_vcycle(hipStream_t s, int m, const CoarseStagLevel<T> *l
     A.L[n].res = p3(lv[n].res);
     A.L[n].K = lv[n].K;
   }
-  hipLaunchKernelGGL(k_stag_coarse_vcycle<T>, dim3(1), dim3(CTPB), 0, s, A);
+  // visc_type -1 / 1: independent components, one workgroup each
+  const int dim = lv[0].g.dim;
+  A.split = (visc == -1 || visc == 1) && tuning().coarse_split ? 1 : 0;
+  hipLaunchKernelGGL(k_stag_coarse_vcycle<T>, dim3(A.split ? dim : 1), dim3(CTPB), 0, s, A);
   post_launch(s, kStagCoarseName);
 }
 

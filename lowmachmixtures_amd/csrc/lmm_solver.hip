@@ -1529,6 +1529,8 @@ static void apply_options(lmm_ctx *c, const lmm_kernel_options &o) {
   c->tune.rb_pairs = (pairs && pairs[0] == '1') ? 1 : 2;
   const char *pt = getenv("LMM_PROLONG_TILED");
   c->tune.prolong_tiled = (pt && pt[0] == '0') ? 0 : 1;
+  const char *cs = getenv("LMM_COARSE_SPLIT");
+  c->tune.coarse_split = (cs && cs[0] == '0') ? 0 : 1;
   c->precon_graph = o.precon_graph != 0;
   c->kopt = o;
   drop_graphs(c);
