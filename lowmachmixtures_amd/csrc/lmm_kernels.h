@@ -29,6 +29,7 @@ struct Tuning {
   int rb_pairs = 2;     // FP32 three-component sweep: point pairs per thread (2: 64 x 16 tiles)
   int prolong_tiled = 1;  // 3D staggered prolongation from coarse values staged in LDS
   int coarse_split = 1;   // coarse staggered V-cycle: one workgroup per component (visc_type -1 / 1)
+  int rr_three = 1;       // fused residual + restriction: the three components in one launch
 };
 // The settings the launchers on this thread use: those of the context whose
 // library call is running on it (TuningScope), else the defaults.

@@ -571,17 +571,10 @@ void launch_rb(hipStream_t s, const Grid &g, int tz, int tx, const T *pin, T *po
 // residuals in the order of the reference, so the coarse right-hand side is
 // bit for bit that of the two separate passes.
 template <typename T, typename P, int A>
-__global__ void __launch_bounds__(256) k_resid_restrict(Grid g, Grid gc, int tz, int ntx, int nty, int ntiles,
-                                                        const T *__restrict__ pin, T *__restrict__ cout,
-                                                        typename P::Args args0, const T *__restrict__ q0,
-                                                        const T *__restrict__ q1, const T *__restrict__ q2,
-                                                        const T *__restrict__ q3, const T *__restrict__ q4) {
+__device__ __forceinline__ void rr_tile(const Grid &g, const Grid &gc, int tz, int ntx, int nty, int tile,
+                                        const T *__restrict__ pin, T *__restrict__ cout, const typename P::Args &args,
+                                        T (*sp)[HY][HX], T (*sres)[FY + 1][FX + 1]) {
   using Coef = typename P::Coef;
-  const typename P::Args args = P::bind(args0, q0, q1, q2, q3, q4);
-  __shared__ T sp[4][HY][HX];
-  __shared__ T sres[4][FY + 1][FX + 1];  // residuals, row / column 0 = the low-side halo
-  int w = blockIdx.x, tile = w;
-  if (ntiles % 8 == 0) tile = (w % 8) * (ntiles / 8) + w / 8;
   const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
   const int i0 = bx * FX, j0 = by * FY, k0 = bz * tz;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
@@ -685,6 +678,52 @@ __global__ void __launch_bounds__(256) k_resid_restrict(Grid g, Grid gc, int tz,
     step(kk, pva, pvb);
     if (kk + 1 < ke) step(kk + 1, pvb, pva);
   }
+}
+
+template <typename T, typename P, int A>
+__global__ void __launch_bounds__(256) k_resid_restrict(Grid g, Grid gc, int tz, int ntx, int nty, int ntiles,
+                                                        const T *__restrict__ pin, T *__restrict__ cout,
+                                                        typename P::Args args0, const T *__restrict__ q0,
+                                                        const T *__restrict__ q1, const T *__restrict__ q2,
+                                                        const T *__restrict__ q3, const T *__restrict__ q4) {
+  __shared__ T sp[4][HY][HX];
+  __shared__ T sres[4][FY + 1][FX + 1];  // residuals, row / column 0 = the low-side halo
+  int w = blockIdx.x, tile = w;
+  if (ntiles % 8 == 0) tile = (w % 8) * (ntiles / 8) + w / 8;
+  rr_tile<T, P, A>(g, gc, tz, ntx, nty, tile, pin, cout, P::bind(args0, q0, q1, q2, q3, q4), sp, sres);
+}
+
+// The three components' residual + restriction in one launch: slots (tile,
+// component) numbered as in the three-component sweep, so the components of
+// a tile run side by side on one XCD and share the lines of beta_cc and the
+// edge arrays in its L2.  The residual reads phi only, so the components are
+// independent for every visc_type.
+template <typename T, int V>
+__global__ void __launch_bounds__(256)
+    k_resid_restrict3(Grid g, Grid gc, int tz, int ntx, int nty, int ntiles, const T *__restrict__ pin0,
+                      T *__restrict__ cout0, const T *__restrict__ pin1, T *__restrict__ cout1,
+                      const T *__restrict__ pin2, T *__restrict__ cout2, typename StagRB<T, 0, V>::Args a0,
+                      typename StagRB<T, 1, V>::Args a1, typename StagRB<T, 2, V>::Args a2,
+                      const T *__restrict__ q00, const T *__restrict__ q01, const T *__restrict__ q02,
+                      const T *__restrict__ q03, const T *__restrict__ q04, const T *__restrict__ q10,
+                      const T *__restrict__ q11, const T *__restrict__ q12, const T *__restrict__ q13,
+                      const T *__restrict__ q14, const T *__restrict__ q20, const T *__restrict__ q21,
+                      const T *__restrict__ q22, const T *__restrict__ q23, const T *__restrict__ q24) {
+  using P0 = StagRB<T, 0, V>;
+  using P1 = StagRB<T, 1, V>;
+  using P2 = StagRB<T, 2, V>;
+  __shared__ T sp[4][HY][HX];
+  __shared__ T sres[4][FY + 1][FX + 1];
+  const int nslots = 3 * ntiles, w = blockIdx.x;
+  int slot = w;
+  if (nslots % 8 == 0) slot = (w % 8) * (nslots / 8) + w / 8;
+  const int tile = slot / 3, comp = slot - 3 * tile;
+  if (comp == 0)
+    rr_tile<T, P0, 0>(g, gc, tz, ntx, nty, tile, pin0, cout0, P0::bind(a0, q00, q01, q02, q03, q04), sp, sres);
+  else if (comp == 1)
+    rr_tile<T, P1, 1>(g, gc, tz, ntx, nty, tile, pin1, cout1, P1::bind(a1, q10, q11, q12, q13, q14), sp, sres);
+  else
+    rr_tile<T, P2, 2>(g, gc, tz, ntx, nty, tile, pin2, cout2, P2::bind(a2, q20, q21, q22, q23, q24), sp, sres);
 }
 
 template <typename T, int A, int VISC>
@@ -1242,11 +1281,8 @@ static int resid_restrict_tz(const Grid &g, const Grid &gc, int es) {
 
 namespace {
 template <typename T, int A, int VISC>
-void launch_rr_one(hipStream_t s, const Grid &g, const Grid &gc, int tz, T *const phi_all[3], const T *rhs, T *crhs,
-                   const StagCoef<T> &K) {
-  using P = StagRB<T, A, VISC>;
-  const T *phi = phi_all[A];
-  typename P::Args a = {};
+typename StagRB<T, A, VISC>::Args rr_args(const Grid &g, T *const phi_all[3], const T *rhs, const StagCoef<T> &K) {
+  typename StagRB<T, A, VISC>::Args a = {};
   a.omega = (T)0;
   a.dxsqinv = (T)(1.0 / (g.dx * g.dx));
   a.rhs = rhs;
@@ -1261,6 +1297,33 @@ void launch_rr_one(hipStream_t s, const Grid &g, const Grid &gc, int tz, T *cons
     a.e1 = (VISC == -1) ? K.bed[A == 2 ? 1 : 0] : nullptr;
     a.e2 = (VISC == -1) ? K.bed[A == 0 ? 1 : 2] : nullptr;
   }
+  return a;
+}
+
+template <typename T, int VISC>
+void launch_rr_three(hipStream_t s, const Grid &g, const Grid &gc, int tz, T *const phi[3], T *const rhs[3],
+                     T *const crhs[3], const StagCoef<T> &K) {
+  const auto a0 = rr_args<T, 0, VISC>(g, phi, rhs[0], K);
+  const auto a1 = rr_args<T, 1, VISC>(g, phi, rhs[1], K);
+  const auto a2 = rr_args<T, 2, VISC>(g, phi, rhs[2], K);
+  const T *q0[5], *q1[5], *q2[5];
+  StagRB<T, 0, VISC>::ptrs(a0, q0);
+  StagRB<T, 1, VISC>::ptrs(a1, q1);
+  StagRB<T, 2, VISC>::ptrs(a2, q2);
+  const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
+  const int ntiles = ntx * nty * ntz;
+  hipLaunchKernelGGL((k_resid_restrict3<T, VISC>), dim3(3 * ntiles), dim3(256), 0, s, g, gc, tz, ntx, nty, ntiles,
+                     phi[0], crhs[0], phi[1], crhs[1], phi[2], crhs[2], a0, a1, a2, q0[0], q0[1], q0[2], q0[3], q0[4],
+                     q1[0], q1[1], q1[2], q1[3], q1[4], q2[0], q2[1], q2[2], q2[3], q2[4]);
+  post_launch(s, kResidRestrictName);
+}
+
+template <typename T, int A, int VISC>
+void launch_rr_one(hipStream_t s, const Grid &g, const Grid &gc, int tz, T *const phi_all[3], const T *rhs, T *crhs,
+                   const StagCoef<T> &K) {
+  using P = StagRB<T, A, VISC>;
+  const T *phi = phi_all[A];
+  const typename P::Args a = rr_args<T, A, VISC>(g, phi_all, rhs, K);
   const T *q[5];
   P::ptrs(a, q);
   const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
@@ -1282,6 +1345,10 @@ bool launch_stag_resid_restrict(hipStream_t s, const Grid &g, const Grid &gc, in
   const int tz = resid_restrict_tz(g, gc, sizeof(T));
 #define RR(V)                                                         \
   do {                                                                \
+    if (tuning().rr_three) {                                          \
+      launch_rr_three<T, V>(s, g, gc, tz, phi, rhs, crhs, K);         \
+      break;                                                          \
+    }                                                                 \
     launch_rr_one<T, 0, V>(s, g, gc, tz, phi, rhs[0], crhs[0], K);    \
     launch_rr_one<T, 1, V>(s, g, gc, tz, phi, rhs[1], crhs[1], K);    \
     launch_rr_one<T, 2, V>(s, g, gc, tz, phi, rhs[2], crhs[2], K);    \

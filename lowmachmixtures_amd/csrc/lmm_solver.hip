@@ -1531,6 +1531,8 @@ static void apply_options(lmm_ctx *c, const lmm_kernel_options &o) {
   c->tune.prolong_tiled = (pt && pt[0] == '0') ? 0 : 1;
   const char *cs = getenv("LMM_COARSE_SPLIT");
   c->tune.coarse_split = (cs && cs[0] == '0') ? 0 : 1;
+  const char *r3 = getenv("LMM_RR_THREE");
+  c->tune.rr_three = (r3 && r3[0] == '0') ? 0 : 1;
   c->precon_graph = o.precon_graph != 0;
   c->kopt = o;
   drop_graphs(c);
