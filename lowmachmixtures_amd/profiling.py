@@ -86,12 +86,17 @@ def fetch_factor(fetch):
 # reads every array it needs once and writes each output once.  A V-cycle's
 # first finest-level sweep starts from the zero guess without reading phi:
 # the sweeps are averaged over a V-cycle's four (13 + 3 * 16 arrays for the
-# three-component sweep, 5 + 3 * 6 for the cell-centred one).
+# three-component sweep, 5 + 3 * 6 for the cell-centred one).  The fused
+# residual + restriction writes 1/8 of an array per component.
 def level0_arrays(name):
     if "k_rb_march2x3" in name and ", -1>" in name:
         return 15.25, 3
     if "k_rb_march2<" in name and "CCRB" in name:
         return 5.75, 1
+    if "k_resid_restrict3" in name and ", -1>" in name:
+        # three components: phi, rhs, alpha each, beta and the three edge
+        # arrays once, the three coarse right-hand sides written
+        return 13.375, 0.375
     if "k_resid_restrict" in name and ", -1>" in name:
         return 6.125, 0.125
     return None, None
