@@ -570,11 +570,13 @@ void launch_rb(hipStream_t s, const Grid &g, int tz, int tx, const T *pin, T *po
 // the operator of stag_applyop_3d and each coarse value sums its 12 fine
 // residuals in the order of the reference, so the coarse right-hand side is
 // bit for bit that of the two separate passes.
-template <typename T, typename P, int A>
+template <typename T, typename P, int A, int TX = 64>
 __device__ __forceinline__ void rr_tile(const Grid &g, const Grid &gc, int tz, int ntx, int nty, int tile,
                                         const T *__restrict__ pin, T *__restrict__ cout, const typename P::Args &args,
-                                        T (*sp)[HY][HX], T (*sres)[FY + 1][FX + 1]) {
+                                        T (*sp)[512 / TX + 4][TX + 4], T (*sres)[512 / TX + 1][TX + 1]) {
   using Coef = typename P::Coef;
+  // tile TX x (512 / TX): 64 x 8, or 32 x 16 on levels 32 points wide
+  constexpr int FX = TX, FY = 512 / TX, HX = FX + 4, HY = FY + 4;
   const int bx = tile % ntx, by = (tile / ntx) % nty, bz = tile / (ntx * nty);
   const int i0 = bx * FX, j0 = by * FY, k0 = bz * tz;
   const int nx = g.n[0], ny = g.n[1], nz = g.n[2];
@@ -698,7 +700,7 @@ __global__ void __launch_bounds__(256) k_resid_restrict(Grid g, Grid gc, int tz,
 // a tile run side by side on one XCD and share the lines of beta_cc and the
 // edge arrays in its L2.  The residual reads phi only, so the components are
 // independent for every visc_type.
-template <typename T, int V>
+template <typename T, int V, int TX>
 __global__ void __launch_bounds__(256)
     k_resid_restrict3(Grid g, Grid gc, int tz, int ntx, int nty, int ntiles, const T *__restrict__ pin0,
                       T *__restrict__ cout0, const T *__restrict__ pin1, T *__restrict__ cout1,
@@ -712,18 +714,18 @@ __global__ void __launch_bounds__(256)
   using P0 = StagRB<T, 0, V>;
   using P1 = StagRB<T, 1, V>;
   using P2 = StagRB<T, 2, V>;
-  __shared__ T sp[4][HY][HX];
-  __shared__ T sres[4][FY + 1][FX + 1];
+  __shared__ T sp[4][512 / TX + 4][TX + 4];
+  __shared__ T sres[4][512 / TX + 1][TX + 1];
   const int nslots = 3 * ntiles, w = blockIdx.x;
   int slot = w;
   if (nslots % 8 == 0) slot = (w % 8) * (nslots / 8) + w / 8;
   const int tile = slot / 3, comp = slot - 3 * tile;
   if (comp == 0)
-    rr_tile<T, P0, 0>(g, gc, tz, ntx, nty, tile, pin0, cout0, P0::bind(a0, q00, q01, q02, q03, q04), sp, sres);
+    rr_tile<T, P0, 0, TX>(g, gc, tz, ntx, nty, tile, pin0, cout0, P0::bind(a0, q00, q01, q02, q03, q04), sp, sres);
   else if (comp == 1)
-    rr_tile<T, P1, 1>(g, gc, tz, ntx, nty, tile, pin1, cout1, P1::bind(a1, q10, q11, q12, q13, q14), sp, sres);
+    rr_tile<T, P1, 1, TX>(g, gc, tz, ntx, nty, tile, pin1, cout1, P1::bind(a1, q10, q11, q12, q13, q14), sp, sres);
   else
-    rr_tile<T, P2, 2>(g, gc, tz, ntx, nty, tile, pin2, cout2, P2::bind(a2, q20, q21, q22, q23, q24), sp, sres);
+    rr_tile<T, P2, 2, TX>(g, gc, tz, ntx, nty, tile, pin2, cout2, P2::bind(a2, q20, q21, q22, q23, q24), sp, sres);
 }
 
 template <typename T, int A, int VISC>
@@ -1267,13 +1269,19 @@ void launch_cc_rb_fused(hipStream_t s, const Grid &g, const T *pin, T *pout, con
   launch_rb<T, CCRB<T>>(s, g, tz, tx, pin, pout, args);
 }
 
-// z-slab thickness of the fused residual + restriction, or 0 when the level
-// cannot use it (3D, x a multiple of 64, y of 8, z of 2, coarse = fine / 2)
-static int resid_restrict_tz(const Grid &g, const Grid &gc, int es) {
-  if (g.dim != 3 || g.n[0] % FX != 0 || g.n[1] % FY != 0 || g.n[2] % 2 != 0) return 0;
+// z-slab thickness and tile width of the fused residual + restriction, or 0
+// when the level cannot use it (3D, x a multiple of 64 and y of 8, or -- the
+// three-component launch only -- x of 32 and y of 16; z of 2; coarse = fine / 2)
+static int resid_restrict_tz(const Grid &g, const Grid &gc, int es, int &tx) {
+  tx = FX;
+  if (g.dim != 3 || g.n[2] % 2 != 0) return 0;
   for (int d = 0; d < 3; d++)
     if (gc.n[d] * 2 != g.n[d]) return 0;
-  const int per_plane_set = (g.n[0] / FX) * (g.n[1] / FY);
+  if (g.n[0] % FX != 0 || g.n[1] % FY != 0) {
+    if (!tuning().rr_three || g.n[0] % FX2 != 0 || g.n[1] % FY2 != 0) return 0;
+    tx = FX2;
+  }
+  const int per_plane_set = (g.n[0] / tx) * (g.n[1] / (512 / tx));
   int tz = 32;
   while (tz > 2 && (g.n[2] % tz != 0 || per_plane_set * (g.n[2] / tz) < rb_target(es))) tz /= 2;
   return (g.n[2] % tz == 0) ? tz : 0;
@@ -1301,7 +1309,7 @@ typename StagRB<T, A, VISC>::Args rr_args(const Grid &g, T *const phi_all[3], co
 }
 
 template <typename T, int VISC>
-void launch_rr_three(hipStream_t s, const Grid &g, const Grid &gc, int tz, T *const phi[3], T *const rhs[3],
+void launch_rr_three(hipStream_t s, const Grid &g, const Grid &gc, int tz, int tx, T *const phi[3], T *const rhs[3],
                      T *const crhs[3], const StagCoef<T> &K) {
   const auto a0 = rr_args<T, 0, VISC>(g, phi, rhs[0], K);
   const auto a1 = rr_args<T, 1, VISC>(g, phi, rhs[1], K);
@@ -1310,11 +1318,13 @@ void launch_rr_three(hipStream_t s, const Grid &g, const Grid &gc, int tz, T *co
   StagRB<T, 0, VISC>::ptrs(a0, q0);
   StagRB<T, 1, VISC>::ptrs(a1, q1);
   StagRB<T, 2, VISC>::ptrs(a2, q2);
-  const int ntx = g.n[0] / FX, nty = g.n[1] / FY, ntz = g.n[2] / tz;
+  const int fy = 512 / tx;
+  const int ntx = g.n[0] / tx, nty = g.n[1] / fy, ntz = g.n[2] / tz;
   const int ntiles = ntx * nty * ntz;
-  hipLaunchKernelGGL((k_resid_restrict3<T, VISC>), dim3(3 * ntiles), dim3(256), 0, s, g, gc, tz, ntx, nty, ntiles,
-                     phi[0], crhs[0], phi[1], crhs[1], phi[2], crhs[2], a0, a1, a2, q0[0], q0[1], q0[2], q0[3], q0[4],
-                     q1[0], q1[1], q1[2], q1[3], q1[4], q2[0], q2[1], q2[2], q2[3], q2[4]);
+  auto kern = (tx == FX2) ? k_resid_restrict3<T, VISC, FX2> : k_resid_restrict3<T, VISC, FX>;
+  hipLaunchKernelGGL(kern, dim3(3 * ntiles), dim3(256), 0, s, g, gc, tz, ntx, nty, ntiles, phi[0], crhs[0], phi[1],
+                     crhs[1], phi[2], crhs[2], a0, a1, a2, q0[0], q0[1], q0[2], q0[3], q0[4], q1[0], q1[1], q1[2],
+                     q1[3], q1[4], q2[0], q2[1], q2[2], q2[3], q2[4]);
   post_launch(s, kResidRestrictName);
 }
 
@@ -1335,18 +1345,20 @@ void launch_rr_one(hipStream_t s, const Grid &g, const Grid &gc, int tz, T *cons
 }  // namespace
 
 bool stag_resid_restrict_ok(const Grid &g, const Grid &gc, int visc) {
-  return (visc == -1 || visc == 1 || visc == 2 || visc == 3) && resid_restrict_tz(g, gc, 4) > 0;
+  int tx;
+  return (visc == -1 || visc == 1 || visc == 2 || visc == 3) && resid_restrict_tz(g, gc, 4, tx) > 0;
 }
 
 template <typename T>
 bool launch_stag_resid_restrict(hipStream_t s, const Grid &g, const Grid &gc, int visc, T *const phi[3],
                                 T *const rhs[3], T *const crhs[3], const StagCoef<T> &K) {
   if (!stag_resid_restrict_ok(g, gc, visc)) return false;
-  const int tz = resid_restrict_tz(g, gc, sizeof(T));
+  int tx = FX;
+  const int tz = resid_restrict_tz(g, gc, sizeof(T), tx);
 #define RR(V)                                                         \
   do {                                                                \
     if (tuning().rr_three) {                                          \
-      launch_rr_three<T, V>(s, g, gc, tz, phi, rhs, crhs, K);         \
+      launch_rr_three<T, V>(s, g, gc, tz, tx, phi, rhs, crhs, K);     \
       break;                                                          \
     }                                                                 \
     launch_rr_one<T, 0, V>(s, g, gc, tz, phi, rhs[0], crhs[0], K);    \
