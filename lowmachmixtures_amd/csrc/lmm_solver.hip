@@ -3184,7 +3184,7 @@ static int check_step(lmm_ctx *c, const char *what, lmm_field *umac[3], lmm_fiel
 }
 
 // A time step's temporaries, views of the context's scratch c->iw: 10 face
-// vectors, 2 face vectors of ns components, ns + 5 cell arrays, ne
+// vectors, 2 face vectors of ns components, 5 cell arrays, ne
 // node/edge arrays.
 struct StepVec {
   lmm_field f[3];
@@ -3193,7 +3193,7 @@ struct StepVec {
 };
 struct StepWork {
   StepVec mold, mtemp, m_a, m_d, m_s, rv, dumac, rfo, gradpi, rf, rho_fc, flux_total;
-  lmm_field ru, bp, dpi, divu, eh, kh;
+  lmm_field bp, dpi, divu, eh, kh;
   lmm_field eeh[3];
   lmm_field *eehp[3];
 };
@@ -3202,7 +3202,7 @@ static int step_work(lmm_ctx *c, const char *what, StepWork &W) {
   const int ns = c->species.nspecies, dim = c->g.dim, ne = ne_of(dim);
   const int ek = dim == 2 ? LMM_NODE : LMM_EDGE_XY;
   const long N = c->g.N;
-  const long need = (long)(10 * dim + 2 * dim * ns + ns + 5 + ne) * N;
+  const long need = (long)(10 * dim + 2 * dim * ns + 5 + ne) * N;
   if (c->iw_n < need) {
     dfree(c->iw);
     c->iw = dalloc<double>(c->stream, need);
@@ -3226,7 +3226,6 @@ static int step_work(lmm_ctx *c, const char *what, StepWork &W) {
     faces(*v, 1);
   faces(W.rho_fc, ns);
   faces(W.flux_total, ns);
-  W.ru = lmm_field{c, LMM_CELL, ns, take((long)ns * N)};
   W.bp = lmm_field{c, LMM_CELL, 1, take(N)};
   W.dpi = lmm_field{c, LMM_CELL, 1, take(N)};
   W.divu = lmm_field{c, LMM_CELL, 1, take(N)};
@@ -3267,7 +3266,6 @@ int lmm_advance_timestep_inertial(lmm_ctx *c, lmm_field *umac[3], lmm_field *rho
   lmm_field &bp_f = W.bp, &dpi_f = W.dpi, &divu_f = W.divu, &eh_f = W.eh, &kh_f = W.kh;
   lmm_field *eeh_f = W.eeh;
   lmm_field **eeh = W.eehp;
-  (void)mtemp;
   double *u[3] = {nullptr, nullptr, nullptr};
   for (int d = 0; d < dim; d++) u[d] = umac[d]->d;
   hipStream_t s = c->stream;
