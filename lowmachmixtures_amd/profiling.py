@@ -170,31 +170,43 @@ def summarize(d, n, command, out_trace, out_counters):
 
 
 def inertial_steps(d, n, command, out_trace):
-    """Kernel-trace summary of the bench's inertial_step line: its steps are
-    delimited by their first k_adv_s launch (two per step: predictor and
-    corrector); the bench runs 4 FP64 steps then 4 FP32-MG steps, the first of
-    each a warm-up.  Per-step wall time (first to last dispatch), kernel busy
-    time and the per-step kernel breakdown of the timed steps."""
+    """Kernel-trace summary of the bench's inertial_step line: inertial steps
+    are delimited by their first k_adv_s launch (two per step: predictor and
+    corrector), overdamped steps by their first k_zero_minus; the bench runs 4 FP64 inertial steps, 4 FP32-MG inertial
+    steps, then 4 + 4 overdamped steps, the first of each group a warm-up.
+    Per-step wall time (first to last dispatch), kernel busy time and the
+    per-step kernel breakdown of each group's timed steps."""
     rows = sorted(((s_, e_, name) for name, s_, e_ in _trace_rows(d)), key=lambda r: r[0])
-    adv = [i for i, r in enumerate(rows) if "k_adv_s" in r[2]]
-    starts = adv[0::2]
-    # a step starts a few dispatches before its first k_adv_s (the face averages of rho)
-    bounds = [max(0, i - 8) for i in starts] + [len(rows)]
-    steps = [rows[bounds[k]:bounds[k + 1]] for k in range(len(starts))]
-    t = [f"rocprofv3 kernel trace: the inertial time step, 3D periodic {n}^3, 4 species (bench.py inertial_step line)",
+    # the overdamped steps are the ones with k_zero_minus (0 - grad pi, once
+    # per stage); an inertial step starts a few dispatches before its first
+    # k_adv_s (the face averages of rho), an overdamped one a few before its
+    # first k_zero_minus (face averages, noise fill, grad pi)
+    zm = [i for i, r in enumerate(rows) if "k_zero_minus" in r[2]]
+    first_od = zm[0] if zm else len(rows)
+    adv = [i for i, r in enumerate(rows) if "k_adv_s" in r[2] and i < first_od]
+    bounds = [max(0, i - 8) for i in adv[0::2]] + [max(0, i - 12) for i in zm[0::2]] + [len(rows)]
+    steps = [rows[bounds[k]:bounds[k + 1]] for k in range(len(bounds) - 1)]
+    t = [f"rocprofv3 kernel trace: the inertial and overdamped time steps, 3D periodic {n}^3, 4 species "
+         "(bench.py inertial_step line)",
          "", "Command (from the repository root on an MI355X, gfx950):",
          f"  cd /tmp && export TMPDIR=/tmp && rocprofv3 --kernel-trace --stats --output-format csv -d <dir> -o run -- {command}",
          "Reduced by python -m lowmachmixtures_amd.profiling steps <dir> <n> <command> <out> from run_kernel_trace.csv:",
-         "steps delimited by their first k_adv_s dispatch; FP64 steps first, then FP32-MG, the first of each a warm-up;",
-         "a form's last step runs to the next form's first dispatch (downloads and the next context's set-up).",
+         "inertial steps delimited by their first k_adv_s dispatch, overdamped ones by their first k_zero_minus;",
+         "groups of inertial FP64, inertial FP32-MG, overdamped FP64, overdamped FP32-MG steps, the first of each a",
+         "warm-up; a group's last step runs to the next group's first dispatch (downloads, the next context's set-up).",
          ""]
-    for k, st in enumerate(steps):
-        wall = (st[-1][1] - st[0][0]) * 1e-6
-        busy = sum(e_ - s_ for s_, e_, _ in st) * 1e-6
-        t.append(f"  step {k} ({'fp64' if k < len(steps) // 2 else 'fp32_mg'}): wall {wall:7.2f} ms, kernel busy "
-                 f"{busy:7.2f} ms, {len(st)} dispatches")
-    half = len(steps) // 2
-    for label, sel in (("FP64 solves", steps[1:half]), ("FP32-MG solves", steps[half + 1:])):
+    labels = ["inertial, FP64 solves", "inertial, FP32-MG solves", "overdamped, FP64 solves",
+              "overdamped, FP32-MG solves"]
+    ngroups = 4 if len(steps) >= 16 else 2
+    per = max(1, len(steps) // ngroups)
+    groups = [steps[g * per:(g + 1) * per] for g in range(ngroups)]
+    for g, grp in enumerate(groups):
+        for k, st in enumerate(grp):
+            wall = (st[-1][1] - st[0][0]) * 1e-6
+            busy = sum(e_ - s_ for s_, e_, _ in st) * 1e-6
+            t.append(f"  step {k} ({labels[g]}): wall {wall:7.2f} ms, kernel busy {busy:7.2f} ms, "
+                     f"{len(st)} dispatches")
+    for label, sel in ((labels[g], grp[1:]) for g, grp in enumerate(groups)):
         if not sel:
             continue
         tot = defaultdict(lambda: [0, 0])
